@@ -1,0 +1,318 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+The reference ships no tests, fixtures or golden vectors, so each oracle function is checked here against
+an independent fact: polynomial exactness and the Nyquist symbol of the stencils, the C-PML profile shapes
+recovered from the tables, closed-form properties of the Ricker wavelet, the isotropic reduction and the
+analytic 2-D Green's function of the forward, a dot-product test and finite-difference directional
+derivatives of the exact adjoint, and the relation between the reference-compatible and exact gradients.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from fwi_amd import models as M
+
+
+def _f(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+# ------------------------------------------------------------------------------------------------
+# Stencils (parallelcalls.cu:57, :127)
+# ------------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_d2_polynomial_exactness(mode):
+    """8th-order D2 is exact on polynomials of degree <= 9 (error term ~ h^8 u^(10))."""
+    nx, nz, h = 40, 24, 0.5
+    x = (np.arange(nx) - nx / 2) * h / 10.0
+    for deg in range(0, 10):
+        u = np.tile(x ** deg, (nz, 1))
+        P = _f(u)
+        F = _f(u.T[:nz, :nx] if False else u)
+        p2 = np.zeros((nz, nx), np.float32)
+        r2 = np.zeros((nz, nx), np.float32)
+        O.lib().orc_SecondDerivate_P(mode, O.ptr(P), O.ptr(F), O.ptr(p2), O.ptr(r2), np.float32(h * h), nx, nz)
+        # d2/dx2 of (ix*h/10 - c)^deg with respect to the grid coordinate ix*h
+        exact = deg * (deg - 1) * x ** max(deg - 2, 0) / 100.0 if deg >= 2 else 0 * x
+        got = p2[10, 4:nx - 4]
+        ref = np.broadcast_to(exact, (nx,))[4:nx - 4]
+        scale = max(np.abs(u).max() / (h * h), 1.0)
+        assert np.abs(got - ref).max() <= 2e-5 * scale, (deg, np.abs(got - ref).max())
+
+
+def test_d2_nyquist_symbol():
+    """On (-1)^ix the D2 stencil returns (-205/72 - 2*8/5 - 2/5 - 2*8/315 - 2/560)/h^2 = -6.50159/h^2."""
+    nx, nz = 32, 16
+    u = np.tile((-1.0) ** np.arange(nx), (nz, 1))
+    p2 = np.zeros((nz, nx), np.float32)
+    r2 = np.zeros((nz, nx), np.float32)
+    O.lib().orc_SecondDerivate_P(0, O.ptr(_f(u)), O.ptr(_f(u)), O.ptr(p2), O.ptr(r2), np.float32(1.0), nx, nz)
+    assert abs(p2[8, 10] - (-6.501587301587)) < 1e-5
+    # z-derivative of a field constant in z is zero
+    assert np.abs(r2[4:-4, 4:-4]).max() < 1e-12
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_d1_polynomial_exactness(mode):
+    """8th-order D1 (via PSI_F with a=1, b=0 so psi = D1 P) is exact on degree <= 8."""
+    nx, nz, cpml, h = 40, 40, 40, 1.0
+    x = (np.arange(nx) - nx / 2) / 8.0
+    ones = np.ones(nx, np.float32)
+    zeros = np.zeros(nx, np.float32)
+    for deg in range(0, 9):
+        P = _f(np.tile(x ** deg, (nz, 1)))
+        F = _f(np.tile((x ** deg)[:, None], (1, nx)))  # same profile along z
+        psx = np.zeros((nz, nx), np.float32)
+        psz = np.zeros((nz, nx), np.float32)
+        O.lib().orc_PSI_F(mode, O.ptr(P), O.ptr(F), O.ptr(ones), O.ptr(zeros), O.ptr(ones), O.ptr(zeros),
+                          O.ptr(psx), O.ptr(psz), nx, nz, np.float32(h), cpml)
+        exact = deg * x ** max(deg - 1, 0) / 8.0 if deg >= 1 else 0 * x
+        ref = np.broadcast_to(exact, (nx,))[4:nx - 4]
+        tol = 2e-5 * max(np.abs(P).max(), 1.0)
+        assert np.abs(psx[12, 4:nx - 4] - ref).max() <= tol, deg
+        assert np.abs(psz[4:nx - 4, 12] - ref).max() <= tol, deg
+
+
+# ------------------------------------------------------------------------------------------------
+# C-PML table (parallelcalls.cu:18-46)
+# ------------------------------------------------------------------------------------------------
+
+def test_cpml_table_profiles():
+    """Invert the table: d+alpha = -ln(b)/dt and d = a (d+alpha)/(b-1).  The damping must be quadratic in
+    the PML depth, alpha linear (zero at the outer edge), and the outer edge gives a = b - 1."""
+    p = M.CpmlParams(cpml=20, dh=10.0, dt=1e-3, f=10.0)
+    a, b = O.cpml_table(p, round_b=False)
+    assert np.all(b > 0) and np.all(b < 1) and np.all(a <= 0)
+    s = -np.log(b) / p.dt
+    d = a * s / (b - 1)
+    al = s - d
+    depth = (p.cpml - np.arange(p.cpml)) * p.dh / p.L
+    np.testing.assert_allclose(d, p.d0 * p.vmax * depth ** 2, rtol=1e-9)
+    np.testing.assert_allclose(al, math.pi * p.f * (1 - depth), rtol=1e-9, atol=1e-9)
+    assert abs(a[0] - (b[0] - 1)) < 1e-12
+    # theoretical reflection coefficient of the quadratic profile: exp(-2/3 d0 L) = R (at unit speed)
+    assert abs(math.exp(-2.0 / 3.0 * p.d0 * p.L) - p.R) / p.R < 1e-5
+
+
+# ------------------------------------------------------------------------------------------------
+# Ricker (main.c:328-333)
+# ------------------------------------------------------------------------------------------------
+
+def test_ricker_closed_form():
+    nt, f, dt = 1000, 10.0, 1e-3
+    s = O.ricker(nt, f, dt).astype(np.float64)
+    # peak 1 at t = 1/f, i.e. (it+1)*dt = 0.1 -> it = 99
+    assert int(np.argmax(s)) == 99 and abs(s[99] - 1.0) < 1e-6
+    # zero crossings at pi f |t - 1/f| = 1/sqrt(2)
+    tz = 1 / f + 1 / (math.sqrt(2) * math.pi * f)
+    iz = int(round(tz / dt)) - 1
+    assert abs(s[iz]) < 0.05 and s[iz - 3] * s[iz + 3] < 0
+    # zero mean, symmetry about the peak
+    assert abs(s.sum() * dt) < 1e-4
+    np.testing.assert_allclose(s[99 - 50:99], s[100:150][::-1], atol=1e-5)
+    np.testing.assert_array_equal(O.ricker(nt, f, dt), M.ricker(nt, f, dt))
+
+
+# ------------------------------------------------------------------------------------------------
+# Forward
+# ------------------------------------------------------------------------------------------------
+
+def _homog(nx, nz, v, eps=0.0, dl=0.0):
+    return (np.full((nz, nx), v, np.float32), np.full((nz, nx), eps, np.float32), np.full((nz, nx), dl, np.float32))
+
+
+@pytest.mark.parametrize("mode", ["mix", "f32", "f64"])
+def test_isotropic_reduction(mode):
+    """epsilon = delta = 0 gives v_hor = v_nmo = v_po, so P and F stay identical: the gather is 2P."""
+    nx, nz, nt, cpml = 64, 56, 200, 12
+    p = M.CpmlParams(cpml=cpml, dh=10.0, dt=1e-3, f=12.0)
+    T = np.float64 if mode == "f64" else np.float32
+    vp, eps, dl = M.gradient_model(nx, nz, seed=1, cpml=cpml, sigma=6.0)
+    sq = O.pre_variables(vp.astype(T), 0 * eps.astype(T), 0 * dl.astype(T), dtype=T)
+    assert np.array_equal(sq[0], sq[1]) and np.array_equal(sq[2], sq[1])
+    ta, tb = O.cpml_table(p)
+    src = O.ricker(nt, p.f, p.dt)
+    shot = M.reference_shot(nx, cpml, 30, M.depth_row(2))
+    g = O.forward(mode, sq, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    assert np.isfinite(g).all() and np.abs(g).max() > 0.1
+
+
+def _green2d_trace(r, v, t, q):
+    """Analytic P(r, t) = int q(tau) G(r, t - tau) dtau for P_tt = v^2 lap P + q(t) delta(x), with
+    G = H(vt - r) / (2 pi v sqrt(v^2 t^2 - r^2)); substitution t - tau = (r/v) cosh w removes the
+    singularity: P = (1 / 2 pi v^2) int_0^{acosh(vt/r)} q(t - (r/v) cosh w) dw."""
+    out = np.zeros_like(t)
+    for k, tk in enumerate(t):
+        if v * tk <= r:
+            continue
+        W = math.acosh(v * tk / r)
+        w = np.linspace(0.0, W, 4001)
+        out[k] = np.trapezoid(q(tk - (r / v) * np.cosh(w)), w) / (2 * math.pi * v * v)
+    return out
+
+
+def test_forward_matches_2d_green_function():
+    """Homogeneous isotropic medium: the FP64 forward trace at 400 m offset agrees with the analytic 2-D
+    Green's function convolved with the discrete source (correlation > 0.999, amplitude within 2 %)."""
+    nx = nz = 200
+    cpml, dh, dt, f, v, nt = 20, 10.0, 1e-3, 8.0, 2500.0, 520
+    p = M.CpmlParams(cpml=cpml, dh=dh, dt=dt, f=f)
+    sq = O.pre_variables(*[x.astype(np.float64) for x in _homog(nx, nz, v)], dtype=np.float64)
+    ta, tb = O.cpml_table(p)
+    src = O.ricker(nt, f, dt).astype(np.float64)
+    isx = isz = 100
+    shot = M.Shot(0, nx, isx, isz, isz, cpml, nx - 2 * cpml)
+    g = O.forward("f64", sq, (ta, tb), src, nx, nz, nt, cpml, dh, dt, shot)
+    off = 40
+    trace = g[:, isx + off - cpml]
+    # source term: step n adds s[n] to the cell, i.e. q(n dt) = s[n] dh^2 / dt^2 with s[n] = ricker((n+1) dt)
+    def q(tau):
+        tt = tau + dt
+        a = math.pi * f * (tt - 1 / f)
+        val = (1 - 2 * a * a) * np.exp(-a * a)
+        return np.where(tau >= 0, val, 0.0) * dh * dh / (dt * dt)
+    t = (np.arange(nt) + 1) * dt
+    ana = 2.0 * _green2d_trace(off * dh, v, t, q)  # gather = P + F = 2P
+    corr = np.dot(trace, ana) / (np.linalg.norm(trace) * np.linalg.norm(ana))
+    ratio = np.linalg.norm(trace) / np.linalg.norm(ana)
+    assert corr > 0.999, corr
+    assert abs(ratio - 1) < 0.02, ratio
+
+
+def test_forward_linear_in_source():
+    nx, nz, nt, cpml = 56, 48, 160, 10
+    p = M.CpmlParams(cpml=cpml, dh=10.0, dt=1e-3, f=12.0)
+    vp, eps, dl = M.gradient_model(nx, nz, seed=3, cpml=cpml, sigma=5.0)
+    sq = O.pre_variables(vp.astype(np.float64), eps.astype(np.float64), dl.astype(np.float64), dtype=np.float64)
+    ta, tb = O.cpml_table(p)
+    rng = np.random.default_rng(0)
+    s1 = rng.standard_normal(nt)
+    s2 = rng.standard_normal(nt)
+    shot = M.reference_shot(nx, cpml, 25, M.depth_row(3))
+    args = ((ta, tb),)
+    g1 = O.forward("f64", sq, *args, s1, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    g2 = O.forward("f64", sq, *args, s2, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    g12 = O.forward("f64", sq, *args, 2 * s1 - 3 * s2, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    np.testing.assert_allclose(g12, 2 * g1 - 3 * g2, rtol=0, atol=1e-10 * np.abs(g12).max())
+
+
+# ------------------------------------------------------------------------------------------------
+# Exact adjoint and gradients
+# ------------------------------------------------------------------------------------------------
+
+def _small_setup(nx=48, nz=40, cpml=10, nt=300, seed=2):
+    p = M.CpmlParams(cpml=cpml, dh=10.0, dt=1e-3, f=12.0)
+    ta, tb = O.cpml_table(p)
+    vp, eps, dl = M.gradient_model(nx, nz, seed=seed, cpml=cpml, sigma=5.0)
+    vt = M.gradient_model(nx, nz, seed=seed + 3, cpml=cpml, sigma=5.0)
+    return p, ta, tb, (vp, eps, dl), vt
+
+
+def test_exact_adjoint_dot_product():
+    """<M x, y> = <x, M^T y> for one linear forward step including all C-PML memory variables (FP64)."""
+    nx, nz, cpml = 48, 40, 10
+    p, ta, tb, m, _ = _small_setup(nx, nz, cpml)
+    sq = O.pre_variables(*[x.astype(np.float64) for x in m], dtype=np.float64)
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz))
+    om = (X >= 4) & (X < nx - 4) & (Z >= 4) & (Z < nz - 4)
+    chix = om & ((X < cpml) | (X > nx - cpml - 1))
+    chiz = om & (Z > nz - cpml - 1)
+    rng = np.random.default_rng(7)
+
+    def rnd():
+        masks = [om] * 4 + [chix, chix, chiz, chiz]
+        return np.ascontiguousarray(np.stack([rng.standard_normal((nz, nx)) * mk for mk in masks]))
+
+    x, y = rnd(), rnd()
+    Mx = np.zeros_like(x)
+    MTy = np.zeros_like(y)
+    L = O.lib()
+    L.orc_step_fwd_f64(O.ptr(sq[0]), O.ptr(sq[1]), O.ptr(sq[2]), O.ptr(ta), O.ptr(tb), nx, nz, cpml, p.dh, p.dt,
+                       O.ptr(x), O.ptr(Mx))
+    L.orc_step_adj_f64(O.ptr(sq[0]), O.ptr(sq[1]), O.ptr(sq[2]), O.ptr(ta), O.ptr(tb), nx, nz, cpml, p.dh, p.dt,
+                       O.ptr(y), O.ptr(MTy))
+    a, b = np.sum(Mx * y), np.sum(x * MTy)
+    assert abs(a - b) / abs(a) < 1e-12
+    # a planted error in the transpose (C-PML memory adjoint dropped) must fail the same test
+    y2 = y.copy()
+    y2[4:] = 0
+    MTy2 = np.zeros_like(y2)
+    L.orc_step_adj_f64(O.ptr(sq[0]), O.ptr(sq[1]), O.ptr(sq[2]), O.ptr(ta), O.ptr(tb), nx, nz, cpml, p.dh, p.dt,
+                       O.ptr(y2), O.ptr(MTy2))
+    assert abs(np.sum(x * MTy2) - b) / abs(b) > 1e-6
+
+
+def _fd_check(exact_mode=True):
+    nx, nz, cpml, nt = 48, 40, 10, 300
+    p, ta, tb, m, vt = _small_setup(nx, nz, cpml, nt)
+    src = O.ricker(nt, p.f, p.dt).astype(np.float64)
+    shot = M.reference_shot(nx, cpml, 24, M.depth_row(2))
+    sqt = O.pre_variables(*[x.astype(np.float64) for x in vt], dtype=np.float64)
+    obs = O.forward("f64", sqt, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    m0 = [x.astype(np.float64) for x in m]
+
+    def J(mm):
+        s = O.pre_variables(*mm, dtype=np.float64)
+        g = O.forward("f64", s, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt, shot)
+        return 0.5 * np.sum((g - obs) ** 2)
+
+    sq = O.pre_variables(*m0, dtype=np.float64)
+    phi, _, _, gh, gp, gn = O.gradient("f64", exact_mode, sq, (ta, tb), src, obs, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    return J, m0, phi, O.chain_rule(*m0, gh, gp, gn), (nx, nz)
+
+
+def test_exact_gradient_fd_directional_derivatives():
+    """Central finite differences of the FP64 misfit along smooth directions of vp, epsilon and delta
+    converge to <grad, dm> with O(h^2) until round-off."""
+    J, m0, phi, grads, (nx, nz) = _fd_check(True)
+    assert abs(phi - J(m0)) <= 1e-12 * phi
+    dm = M.smooth_perturbation(nx, nz, seed=4, sigma=3.0)
+    scale = [300.0, 0.05, 0.05]
+    for comp in range(3):
+        d = [np.zeros((nz, nx)) for _ in range(3)]
+        d[comp] = dm[comp] * scale[comp]
+        pred = sum(np.sum(g * dd) for g, dd in zip(grads, d))
+        errs = []
+        for h in (1e-2, 1e-3, 1e-4):
+            fd = (J([a + h * b for a, b in zip(m0, d)]) - J([a - h * b for a, b in zip(m0, d)])) / (2 * h)
+            errs.append(abs(fd - pred) / abs(pred))
+        assert errs[2] < 1e-7, (comp, errs)
+        assert errs[0] / errs[1] > 50, (comp, errs)  # second-order convergence
+
+
+def test_refcompat_gradient_relates_to_exact():
+    """The reference's gradient is the exact one scaled by 1/dt with a one-step lag of the forward field:
+    away from the C-PML the two agree in shape (correlation > 0.99) and roughly in size."""
+    nx, nz, cpml, nt = 48, 40, 10, 300
+    p, ta, tb, m, vt = _small_setup(nx, nz, cpml, nt)
+    src = O.ricker(nt, p.f, p.dt).astype(np.float64)
+    shot = M.reference_shot(nx, cpml, 24, M.depth_row(2))
+    sqt = O.pre_variables(*[x.astype(np.float64) for x in vt], dtype=np.float64)
+    obs = O.forward("f64", sqt, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    sq = O.pre_variables(*[x.astype(np.float64) for x in m], dtype=np.float64)
+    _, _, _, eh, ep, en = O.gradient("f64", True, sq, (ta, tb), src, obs, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    _, _, _, rh, rp, rn = O.gradient("f64", False, sq, (ta, tb), src, obs, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz))
+    inner = (X >= cpml + 4) & (X < nx - cpml - 4) & (Z >= 4) & (Z < nz - cpml - 4)
+    for e, r in ((eh, rh), (ep, rp), (en, rn)):
+        a, b = e[inner], r[inner] * p.dt
+        assert np.corrcoef(a, b)[0, 1] > 0.99
+        assert 0.7 < np.linalg.norm(b) / np.linalg.norm(a) < 1.4
+
+
+def test_misfit_reference_float_sum_close_to_fp64():
+    J, m0, phi, _, _ = _fd_check(True)
+    nx, nz, cpml, nt = 48, 40, 10, 300
+    p, ta, tb, m, vt = _small_setup(nx, nz, cpml, nt)
+    src = O.ricker(nt, p.f, p.dt)
+    shot = M.reference_shot(nx, cpml, 24, M.depth_row(2))
+    sqt = O.pre_variables(*vt)
+    obs = O.forward("mix", sqt, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    sq = O.pre_variables(*m)
+    phi64, phi32, syn, *_ = O.gradient("mix", False, sq, (ta, tb), src, obs, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    res = syn.astype(np.float64) - obs.astype(np.float64)
+    assert abs(phi64 - 0.5 * np.sum(res ** 2)) < 1e-9 * phi64
+    assert abs(phi32 - phi64) < 1e-4 * phi64
