@@ -1,0 +1,269 @@
+#!/usr/bin/env python3
+"""Benchmark: one full FWI gradient of the 1600x400, 64-shot survey per step on one MI355X.
+
+A "step" is one gradient (all shots: forward with checkpoints, residual, FP64 misfit, adjoint with
+recomputation, imaging, FP64 cross-shot sum) of the batched path.  value = Gcell-updates/s (fwd+adj) =
+2 * sum_shots(nx*nz) * nt / seconds per gradient; recomputation is not counted as work.
+
+Beside it, on the same workload:
+  * plain per-shot GPU path: the reference's per-shot flow on the legacy C ABI (launchPropagation with a
+    pinned-host snapshot every step, launchBackPropagation), timed on one shot for a bounded number of
+    steps and scaled to the full survey (it is linear in shots and steps);
+  * roofline of the dominant kernel (the adjoint step): algorithmic bytes per launch over its average
+    duration, measured with HIP events on the stream the kernel runs on during the timed region;
+  * cpu_baseline: the CPU oracle (f32, exact gradient, OpenMP) on one shot for a bounded number of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="survey1600")
+    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--nt", type=int, default=0)
+    ap.add_argument("--mode", choices=["exact", "refcompat"], default="exact")
+    ap.add_argument("--ckpt", type=int, default=0)
+    ap.add_argument("--plain-steps", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timing-stride", type=int, default=4)
+    return ap.parse_args()
+
+
+def plain_path_seconds(w, model_sq_dev, src_dev, ab, nsteps):
+    """Reference flow on the legacy ABI for one shot and `nsteps` steps: forward with a pinned snapshot
+    per step (launchPropagation), residual, adjoint with the snapshot copy back (launchBackPropagation)."""
+    import numpy as np
+    import torch
+
+    from fwi_amd import _lib
+
+    L = _lib.load()
+    nx, nz, cpml = w.nx, w.nz, w.cpml
+    n = nx * nz
+    nt = nsteps
+    shot = w.shots[len(w.shots) // 2]
+    z = lambda k: torch.zeros(k, dtype=torch.float32, device="cuda")
+    names = ("p2", "r2", "psx", "psz", "zx", "zz", "atp", "atf", "PF", "aux1", "aux2", "aux3", "aux4", "p2cx", "p2cz",
+             "r2dx", "r2cz", "psixcx", "psixcz", "atenxcx", "atenxcz", "atenzdx", "atenzcz", "gcx", "gcz", "gdx")
+    f = {k: z(n) for k in names}
+    ax, bx, az, bz = z(nx), z(nx), z(nz), z(nz)
+    p = w.cpml_params
+    L.CPML_x_h(ax.data_ptr(), bx.data_ptr(), cpml, p.vmax, nx, p.dt, p.dh, p.f, p.d0, p.L)
+    L.CPML_z_h(az.data_ptr(), bz.data_ptr(), cpml, p.vmax, nz, p.dt, p.dh, p.f, p.d0, p.L)
+    vh, vp, vn = model_sq_dev
+    Pa, Pb, Pre2 = z(2 * n), z(2 * n), z(2 * n)
+    PFh = torch.empty(2 * nt * n, dtype=torch.float32, pin_memory=True)
+    shot1 = z(nx * nt)
+    shot2 = z(nx * nt)
+    Res = z(nx * nt)
+    dh = np.float32(w.dh)
+    dh2 = dh * dh
+    dt = np.float32(w.dt)
+    C = np.float32(dt * dt) / dh2
+    s1 = _lib.create_blocking_stream()
+    s2 = _lib.create_blocking_stream()
+    ptr = lambda k: f[k].data_ptr()
+    sx, sz = shot.isx + 1, shot.isz + 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Ppre, Ppast = Pa, Pb
+    for it in range(nt):
+        L.launchPropagation(s1, s2, PFh.data_ptr() + 8 * it * n, ptr("PF"), Ppast.data_ptr(), Ppre.data_ptr(),
+                            Ppast.data_ptr() + 4 * n, Ppre.data_ptr() + 4 * n, ptr("p2"), ptr("r2"), dh2, nx, nz,
+                            ax.data_ptr(), bx.data_ptr(), az.data_ptr(), bz.data_ptr(), ptr("psx"), ptr("psz"), dh,
+                            cpml, ptr("zx"), ptr("zz"), ptr("atp"), ptr("atf"), vh.data_ptr(), vp.data_ptr(),
+                            vn.data_ptr(), src_dev.data_ptr(), shot2.data_ptr(), C, sx, sz, it)
+        Ppre, Ppast = Ppast, Ppre
+    L.res_h(Res.data_ptr(), shot1.data_ptr(), shot2.data_ptr(), nx, nt)
+    torch.cuda.synchronize()
+    Pa.zero_()
+    Pb.zero_()
+    Ppre, Ppast = Pa, Pb
+    for it in range(nt):
+        L.launchBackPropagation(s1, s2, nx, nz, nt, PFh.data_ptr() + 8 * (nt - it - 1) * n, Ppre.data_ptr(),
+                                Ppast.data_ptr(), Ppre.data_ptr() + 4 * n, Ppast.data_ptr() + 4 * n, ptr("aux1"),
+                                ptr("aux2"), ptr("aux3"), ptr("aux4"), vh.data_ptr(), vp.data_ptr(), vn.data_ptr(),
+                                ptr("p2cx"), ptr("p2cz"), ptr("r2dx"), ptr("r2cz"), dh2, Pre2.data_ptr(),
+                                Pre2.data_ptr() + 4 * n, ax.data_ptr(), bx.data_ptr(), az.data_ptr(), bz.data_ptr(),
+                                ptr("psixcx"), ptr("psixcz"), ptr("psx"), ptr("psz"), dh, cpml, ptr("zx"), ptr("zz"),
+                                ptr("atf"), ptr("atp"), ptr("atenxcx"), ptr("atenxcz"), ptr("atenzdx"),
+                                ptr("atenzcz"), Res.data_ptr(), C, ptr("PF"), it, sx, sz, dt, ptr("gcx"), ptr("gcz"),
+                                ptr("gdx"))
+        Ppre, Ppast = Ppast, Ppre
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    _lib.check_last("plain path")
+    _lib.destroy_stream(s1)
+    _lib.destroy_stream(s2)
+    del PFh
+    return sec
+
+
+def cpu_baseline(w, nsteps):
+    """Oracle (f32 arithmetic, exact gradient, OpenMP over all host threads) on one shot of the workload
+    for nsteps steps: forward + adjoint + imaging.  Returns (Gcell-updates/s, threads, seconds)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    shot = w.shots[len(w.shots) // 2]
+    tab = O.cpml_table(w.cpml_params)
+    src = O.ricker(nsteps, w.f, w.dt)
+    sq = O.pre_variables(*w.init_model)
+    obs = np.zeros((nsteps, shot.nrec), np.float32)
+    t0 = time.perf_counter()
+    O.gradient("f32", True, sq, tab, src, obs, shot.nx, w.nz, nsteps, w.cpml, w.dh, w.dt, shot)
+    sec = time.perf_counter() - t0
+    return 2.0 * shot.nx * w.nz * nsteps / sec / 1e9, threads, sec
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (shots would shard across GPUs "
+              "with one FP64 all-reduce of the gradient; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from fwi_amd import models as M
+    from fwi_amd.survey import Plan, cpml_table, pre_variables
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    w = M.workload(args.workload, nshots=args.shots, nt=args.nt or None)
+    nt = w.nt
+    log(f"workload {w.name}: {w.nx}x{w.nz}, {len(w.shots)} shots, nt={nt}, dt={w.dt}, f={w.f}")
+    ab = cpml_table(w.cpml_params)
+    src = M.ricker(nt, w.f, w.dt)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+    true_sq = pre_variables(*(dev(x) for x in w.true_model))
+    init_sq = pre_variables(*(dev(x) for x in w.init_model))
+    grad_mode = 1 if args.mode == "exact" else 0
+    plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt)
+    log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, workspace {plan.workspace_bytes / 1e9:.1f} GB")
+    t0 = time.perf_counter()
+    obs = plan.forward(true_sq)
+    torch.cuda.synchronize()
+    log(f"observed data modelled in {time.perf_counter() - t0:.2f} s (includes tile tuning)")
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        plan.gradient(init_sq, obs)
+        torch.cuda.synchronize()
+        log(f"warmup gradient {i}: {time.perf_counter() - t0:.3f} s")
+    plan.set_timing(args.timing_stride)
+    torch.cuda.synchronize()
+    times = []
+    phi = None
+    t_all = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        phi, grad, _ = plan.gradient(init_sq, obs)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f"gradient {i}: {times[-1]:.3f} s, misfit {phi:.6e}")
+    total = time.perf_counter() - t_all
+    sec = total / args.steps
+    ncell, nstrip = plan.cells()
+    work = 2.0 * ncell * nt
+    gcells = work / sec / 1e9
+    assert np.isfinite(phi) and torch.isfinite(grad).all().item()
+
+    # roofline of the adjoint step kernel (dominant): algorithmic bytes per launch / mean event time
+    adj_ms, adj_n = plan.kernel_time(2)
+    fwd_ms, fwd_n = plan.kernel_time(0)
+    rec_ms, rec_n = plan.kernel_time(1)
+    nmodel = w.nx * w.nz
+    adj_bytes = 56.0 * ncell + 12.0 * nmodel
+    adj_avg = adj_ms / max(adj_n, 1) / 1e3
+    achieved = adj_bytes / adj_avg / 1e9 if adj_n else None
+    fwd_bytes = 24.0 * ncell + 12.0 * nmodel
+    fwd_avg = fwd_ms / max(fwd_n, 1) / 1e3
+    log(f"adjoint step: {adj_n} timed launches, mean {adj_avg * 1e6:.1f} us -> {achieved:.0f} GB/s algorithmic")
+    log(f"forward step: {fwd_n} timed launches, mean {fwd_avg * 1e6:.1f} us -> "
+        f"{fwd_bytes / max(fwd_avg, 1e-12) / 1e9:.0f} GB/s; storing step mean "
+        f"{rec_ms / max(rec_n, 1) * 1e3:.1f} us")
+
+    plain = None
+    if not args.no_plain:
+        ns = min(args.plain_steps, nt)
+        # the snapshot buffer of the plain path is 2*ns*nx*nz floats of pinned host memory
+        s_plain = plain_path_seconds(w, init_sq, torch.from_numpy(src[:ns].copy()).cuda(), ab, ns)
+        plain = s_plain * (nt / ns) * len(w.shots)
+        log(f"plain per-shot path: {s_plain:.3f} s for 1 shot x {ns} steps -> {plain:.1f} s per gradient")
+
+    cpu = None
+    if not args.no_cpu:
+        ns = min(args.cpu_steps, nt)
+        cg, threads, csec = cpu_baseline(w, ns)
+        cpu = {"value": round(cg, 4), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"oracle f32 exact gradient, 1 shot of {w.name} ({w.nx}x{w.nz}), {ns} of {nt} steps, "
+                         f"fwd+adj, {csec:.1f} s"}
+        log(f"cpu baseline: {cg:.3f} Gcell/s on {threads} threads ({csec:.1f} s)")
+
+    line = {
+        "metric": "Gcell-updates/s fwd+adj; s per FWI gradient, batched vs plain, 1 MI355X",
+        "value": round(gcells, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded layered VTI model with faults, Ricker source)",
+        "config": {
+            "workload": f"{w.name}: {w.nx}x{w.nz} dh={w.dh} m, {len(w.shots)} shots, {w.shots[0].nrec} receivers, "
+                        f"nt={nt}, dt={w.dt} s, {w.f} Hz Ricker, cpml={w.cpml}",
+            "grad_mode": args.mode,
+            "ckpt_interval": plan.ckpt_interval,
+            "batch_shots": plan.batch_size,
+            "rows_per_item": plan.rows_per_item,
+            "s_per_gradient_batched": round(sec, 4),
+            "s_per_gradient_plain": round(plain, 3) if plain else None,
+            "speedup_vs_plain": round(plain / sec, 2) if plain else None,
+            "misfit": phi,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_adj_ex_main" if grad_mode == 1 else "k_adj_ref_main",
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": None,
+            "bytes_per_launch": adj_bytes,
+            "mean_launch_us": round(adj_avg * 1e6, 2),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

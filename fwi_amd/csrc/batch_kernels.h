@@ -1,0 +1,144 @@
+// Device-side descriptors and kernel parameter blocks of the batched FWI path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fwi {
+
+// One shot of a batch as the kernels see it.
+struct ShotDev {
+  int nx;         // window width (cells)
+  int x0;         // model column of window column 0
+  int isx, isz;   // source cell
+  int irz, ir0;   // receiver row and first receiver column
+  int nrec;       // receivers
+  int nstrip;     // C-PML cells of the window (chi_x U chi_z)
+  long long off;  // offset of the shot's nx*nz fields in the batch field arrays (cells)
+  long long soff; // offset of the shot's strip cells in packed strip arrays
+  long long goff; // offset of the shot's gather [nt][nrec] in the batch gather arrays
+};
+
+// C-PML cells of a window: the side strips Omega & (ix < cpml | ix > nx-cpml-1) for rows 4..nz-5, then
+// the bottom strip rows nz-cpml..nz-5 for the middle columns cpml..nx-cpml-1.  Enumerated row by row.
+__host__ __device__ inline int strip_count(int nx, int nz, int cpml) {
+  int w = cpml - 4;
+  return 2 * w * (nz - 8) + (nx - 2 * cpml) * (cpml - 4);
+}
+__host__ __device__ inline void strip_cell(int k, int nx, int nz, int cpml, int& ix, int& iz) {
+  int w = cpml - 4;
+  int nside = 2 * w * (nz - 8);
+  if (k < nside) {
+    int row = k / (2 * w), j = k - row * 2 * w;
+    iz = 4 + row;
+    ix = j < w ? 4 + j : nx - cpml + (j - w);
+  } else {
+    int kk = k - nside, nmid = nx - 2 * cpml;
+    int row = kk / nmid;
+    iz = nz - cpml + row;
+    ix = cpml + (kk - row * nmid);
+  }
+}
+
+struct ModelArgs {
+  const float* vh;
+  const float* vp;
+  const float* vn;
+  int nxm;  // model width
+};
+
+struct GridArgs {
+  const ShotDev* shots;
+  int nz, cpml;
+  float dh, dh2;
+  const float* ta;  // C-PML a by distance from the edge
+  const float* tb;  // C-PML b
+};
+
+// Forward step.  P/F "cur" hold step n, "old" hold n-1 and receive n+1.
+struct FwdArgs {
+  GridArgs g;
+  ModelArgs m;
+  const int4* items;  // (shot, band x0, z0, z1)
+  int nitems;
+  float C;            // dt^2/dh^2
+  const float* src;
+  const float* Pc;
+  float* Po;
+  const float* Fc;
+  float* Fo;
+  float* psx;
+  float* psz;
+  float* zx;
+  float* zz;
+  float* gather;  // batch gathers or null
+  float* qa;      // per-step store or null
+  float* qb;
+  int it;
+};
+
+// Strip (C-PML) blocks: (shot, first strip cell) per 256-thread block.
+struct StripArgs {
+  GridArgs g;
+  ModelArgs m;
+  const int2* blocks;
+  int nblocks;
+};
+
+// Adjoint step, reference-compatible (4 auxiliary fields, forward-recursion C-PML).
+struct AdjRefArgs {
+  GridArgs g;
+  ModelArgs m;
+  const int4* items;
+  int nitems;
+  float dh2C;  // dh2*C
+  float dt;
+  const float* Lc;  // lambda_P cur
+  float* Lo;
+  const float* Mc;  // lambda_F cur
+  float* Mo;
+  float* s1;
+  float* s2;
+  float* s3;
+  float* s4;
+  float* z1;
+  float* z2;
+  float* z3;
+  float* z4;
+  const float* res;  // batch residual gathers
+  int rit;           // residual time index injected at this step
+  const float* qa;   // forward D2x P / D2z F of the paired step, or null
+  const float* qb;
+  float* gcx;
+  float* gcz;
+  float* gdx;
+};
+
+// Adjoint step, exact discrete transpose.
+struct AdjExArgs {
+  GridArgs g;
+  ModelArgs m;
+  const int4* items;
+  int nitems;
+  float dt2;
+  const float* Lc;
+  float* Lo;
+  const float* Mc;
+  float* Mo;
+  float* sx;   // psi-bar x
+  float* zx;   // zeta-bar x
+  float* sz;
+  float* zz;
+  float* ztx;  // chi_x (zeta-bar_in + a-bar) of this step
+  float* ztz;
+  float* ptx;  // a_x * psi-bar_tot of this step
+  float* ptz;
+  const float* res;
+  int rit;
+  const float* qa;  // forward a^n, b^n of the paired step, or null
+  const float* qb;
+  float* gh;
+  float* gp;
+  float* gn;
+};
+
+}  // namespace fwi

@@ -1,0 +1,69 @@
+// Shared device helpers for the FWI kernels (gfx950).
+//
+// All arithmetic is written in the evaluation order of the reference kernels (src/parallelcalls.cu) and
+// the library is compiled with -ffp-contract=off, so every stored value is the IEEE result of the written
+// expression.  Stencil sums use S = double for the reference's mixed precision (its coefficients are
+// double literals, parallelcalls.cu:57,127) or S = float for the FP32 path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fwi {
+
+constexpr int kRim = 4;     // stencil radius; the rim of 4 cells is never updated
+constexpr int kLayer = 15;  // rows iz <= 15 are excluded from model updates (include/parallelcalls.h:6)
+
+// 8th-order second derivative from the 9 samples u[-4..4]; pair sums in float, weighted sum in S.
+template <class S>
+__device__ __forceinline__ float d2s(float m4, float m3, float m2, float m1, float c, float p1, float p2,
+                                     float p3, float p4, float h2) {
+  const S c0 = (S)(205.0 / 72.0), c1 = (S)(8.0 / 5.0), c2 = (S)(1.0 / 5.0), c3 = (S)(8.0 / 315.0),
+          c4 = (S)(1.0 / 560.0);
+  S r = -c0 * (S)c + c1 * (S)(p1 + m1) - c2 * (S)(p2 + m2) + c3 * (S)(p3 + m3) - c4 * (S)(p4 + m4);
+  return (float)(r / (S)h2);
+}
+
+// 8th-order first derivative from u[-4..4] (the centre sample is not used).
+template <class S>
+__device__ __forceinline__ float d1s(float m4, float m3, float m2, float m1, float p1, float p2, float p3,
+                                     float p4, float h) {
+  const S c1 = (S)(4.0 / 5.0), c2 = (S)(1.0 / 5.0), c3 = (S)(4.0 / 105.0), c4 = (S)(1.0 / 280.0);
+  S r = c1 * (S)(p1 - m1) - c2 * (S)(p2 - m2) + c3 * (S)(p3 - m3) - c4 * (S)(p4 - m4);
+  return (float)(r / (S)h);
+}
+
+// Strided loads of the 9-point stencil around index i.
+template <class S>
+__device__ __forceinline__ float d2_at(const float* __restrict__ u, long i, long st, float h2) {
+  return d2s<S>(u[i - 4 * st], u[i - 3 * st], u[i - 2 * st], u[i - st], u[i], u[i + st], u[i + 2 * st],
+                u[i + 3 * st], u[i + 4 * st], h2);
+}
+template <class S>
+__device__ __forceinline__ float d1_at(const float* __restrict__ u, long i, long st, float h) {
+  return d1s<S>(u[i - 4 * st], u[i - 3 * st], u[i - 2 * st], u[i - st], u[i + st], u[i + 2 * st],
+                u[i + 3 * st], u[i + 4 * st], h);
+}
+
+// C-PML coefficient of cell i on an axis of n cells from the per-distance table tab[e], e = distance from
+// the edge (0 = outermost).  Both ends of x carry C-PML; along z only the bottom (the top is a free surface).
+__device__ __forceinline__ float coef_x(const float* __restrict__ tab, int i, int n, int cpml) {
+  if (i < cpml) return tab[i];
+  if (i > n - cpml - 1) return tab[n - 1 - i];
+  return 0.f;
+}
+__device__ __forceinline__ float coef_z(const float* __restrict__ tab, int i, int n, int cpml) {
+  return (i > n - cpml - 1) ? tab[n - 1 - i] : 0.f;
+}
+
+// Map a linear workgroup id onto a logical id so that the workgroups one XCD receives (hardware deals
+// them round-robin over 8 XCDs) form one contiguous block of logical work.  Bijective for any n.
+__device__ __forceinline__ int xcd_logical_block(int bid, int n) {
+  const int nx = 8;
+  int q = n / nx, r = n % nx;
+  int xcd = bid % nx, k = bid / nx;
+  return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
+}  // namespace fwi

@@ -1,0 +1,761 @@
+// Host orchestration of the batched FWI path (include/fwi.h).
+//
+// Replaces the per-shot loop of src/main.c:396-538: every batch of shots runs
+//   forward:  nt steps (one C-PML strip launch + one fused step launch each), a checkpoint of the state
+//             before every K-th step, the model-multiplied terms of the last segment kept;
+//   residual: res = syn - obs and the FP64 misfit;
+//   adjoint:  nt steps backwards; whenever the imaging needs a forward step that is not kept, the
+//             segment is recomputed from its checkpoint (K forward steps storing the imaging terms);
+//   sum:      FP64 cross-shot sum of the per-shot gradients into the model gradient.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fwi.h"
+#include "batch_kernels.h"
+#include "fwi_common.h"
+#include "fwi_internal.h"
+
+#include "batch_kernels.hip"
+
+using namespace fwi;
+
+namespace {
+
+constexpr int kResidualBlocks = 1024;
+
+struct Batch {
+  int s0 = 0, count = 0;
+  long long ncell = 0, nstrip = 0, ngather = 0, gather_off = 0;
+  ShotDev* d_shots = nullptr;
+  int4* d_items = nullptr;
+  int nitems = 0;
+  int2* d_sblocks = nullptr;
+  int nsblocks = 0;
+  std::vector<ShotDev> h_shots;
+  std::vector<int2> h_sblocks;
+};
+
+#define CK(x)                                          \
+  do {                                                 \
+    hipError_t e_ = (x);                               \
+    if (e_ != hipSuccess) return fail(e_, #x);         \
+  } while (0)
+
+}  // namespace
+
+struct fwi_plan {
+  // survey
+  int nx = 0, nz = 0, nt = 0, cpml = 0;
+  float dh = 0, dt = 0;
+  std::vector<fwi_shot> shots;
+  std::vector<float> wavelet, ta, tb;
+  fwi_options opt{};
+  int K = 1, J = 1;  // checkpoint interval and number of segments
+  int tz = 32;       // rows per work item
+  bool tuned = false;
+  std::vector<Batch> batches;
+  long long gather_total = 0;
+  // device
+  float* d_src = nullptr;
+  float* d_ta = nullptr;
+  float* d_tb = nullptr;
+  float* d_work = nullptr;  // all per-cell workspace of the largest batch
+  size_t work_bytes = 0;
+  long long cap_cell = 0, cap_strip = 0, cap_gather = 0;
+  double* d_partial = nullptr;
+  double* d_misfit = nullptr;
+  // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute),
+  // 2 = adjoint step; every `timing_stride`-th launch of a kind is bracketed by events on its stream
+  int timing_stride = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_open;  // (kind, event index of the start; end = start + 1)
+  double kt_ms[3] = {0, 0, 0};
+  long long kt_count[3] = {0, 0, 0};
+  long long launch_no[3] = {0, 0, 0};
+
+  int timed_begin(int kind, hipStream_t st) {
+    if (timing_stride <= 0 || (launch_no[kind]++ % timing_stride) != 0) return -1;
+    size_t idx = ev_open.size() * 2;
+    while (ev_pool.size() < idx + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      ev_pool.push_back(e);
+    }
+    (void)hipEventRecord(ev_pool[idx], st);
+    ev_open.push_back({kind, (int)idx});
+    return (int)idx;
+  }
+  void timed_end(int idx, hipStream_t st) {
+    if (idx >= 0) (void)hipEventRecord(ev_pool[idx + 1], st);
+  }
+  void timing_collect() {
+    if (ev_open.empty()) return;
+    (void)hipEventSynchronize(ev_pool[ev_open.back().second + 1]);
+    for (auto& o : ev_open) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ev_pool[o.second], ev_pool[o.second + 1]) == hipSuccess) {
+        kt_ms[o.first] += ms;
+        kt_count[o.first] += 1;
+      }
+    }
+    ev_open.clear();
+  }
+
+  // workspace views (sized for cap_cell)
+  float* fwd(int k) const { return d_work + (size_t)k * cap_cell; }                 // 0..7
+  float* adj(int k) const { return d_work + (size_t)(8 + k) * cap_cell; }           // 0..11
+  float* sgrad(int k) const { return d_work + (size_t)(20 + k) * cap_cell; }         // 0..2
+  float* qbuf() const { return d_work + (size_t)23 * cap_cell; }                    // K * 2 * cap_cell
+  float* ckpt(int j) const {                                                        // j = 1..J-1
+    return d_work + (size_t)(23 + 2 * (size_t)K) * cap_cell + (size_t)(j - 1) * (4 * cap_cell + 4 * cap_strip);
+  }
+  float* gathers() const {
+    return d_work + (size_t)(23 + 2 * (size_t)K) * cap_cell + (size_t)(J - 1) * (4 * cap_cell + 4 * cap_strip);
+  }
+  float* resid() const { return gathers() + cap_gather; }
+  std::string err;
+
+  int fail(hipError_t e, const char* what) {
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    set_message(err);
+    return (int)e;
+  }
+  int fail(const std::string& msg) {
+    err = msg;
+    set_message(err);
+    return 1;
+  }
+
+  ~fwi_plan() {
+    for (auto& b : batches) {
+      (void)hipFree(b.d_shots);
+      (void)hipFree(b.d_items);
+      (void)hipFree(b.d_sblocks);
+    }
+    (void)hipFree(d_src);
+    (void)hipFree(d_ta);
+    (void)hipFree(d_tb);
+    (void)hipFree(d_work);
+    (void)hipFree(d_partial);
+    (void)hipFree(d_misfit);
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  int build_items(Batch& b, int rows) {
+    std::vector<int4> items;
+    for (int s = 0; s < b.count; ++s) {
+      const ShotDev& sd = b.h_shots[s];
+      for (int z0 = kRim; z0 < nz - kRim; z0 += rows) {
+        int z1 = std::min(z0 + rows, nz - kRim);
+        for (int x0 = 0; x0 < sd.nx; x0 += 64) items.push_back(make_int4(s, x0, z0, z1));
+      }
+    }
+    (void)hipFree(b.d_items);
+    b.d_items = nullptr;
+    b.nitems = (int)items.size();
+    CK(hipMalloc(&b.d_items, items.size() * sizeof(int4)));
+    CK(hipMemcpy(b.d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    return 0;
+  }
+
+  GridArgs grid_args(const Batch& b) const {
+    GridArgs g;
+    g.shots = b.d_shots;
+    g.nz = nz;
+    g.cpml = cpml;
+    g.dh = dh;
+    g.dh2 = dh * dh;
+    g.ta = d_ta;
+    g.tb = d_tb;
+    return g;
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  // one forward step of a batch
+  int fwd_step(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
+               hipStream_t st) {
+    const int cur = n & 1;
+    float* Pc = fwd(cur);
+    float* Po = fwd(cur ^ 1);
+    float* Fc = fwd(2 + cur);
+    float* Fo = fwd(2 + (cur ^ 1));
+    StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
+    const bool mixed = opt.arith == FWI_ARITH_MIXED;
+    if (b.nsblocks > 0) {
+      if (mixed)
+        hipLaunchKernelGGL(k_fwd_strip<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fwd(4), fwd(5));
+      else
+        hipLaunchKernelGGL(k_fwd_strip<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fwd(4), fwd(5));
+    }
+    FwdArgs a;
+    a.g = grid_args(b);
+    a.m = m;
+    a.items = b.d_items;
+    a.nitems = b.nitems;
+    a.C = (dt * dt) / (dh * dh);
+    a.src = d_src;
+    a.Pc = Pc;
+    a.Po = Po;
+    a.Fc = Fc;
+    a.Fo = Fo;
+    a.psx = fwd(4);
+    a.psz = fwd(5);
+    a.zx = fwd(6);
+    a.zz = fwd(7);
+    a.gather = gather;
+    a.qa = qa;
+    a.qb = qb;
+    a.it = n;
+    dim3 grid((b.nitems + 3) / 4);
+    const int tev = timed_begin(qm == 0 ? 0 : 1, st);
+#define FWD_LAUNCH(QMV, SV) hipLaunchKernelGGL((k_fwd_main<QMV, SV>), grid, dim3(256), 0, st, a)
+    if (mixed) {
+      if (qm == 0) FWD_LAUNCH(0, double);
+      else if (qm == 1) FWD_LAUNCH(1, double);
+      else FWD_LAUNCH(2, double);
+    } else {
+      if (qm == 0) FWD_LAUNCH(0, float);
+      else if (qm == 1) FWD_LAUNCH(1, float);
+      else FWD_LAUNCH(2, float);
+    }
+#undef FWD_LAUNCH
+    timed_end(tev, st);
+    CK(hipGetLastError());
+    return 0;
+  }
+
+  // one adjoint step k (produces m^k from m^{k+1}, m^{k+2}); q slot pointers may be null
+  int adj_step(const Batch& b, const ModelArgs& m, int k, const float* qa, const float* qb, hipStream_t st) {
+    const int cur = k & 1;
+    float* Lc = adj(cur);
+    float* Lo = adj(cur ^ 1);
+    float* Mc = adj(2 + cur);
+    float* Mo = adj(2 + (cur ^ 1));
+    const bool mixed = opt.arith == FWI_ARITH_MIXED;
+    StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
+    dim3 grid((b.nitems + 3) / 4);
+    if (opt.grad_mode == FWI_GRAD_REFCOMPAT) {
+      if (b.nsblocks > 0) {
+        if (mixed)
+          hipLaunchKernelGGL(k_adj_ref_strip<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, Lc, Mc, adj(4), adj(5),
+                             adj(6), adj(7));
+        else
+          hipLaunchKernelGGL(k_adj_ref_strip<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, Lc, Mc, adj(4), adj(5),
+                             adj(6), adj(7));
+      }
+      AdjRefArgs a;
+      a.g = grid_args(b);
+      a.m = m;
+      a.items = b.d_items;
+      a.nitems = b.nitems;
+      a.dh2C = (dh * dh) * ((dt * dt) / (dh * dh));
+      a.dt = dt;
+      a.Lc = Lc;
+      a.Lo = Lo;
+      a.Mc = Mc;
+      a.Mo = Mo;
+      a.s1 = adj(4);
+      a.s2 = adj(5);
+      a.s3 = adj(6);
+      a.s4 = adj(7);
+      a.z1 = adj(8);
+      a.z2 = adj(9);
+      a.z3 = adj(10);
+      a.z4 = adj(11);
+      a.res = resid();
+      a.rit = k - 1;
+      a.qa = qa;
+      a.qb = qb;
+      a.gcx = sgrad(0);
+      a.gcz = sgrad(1);
+      a.gdx = sgrad(2);
+      const int tev = timed_begin(2, st);
+      if (mixed)
+        hipLaunchKernelGGL(k_adj_ref_main<double>, grid, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL(k_adj_ref_main<float>, grid, dim3(256), 0, st, a);
+      timed_end(tev, st);
+    } else {
+      AdjExArgs a;
+      a.g = grid_args(b);
+      a.m = m;
+      a.items = b.d_items;
+      a.nitems = b.nitems;
+      a.dt2 = dt * dt;
+      a.Lc = Lc;
+      a.Lo = Lo;
+      a.Mc = Mc;
+      a.Mo = Mo;
+      a.sx = adj(4);
+      a.zx = adj(5);
+      a.sz = adj(6);
+      a.zz = adj(7);
+      a.ztx = adj(8);
+      a.ztz = adj(9);
+      a.ptx = adj(10);
+      a.ptz = adj(11);
+      a.res = resid();
+      a.rit = k - 1;
+      a.qa = qa;
+      a.qb = qb;
+      a.gh = sgrad(0);
+      a.gp = sgrad(1);
+      a.gn = sgrad(2);
+      if (b.nsblocks > 0) {
+        hipLaunchKernelGGL(k_adj_ex_strip1, dim3(b.nsblocks), dim3(256), 0, st, sa, a.dt2, Lc, Mc, a.zx, a.zz, a.ztx,
+                           a.ztz);
+        if (mixed)
+          hipLaunchKernelGGL(k_adj_ex_strip2<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, a);
+        else
+          hipLaunchKernelGGL(k_adj_ex_strip2<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, a);
+      }
+      const int tev = timed_begin(2, st);
+      if (mixed)
+        hipLaunchKernelGGL(k_adj_ex_main<double>, grid, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL(k_adj_ex_main<float>, grid, dim3(256), 0, st, a);
+      timed_end(tev, st);
+    }
+    CK(hipGetLastError());
+    return 0;
+  }
+
+  int strip_copy(const Batch& b, int dir, float* packed, hipStream_t st) {
+    if (b.nsblocks == 0) return 0;
+    StripArgs sa{grid_args(b), ModelArgs{nullptr, nullptr, nullptr, nx}, b.d_sblocks, b.nsblocks};
+    hipLaunchKernelGGL(k_strip_copy, dim3(b.nsblocks), dim3(256), 0, st, sa, dir, fwd(4), fwd(5), fwd(6), fwd(7),
+                       packed, b.nstrip);
+    CK(hipGetLastError());
+    return 0;
+  }
+
+  int zero_fwd(const Batch& b, hipStream_t st) {
+    CK(hipMemsetAsync(fwd(0), 0, 8 * (size_t)cap_cell * sizeof(float), st));
+    (void)b;
+    return 0;
+  }
+
+  int save_ckpt(const Batch& b, int j, hipStream_t st) {
+    float* c = ckpt(j);
+    for (int f = 0; f < 4; ++f)
+      CK(hipMemcpyAsync(c + (size_t)f * b.ncell, fwd(f), b.ncell * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return strip_copy(b, 0, c + 4 * (size_t)b.ncell, st);
+  }
+  int load_ckpt(const Batch& b, int j, hipStream_t st) {
+    float* c = ckpt(j);
+    for (int f = 0; f < 4; ++f)
+      CK(hipMemcpyAsync(fwd(f), c + (size_t)f * b.ncell, b.ncell * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return strip_copy(b, 1, c + 4 * (size_t)b.ncell, st);
+  }
+
+  int residual(const Batch& b, const float* syn, const float* obs, float* res, double* out, int accumulate,
+               hipStream_t st) {
+    hipLaunchKernelGGL(k_residual, dim3(kResidualBlocks), dim3(256), 0, st, syn, obs + b.gather_off, res, b.ngather,
+                       d_partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, d_partial, kResidualBlocks, 0.5, out, accumulate);
+    CK(hipGetLastError());
+    return 0;
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  int forward_batch(const Batch& b, const ModelArgs& m, float* gather, hipStream_t st) {
+    if (int r = zero_fwd(b, st)) return r;
+    for (int n = 0; n < nt; ++n)
+      if (int r = fwd_step(b, m, n, 0, nullptr, nullptr, gather, st)) return r;
+    return 0;
+  }
+
+  int gradient_batch(const Batch& b, const ModelArgs& m, const float* obs, double* grad, int first_batch,
+                     float* syn_out, hipStream_t st) {
+    const int qm = opt.grad_mode == FWI_GRAD_EXACT ? 1 : 2;
+    const int lag = opt.grad_mode == FWI_GRAD_EXACT ? 1 : 2;
+    float* gather = syn_out ? syn_out + b.gather_off : gathers();
+    const size_t slot = 2 * (size_t)cap_cell;
+    if (int r = zero_fwd(b, st)) return r;
+    // forward with checkpoints; the last segment keeps its imaging terms
+    const int slast = (J - 1) * K;
+    for (int n = 0; n < nt; ++n) {
+      if (n > 0 && n % K == 0)
+        if (int r = save_ckpt(b, n / K, st)) return r;
+      float* qa = nullptr;
+      float* qb = nullptr;
+      if (n >= slast) {
+        qa = qbuf() + (size_t)(n - slast) * slot;
+        qb = qa + cap_cell;
+      }
+      if (int r = fwd_step(b, m, n, n >= slast ? qm : 0, qa, qb, gather, st)) return r;
+    }
+    if (int r = residual(b, gather, obs, resid(), d_misfit, first_batch ? 0 : 1, st)) return r;
+    // adjoint
+    CK(hipMemsetAsync(adj(0), 0, 12 * (size_t)cap_cell * sizeof(float), st));
+    CK(hipMemsetAsync(sgrad(0), 0, 3 * (size_t)cap_cell * sizeof(float), st));
+    int seg = J - 1;
+    for (int k = nt; k >= 1; --k) {
+      const int n = k - lag;
+      const float* qa = nullptr;
+      const float* qb = nullptr;
+      if (n >= 0) {
+        while (n < seg * K) {
+          --seg;
+          if (seg > 0) {
+            if (int r = load_ckpt(b, seg, st)) return r;
+          } else {
+            if (int r = zero_fwd(b, st)) return r;
+          }
+          const int e = std::min((seg + 1) * K, nt);
+          for (int nn = seg * K; nn < e; ++nn) {
+            float* wa = qbuf() + (size_t)(nn - seg * K) * slot;
+            if (int r = fwd_step(b, m, nn, qm, wa, wa + cap_cell, nullptr, st)) return r;
+          }
+        }
+        qa = qbuf() + (size_t)(n - seg * K) * slot;
+        qb = qa + cap_cell;
+      }
+      if (int r = adj_step(b, m, k, qa, qb, st)) return r;
+    }
+    const long long nmodel = (long long)nx * nz;
+    hipLaunchKernelGGL(k_cross_shot_sum, dim3((unsigned)((nmodel + 255) / 256)), dim3(256), 0, st, b.d_shots, b.count,
+                       nx, nz, sgrad(0), sgrad(1), sgrad(2), grad, nmodel);
+    CK(hipGetLastError());
+    return 0;
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  // Pick the rows per work item by timing the fused forward and adjoint steps of the largest batch.
+  int tune(const ModelArgs& m, hipStream_t st) {
+    if (opt.rows_per_item > 0 || tuned) {
+      tuned = true;
+      return 0;
+    }
+    const int cands[] = {16, 24, 32, 48, 64, 96};
+    Batch& b = batches[0];
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_tz = tz;
+    for (int c : cands) {
+      if (c > nz - 8 && c != cands[0]) continue;
+      if (int r = build_items(b, c)) return r;
+      if (int r = zero_fwd(b, st)) return r;
+      CK(hipMemsetAsync(adj(0), 0, 12 * (size_t)cap_cell * sizeof(float), st));
+      for (int w = 0; w < 2; ++w) {
+        if (int r = fwd_step(b, m, w, 0, nullptr, nullptr, nullptr, st)) return r;
+      }
+      CK(hipEventRecord(e0, st));
+      for (int n = 0; n < 6; ++n) {
+        if (int r = fwd_step(b, m, n, 0, nullptr, nullptr, nullptr, st)) return r;
+        if (int r = adj_step(b, m, nt - n, nullptr, nullptr, st)) return r;
+      }
+      CK(hipEventRecord(e1, st));
+      CK(hipEventSynchronize(e1));
+      float ms = 0;
+      CK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_tz = c;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tz = best_tz;
+    for (auto& bb : batches)
+      if (int r = build_items(bb, tz)) return r;
+    tuned = true;
+    return 0;
+  }
+};
+
+namespace {
+
+int validate(const fwi_survey* s, std::string& why) {
+  if (!s || !s->shots || s->nshots <= 0) return why = "survey has no shots", 1;
+  if (s->nt <= 0 || s->nx < 16 || s->nz < 16) return why = "grid or time axis too small", 1;
+  if (s->cpml <= 4) return why = "cpml must exceed the 4-cell stencil rim", 1;
+  if (s->nz - s->cpml < 8) return why = "nz too small for the bottom C-PML", 1;
+  if (!(s->dh > 0) || !(s->dt > 0)) return why = "dh and dt must be positive", 1;
+  if (!s->wavelet || !s->cpml_a || !s->cpml_b) return why = "wavelet and C-PML table are required", 1;
+  for (int i = 0; i < s->nshots; ++i) {
+    const fwi_shot& h = s->shots[i];
+    char buf[160];
+    auto bad = [&](const char* w) {
+      std::snprintf(buf, sizeof buf, "shot %d: %s", i, w);
+      why = buf;
+      return 1;
+    };
+    if (h.x0 < 0 || h.nx < 2 * s->cpml + 8 || h.x0 + h.nx > s->nx) return bad("window outside the model or too narrow");
+    if (h.isx < 4 || h.isx > h.nx - 5 || h.isz < 4 || h.isz > s->nz - 5) return bad("source outside the interior");
+    if (h.nrec < 0 || h.irz < 4 || h.irz > s->nz - 5 || (h.nrec > 0 && (h.ir0 < 4 || h.ir0 + h.nrec > h.nx - 4)))
+      return bad("receivers outside the interior");
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void fwi_options_default(fwi_options* o) {
+  std::memset(o, 0, sizeof *o);
+  o->grad_mode = FWI_GRAD_EXACT;
+  o->arith = FWI_ARITH_FP32;
+}
+
+int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out) {
+  *out = nullptr;
+  std::string why;
+  if (validate(s, why)) {
+    set_message(why);
+    return 1;
+  }
+  auto* p = new fwi_plan();
+  p->nx = s->nx;
+  p->nz = s->nz;
+  p->nt = s->nt;
+  p->cpml = s->cpml;
+  p->dh = s->dh;
+  p->dt = s->dt;
+  p->shots.assign(s->shots, s->shots + s->nshots);
+  p->wavelet.assign(s->wavelet, s->wavelet + s->nt);
+  p->ta.assign(s->cpml_a, s->cpml_a + s->cpml);
+  p->tb.assign(s->cpml_b, s->cpml_b + s->cpml);
+  if (opt)
+    p->opt = *opt;
+  else
+    fwi_options_default(&p->opt);
+  if (p->opt.grad_mode != FWI_GRAD_EXACT && p->opt.grad_mode != FWI_GRAD_REFCOMPAT) {
+    delete p;
+    set_message("unknown grad_mode");
+    return 1;
+  }
+  const int nt = p->nt;
+  p->K = p->opt.ckpt_interval > 0 ? std::min(p->opt.ckpt_interval, nt) : (int)std::ceil(std::sqrt((double)nt));
+  p->J = (nt + p->K - 1) / p->K;
+  p->tz = p->opt.rows_per_item > 0 ? p->opt.rows_per_item : 32;
+
+  // per-shot sizes
+  const int ns = s->nshots;
+  std::vector<long long> cells(ns), strips(ns), gath(ns);
+  for (int i = 0; i < ns; ++i) {
+    cells[i] = (long long)p->shots[i].nx * p->nz;
+    strips[i] = strip_count(p->shots[i].nx, p->nz, p->cpml);
+    gath[i] = (long long)nt * p->shots[i].nrec;
+  }
+  // per-cell floats: 8 fwd + 12 adj + 3 grad + 2K imaging terms + 4 per checkpoint
+  const double per_cell = 23.0 + 2.0 * p->K + 4.0 * (p->J - 1);
+  const double per_strip = 4.0 * (p->J - 1);
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+    delete p;
+    set_message("hipMemGetInfo failed (no GPU?)");
+    return 1;
+  }
+  double budget = p->opt.mem_budget > 0 ? p->opt.mem_budget : 0.85 * (double)freeb;
+  // greedy batches in shot order
+  int i = 0;
+  long long gather_off = 0;
+  while (i < ns) {
+    Batch b;
+    b.s0 = i;
+    double bytes = 0;
+    while (i < ns) {
+      double add = 4.0 * (per_cell * cells[i] + per_strip * strips[i] + 2.0 * gath[i]);
+      if (b.count > 0 && (bytes + add > budget || (p->opt.max_batch > 0 && b.count >= p->opt.max_batch))) break;
+      bytes += add;
+      b.ncell += cells[i];
+      b.nstrip += strips[i];
+      b.ngather += gath[i];
+      ++b.count;
+      ++i;
+    }
+    b.gather_off = gather_off;
+    gather_off += b.ngather;
+    p->batches.push_back(std::move(b));
+  }
+  p->gather_total = gather_off;
+  for (auto& b : p->batches) {
+    p->cap_cell = std::max(p->cap_cell, b.ncell);
+    p->cap_strip = std::max(p->cap_strip, b.nstrip);
+    p->cap_gather = std::max(p->cap_gather, b.ngather);
+  }
+  const size_t nfloat = (size_t)(per_cell * p->cap_cell) + (size_t)(per_strip * p->cap_strip) + 2 * p->cap_gather;
+  p->work_bytes = nfloat * sizeof(float);
+  int rc = 0;
+  auto ck = [&](hipError_t e, const char* w) {
+    if (e != hipSuccess && rc == 0) rc = p->fail(e, w);
+  };
+  ck(hipMalloc(&p->d_work, p->work_bytes), "hipMalloc(workspace)");
+  ck(hipMalloc(&p->d_src, nt * sizeof(float)), "hipMalloc(src)");
+  ck(hipMalloc(&p->d_ta, p->cpml * sizeof(float)), "hipMalloc(ta)");
+  ck(hipMalloc(&p->d_tb, p->cpml * sizeof(float)), "hipMalloc(tb)");
+  ck(hipMalloc(&p->d_partial, kResidualBlocks * sizeof(double)), "hipMalloc(partial)");
+  ck(hipMalloc(&p->d_misfit, sizeof(double)), "hipMalloc(misfit)");
+  if (rc == 0) ck(hipMemcpy(p->d_src, p->wavelet.data(), nt * sizeof(float), hipMemcpyHostToDevice), "copy src");
+  if (rc == 0) ck(hipMemcpy(p->d_ta, p->ta.data(), p->cpml * sizeof(float), hipMemcpyHostToDevice), "copy ta");
+  if (rc == 0) ck(hipMemcpy(p->d_tb, p->tb.data(), p->cpml * sizeof(float), hipMemcpyHostToDevice), "copy tb");
+  if (rc == 0) ck(hipMemset(p->d_work, 0, p->work_bytes), "zero workspace");
+  // per-batch tables
+  for (auto& b : p->batches) {
+    if (rc) break;
+    long long off = 0, soff = 0, goff = 0;
+    for (int k = 0; k < b.count; ++k) {
+      const fwi_shot& h = p->shots[b.s0 + k];
+      ShotDev d;
+      d.nx = h.nx;
+      d.x0 = h.x0;
+      d.isx = h.isx;
+      d.isz = h.isz;
+      d.irz = h.irz;
+      d.ir0 = h.ir0;
+      d.nrec = h.nrec;
+      d.nstrip = strip_count(h.nx, p->nz, p->cpml);
+      d.off = off;
+      d.soff = soff;
+      d.goff = goff;
+      off += (long long)h.nx * p->nz;
+      soff += d.nstrip;
+      goff += (long long)nt * h.nrec;
+      b.h_shots.push_back(d);
+      for (int k0 = 0; k0 < d.nstrip; k0 += 256) b.h_sblocks.push_back(make_int2(k, k0));
+    }
+    b.nsblocks = (int)b.h_sblocks.size();
+    ck(hipMalloc(&b.d_shots, b.h_shots.size() * sizeof(ShotDev)), "hipMalloc(shots)");
+    if (rc == 0)
+      ck(hipMemcpy(b.d_shots, b.h_shots.data(), b.h_shots.size() * sizeof(ShotDev), hipMemcpyHostToDevice), "copy shots");
+    if (rc == 0 && b.nsblocks > 0) {
+      ck(hipMalloc(&b.d_sblocks, b.h_sblocks.size() * sizeof(int2)), "hipMalloc(sblocks)");
+      if (rc == 0)
+        ck(hipMemcpy(b.d_sblocks, b.h_sblocks.data(), b.h_sblocks.size() * sizeof(int2), hipMemcpyHostToDevice),
+           "copy sblocks");
+    }
+    if (rc == 0) rc = p->build_items(b, p->tz);
+  }
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  if (p->opt.rows_per_item > 0) p->tuned = true;
+  *out = p;
+  return 0;
+}
+
+void fwi_plan_destroy(fwi_plan* p) { delete p; }
+
+int fwi_plan_set_wavelet(fwi_plan* p, const float* w) {
+  p->wavelet.assign(w, w + p->nt);
+  hipError_t e = hipMemcpy(p->d_src, w, p->nt * sizeof(float), hipMemcpyHostToDevice);
+  return e == hipSuccess ? 0 : p->fail(e, "set_wavelet");
+}
+
+int fwi_plan_set_cpml(fwi_plan* p, const float* a, const float* b) {
+  p->ta.assign(a, a + p->cpml);
+  p->tb.assign(b, b + p->cpml);
+  hipError_t e = hipMemcpy(p->d_ta, a, p->cpml * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->d_tb, b, p->cpml * sizeof(float), hipMemcpyHostToDevice);
+  return e == hipSuccess ? 0 : p->fail(e, "set_cpml");
+}
+
+long long fwi_plan_gather_size(const fwi_plan* p) { return p->gather_total; }
+int fwi_plan_batch_size(const fwi_plan* p) { return p->batches.empty() ? 0 : p->batches[0].count; }
+double fwi_plan_workspace_bytes(const fwi_plan* p) { return (double)p->work_bytes; }
+
+int fwi_forward(fwi_plan* p, const float* vh, const float* vp, const float* vn, float* gathers, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  ModelArgs m{vh, vp, vn, p->nx};
+  if (int r = p->tune(m, st)) return r;
+  for (auto& b : p->batches)
+    if (int r = p->forward_batch(b, m, gathers + b.gather_off, st)) return r;
+  p->timing_collect();
+  return 0;
+}
+
+int fwi_misfit(fwi_plan* p, const float* vh, const float* vp, const float* vn, const float* obs, double* misfit,
+               void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  ModelArgs m{vh, vp, vn, p->nx};
+  if (int r = p->tune(m, st)) return r;
+  bool first = true;
+  for (auto& b : p->batches) {
+    if (int r = p->forward_batch(b, m, p->gathers(), st)) return r;
+    if (int r = p->residual(b, p->gathers(), obs, nullptr, p->d_misfit, first ? 0 : 1, st)) return r;
+    first = false;
+  }
+  hipError_t e = hipMemcpyAsync(misfit, p->d_misfit, sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? 0 : p->fail(e, "misfit readback");
+}
+
+int fwi_gradient(fwi_plan* p, const float* vh, const float* vp, const float* vn, const float* obs, double* grad,
+                 double* misfit, float* syn, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  ModelArgs m{vh, vp, vn, p->nx};
+  if (int r = p->tune(m, st)) return r;
+  hipError_t e = hipMemsetAsync(grad, 0, 3 * (size_t)p->nx * p->nz * sizeof(double), st);
+  if (e != hipSuccess) return p->fail(e, "zero grad");
+  bool first = true;
+  for (auto& b : p->batches) {
+    if (int r = p->gradient_batch(b, m, obs, grad, first ? 1 : 0, syn, st)) return r;
+    first = false;
+  }
+  if (misfit) {
+    e = hipMemcpyAsync(misfit, p->d_misfit, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return p->fail(e, "misfit readback");
+  }
+  p->timing_collect();
+  return 0;
+}
+
+void fwi_plan_set_timing(fwi_plan* p, int stride) {
+  p->timing_collect();
+  p->timing_stride = stride;
+  for (int k = 0; k < 3; ++k) {
+    p->kt_ms[k] = 0;
+    p->kt_count[k] = 0;
+    p->launch_no[k] = 0;
+  }
+}
+
+int fwi_plan_kernel_time(fwi_plan* p, int kind, double* total_ms, long long* count) {
+  if (kind < 0 || kind > 2) return 1;
+  p->timing_collect();
+  *total_ms = p->kt_ms[kind];
+  *count = p->kt_count[kind];
+  return 0;
+}
+
+long long fwi_plan_cells(const fwi_plan* p, long long* strip_cells) {
+  long long c = 0, s = 0;
+  for (auto& b : p->batches) {
+    c += b.ncell;
+    s += b.nstrip;
+  }
+  if (strip_cells) *strip_cells = s;
+  return c;
+}
+
+int fwi_plan_ckpt_interval(const fwi_plan* p) { return p->K; }
+int fwi_plan_rows_per_item(const fwi_plan* p) { return p->tz; }
+
+int fwi_chain_rule(const float* vp, const float* eps, const float* dl, const double* g, double* out, long long n,
+                   void* stream) {
+  hipLaunchKernelGGL(k_chain_rule, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, vp, eps, dl,
+                     g, out, n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_message(std::string("chain_rule: ") + hipGetErrorString(e));
+    return (int)e;
+  }
+  return 0;
+}
+
+}  // extern "C"

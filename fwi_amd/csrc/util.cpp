@@ -1,0 +1,59 @@
+// Error bookkeeping and host-side setup helpers.
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "fwi_internal.h"
+
+namespace fwi {
+
+namespace {
+thread_local int g_last_error = 0;
+thread_local std::string g_message;
+constexpr double kPi = 3.141592653589793;  // include/parallelcalls.h:3
+}  // namespace
+
+void record_error(hipError_t e) {
+  if (e != hipSuccess && g_last_error == 0) g_last_error = (int)e;
+}
+
+void set_message(const std::string& msg) { g_message = msg; }
+
+void cpml_table_host(int cpml, float vmax, float dt, float dh, float f, float d0, float L, std::vector<float>& a,
+                     std::vector<float>& b) {
+  a.assign((size_t)(cpml > 0 ? cpml : 0), 0.f);
+  b.assign(a.size(), 0.f);
+  for (int e = 0; e < cpml; ++e) {
+    double l = (double)(cpml - e) * (double)dh;
+    double LL = (double)L;
+    double d = (double)d0 * (double)vmax * (l / LL) * (l / LL);
+    double al = kPi * (double)f * (LL - l) / LL;
+    float bf = (float)std::exp(-(d + al) * (double)dt);
+    b[e] = bf;
+    a[e] = (float)(d * ((double)bf - 1) / (d + al));
+  }
+}
+
+}  // namespace fwi
+
+extern "C" {
+
+int fwi_last_error(void) {
+  int e = fwi::g_last_error;
+  fwi::g_last_error = 0;
+  return e;
+}
+
+const char* fwi_error_message(void) { return fwi::g_message.c_str(); }
+
+// Host evaluation of the C-PML table (see cpml_table_host); writes cpml entries to a and b.
+void fwi_cpml_table(int cpml, float vmax, float dt, float dh, float f, float d0, float L, float* a, float* b) {
+  std::vector<float> ta, tb;
+  fwi::cpml_table_host(cpml, vmax, dt, dh, f, d0, L, ta, tb);
+  for (int e = 0; e < cpml; ++e) {
+    a[e] = ta[e];
+    b[e] = tb[e];
+  }
+}
+
+}  // extern "C"

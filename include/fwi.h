@@ -1,0 +1,120 @@
+/* fwi.h -- batched FWI gradient API (this library's own entry points).
+ *
+ * The reference computes one shot per MPI rank through the per-step launchers of
+ * include/fwi_serialcalls.h (src/main.c:396-538): nt x launchPropagation with a full-wavefield copy to
+ * pinned host memory, the residual, nt x launchBackPropagation, then an MPI float reduction of the
+ * gradient.  The functions below replace that whole per-iteration block (main.c:396-538) and the
+ * forward-only surveys of main.c:334-366 and :695-737 with on-device shot batching: all shots of a batch
+ * advance together in one launch per time step, the adjoint pass recomputes the forward field from
+ * checkpoints every K ~ sqrt(nt) steps instead of storing it, and the cross-shot sum, misfit and
+ * gradient are reduced on the device in FP64.
+ *
+ * All array arguments are device pointers unless marked host.  Fields are x fastest (ix + nx*iz).
+ * Status-returning functions return 0 on success and a nonzero code with fwi_error_message() set on
+ * failure; they never fall back to another implementation.
+ */
+#ifndef FWI_H
+#define FWI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One shot in window-local coordinates: its grid is model columns [x0, x0+nx) with its own C-PML on the
+ * window's left/right edges (and the bottom); source at (isx, isz); receivers on row irz at columns
+ * [ir0, ir0+nrec).  The reference's shot (propagator_F, src/parallelcalls.cu:107-116) is x0=0, nx=model
+ * width, isx=sx-1, isz=irz=sz-1, ir0=CPML, nrec=nx-2*CPML.  Source and receivers must lie in the updated
+ * interior (4 <= i <= n-5). */
+typedef struct fwi_shot {
+  int x0, nx;
+  int isx, isz;
+  int irz, ir0, nrec;
+} fwi_shot;
+
+typedef struct fwi_survey {
+  int nx, nz;            /* model grid (cells) */
+  int nt;                /* time steps */
+  int cpml;              /* C-PML width in cells (main.c:38), > 4 */
+  float dh, dt;          /* grid spacing [m], time step [s] */
+  int nshots;
+  const fwi_shot* shots; /* host array of nshots */
+  const float* wavelet;  /* host, nt samples (one source wavelet for every shot, main.c:328-333) */
+  const float* cpml_a;   /* host, cpml entries: C-PML a by distance from the edge (fwi_cpml_table) */
+  const float* cpml_b;   /* host, cpml entries: C-PML b */
+} fwi_survey;
+
+enum { FWI_GRAD_REFCOMPAT = 0, FWI_GRAD_EXACT = 1 };
+enum { FWI_ARITH_FP32 = 0, FWI_ARITH_MIXED = 1 };
+
+typedef struct fwi_options {
+  int grad_mode;       /* FWI_GRAD_EXACT (true discrete gradient) or FWI_GRAD_REFCOMPAT (the reference's) */
+  int arith;           /* FWI_ARITH_FP32, or FWI_ARITH_MIXED: stencil sums in FP64 as the reference */
+  int ckpt_interval;   /* K: 0 = ceil(sqrt(nt)); >= nt stores every step (no recomputation) */
+  int max_batch;       /* shots per batch, 0 = as many as mem_budget allows */
+  double mem_budget;   /* bytes of device memory the plan may use, 0 = 85 % of free memory */
+  int rows_per_item;   /* z rows per wave in the stencil kernels, 0 = chosen by the tuner */
+} fwi_options;
+
+typedef struct fwi_plan fwi_plan;
+
+/* Fill *opt with defaults (exact gradient, FP32, K = ceil(sqrt(nt)), automatic batch and tile). */
+void fwi_options_default(fwi_options* opt);
+
+/* Validate the survey, size the batches, allocate the device workspace. */
+int fwi_plan_create(const fwi_survey* survey, const fwi_options* opt, fwi_plan** plan);
+void fwi_plan_destroy(fwi_plan* plan);
+
+/* Replace the wavelet (host, nt) or the C-PML table (host, cpml entries), e.g. for a new frequency stage. */
+int fwi_plan_set_wavelet(fwi_plan* plan, const float* wavelet);
+int fwi_plan_set_cpml(fwi_plan* plan, const float* cpml_a, const float* cpml_b);
+
+/* Size of the gather array of the whole survey: sum over shots of nt*nrec (floats); layout [shot][it][rec]. */
+long long fwi_plan_gather_size(const fwi_plan* plan);
+/* Shots per batch and number of batches chosen by the plan. */
+int fwi_plan_batch_size(const fwi_plan* plan);
+/* Device bytes held by the plan. */
+double fwi_plan_workspace_bytes(const fwi_plan* plan);
+
+/* Forward modelling of every shot (observed data: main.c:334-366).  v_hor, v_po, v_nmo: model grid. */
+int fwi_forward(fwi_plan* plan, const float* v_hor, const float* v_po, const float* v_nmo, float* gathers,
+                void* stream);
+
+/* Misfit only (a line-search trial: main.c:695-737): *misfit = 1/2 sum (syn - obs)^2 in FP64 (host out). */
+int fwi_misfit(fwi_plan* plan, const float* v_hor, const float* v_po, const float* v_nmo, const float* obs,
+               double* misfit, void* stream);
+
+/* One FWI gradient (main.c:396-538): forward with checkpoints, residual, FP64 misfit (host out), adjoint
+ * with recomputation, imaging, FP64 cross-shot sum.  grad (device, FP64, 3*nx*nz) receives, in order, the
+ * gradient with respect to v_hor (the reference's gcx), v_po (gcz) and v_nmo (gdx); overwritten.
+ * syn (optional, may be NULL): the synthetic gathers.  In FWI_GRAD_REFCOMPAT mode the result reproduces
+ * the reference's formula (serialcalls.cu:68-90, a dt-scaled imaging with a one-step lag). */
+int fwi_gradient(fwi_plan* plan, const float* v_hor, const float* v_po, const float* v_nmo, const float* obs,
+                 double* grad, double* misfit, float* syn, void* stream);
+
+/* Chain rule to (vp, epsilon, delta) for v_po = vp^2, v_hor = vp^2(1+2 eps), v_nmo = vp^2(1+2 delta):
+ * out = [dJ/dvp, dJ/deps, dJ/ddelta] (FP64, 3*n) from grad = [dJ/dv_hor, dJ/dv_po, dJ/dv_nmo]. */
+int fwi_chain_rule(const float* vp, const float* eps, const float* delta, const double* grad, double* out, long long n,
+                   void* stream);
+
+/* Instrumentation (no counterpart in the reference, whose only timer is clock() around whole phases,
+ * main.c:323,368-372).  With stride > 0, every stride-th launch of each fused step kernel is bracketed by
+ * HIP events on the stream it runs on; fwi_plan_kernel_time returns the summed event time and the
+ * number of timed launches of a kind (0 forward step, 1 forward step storing imaging terms, 2 adjoint
+ * step) since the last fwi_plan_set_timing call. */
+void fwi_plan_set_timing(fwi_plan* plan, int stride);
+int fwi_plan_kernel_time(fwi_plan* plan, int kind, double* total_ms, long long* count);
+/* Cells of all shot windows (nx_s*nz summed over shots) and, in *strip_cells, their C-PML cells. */
+long long fwi_plan_cells(const fwi_plan* plan, long long* strip_cells);
+/* Checkpoint interval K and z rows per work item in use. */
+int fwi_plan_ckpt_interval(const fwi_plan* plan);
+int fwi_plan_rows_per_item(const fwi_plan* plan);
+
+/* Host helpers. */
+void fwi_cpml_table(int cpml, float vmax, float dt, float dh, float f, float d0, float L, float* a, float* b);
+const char* fwi_error_message(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FWI_H */

@@ -1,0 +1,70 @@
+"""Shared test helpers: workload setup on the CPU oracle and on the GPU."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+
+import oracle_lib as O
+from fwi_amd import models as M
+
+
+def say(msg: str):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def small_workload(nx=96, nz=80, nt=240, cpml=12, f=14.0, nshots=2, seed=0, dt=1e-3, dh=10.0):
+    """A quick workload of the plain200 kind at reduced size (same model family, same geometry rules)."""
+    vt = M.gradient_model(nx, nz, seed=seed, cpml=cpml, sigma=8.0)
+    vi = M.smooth_model(*vt, width=10.0)
+    if nshots == 1:
+        shots = [M.reference_shot(nx, cpml, nx // 2, M.depth_row(2))]
+    else:
+        shots = M.even_shots(nx, cpml, nshots, M.depth_row(2))
+    return M.Workload("small", nx, nz, dh, dt, nt, f, cpml, vt, vi, shots)
+
+
+def oracle_setup(w, mode="f32"):
+    """Oracle-side tables, wavelet, squared-velocity models (true, initial) of a workload."""
+    T = np.float64 if mode == "f64" else np.float32
+    p = w.cpml_params
+    ta, tb = O.cpml_table(p)
+    src = O.ricker(w.nt, w.f, w.dt)
+    sq_true = O.pre_variables(*[x.astype(T) for x in w.true_model], dtype=T)
+    sq_init = O.pre_variables(*[x.astype(T) for x in w.init_model], dtype=T)
+    return (ta, tb), src, sq_true, sq_init
+
+
+def window_model(sq, shot):
+    return tuple(np.ascontiguousarray(x[:, shot.x0:shot.x0 + shot.nx]) for x in sq)
+
+
+def oracle_gathers(w, mode, sq, tab, src):
+    out = []
+    for s in w.shots:
+        out.append(O.forward(mode, window_model(sq, s), tab, src, s.nx, w.nz, w.nt, w.cpml, w.dh, w.dt, s))
+    return out
+
+
+def oracle_gradient(w, mode, exact, sq, tab, src, obs_list):
+    """Sum over shots (FP64) of the oracle gradients, placed into model columns; total FP64 misfit."""
+    T = np.float64 if mode == "f64" else np.float32
+    G = np.zeros((3, w.nz, w.nx))
+    phi = 0.0
+    syns = []
+    for s, obs in zip(w.shots, obs_list):
+        ph, _, syn, gh, gp, gn = O.gradient(mode, exact, window_model(sq, s), tab, src.astype(T), obs, s.nx, w.nz, w.nt,
+                                            w.cpml, w.dh, w.dt, s)
+        phi += ph
+        syns.append(syn)
+        for k, g in enumerate((gh, gp, gn)):
+            G[k][:, s.x0:s.x0 + s.nx] += g.astype(np.float64)
+    return phi, G, syns

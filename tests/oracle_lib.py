@@ -86,10 +86,13 @@ def ptr(a: np.ndarray):
 # ------------------------------------------------------------------------------------------------
 
 def cpml_table(p, round_b=True):
-    """(a, b) C-PML table of p.cpml entries in FP64, indexed by distance from the edge."""
+    """(a, b) C-PML table of p.cpml entries in FP64, indexed by distance from the edge.  The parameters are
+    the reference's float values (main.c:38-52) promoted to double."""
     a = np.zeros(p.cpml)
     b = np.zeros(p.cpml)
-    lib().orc_cpml_table(ptr(a), ptr(b), p.cpml, p.vmax, p.dt, p.dh, p.f, p.d0, p.L, 1 if round_b else 0)
+    f32 = lambda v: float(np.float32(v))
+    lib().orc_cpml_table(ptr(a), ptr(b), p.cpml, f32(p.vmax), f32(p.dt), f32(p.dh), f32(p.f), f32(p.d0), f32(p.L),
+                         1 if round_b else 0)
     return a, b
 
 

@@ -1,0 +1,241 @@
+"""GPU parity of the legacy C-ABI launchers (include/fwi_serialcalls.h) against the oracle.
+
+The launchers keep the reference's arithmetic (stencil sums in double, pointwise updates in float) and
+the oracle's "mix" instantiation restates the same expressions, so the comparison bar is bit-level:
+tolerance rel-L2 <= 1e-6 is written in each test and the measured difference is printed (expected 0).
+The drive loops follow src/main.c:334-366 (observed data) and :410-496 (per-shot gradient).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from helpers import rel_l2, say
+from fwi_amd import models as M
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+
+    from fwi_amd import _lib
+
+    L = _lib.load()
+    return torch, L, _lib
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def _setup_legacy(torch, L, w, model):
+    nx, nz, nt, cpml = w.nx, w.nz, w.nt, w.cpml
+    p = w.cpml_params
+    d = {}
+    vp, eps, dl = (_dev(torch, x) for x in model)
+    z = lambda n: torch.zeros(n, dtype=torch.float32, device="cuda")
+    d["vh"], d["vp"], d["vn"] = z(nx * nz), z(nx * nz), z(nx * nz)
+    L.pre_variables_h(vp.data_ptr(), eps.data_ptr(), dl.data_ptr(), d["vh"].data_ptr(), d["vp"].data_ptr(),
+                      d["vn"].data_ptr(), nx, nz)
+    d["ax"], d["bx"], d["az"], d["bz"] = z(nx), z(nx), z(nz), z(nz)
+    L.CPML_x_h(d["ax"].data_ptr(), d["bx"].data_ptr(), cpml, p.vmax, nx, p.dt, p.dh, p.f, p.d0, p.L)
+    L.CPML_z_h(d["az"].data_ptr(), d["bz"].data_ptr(), cpml, p.vmax, nz, p.dt, p.dh, p.f, p.d0, p.L)
+    d["src"] = _dev(torch, O.ricker(nt, w.f, w.dt))
+    return d
+
+
+def _legacy_forward(torch, L, w, d, shot):
+    """main.c:336-365: zero state, nt x (PSI_F, SecondDerivate_P, ZETA_F, propagator_F), pointer swap."""
+    nx, nz, nt, cpml = w.nx, w.nz, w.nt, w.cpml
+    n = nx * nz
+    dh = np.float32(w.dh)
+    dh2 = dh * dh
+    C = np.float32(np.float32(w.dt) * np.float32(w.dt)) / dh2
+    z = lambda k: torch.zeros(k, dtype=torch.float32, device="cuda")
+    Pa, Pb = z(2 * n), z(2 * n)
+    f = {k: z(n) for k in ("p2", "r2", "psx", "psz", "zx", "zz", "atp", "atf", "PF")}
+    shot_g = z(nx * nt)
+    Ppre, Ppast = Pa, Pb
+    sx, sz = shot.isx + 1, shot.isz + 1
+    for it in range(nt):
+        Fpre, Fpast = Ppre.data_ptr() + 4 * n, Ppast.data_ptr() + 4 * n
+        L.PSI_F_h(Ppre.data_ptr(), Fpre, d["ax"].data_ptr(), d["bx"].data_ptr(), d["az"].data_ptr(),
+                  d["bz"].data_ptr(), f["psx"].data_ptr(), f["psz"].data_ptr(), nx, nz, dh, cpml)
+        L.SecondDerivate_P_h(Ppre.data_ptr(), Fpre, f["p2"].data_ptr(), f["r2"].data_ptr(), dh2, nx, nz)
+        L.ZETA_F_h(f["p2"].data_ptr(), f["r2"].data_ptr(), Ppre.data_ptr(), Fpre, d["ax"].data_ptr(),
+                   d["bx"].data_ptr(), d["az"].data_ptr(), d["bz"].data_ptr(), f["psx"].data_ptr(),
+                   f["psz"].data_ptr(), f["zx"].data_ptr(), f["zz"].data_ptr(), f["atp"].data_ptr(),
+                   f["atf"].data_ptr(), nx, nz, dh, dh2, cpml)
+        L.propagator_F_h(f["p2"].data_ptr(), f["r2"].data_ptr(), f["PF"].data_ptr(), Ppre.data_ptr(), Fpre,
+                         Ppast.data_ptr(), Fpast, d["vh"].data_ptr(), d["vp"].data_ptr(), d["vn"].data_ptr(),
+                         f["atp"].data_ptr(), f["atf"].data_ptr(), d["src"].data_ptr(), shot_g.data_ptr(), C, dh2,
+                         nx, nz, sx, sz, it, cpml)
+        Ppre, Ppast = Ppast, Ppre
+    torch.cuda.synchronize()
+    return shot_g.cpu().numpy().reshape(nt, nx)
+
+
+def _oracle_sq(w, model):
+    return O.pre_variables(*[x.astype(np.float32) for x in model])
+
+
+def test_legacy_forward_plain200_matches_oracle(env):
+    torch, L, _lib = env
+    w = M.workload("plain200")
+    say("legacy forward plain200 200x200 nt=1000 (main.c observed-data loop)")
+    d = _setup_legacy(torch, L, w, w.true_model)
+    g = _legacy_forward(torch, L, w, d, w.shots[0])
+    _lib.check_last("legacy forward")
+    tab = O.cpml_table(w.cpml_params)
+    ref = O.forward("mix", _oracle_sq(w, w.true_model), tab, O.ricker(w.nt, w.f, w.dt), w.nx, w.nz, w.nt, w.cpml,
+                    w.dh, w.dt, w.shots[0])
+    s = w.shots[0]
+    got = g[:, s.ir0:s.ir0 + s.nrec]
+    err = rel_l2(got, ref)
+    print(f"legacy forward vs oracle(mix): rel-L2 {err:.3e}, max|diff| {np.abs(got - ref).max():.3e}")
+    assert err <= TOL
+    # the C-PML columns of the reference gather stay zero
+    assert np.all(g[:, :w.cpml] == 0) and np.all(g[:, w.nx - w.cpml:] == 0)
+
+
+def test_legacy_gradient_matches_oracle(env):
+    """main.c:410-496 with launchPropagation / res_h / launchBackPropagation and pinned PF_h snapshots."""
+    torch, L, _lib = env
+    w = M.workload("plain200", nt=400)
+    nx, nz, nt, cpml = w.nx, w.nz, w.nt, w.cpml
+    n = nx * nz
+    say("legacy gradient plain200 nt=400 (launchPropagation/launchBackPropagation, streams)")
+    dobs = _setup_legacy(torch, L, w, w.true_model)
+    shot = w.shots[0]
+    obs_g = _legacy_forward(torch, L, w, dobs, shot)
+    d = _setup_legacy(torch, L, w, w.init_model)
+    z = lambda k: torch.zeros(k, dtype=torch.float32, device="cuda")
+    dh = np.float32(w.dh)
+    dh2 = dh * dh
+    dt = np.float32(w.dt)
+    C = np.float32(dt * dt) / dh2
+    s1 = _lib.create_blocking_stream()
+    s2 = _lib.create_blocking_stream()
+    try:
+        PFh = torch.zeros(2 * nt * n, dtype=torch.float32, pin_memory=True)
+        f = {k: z(n) for k in ("p2", "r2", "psx", "psz", "zx", "zz", "atp", "atf", "PF", "aux1", "aux2", "aux3",
+                               "aux4", "p2cx", "p2cz", "r2dx", "r2cz", "psixcx", "psixcz", "atenxcx", "atenxcz",
+                               "atenzdx", "atenzcz", "gcx", "gcz", "gdx")}
+        Pa, Pb, Pre2 = z(2 * n), z(2 * n), z(2 * n)
+        shot1 = torch.from_numpy(obs_g.reshape(-1).copy()).cuda()
+        shot2 = z(nx * nt)
+        Res = z(nx * nt)
+        sx, sz = shot.isx + 1, shot.isz + 1
+        Ppre, Ppast = Pa, Pb
+        ptr = lambda k: f[k].data_ptr()
+        for it in range(nt):
+            L.launchPropagation(s1, s2, PFh.data_ptr() + 4 * 2 * it * n, ptr("PF"), Ppast.data_ptr(), Ppre.data_ptr(),
+                                Ppast.data_ptr() + 4 * n, Ppre.data_ptr() + 4 * n, ptr("p2"), ptr("r2"), dh2, nx, nz,
+                                d["ax"].data_ptr(), d["bx"].data_ptr(), d["az"].data_ptr(), d["bz"].data_ptr(),
+                                ptr("psx"), ptr("psz"), dh, cpml, ptr("zx"), ptr("zz"), ptr("atp"), ptr("atf"),
+                                d["vh"].data_ptr(), d["vp"].data_ptr(), d["vn"].data_ptr(), d["src"].data_ptr(),
+                                shot2.data_ptr(), C, sx, sz, it)
+            Ppre, Ppast = Ppast, Ppre
+        L.res_h(Res.data_ptr(), shot1.data_ptr(), shot2.data_ptr(), nx, nt)
+        torch.cuda.synchronize()
+        res = Res.cpu().numpy()
+        misfit32 = np.float32(0)
+        for v in res:
+            misfit32 = np.float32(misfit32 + np.float32(v) ** 2)
+        # adjoint state reuses the forward C-PML buffers as the reference does (main.c:184-193)
+        for k in ("psx", "psz", "zx", "zz", "atp", "atf", "psixcx", "psixcz", "atenxcx", "atenxcz", "atenzdx",
+                  "atenzcz"):
+            f[k].zero_()
+        Pa.zero_()
+        Pb.zero_()
+        Ppre, Ppast = Pa, Pb
+        for it in range(nt):
+            L.launchBackPropagation(s1, s2, nx, nz, nt, PFh.data_ptr() + 4 * (nt - it - 1) * n * 2, Ppre.data_ptr(),
+                                    Ppast.data_ptr(), Ppre.data_ptr() + 4 * n, Ppast.data_ptr() + 4 * n, ptr("aux1"),
+                                    ptr("aux2"), ptr("aux3"), ptr("aux4"), d["vh"].data_ptr(), d["vp"].data_ptr(),
+                                    d["vn"].data_ptr(), ptr("p2cx"), ptr("p2cz"), ptr("r2dx"), ptr("r2cz"), dh2,
+                                    Pre2.data_ptr(), Pre2.data_ptr() + 4 * n, d["ax"].data_ptr(), d["bx"].data_ptr(),
+                                    d["az"].data_ptr(), d["bz"].data_ptr(), ptr("psixcx"), ptr("psixcz"), ptr("psx"),
+                                    ptr("psz"), dh, cpml, ptr("zx"), ptr("zz"), ptr("atf"), ptr("atp"),
+                                    ptr("atenxcx"), ptr("atenxcz"), ptr("atenzdx"), ptr("atenzcz"), Res.data_ptr(), C,
+                                    ptr("PF"), it, sx, sz, dt, ptr("gcx"), ptr("gcz"), ptr("gdx"))
+            Ppre, Ppast = Ppast, Ppre
+        torch.cuda.synchronize()
+        _lib.check_last("legacy gradient")
+        got = [f[k].cpu().numpy().reshape(nz, nx) for k in ("gcx", "gcz", "gdx")]
+    finally:
+        _lib.destroy_stream(s1)
+        _lib.destroy_stream(s2)
+    tab = O.cpml_table(w.cpml_params)
+    obs = obs_g[:, shot.ir0:shot.ir0 + shot.nrec].copy()
+    phi, phi32, syn, gh, gp, gn = O.gradient("mix", False, _oracle_sq(w, w.init_model), tab,
+                                             O.ricker(nt, w.f, w.dt), obs, nx, nz, nt, cpml, w.dh, w.dt, shot)
+    for name, a, b in zip(("gcx", "gcz", "gdx"), got, (gh, gp, gn)):
+        err = rel_l2(a, b)
+        print(f"legacy {name} vs oracle(mix): rel-L2 {err:.3e}")
+        assert err <= TOL
+    assert abs(float(misfit32) / 2 - phi32) <= 1e-6 * phi32
+
+
+@pytest.mark.parametrize("kernel", ["NORMAL", "UpdateModel", "compute_sy", "compute_qi", "compute_ri", "update_r",
+                                    "update_LBFGS"])
+def test_legacy_optimizer_kernels(env, kernel):
+    torch, L, _lib = env
+    say(f"legacy optimizer kernel {kernel}")
+    nx, nz, m = 70, 50, 3
+    n = nx * nz
+    rng = np.random.default_rng(11)
+    f32 = lambda *s: np.ascontiguousarray(rng.standard_normal(s).astype(np.float32))
+    big = lambda *s: np.ascontiguousarray((2.5e6 + 1e6 * rng.standard_normal(s)).astype(np.float32))
+    P = O.ptr
+    if kernel == "NORMAL":
+        h = [f32(n) for _ in range(3)]
+        dv = [_dev(torch, x) for x in h]
+        L.NORMAL_h(*[t.data_ptr() for t in dv], nx, nz, 1.5, 2.5, 3.5)
+        O.lib().orc_NORMAL(*[P(x) for x in h], nx, nz, 1.5, 2.5, 3.5)
+    elif kernel == "UpdateModel":
+        h = [f32(n) for _ in range(3)] + [big(n) for _ in range(3)]
+        dv = [_dev(torch, x) for x in h]
+        L.UpdateModel_h(*[t.data_ptr() for t in dv], 3e5, 2e5, 2.5e5, nx, nz)
+        O.lib().orc_UpdateModel(*[P(x) for x in h], 3e5, 2e5, 2.5e5, nx, nz)
+    elif kernel == "compute_sy":
+        h = [np.zeros(m * n, np.float32) for _ in range(6)] + [f32(n) for _ in range(12)]
+        dv = [_dev(torch, x) for x in h]
+        L.compute_sy_h(*[t.data_ptr() for t in dv], nx, nz, 2)
+        O.lib().orc_compute_sy(*[P(x) for x in h], nx, nz, 2)
+    elif kernel == "compute_qi":
+        h = [f32(n) for _ in range(3)] + [f32(m * n) for _ in range(3)]
+        dv = [_dev(torch, x) for x in h]
+        L.compute_qi_h(*[t.data_ptr() for t in dv], 0.3, -0.7, 1.1, nx, nz, 1)
+        O.lib().orc_compute_qi(*[P(x) for x in h], 0.3, -0.7, 1.1, nx, nz, 1)
+    elif kernel == "compute_ri":
+        h = [f32(n) for _ in range(6)]
+        dv = [_dev(torch, x) for x in h]
+        L.compute_ri_h(*[t.data_ptr() for t in dv], 0.3, -0.7, 1.1, nx, nz)
+        O.lib().orc_compute_ri(*[P(x) for x in h], 0.3, -0.7, 1.1, nx, nz)
+    elif kernel == "update_r":
+        h = [f32(n) for _ in range(3)] + [f32(m * n) for _ in range(3)]
+        dv = [_dev(torch, x) for x in h]
+        L.update_r_h(*[t.data_ptr() for t in dv], 0.3, -0.7, 1.1, 0.2, 0.1, -0.4, nx, nz, 2)
+        O.lib().orc_update_r(*[P(x) for x in h], 0.3, -0.7, 1.1, 0.2, 0.1, -0.4, nx, nz, 2)
+    else:
+        h = [f32(n) * 1e5 for _ in range(3)] + [big(n) for _ in range(6)]
+        h = [np.ascontiguousarray(x) for x in h]
+        for flags in ((1, 1, 1, 0, 0), (1, 1, 1, 1, 1)):
+            hh = [x.copy() for x in h]
+            dv = [_dev(torch, x) for x in hh]
+            L.update_LBFGS_h(*[t.data_ptr() for t in dv], nx, nz, 0.5, 0.25, 0.125, *flags)
+            O.lib().orc_update_LBFGS(*[P(x) for x in hh], nx, nz, 0.5, 0.25, 0.125, *flags)
+            torch.cuda.synchronize()
+            for a, b in zip(dv, hh):
+                np.testing.assert_array_equal(a.cpu().numpy(), b)
+        return
+    torch.cuda.synchronize()
+    _lib.check_last(kernel)
+    for a, b in zip(dv, h):
+        np.testing.assert_array_equal(a.cpu().numpy(), b)

@@ -85,13 +85,14 @@ def test_cpml_table_profiles():
     the PML depth, alpha linear (zero at the outer edge), and the outer edge gives a = b - 1."""
     p = M.CpmlParams(cpml=20, dh=10.0, dt=1e-3, f=10.0)
     a, b = O.cpml_table(p, round_b=False)
+    f32 = lambda v: float(np.float32(v))  # the reference's parameters are floats (main.c:38-52)
     assert np.all(b > 0) and np.all(b < 1) and np.all(a <= 0)
-    s = -np.log(b) / p.dt
+    s = -np.log(b) / f32(p.dt)
     d = a * s / (b - 1)
     al = s - d
-    depth = (p.cpml - np.arange(p.cpml)) * p.dh / p.L
-    np.testing.assert_allclose(d, p.d0 * p.vmax * depth ** 2, rtol=1e-9)
-    np.testing.assert_allclose(al, math.pi * p.f * (1 - depth), rtol=1e-9, atol=1e-9)
+    depth = (p.cpml - np.arange(p.cpml)) * f32(p.dh) / f32(p.L)
+    np.testing.assert_allclose(d, f32(p.d0) * f32(p.vmax) * depth ** 2, rtol=1e-9)
+    np.testing.assert_allclose(al, math.pi * f32(p.f) * (1 - depth), rtol=1e-9, atol=1e-9)
     assert abs(a[0] - (b[0] - 1)) < 1e-12
     # theoretical reflection coefficient of the quadratic profile: exp(-2/3 d0 L) = R (at unit speed)
     assert abs(math.exp(-2.0 / 3.0 * p.d0 * p.L) - p.R) / p.R < 1e-5
