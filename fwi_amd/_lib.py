@@ -74,6 +74,15 @@ BATCHED = {
     "fwi_cpml_table": ([c_int, c_float, c_float, c_float, c_float, c_float, c_float, ctypes.POINTER(c_float),
                         ctypes.POINTER(c_float)], None),
     "fwi_error_message": ([], ctypes.c_char_p),
+    "fwi_lbfgs_create": ([c_int, c_int, c_int, ctypes.POINTER(vp_)], c_int),
+    "fwi_lbfgs_destroy": ([vp_], None),
+    "fwi_lbfgs_sd_step": ([vp_, _F, _F, _F, vp_, ctypes.POINTER(c_double), vp_], c_int),
+    "fwi_lbfgs_direction": ([vp_, _F, _F, _F, vp_, vp_], c_int),
+    "fwi_lbfgs_trial": ([vp_, _F, _F, _F, ctypes.POINTER(c_double), vp_], c_int),
+    "fwi_lbfgs_restore": ([vp_, _F, _F, _F, vp_], c_int),
+    "fwi_lbfgs_get_direction": ([vp_, vp_, vp_], c_int),
+    "fwi_lbfgs_pairs": ([vp_], c_int),
+    "fwi_dot3": ([vp_, vp_, c_ll, ctypes.POINTER(c_double), vp_], c_int),
     "fwi_plan_set_timing": ([vp_, c_int], None),
     "fwi_plan_kernel_time": ([vp_, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll)], c_int),
     "fwi_plan_cells": ([vp_, ctypes.POINTER(c_ll)], c_ll),

@@ -21,7 +21,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfwi.so")
 ARCH = os.environ.get("FWI_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["legacy.hip", "plan.hip", "util.cpp"]
+SOURCES = ["legacy.hip", "plan.hip", "lbfgs.hip", "util.cpp"]
 HEADERS = ["fwi_common.h", "fwi_internal.h", "batch_kernels.h", "batch_kernels.hip"]
 
 COMMON = [

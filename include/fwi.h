@@ -96,6 +96,34 @@ int fwi_gradient(fwi_plan* plan, const float* v_hor, const float* v_po, const fl
 int fwi_chain_rule(const float* vp, const float* eps, const float* delta, const double* grad, double* out, long long n,
                    void* stream);
 
+/* ---------------------------------------------------------------------------------------------------
+ * Optimizer (main.c:539-800) on the device in FP64.  Three blocks in the order of fwi_gradient's output:
+ * v_hor, v_po, v_nmo (n = nx*nz each; grad is FP64 3*n, the model three FP32 fields).  Model updates touch
+ * rows iz > 15 only (include/parallelcalls.h:6) and end with v_hor >= v_nmo (parallelcalls.cu:282,400).
+ * --------------------------------------------------------------------------------------------------- */
+typedef struct fwi_lbfgs fwi_lbfgs;
+/* History of m (s, y) pairs per block (the reference uses m = 5, main.c:59). */
+int fwi_lbfgs_create(int nx, int nz, int m, fwi_lbfgs** out);
+void fwi_lbfgs_destroy(fwi_lbfgs* h);
+/* First iteration (main.c:539-568, NORMAL_h + UpdateModel_h): back up m and g, then
+ * m -= alpha_k * g_k / ||g_k|| (FP64 norm over the whole grid); alpha is host double[3]. */
+int fwi_lbfgs_sd_step(fwi_lbfgs* h, float* v_hor, float* v_po, float* v_nmo, const double* grad, const double* alpha,
+                      void* stream);
+/* Later iterations (main.c:569-689): push the pair (m - m_back, g - g_back), run the two-loop recursion
+ * per block with FP64 dot products (H0 = gamma I, gamma = s.y / y.y of the newest pair) into the
+ * direction r, then back up m and g. */
+int fwi_lbfgs_direction(fwi_lbfgs* h, const float* v_hor, const float* v_po, const float* v_nmo, const double* grad,
+                        void* stream);
+/* Trial model of the line search (update_LBFGS_h with flags 1,1,1,0,0): m = m_back - alpha_k r_k. */
+int fwi_lbfgs_trial(fwi_lbfgs* h, float* v_hor, float* v_po, float* v_nmo, const double* alpha, void* stream);
+/* Restore the backed-up model (main.c:777-781). */
+int fwi_lbfgs_restore(fwi_lbfgs* h, float* v_hor, float* v_po, float* v_nmo, void* stream);
+/* Copy the current direction r (FP64 3*n) and the number of stored pairs. */
+int fwi_lbfgs_get_direction(fwi_lbfgs* h, double* out, void* stream);
+int fwi_lbfgs_pairs(const fwi_lbfgs* h);
+/* Deterministic FP64 dot products of three blocks of n: out (host double[3]). */
+int fwi_dot3(const double* a, const double* b, long long n, double* out, void* stream);
+
 /* Instrumentation (no counterpart in the reference, whose only timer is clock() around whole phases,
  * main.c:323,368-372).  With stride > 0, every stride-th launch of each fused step kernel is bracketed by
  * HIP events on the stream it runs on; fwi_plan_kernel_time returns the summed event time and the
