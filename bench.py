@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-stride", type=int, default=4)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device-to-device (a kernel of known traffic for PMC scaling)")
     return ap.parse_args()
 
 
@@ -154,6 +156,13 @@ def main():
     from fwi_amd.survey import Plan, cpml_table, pre_variables
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.calibrate_copy:
+        a = torch.ones(256 * 2 ** 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB written")
+        del a, b
     w = M.workload(args.workload, nshots=args.shots, nt=args.nt or None)
     nt = w.nt
     log(f"workload {w.name}: {w.nx}x{w.nz}, {len(w.shots)} shots, nt={nt}, dt={w.dt}, f={w.f}")

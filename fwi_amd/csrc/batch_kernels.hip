@@ -54,6 +54,7 @@ __global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __r
 // a = p2 + aten_p, b = r2 + aten_f (exact gradient), 2 p2, r2 (reference-compatible gradient).
 template <int QM, class S>
 __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
+  __shared__ float xbuf[4][72];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
   if (item >= a.nitems) return;
@@ -61,7 +62,10 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
   const ShotDev sd = a.g.shots[itm.x];
   const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
   const int ix = itm.y + lane;
+  const bool inb = ix < nx;
   const bool act = ix >= kRim && ix < nx - kRim;
+  const int hx = itm.y + halo_col(lane);
+  const bool hin = lane < 8 && hx >= 0 && hx < nx;
   const bool xstrip = ix < cpml || ix > nx - cpml - 1;
   const float axv = act ? coef_x(a.g.ta, ix, nx, cpml) : 0.f;
   const float bxv = act ? coef_x(a.g.tb, ix, nx, cpml) : 0.f;
@@ -73,17 +77,22 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
   const float* __restrict__ vh = a.m.vh + sd.x0;
   const float* __restrict__ vp = a.m.vp + sd.x0;
   const float* __restrict__ vn = a.m.vn + sd.x0;
+  float* buf = xbuf[wave];
   const float srcv = a.src[a.it];
   const int z0 = itm.z, z1 = itm.w;
   float fq[9];
 #pragma unroll
   for (int k = 0; k < 8; ++k) fq[k] = act ? F[ix + (long)nx * (z0 - 4 + k)] : 0.f;
   for (int z = z0; z < z1; ++z) {
-    const long i = ix + (long)nx * z;
+    const long row = (long)nx * z;
+    const long i = ix + row;
     fq[8] = act ? F[i + 4L * nx] : 0.f;
+    const float c = inb ? P[i] : 0.f;
+    const float h = hin ? P[hx + row] : 0.f;
+    float u[9];
+    xrow9(buf, lane, c, h, u);
     if (act) {
-      const float c = P[i];
-      const float p2 = d2s<S>(P[i - 4], P[i - 3], P[i - 2], P[i - 1], c, P[i + 1], P[i + 2], P[i + 3], P[i + 4], dh2);
+      const float p2 = d2s<S>(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8], dh2);
       const float r2 = d2s<S>(fq[0], fq[1], fq[2], fq[3], fq[4], fq[5], fq[6], fq[7], fq[8], dh2);
       float atp = 0.f, atf = 0.f;
       if (xstrip || z > nz - cpml - 1) {
@@ -182,6 +191,7 @@ __global__ void __launch_bounds__(256) k_adj_ref_strip(StripArgs a, const float*
 // one-step snapshot lag, serialcalls.cu:60,76).
 template <class S>
 __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
+  __shared__ float xbuf[4][72];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
   if (item >= a.nitems) return;
@@ -189,7 +199,10 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
   const ShotDev sd = a.g.shots[itm.x];
   const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
+  const bool inb = ix < nx;
   const bool act = ix >= kRim && ix < nx - kRim;
+  const int hx = itm.y + halo_col(lane);
+  const bool hin = lane < 8 && hx >= 0 && hx < nx;
   const bool xstrip = ix < cpml || ix > nx - cpml - 1;
   const float axv = act ? coef_x(a.g.ta, ix, nx, cpml) : 0.f;
   const float bxv = act ? coef_x(a.g.tb, ix, nx, cpml) : 0.f;
@@ -201,28 +214,50 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
   const float* __restrict__ vh = a.m.vh + sd.x0;
   const float* __restrict__ vp = a.m.vp + sd.x0;
   const float* __restrict__ vn = a.m.vn + sd.x0;
+  float* buf = xbuf[wave];
   const int z0 = itm.z, z1 = itm.w;
-  float q2[9], q4[9];  // aux2 = v_po lP, aux4 = v_po lF along z
+  // z queues: aux2 = v_po lP and aux4 = v_po lF at rows z-4..z+4; lP, lF themselves at rows z..z+4
+  float q2[9], q4[9], lq[5], mq[5];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int r = z0 - 4 + k;
+    const float l = inb ? L[ix + (long)nx * r] : 0.f;
+    const float m = inb ? M[ix + (long)nx * r] : 0.f;
     const float v = act ? vp[ix + (long)nxm * r] : 0.f;
-    q2[k] = act ? v * L[ix + (long)nx * r] : 0.f;
-    q4[k] = act ? v * M[ix + (long)nx * r] : 0.f;
+    q2[k] = act ? v * l : 0.f;
+    q4[k] = act ? v * m : 0.f;
+    if (k >= 4) {
+      lq[k - 4] = l;
+      mq[k - 4] = m;
+    }
   }
   for (int z = z0; z < z1; ++z) {
     const long i = ix + (long)nx * z;
     const long mi = ix + (long)nxm * z;
+    {
+      const float l = inb ? L[i + 4L * nx] : 0.f;
+      const float m = inb ? M[i + 4L * nx] : 0.f;
+      const float v = act ? vp[mi + 4L * nxm] : 0.f;
+      lq[4] = l;
+      mq[4] = m;
+      q2[8] = act ? v * l : 0.f;
+      q4[8] = act ? v * m : 0.f;
+    }
+    // x-direction auxiliaries aux1 = v_hor lP and aux3 = v_nmo lF of the band row and its halo
+    float u1o = 0.f, u3o = 0.f, u1h = 0.f, u3h = 0.f;
+    if (inb) {
+      u1o = vh[mi] * lq[0];
+      u3o = vn[mi] * mq[0];
+    }
+    if (hin) {
+      const long hi = hx + (long)nx * z, hm = hx + (long)nxm * z;
+      u1h = vh[hm] * L[hi];
+      u3h = vn[hm] * M[hi];
+    }
+    float u1[9], u3[9];
+    xrow9(buf, lane, u1o, u1h, u1);
+    xrow9(buf, lane, u3o, u3h, u3);
     if (act) {
-      const float v = vp[mi + 4L * nxm];
-      q2[8] = v * L[i + 4L * nx];
-      q4[8] = v * M[i + 4L * nx];
-      float u1[9], u3[9];
-#pragma unroll
-      for (int j = 0; j < 9; ++j) {
-        u1[j] = vh[mi + j - 4] * L[i + j - 4];
-        u3[j] = vn[mi + j - 4] * M[i + j - 4];
-      }
       const float p2cx = d2s<S>(u1[0], u1[1], u1[2], u1[3], u1[4], u1[5], u1[6], u1[7], u1[8], dh2);
       const float r2dx = d2s<S>(u3[0], u3[1], u3[2], u3[3], u3[4], u3[5], u3[6], u3[7], u3[8], dh2);
       const float p2cz = d2s<S>(q2[0], q2[1], q2[2], q2[3], q2[4], q2[5], q2[6], q2[7], q2[8], dh2);
@@ -248,8 +283,8 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
         t3 = n3 + d3;
         t4 = n4 + d4;
       }
-      float ln = 2.f * L[i] - Lo[i] + a.dh2C * (p2cx + t1 + r2dx + t3);
-      float mn = 2.f * M[i] - Mo[i] + a.dh2C * (p2cz + r2cz + t2 + t4);
+      float ln = 2.f * lq[0] - Lo[i] + a.dh2C * (p2cx + t1 + r2dx + t3);
+      float mn = 2.f * mq[0] - Mo[i] + a.dh2C * (p2cz + r2cz + t2 + t4);
       if (z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec) {
         const float r = a.res[sd.goff + (long)a.rit * sd.nrec + (ix - sd.ir0)];
         ln += r;
@@ -269,6 +304,11 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
     for (int k = 0; k < 8; ++k) {
       q2[k] = q2[k + 1];
       q4[k] = q4[k + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      lq[k] = lq[k + 1];
+      mq[k] = mq[k + 1];
     }
   }
 }
@@ -370,6 +410,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
 //   g_hor += dt^2 mP^k a^{k-1}, g_nmo += dt^2 mF^k a^{k-1}, g_po += dt^2 (mP^k + mF^k) b^{k-1}.
 template <class S>
 __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
+  __shared__ float xbuf[4][72];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
   if (item >= a.nitems) return;
@@ -377,9 +418,14 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const ShotDev sd = a.g.shots[itm.x];
   const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
+  const bool inb = ix < nx;
   const bool act = ix >= kRim && ix < nx - kRim;
+  const int hx = itm.y + halo_col(lane);
+  const bool hact = lane < 8 && hx >= kRim && hx < nx - kRim;
   // bands within reach of a side strip need the C-PML terms in x
   const bool xnear = itm.y < cpml + 4 || itm.y + 64 > nx - cpml - 4;
+  const float axo = act ? coef_x(a.g.ta, ix, nx, cpml) : 0.f;
+  const float axh = hact ? coef_x(a.g.ta, hx, nx, cpml) : 0.f;
   const float dh = a.g.dh, dh2 = a.g.dh2, dt2 = a.dt2;
   const float* __restrict__ L = a.Lc + sd.off;
   const float* __restrict__ M = a.Mc + sd.off;
@@ -389,49 +435,60 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const float* __restrict__ vp = a.m.vp + sd.x0;
   const float* __restrict__ vn = a.m.vn + sd.x0;
   const float* __restrict__ ztz = a.ztz + sd.off;
+  const float* __restrict__ ztx = a.ztx + sd.off;
+  float* buf = xbuf[wave];
   const int z0 = itm.z, z1 = itm.w;
   const int zc = nz - cpml - 4;  // rows at or below this need the C-PML terms in z
-  float gq[9];  // gbar_z along z
+  float gq[9];         // gbar_z along z (rows z-4..z+4)
+  float lq[5], mq[5];  // mP, mF at rows z..z+4
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int r = z0 - 4 + k;
+    const long ir = ix + (long)nx * r;
+    const float l = act ? L[ir] : 0.f;
+    const float m = act ? M[ir] : 0.f;
     float v = 0.f;
-    if (act && r >= kRim) {
-      const long i = ix + (long)nx * r;
-      v = dt2 * (vp[ix + (long)nxm * r] * (L[i] + M[i])) + coef_z(a.g.ta, r, nz, cpml) * ztz[i];
-    }
+    if (act && r >= kRim) v = dt2 * (vp[ix + (long)nxm * r] * (l + m)) + coef_z(a.g.ta, r, nz, cpml) * ztz[ir];
     gq[k] = v;
+    if (k >= 4) {
+      lq[k - 4] = l;
+      mq[k - 4] = m;
+    }
   }
   for (int z = z0; z < z1; ++z) {
     const long i = ix + (long)nx * z;
     const long mi = ix + (long)nxm * z;
     {
       const int r = z + 4;
+      const float l = act ? L[i + 4L * nx] : 0.f;
+      const float m = act ? M[i + 4L * nx] : 0.f;
+      lq[4] = l;
+      mq[4] = m;
       float v = 0.f;
-      if (act && r < nz - kRim) {
-        const long ir = i + 4L * nx;
-        v = dt2 * (vp[mi + 4L * nxm] * (L[ir] + M[ir])) + coef_z(a.g.ta, r, nz, cpml) * ztz[ir];
-      }
+      if (act && r < nz - kRim)
+        v = dt2 * (vp[mi + 4L * nxm] * (l + m)) + coef_z(a.g.ta, r, nz, cpml) * ztz[i + 4L * nx];
       gq[8] = v;
     }
+    // gbar_x of the band row (zero outside Omega) and of the halo columns
+    float go = 0.f, gh = 0.f;
     if (act) {
-      float gx[9];
-#pragma unroll
-      for (int j = 0; j < 9; ++j) {
-        const int jx = ix + j - 4;
-        float v = 0.f;
-        if (jx >= kRim && jx < nx - kRim) {
-          v = dt2 * (vh[mi + j - 4] * L[i + j - 4] + vn[mi + j - 4] * M[i + j - 4]);
-          if (xnear) v = v + coef_x(a.g.ta, jx, nx, cpml) * a.ztx[sd.off + i + j - 4];
-        }
-        gx[j] = v;
-      }
+      go = dt2 * (vh[mi] * lq[0] + vn[mi] * mq[0]);
+      if (xnear) go = go + axo * ztx[i];
+    }
+    if (hact) {
+      const long hi = hx + (long)nx * z, hm = hx + (long)nxm * z;
+      gh = dt2 * (vh[hm] * L[hi] + vn[hm] * M[hi]);
+      if (xnear) gh = gh + axh * ztx[hi];
+    }
+    float gx[9];
+    xrow9(buf, lane, go, gh, gx);
+    if (act) {
       float ap = d2s<S>(gx[0], gx[1], gx[2], gx[3], gx[4], gx[5], gx[6], gx[7], gx[8], dh2);
       float af = d2s<S>(gq[0], gq[1], gq[2], gq[3], gq[4], gq[5], gq[6], gq[7], gq[8], dh2);
       if (xnear) ap = ap - d1_at<S>(a.ptx, sd.off + i, 1, dh);
       if (z >= zc) af = af - d1_at<S>(a.ptz, sd.off + i, nx, dh);
-      float ln = 2.f * L[i] + ap - Lo[i];
-      float mn = 2.f * M[i] + af - Mo[i];
+      float ln = 2.f * lq[0] + ap - Lo[i];
+      float mn = 2.f * mq[0] + af - Mo[i];
       if (z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec) {
         const float r = a.res[sd.goff + (long)a.rit * sd.nrec + (ix - sd.ir0)];
         ln += r;
@@ -449,6 +506,11 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) gq[k] = gq[k + 1];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      lq[k] = lq[k + 1];
+      mq[k] = mq[k + 1];
+    }
   }
 }
 

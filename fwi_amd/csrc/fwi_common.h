@@ -57,6 +57,28 @@ __device__ __forceinline__ float coef_z(const float* __restrict__ tab, int i, in
   return (i > n - cpml - 1) ? tab[n - 1 - i] : 0.f;
 }
 
+// Wave-private row exchange through LDS.  Lane l holds the value v of band column l; lanes 0..3 also hold
+// the halo value h of column l-4, lanes 4..7 that of column 64+(l-4).  On return u[k] is the value of
+// column l-4+k.  buf is this wave's 72-float LDS slice.  LDS operations of one wavefront execute in
+// order, so the wavefront-scope fences (which only stop the compiler from moving the accesses) are the
+// whole synchronisation; no workgroup barrier is involved.
+__device__ __forceinline__ void xrow9(float* buf, int lane, float v, float h, float (&u)[9]) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  buf[4 + lane] = v;
+  if (lane < 4) buf[lane] = h;
+  else if (lane < 8) buf[64 + lane] = h;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int k = 0; k < 9; ++k) u[k] = buf[lane + k];
+}
+
+// Band column of the halo value a lane supplies to xrow9: lanes 0..3 -> -4..-1, lanes 4..7 -> 64..67.
+__device__ __forceinline__ int halo_col(int lane) { return lane < 4 ? lane - 4 : 60 + lane; }
+
 // Map a linear workgroup id onto a logical id so that the workgroups one XCD receives (hardware deals
 // them round-robin over 8 XCDs) form one contiguous block of logical work.  Bijective for any n.
 __device__ __forceinline__ int xcd_logical_block(int bid, int n) {
