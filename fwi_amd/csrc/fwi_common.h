@@ -9,8 +9,24 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace fwi {
+
+// Arithmetic of the stencils.  float / double: sums in that type in the reference's expression order
+// (parallelcalls.cu:57,127).  F32D: float, with the second derivative written in difference form
+//   D2 u = sum_k c_k ((u_{+k} - u_0) + (u_{-k} - u_0)) / h^2
+// (exact because the weights sum to zero), which removes the cancellation between the O(|u|) terms of
+// the reference form and keeps the FP32 error of a 4000-step propagation near 3e-6 instead of 1e-4.
+struct F32D {};
+template <class S>
+struct SumType {
+  using type = S;
+};
+template <>
+struct SumType<F32D> {
+  using type = float;
+};
 
 constexpr int kRim = 4;     // stencil radius; the rim of 4 cells is never updated
 constexpr int kLayer = 15;  // rows iz <= 15 are excluded from model updates (include/parallelcalls.h:6)
@@ -19,16 +35,25 @@ constexpr int kLayer = 15;  // rows iz <= 15 are excluded from model updates (in
 template <class S>
 __device__ __forceinline__ float d2s(float m4, float m3, float m2, float m1, float c, float p1, float p2,
                                      float p3, float p4, float h2) {
-  const S c0 = (S)(205.0 / 72.0), c1 = (S)(8.0 / 5.0), c2 = (S)(1.0 / 5.0), c3 = (S)(8.0 / 315.0),
-          c4 = (S)(1.0 / 560.0);
-  S r = -c0 * (S)c + c1 * (S)(p1 + m1) - c2 * (S)(p2 + m2) + c3 * (S)(p3 + m3) - c4 * (S)(p4 + m4);
-  return (float)(r / (S)h2);
+  if constexpr (std::is_same<S, F32D>::value) {
+    const float c1 = (float)(8.0 / 5.0), c2 = (float)(1.0 / 5.0), c3 = (float)(8.0 / 315.0),
+                c4 = (float)(1.0 / 560.0);
+    const float r = c1 * ((p1 - c) + (m1 - c)) - c2 * ((p2 - c) + (m2 - c)) + c3 * ((p3 - c) + (m3 - c)) -
+                    c4 * ((p4 - c) + (m4 - c));
+    return r / h2;
+  } else {
+    const S c0 = (S)(205.0 / 72.0), c1 = (S)(8.0 / 5.0), c2 = (S)(1.0 / 5.0), c3 = (S)(8.0 / 315.0),
+            c4 = (S)(1.0 / 560.0);
+    S r = -c0 * (S)c + c1 * (S)(p1 + m1) - c2 * (S)(p2 + m2) + c3 * (S)(p3 + m3) - c4 * (S)(p4 + m4);
+    return (float)(r / (S)h2);
+  }
 }
 
 // 8th-order first derivative from u[-4..4] (the centre sample is not used).
-template <class S>
+template <class S0>
 __device__ __forceinline__ float d1s(float m4, float m3, float m2, float m1, float p1, float p2, float p3,
                                      float p4, float h) {
+  using S = typename SumType<S0>::type;
   const S c1 = (S)(4.0 / 5.0), c2 = (S)(1.0 / 5.0), c3 = (S)(4.0 / 105.0), c4 = (S)(1.0 / 280.0);
   S r = c1 * (S)(p1 - m1) - c2 * (S)(p2 - m2) + c3 * (S)(p3 - m3) - c4 * (S)(p4 - m4);
   return (float)(r / (S)h);

@@ -189,10 +189,12 @@ struct fwi_plan {
     float* Fc = fwd(2 + cur);
     float* Fo = fwd(2 + (cur ^ 1));
     StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
-    const bool mixed = opt.arith == FWI_ARITH_MIXED;
+    const int am = opt.arith;
     if (b.nsblocks > 0) {
-      if (mixed)
+      if (am == FWI_ARITH_MIXED)
         hipLaunchKernelGGL(k_fwd_strip<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fwd(4), fwd(5));
+      else if (am == FWI_ARITH_FP32D)
+        hipLaunchKernelGGL(k_fwd_strip<F32D>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fwd(4), fwd(5));
       else
         hipLaunchKernelGGL(k_fwd_strip<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fwd(4), fwd(5));
     }
@@ -218,10 +220,14 @@ struct fwi_plan {
     dim3 grid((b.nitems + 3) / 4);
     const int tev = timed_begin(qm == 0 ? 0 : 1, st);
 #define FWD_LAUNCH(QMV, SV) hipLaunchKernelGGL((k_fwd_main<QMV, SV>), grid, dim3(256), 0, st, a)
-    if (mixed) {
+    if (am == FWI_ARITH_MIXED) {
       if (qm == 0) FWD_LAUNCH(0, double);
       else if (qm == 1) FWD_LAUNCH(1, double);
       else FWD_LAUNCH(2, double);
+    } else if (am == FWI_ARITH_FP32D) {
+      if (qm == 0) FWD_LAUNCH(0, F32D);
+      else if (qm == 1) FWD_LAUNCH(1, F32D);
+      else FWD_LAUNCH(2, F32D);
     } else {
       if (qm == 0) FWD_LAUNCH(0, float);
       else if (qm == 1) FWD_LAUNCH(1, float);
@@ -240,18 +246,20 @@ struct fwi_plan {
     float* Lo = adj(cur ^ 1);
     float* Mc = adj(2 + cur);
     float* Mo = adj(2 + (cur ^ 1));
-    const bool mixed = opt.arith == FWI_ARITH_MIXED;
+    const int am = opt.arith;
     StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
     dim3 grid((b.nitems + 3) / 4);
+#define ARITH_LAUNCH(KERNEL, GRID, ...)                                                     \
+  do {                                                                                      \
+    if (am == FWI_ARITH_MIXED)                                                              \
+      hipLaunchKernelGGL(KERNEL<double>, GRID, dim3(256), 0, st, __VA_ARGS__);              \
+    else if (am == FWI_ARITH_FP32D)                                                         \
+      hipLaunchKernelGGL(KERNEL<F32D>, GRID, dim3(256), 0, st, __VA_ARGS__);                \
+    else                                                                                    \
+      hipLaunchKernelGGL(KERNEL<float>, GRID, dim3(256), 0, st, __VA_ARGS__);               \
+  } while (0)
     if (opt.grad_mode == FWI_GRAD_REFCOMPAT) {
-      if (b.nsblocks > 0) {
-        if (mixed)
-          hipLaunchKernelGGL(k_adj_ref_strip<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, Lc, Mc, adj(4), adj(5),
-                             adj(6), adj(7));
-        else
-          hipLaunchKernelGGL(k_adj_ref_strip<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, Lc, Mc, adj(4), adj(5),
-                             adj(6), adj(7));
-      }
+      if (b.nsblocks > 0) ARITH_LAUNCH(k_adj_ref_strip, dim3(b.nsblocks), sa, Lc, Mc, adj(4), adj(5), adj(6), adj(7));
       AdjRefArgs a;
       a.g = grid_args(b);
       a.m = m;
@@ -279,10 +287,7 @@ struct fwi_plan {
       a.gcz = sgrad(1);
       a.gdx = sgrad(2);
       const int tev = timed_begin(2, st);
-      if (mixed)
-        hipLaunchKernelGGL(k_adj_ref_main<double>, grid, dim3(256), 0, st, a);
-      else
-        hipLaunchKernelGGL(k_adj_ref_main<float>, grid, dim3(256), 0, st, a);
+      ARITH_LAUNCH(k_adj_ref_main, grid, a);
       timed_end(tev, st);
     } else {
       AdjExArgs a;
@@ -313,18 +318,13 @@ struct fwi_plan {
       if (b.nsblocks > 0) {
         hipLaunchKernelGGL(k_adj_ex_strip1, dim3(b.nsblocks), dim3(256), 0, st, sa, a.dt2, Lc, Mc, a.zx, a.zz, a.ztx,
                            a.ztz);
-        if (mixed)
-          hipLaunchKernelGGL(k_adj_ex_strip2<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, a);
-        else
-          hipLaunchKernelGGL(k_adj_ex_strip2<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, a);
+        ARITH_LAUNCH(k_adj_ex_strip2, dim3(b.nsblocks), sa, a);
       }
       const int tev = timed_begin(2, st);
-      if (mixed)
-        hipLaunchKernelGGL(k_adj_ex_main<double>, grid, dim3(256), 0, st, a);
-      else
-        hipLaunchKernelGGL(k_adj_ex_main<float>, grid, dim3(256), 0, st, a);
+      ARITH_LAUNCH(k_adj_ex_main, grid, a);
       timed_end(tev, st);
     }
+#undef ARITH_LAUNCH
     CK(hipGetLastError());
     return 0;
   }
@@ -535,6 +535,11 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
   if (p->opt.grad_mode != FWI_GRAD_EXACT && p->opt.grad_mode != FWI_GRAD_REFCOMPAT) {
     delete p;
     set_message("unknown grad_mode");
+    return 1;
+  }
+  if (p->opt.arith != FWI_ARITH_FP32 && p->opt.arith != FWI_ARITH_MIXED && p->opt.arith != FWI_ARITH_FP32D) {
+    delete p;
+    set_message("unknown arith");
     return 1;
   }
   const int nt = p->nt;

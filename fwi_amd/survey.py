@@ -19,6 +19,7 @@ GRAD_REFCOMPAT = 0
 GRAD_EXACT = 1
 ARITH_FP32 = 0
 ARITH_MIXED = 1
+ARITH_FP32D = 2
 
 
 def cpml_table(p: CpmlParams):

@@ -13,6 +13,9 @@
 //         float on store, while pointwise updates are float.
 //   f32 : T=float,  S=float  -- everything in float (what the GPU FP32 kernels compute).
 //   f64 : T=double, S=double -- the FP64 oracle for gradient / adjoint checks.
+//   f32d: T=float,  S=F32D   -- float, second derivatives in difference form
+//         sum_k c_k((u_{+k}-u_0)+(u_{-k}-u_0)) (this project's FP32 arithmetic; the weights sum to zero,
+//         so it is the same operator without the cancellation of the O(|u|) terms).
 // Build with -ffp-contract=off so the rounding of every expression is the one written here.
 //
 // Indexing: idx = ix + nx*iz, x fastest (include/parallelcalls.h:5).  Omega = {4<=ix<=nx-5, 4<=iz<=nz-5}
@@ -36,6 +39,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -53,19 +57,38 @@ struct Geo {
   }
 };
 
+struct F32D {};  // tag: float arithmetic, difference-form second derivative
+template <class S>
+struct SumType {
+  using type = S;
+};
+template <>
+struct SumType<F32D> {
+  using type = float;
+};
+
 // 8th-order central first/second derivative.  Reference: the D1 expression of parallelcalls.cu:57-58
 // and the D2 expression of :127-128.  Pair sums/differences are formed in T (float operands), the
 // weighted sum in S, the result is rounded to T -- the order of operations is kept.
-template <class T, class S>
+template <class T, class S0>
 inline T d2(const T* u, long i, long st, T h2) {
+  if constexpr (std::is_same<S0, F32D>::value) {
+    const float c1 = (float)(8.0 / 5.0), c2 = (float)(1.0 / 5.0), c3 = (float)(8.0 / 315.0), c4 = (float)(1.0 / 560.0);
+    const float c = u[i];
+    const float r = c1 * ((u[i + st] - c) + (u[i - st] - c)) - c2 * ((u[i + 2 * st] - c) + (u[i - 2 * st] - c)) +
+                    c3 * ((u[i + 3 * st] - c) + (u[i - 3 * st] - c)) - c4 * ((u[i + 4 * st] - c) + (u[i - 4 * st] - c));
+    return r / h2;
+  }
+  using S = typename SumType<S0>::type;
   const S c0 = (S)(205.0 / 72.0), c1 = (S)(8.0 / 5.0), c2 = (S)(1.0 / 5.0), c3 = (S)(8.0 / 315.0),
           c4 = (S)(1.0 / 560.0);
   S r = -c0 * (S)u[i] + c1 * (S)(T)(u[i + st] + u[i - st]) - c2 * (S)(T)(u[i + 2 * st] + u[i - 2 * st]) +
         c3 * (S)(T)(u[i + 3 * st] + u[i - 3 * st]) - c4 * (S)(T)(u[i + 4 * st] + u[i - 4 * st]);
   return (T)(r / (S)h2);
 }
-template <class T, class S>
+template <class T, class S0>
 inline T d1(const T* u, long i, long st, T h) {
+  using S = typename SumType<S0>::type;
   const S c1 = (S)(4.0 / 5.0), c2 = (S)(1.0 / 5.0), c3 = (S)(4.0 / 105.0), c4 = (S)(1.0 / 280.0);
   S r = c1 * (S)(T)(u[i + st] - u[i - st]) - c2 * (S)(T)(u[i + 2 * st] - u[i - 2 * st]) +
         c3 * (S)(T)(u[i + 3 * st] - u[i - 3 * st]) - c4 * (S)(T)(u[i + 4 * st] - u[i - 4 * st]);
@@ -875,6 +898,7 @@ extern "C" {
 ORC_FWD(mix, float, double)
 ORC_FWD(f32, float, float)
 ORC_FWD(f64, double, double)
+ORC_FWD(f32d, float, F32D)
 
 // One shot's gradient.  mode_exact=0: reference-compatible (serialcalls.cu:68-90 semantics: one-step
 // snapshot lag, dt scaling, forward-recursion C-PML on the adjoint); 1: exact discrete gradient.
@@ -890,6 +914,7 @@ ORC_FWD(f64, double, double)
 ORC_GRAD(mix, float, double)
 ORC_GRAD(f32, float, float)
 ORC_GRAD(f64, double, double)
+ORC_GRAD(f32d, float, F32D)
 
 // One linear forward state step without source (for the dot-product test), FP64.
 // state = 8 fields of nx*nz: P, P_old, F, F_old, psx, zx, psz, zz  ->  out: P', P, F', F, psx', zx', psz', zz'

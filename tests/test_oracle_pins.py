@@ -125,7 +125,7 @@ def _homog(nx, nz, v, eps=0.0, dl=0.0):
     return (np.full((nz, nx), v, np.float32), np.full((nz, nx), eps, np.float32), np.full((nz, nx), dl, np.float32))
 
 
-@pytest.mark.parametrize("mode", ["mix", "f32", "f64"])
+@pytest.mark.parametrize("mode", ["mix", "f32", "f64", "f32d"])
 def test_isotropic_reduction(mode):
     """epsilon = delta = 0 gives v_hor = v_nmo = v_po, so P and F stay identical: the gather is 2P."""
     nx, nz, nt, cpml = 64, 56, 200, 12
@@ -317,3 +317,23 @@ def test_misfit_reference_float_sum_close_to_fp64():
     res = syn.astype(np.float64) - obs.astype(np.float64)
     assert abs(phi64 - 0.5 * np.sum(res ** 2)) < 1e-9 * phi64
     assert abs(phi32 - phi64) < 1e-4 * phi64
+
+
+def test_f32d_forward_is_closer_to_fp64():
+    """The difference-form FP32 arithmetic computes the same operator (weights sum to zero) without the
+    cancellation of the reference form: in a homogeneous isotropic medium its gather differs from the FP64
+    forward by < 5e-6 relative L2, at least 4x less than the reference FP32 expression order."""
+    w = M.workload("plain200")
+    tab = O.cpml_table(w.cpml_params)
+    src = O.ricker(w.nt, w.f, w.dt)
+    model = (np.full((w.nz, w.nx), 2500.0, np.float32), np.zeros((w.nz, w.nx), np.float32),
+             np.zeros((w.nz, w.nx), np.float32))
+    shot = M.Shot(0, w.nx, 100, 100, 100, w.cpml, w.nx - 2 * w.cpml)
+    out = {}
+    for mode in ("f64", "f32", "f32d"):
+        T = np.float64 if mode == "f64" else np.float32
+        sq = O.pre_variables(*[x.astype(T) for x in model], dtype=T)
+        out[mode] = O.forward(mode, sq, tab, src, w.nx, w.nz, w.nt, w.cpml, w.dh, w.dt, shot).astype(np.float64)
+    e = lambda a: np.linalg.norm(a - out["f64"]) / np.linalg.norm(out["f64"])
+    assert e(out["f32d"]) < 5e-6
+    assert e(out["f32d"]) * 4 < e(out["f32"])
