@@ -55,40 +55,46 @@ __global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __r
 template <int QM, class S>
 __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
   __shared__ float xbuf[4][72];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = wave_id(), lane = threadIdx.x & 63;
   const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
   if (item >= a.nitems) return;
   const int4 itm = a.items[item];
   const ShotDev sd = a.g.shots[itm.x];
-  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
+  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
-  const bool inb = ix < nx;
   const bool act = ix >= kRim && ix < nx - kRim;
-  const int hx = itm.y + halo_col(lane);
-  const bool hin = lane < 8 && hx >= 0 && hx < nx;
+  // every lane loads from a valid column; values of columns outside the shot are never consumed
+  const int cx = clampi(ix, 0, nx - 1);
+  const int hx = lane < 8 ? clampi(itm.y + halo_col(lane), 0, nx - 1) : cx;
   const bool xstrip = ix < cpml || ix > nx - cpml - 1;
   const float axv = act ? coef_x(a.g.ta, ix, nx, cpml) : 0.f;
   const float bxv = act ? coef_x(a.g.tb, ix, nx, cpml) : 0.f;
   const float dh = a.g.dh, dh2 = a.g.dh2, C = a.C;
-  const float* __restrict__ P = a.Pc + sd.off;
-  const float* __restrict__ F = a.Fc + sd.off;
-  float* __restrict__ Po = a.Po + sd.off;
-  float* __restrict__ Fo = a.Fo + sd.off;
-  const float* __restrict__ vh = a.m.vh + sd.x0;
-  const float* __restrict__ vp = a.m.vp + sd.x0;
-  const float* __restrict__ vn = a.m.vn + sd.x0;
+  const float* P = a.Pc + sd.off;
+  const float* F = a.Fc + sd.off;
+  float* Po = a.Po + sd.off;
+  float* Fo = a.Fo + sd.off;
+  const float* psx = a.psx + sd.off;
+  const float* psz = a.psz + sd.off;
+  float* zx = a.zx + sd.off;
+  float* zz = a.zz + sd.off;
+  const float* vh = a.m.vh + sd.x0;
+  const float* vp = a.m.vp + sd.x0;
+  const float* vn = a.m.vn + sd.x0;
   float* buf = xbuf[wave];
   const float srcv = a.src[a.it];
   const int z0 = itm.z, z1 = itm.w;
   float fq[9];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) fq[k] = act ? F[ix + (long)nx * (z0 - 4 + k)] : 0.f;
+  for (int k = 0; k < 8; ++k) fq[k] = ldu(F, (unsigned)(cx + nx * (z0 - 4 + k)));
   for (int z = z0; z < z1; ++z) {
-    const long row = (long)nx * z;
-    const long i = ix + row;
-    fq[8] = act ? F[i + 4L * nx] : 0.f;
-    const float c = inb ? P[i] : 0.f;
-    const float h = hin ? P[hx + row] : 0.f;
+    const unsigned o = (unsigned)(cx + nx * z);
+    const unsigned mo = (unsigned)(cx + nxm * z);
+    fq[8] = ldu(F, o + 4u * nx);
+    const float c = ldu(P, o);
+    const float h = ldu(P, (unsigned)(hx + nx * z));
+    const float pold = ldu(Po, o), fold = ldu(Fo, o);
+    const float vhv = ldu(vh, mo), vpv = ldu(vp, mo), vnv = ldu(vn, mo);
     float u[9];
     xrow9(buf, lane, c, h, u);
     if (act) {
@@ -96,39 +102,40 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
       const float r2 = d2s<S>(fq[0], fq[1], fq[2], fq[3], fq[4], fq[5], fq[6], fq[7], fq[8], dh2);
       float atp = 0.f, atf = 0.f;
       if (xstrip || z > nz - cpml - 1) {
-        const long g = sd.off + i;
-        const float tx = d1_at<S>(a.psx, g, 1, dh);
-        const float tz = d1_at<S>(a.psz, g, nx, dh);
+        const unsigned snx = (unsigned)nx;
+        const float tx = d1s<S>(ldu(psx, o - 4), ldu(psx, o - 3), ldu(psx, o - 2), ldu(psx, o - 1), ldu(psx, o + 1),
+                                ldu(psx, o + 2), ldu(psx, o + 3), ldu(psx, o + 4), dh);
+        const float tz = d1s<S>(ldu(psz, o - 4 * snx), ldu(psz, o - 3 * snx), ldu(psz, o - 2 * snx),
+                                ldu(psz, o - snx), ldu(psz, o + snx), ldu(psz, o + 2 * snx), ldu(psz, o + 3 * snx),
+                                ldu(psz, o + 4 * snx), dh);
         const float tpx = p2 + tx;
         const float tfz = r2 + tz;
         const float azv = coef_z(a.g.ta, z, nz, cpml), bzv = coef_z(a.g.tb, z, nz, cpml);
-        const float zxn = bxv * a.zx[g] + axv * tpx;
-        const float zzn = bzv * a.zz[g] + azv * tfz;
-        a.zx[g] = zxn;
-        a.zz[g] = zzn;
+        const float zxn = bxv * ldu(zx, o) + axv * tpx;
+        const float zzn = bzv * ldu(zz, o) + azv * tfz;
+        stu(zx, o, zxn);
+        stu(zz, o, zzn);
         atp = tx + zxn;
         atf = tz + zzn;
       }
       const float tp = dh2 * (p2 + atp);
       const float tf = dh2 * (r2 + atf);
-      const long mi = ix + (long)a.m.nxm * z;
-      const float vhv = vh[mi], vpv = vp[mi], vnv = vn[mi];
-      float pn = 2.f * c - Po[i] + vhv * C * tp + vpv * C * tf;
-      float fn = 2.f * fq[4] - Fo[i] + vnv * C * tp + vpv * C * tf;
+      float pn = 2.f * c - pold + vhv * C * tp + vpv * C * tf;
+      float fn = 2.f * fq[4] - fold + vnv * C * tp + vpv * C * tf;
       if (ix == sd.isx && z == sd.isz) {
         pn += srcv;
         fn += srcv;
       }
-      Po[i] = pn;
-      Fo[i] = fn;
+      stu(Po, o, pn);
+      stu(Fo, o, fn);
       if (a.gather && z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec)
         a.gather[sd.goff + (long)a.it * sd.nrec + (ix - sd.ir0)] = pn + fn;
       if (QM == 1) {
-        a.qa[sd.off + i] = p2 + atp;
-        a.qb[sd.off + i] = r2 + atf;
+        stu(a.qa + sd.off, o, p2 + atp);
+        stu(a.qb + sd.off, o, r2 + atf);
       } else if (QM == 2) {
-        a.qa[sd.off + i] = p2;
-        a.qb[sd.off + i] = r2;
+        stu(a.qa + sd.off, o, p2);
+        stu(a.qb + sd.off, o, r2);
       }
     }
 #pragma unroll
@@ -192,28 +199,27 @@ __global__ void __launch_bounds__(256) k_adj_ref_strip(StripArgs a, const float*
 template <class S>
 __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
   __shared__ float xbuf[4][72];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = wave_id(), lane = threadIdx.x & 63;
   const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
   if (item >= a.nitems) return;
   const int4 itm = a.items[item];
   const ShotDev sd = a.g.shots[itm.x];
   const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
-  const bool inb = ix < nx;
   const bool act = ix >= kRim && ix < nx - kRim;
-  const int hx = itm.y + halo_col(lane);
-  const bool hin = lane < 8 && hx >= 0 && hx < nx;
+  const int cx = clampi(ix, 0, nx - 1);
+  const int hx = lane < 8 ? clampi(itm.y + halo_col(lane), 0, nx - 1) : cx;
   const bool xstrip = ix < cpml || ix > nx - cpml - 1;
   const float axv = act ? coef_x(a.g.ta, ix, nx, cpml) : 0.f;
   const float bxv = act ? coef_x(a.g.tb, ix, nx, cpml) : 0.f;
   const float dh = a.g.dh, dh2 = a.g.dh2;
-  const float* __restrict__ L = a.Lc + sd.off;
-  const float* __restrict__ M = a.Mc + sd.off;
-  float* __restrict__ Lo = a.Lo + sd.off;
-  float* __restrict__ Mo = a.Mo + sd.off;
-  const float* __restrict__ vh = a.m.vh + sd.x0;
-  const float* __restrict__ vp = a.m.vp + sd.x0;
-  const float* __restrict__ vn = a.m.vn + sd.x0;
+  const float* L = a.Lc + sd.off;
+  const float* M = a.Mc + sd.off;
+  float* Lo = a.Lo + sd.off;
+  float* Mo = a.Mo + sd.off;
+  const float* vh = a.m.vh + sd.x0;
+  const float* vp = a.m.vp + sd.x0;
+  const float* vn = a.m.vn + sd.x0;
   float* buf = xbuf[wave];
   const int z0 = itm.z, z1 = itm.w;
   // z queues: aux2 = v_po lP and aux4 = v_po lF at rows z-4..z+4; lP, lF themselves at rows z..z+4
@@ -221,39 +227,35 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int r = z0 - 4 + k;
-    const float l = inb ? L[ix + (long)nx * r] : 0.f;
-    const float m = inb ? M[ix + (long)nx * r] : 0.f;
-    const float v = act ? vp[ix + (long)nxm * r] : 0.f;
-    q2[k] = act ? v * l : 0.f;
-    q4[k] = act ? v * m : 0.f;
+    const float l = ldu(L, (unsigned)(cx + nx * r));
+    const float m = ldu(M, (unsigned)(cx + nx * r));
+    const float v = ldu(vp, (unsigned)(cx + nxm * r));
+    q2[k] = v * l;
+    q4[k] = v * m;
     if (k >= 4) {
       lq[k - 4] = l;
       mq[k - 4] = m;
     }
   }
   for (int z = z0; z < z1; ++z) {
-    const long i = ix + (long)nx * z;
-    const long mi = ix + (long)nxm * z;
+    const unsigned o = (unsigned)(cx + nx * z);
+    const unsigned mo = (unsigned)(cx + nxm * z);
+    const unsigned ho = (unsigned)(hx + nx * z), hmo = (unsigned)(hx + nxm * z);
     {
-      const float l = inb ? L[i + 4L * nx] : 0.f;
-      const float m = inb ? M[i + 4L * nx] : 0.f;
-      const float v = act ? vp[mi + 4L * nxm] : 0.f;
+      const float l = ldu(L, o + 4u * nx);
+      const float m = ldu(M, o + 4u * nx);
+      const float v = ldu(vp, mo + 4u * nxm);
       lq[4] = l;
       mq[4] = m;
-      q2[8] = act ? v * l : 0.f;
-      q4[8] = act ? v * m : 0.f;
+      q2[8] = v * l;
+      q4[8] = v * m;
     }
     // x-direction auxiliaries aux1 = v_hor lP and aux3 = v_nmo lF of the band row and its halo
-    float u1o = 0.f, u3o = 0.f, u1h = 0.f, u3h = 0.f;
-    if (inb) {
-      u1o = vh[mi] * lq[0];
-      u3o = vn[mi] * mq[0];
-    }
-    if (hin) {
-      const long hi = hx + (long)nx * z, hm = hx + (long)nxm * z;
-      u1h = vh[hm] * L[hi];
-      u3h = vn[hm] * M[hi];
-    }
+    const float u1o = ldu(vh, mo) * lq[0];
+    const float u3o = ldu(vn, mo) * mq[0];
+    const float u1h = ldu(vh, hmo) * ldu(L, ho);
+    const float u3h = ldu(vn, hmo) * ldu(M, ho);
+    const float lold = ldu(Lo, o), mold = ldu(Mo, o);
     float u1[9], u3[9];
     xrow9(buf, lane, u1o, u1h, u1);
     xrow9(buf, lane, u3o, u3h, u3);
@@ -264,7 +266,7 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
       const float r2cz = d2s<S>(q4[0], q4[1], q4[2], q4[3], q4[4], q4[5], q4[6], q4[7], q4[8], dh2);
       float t1 = 0.f, t2 = 0.f, t3 = 0.f, t4 = 0.f;
       if (xstrip || z > nz - cpml - 1) {
-        const long g = sd.off + i;
+        const long g = sd.off + o;
         const float d1 = d1_at<S>(a.s1, g, 1, dh);
         const float d2 = d1_at<S>(a.s2, g, nx, dh);
         const float d3 = d1_at<S>(a.s3, g, 1, dh);
@@ -283,21 +285,23 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
         t3 = n3 + d3;
         t4 = n4 + d4;
       }
-      float ln = 2.f * lq[0] - Lo[i] + a.dh2C * (p2cx + t1 + r2dx + t3);
-      float mn = 2.f * mq[0] - Mo[i] + a.dh2C * (p2cz + r2cz + t2 + t4);
+      float ln = 2.f * lq[0] - lold + a.dh2C * (p2cx + t1 + r2dx + t3);
+      float mn = 2.f * mq[0] - mold + a.dh2C * (p2cz + r2cz + t2 + t4);
       if (z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec) {
         const float r = a.res[sd.goff + (long)a.rit * sd.nrec + (ix - sd.ir0)];
         ln += r;
         mn += r;
       }
-      Lo[i] = ln;
-      Mo[i] = mn;
+      stu(Lo, o, ln);
+      stu(Mo, o, mn);
       if (a.qa) {
-        const long g = sd.off + i;
-        const float p2 = a.qa[g], r2 = a.qb[g];
-        a.gcx[g] += a.dt * p2 * ln;
-        a.gcz[g] += a.dt * r2 * (ln + mn);
-        a.gdx[g] += a.dt * p2 * mn;
+        const float p2 = ldu(a.qa + sd.off, o), r2 = ldu(a.qb + sd.off, o);
+        float* gcx = a.gcx + sd.off;
+        float* gcz = a.gcz + sd.off;
+        float* gdx = a.gdx + sd.off;
+        stu(gcx, o, ldu(gcx, o) + a.dt * p2 * ln);
+        stu(gcz, o, ldu(gcz, o) + a.dt * r2 * (ln + mn));
+        stu(gdx, o, ldu(gdx, o) + a.dt * p2 * mn);
       }
     }
 #pragma unroll
@@ -411,97 +415,110 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
 template <class S>
 __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   __shared__ float xbuf[4][72];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = wave_id(), lane = threadIdx.x & 63;
   const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
   if (item >= a.nitems) return;
   const int4 itm = a.items[item];
   const ShotDev sd = a.g.shots[itm.x];
   const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
-  const bool inb = ix < nx;
   const bool act = ix >= kRim && ix < nx - kRim;
-  const int hx = itm.y + halo_col(lane);
-  const bool hact = lane < 8 && hx >= kRim && hx < nx - kRim;
+  const int cx = clampi(ix, 0, nx - 1);
+  const int hxr = itm.y + halo_col(lane);
+  const bool hact = lane < 8 && hxr >= kRim && hxr < nx - kRim;
+  const int hx = lane < 8 ? clampi(hxr, 0, nx - 1) : cx;
   // bands within reach of a side strip need the C-PML terms in x
   const bool xnear = itm.y < cpml + 4 || itm.y + 64 > nx - cpml - 4;
   const float axo = act ? coef_x(a.g.ta, ix, nx, cpml) : 0.f;
   const float axh = hact ? coef_x(a.g.ta, hx, nx, cpml) : 0.f;
   const float dh = a.g.dh, dh2 = a.g.dh2, dt2 = a.dt2;
-  const float* __restrict__ L = a.Lc + sd.off;
-  const float* __restrict__ M = a.Mc + sd.off;
-  float* __restrict__ Lo = a.Lo + sd.off;
-  float* __restrict__ Mo = a.Mo + sd.off;
-  const float* __restrict__ vh = a.m.vh + sd.x0;
-  const float* __restrict__ vp = a.m.vp + sd.x0;
-  const float* __restrict__ vn = a.m.vn + sd.x0;
-  const float* __restrict__ ztz = a.ztz + sd.off;
-  const float* __restrict__ ztx = a.ztx + sd.off;
+  const float* L = a.Lc + sd.off;
+  const float* M = a.Mc + sd.off;
+  float* Lo = a.Lo + sd.off;
+  float* Mo = a.Mo + sd.off;
+  const float* vh = a.m.vh + sd.x0;
+  const float* vp = a.m.vp + sd.x0;
+  const float* vn = a.m.vn + sd.x0;
+  const float* ztz = a.ztz + sd.off;
+  const float* ztx = a.ztx + sd.off;
   float* buf = xbuf[wave];
   const int z0 = itm.z, z1 = itm.w;
+  const int zb = nz - cpml;      // first row of the bottom C-PML strip
   const int zc = nz - cpml - 4;  // rows at or below this need the C-PML terms in z
   float gq[9];         // gbar_z along z (rows z-4..z+4)
   float lq[5], mq[5];  // mP, mF at rows z..z+4
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int r = z0 - 4 + k;
-    const long ir = ix + (long)nx * r;
-    const float l = act ? L[ir] : 0.f;
-    const float m = act ? M[ir] : 0.f;
-    float v = 0.f;
-    if (act && r >= kRim) v = dt2 * (vp[ix + (long)nxm * r] * (l + m)) + coef_z(a.g.ta, r, nz, cpml) * ztz[ir];
-    gq[k] = v;
+    const unsigned ir = (unsigned)(cx + nx * r);
+    const float l = ldu(L, ir);
+    const float m = ldu(M, ir);
+    float v = dt2 * (ldu(vp, (unsigned)(cx + nxm * r)) * (l + m));
+    if (r >= zb) v = v + coef_z(a.g.ta, r, nz, cpml) * ldu(ztz, ir);
+    gq[k] = (act && r >= kRim) ? v : 0.f;
     if (k >= 4) {
       lq[k - 4] = l;
       mq[k - 4] = m;
     }
   }
   for (int z = z0; z < z1; ++z) {
-    const long i = ix + (long)nx * z;
-    const long mi = ix + (long)nxm * z;
+    const unsigned o = (unsigned)(cx + nx * z);
+    const unsigned mo = (unsigned)(cx + nxm * z);
+    const unsigned ho = (unsigned)(hx + nx * z), hmo = (unsigned)(hx + nxm * z);
     {
       const int r = z + 4;
-      const float l = act ? L[i + 4L * nx] : 0.f;
-      const float m = act ? M[i + 4L * nx] : 0.f;
+      const unsigned ir = o + 4u * nx;
+      const float l = ldu(L, ir);
+      const float m = ldu(M, ir);
       lq[4] = l;
       mq[4] = m;
-      float v = 0.f;
-      if (act && r < nz - kRim)
-        v = dt2 * (vp[mi + 4L * nxm] * (l + m)) + coef_z(a.g.ta, r, nz, cpml) * ztz[i + 4L * nx];
-      gq[8] = v;
+      float v = dt2 * (ldu(vp, mo + 4u * nxm) * (l + m));
+      if (r >= zb) v = v + coef_z(a.g.ta, r, nz, cpml) * ldu(ztz, ir);
+      gq[8] = (act && r < nz - kRim) ? v : 0.f;
     }
     // gbar_x of the band row (zero outside Omega) and of the halo columns
-    float go = 0.f, gh = 0.f;
-    if (act) {
-      go = dt2 * (vh[mi] * lq[0] + vn[mi] * mq[0]);
-      if (xnear) go = go + axo * ztx[i];
+    float go = dt2 * (ldu(vh, mo) * lq[0] + ldu(vn, mo) * mq[0]);
+    float gh = dt2 * (ldu(vh, hmo) * ldu(L, ho) + ldu(vn, hmo) * ldu(M, ho));
+    if (xnear) {
+      go = go + axo * ldu(ztx, o);
+      gh = gh + axh * ldu(ztx, ho);
     }
-    if (hact) {
-      const long hi = hx + (long)nx * z, hm = hx + (long)nxm * z;
-      gh = dt2 * (vh[hm] * L[hi] + vn[hm] * M[hi]);
-      if (xnear) gh = gh + axh * ztx[hi];
-    }
+    go = act ? go : 0.f;
+    gh = hact ? gh : 0.f;
+    const float lold = ldu(Lo, o), mold = ldu(Mo, o);
     float gx[9];
     xrow9(buf, lane, go, gh, gx);
     if (act) {
       float ap = d2s<S>(gx[0], gx[1], gx[2], gx[3], gx[4], gx[5], gx[6], gx[7], gx[8], dh2);
       float af = d2s<S>(gq[0], gq[1], gq[2], gq[3], gq[4], gq[5], gq[6], gq[7], gq[8], dh2);
-      if (xnear) ap = ap - d1_at<S>(a.ptx, sd.off + i, 1, dh);
-      if (z >= zc) af = af - d1_at<S>(a.ptz, sd.off + i, nx, dh);
-      float ln = 2.f * lq[0] + ap - Lo[i];
-      float mn = 2.f * mq[0] + af - Mo[i];
+      if (xnear) {
+        const float* pt = a.ptx + sd.off;
+        ap = ap - d1s<S>(ldu(pt, o - 4), ldu(pt, o - 3), ldu(pt, o - 2), ldu(pt, o - 1), ldu(pt, o + 1),
+                         ldu(pt, o + 2), ldu(pt, o + 3), ldu(pt, o + 4), dh);
+      }
+      if (z >= zc) {
+        const float* pt = a.ptz + sd.off;
+        const unsigned snx = (unsigned)nx;
+        af = af - d1s<S>(ldu(pt, o - 4 * snx), ldu(pt, o - 3 * snx), ldu(pt, o - 2 * snx), ldu(pt, o - snx),
+                         ldu(pt, o + snx), ldu(pt, o + 2 * snx), ldu(pt, o + 3 * snx), ldu(pt, o + 4 * snx), dh);
+      }
+      float ln = 2.f * lq[0] + ap - lold;
+      float mn = 2.f * mq[0] + af - mold;
       if (z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec) {
         const float r = a.res[sd.goff + (long)a.rit * sd.nrec + (ix - sd.ir0)];
         ln += r;
         mn += r;
       }
-      Lo[i] = ln;
-      Mo[i] = mn;
+      stu(Lo, o, ln);
+      stu(Mo, o, mn);
       if (a.qa) {
-        const long g = sd.off + i;
-        const float qa = a.qa[g], qb = a.qb[g];
-        a.gh[g] += dt2 * ln * qa;
-        a.gn[g] += dt2 * mn * qa;
-        a.gp[g] += dt2 * (ln + mn) * qb;
+        const float qa = ldu(a.qa + sd.off, o), qb = ldu(a.qb + sd.off, o);
+        float* gh_ = a.gh + sd.off;
+        float* gn_ = a.gn + sd.off;
+        float* gp_ = a.gp + sd.off;
+        stu(gh_, o, ldu(gh_, o) + dt2 * ln * qa);
+        stu(gn_, o, ldu(gn_, o) + dt2 * mn * qa);
+        stu(gp_, o, ldu(gp_, o) + dt2 * (ln + mn) * qb);
       }
     }
 #pragma unroll

@@ -104,6 +104,20 @@ __device__ __forceinline__ void xrow9(float* buf, int lane, float v, float h, fl
 // Band column of the halo value a lane supplies to xrow9: lanes 0..3 -> -4..-1, lanes 4..7 -> 64..67.
 __device__ __forceinline__ int halo_col(int lane) { return lane < 4 ? lane - 4 : 60 + lane; }
 
+// Load/store through a wave-uniform base pointer and a 32-bit element index: the address is
+// base + zext(4 * idx), which the compiler emits as one global access with the base in scalar registers
+// and a 32-bit vector offset (no per-lane 64-bit address arithmetic).  Per-shot fields are far below
+// 2^30 elements.
+__device__ __forceinline__ float ldu(const float* base, unsigned idx) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (idx << 2));
+}
+__device__ __forceinline__ void stu(float* base, unsigned idx, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (idx << 2)) = v;
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// Wave index within the workgroup as a wave-uniform (scalar) value.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 // Map a linear workgroup id onto a logical id so that the workgroups one XCD receives (hardware deals
 // them round-robin over 8 XCDs) form one contiguous block of logical work.  Bijective for any n.
 __device__ __forceinline__ int xcd_logical_block(int bid, int n) {

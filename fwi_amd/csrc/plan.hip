@@ -150,14 +150,18 @@ struct fwi_plan {
   }
 
   // ---------------------------------------------------------------------------------------------
+  // Work items ordered row run -> band -> shot, so that the four waves of a workgroup are four shots on
+  // the same tile (their model rows are shared through the L1) and the items of one XCD form a
+  // contiguous range of tiles (their halo rows stay in that XCD's L2).
   int build_items(Batch& b, int rows) {
     std::vector<int4> items;
-    for (int s = 0; s < b.count; ++s) {
-      const ShotDev& sd = b.h_shots[s];
-      for (int z0 = kRim; z0 < nz - kRim; z0 += rows) {
-        int z1 = std::min(z0 + rows, nz - kRim);
-        for (int x0 = 0; x0 < sd.nx; x0 += 64) items.push_back(make_int4(s, x0, z0, z1));
-      }
+    int maxnx = 0;
+    for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
+    for (int z0 = kRim; z0 < nz - kRim; z0 += rows) {
+      const int z1 = std::min(z0 + rows, nz - kRim);
+      for (int x0 = 0; x0 < maxnx; x0 += 64)
+        for (int s = 0; s < b.count; ++s)
+          if (x0 < b.h_shots[s].nx) items.push_back(make_int4(s, x0, z0, z1));
     }
     (void)hipFree(b.d_items);
     b.d_items = nullptr;
