@@ -218,6 +218,13 @@ def main():
             f"{fwd_bytes / max(fwd_avg, 1e-12) / 1e9:.0f} GB/s; storing step mean "
             f"{rec_ms / max(rec_n, 1) * 1e3:.1f} us")
 
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "fused_steps_survey1600_hbm_traffic.json")
+    kname = "k_adj_ex_main" if grad_mode == 1 else "k_adj_ref_main"
+    if os.path.exists(tfile) and w.name == "survey1600" and len(w.shots) == 64:
+        with open(tfile) as fh:
+            traffic = json.load(fh)["bytes_per_launch"].get(kname)
+
     plain = None
     if not args.no_plain:
         ns = min(args.plain_steps, nt)
@@ -262,12 +269,12 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_adj_ex_main" if grad_mode == 1 else "k_adj_ref_main",
+            "kernel": kname,
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": adj_bytes,
             "mean_launch_us": round(adj_avg * 1e6, 2) if adj_n else None,
         },
