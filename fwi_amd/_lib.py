@@ -70,6 +70,7 @@ BATCHED = {
     "fwi_forward": ([vp_, _F, _F, _F, _F, vp_], c_int),
     "fwi_misfit": ([vp_, _F, _F, _F, _F, ctypes.POINTER(c_double), vp_], c_int),
     "fwi_gradient": ([vp_, _F, _F, _F, _F, vp_, ctypes.POINTER(c_double), _F, vp_], c_int),
+    "fwi_gradient_src": ([vp_, _F, _F, _F, _F, vp_, _F, vp_], c_int),
     "fwi_chain_rule": ([_F, _F, _F, vp_, vp_, c_ll, vp_], c_int),
     "fwi_cpml_table": ([c_int, c_float, c_float, c_float, c_float, c_float, c_float, ctypes.POINTER(c_float),
                         ctypes.POINTER(c_float)], None),

@@ -56,6 +56,24 @@ def _compile(src: str) -> str:
     return out
 
 
+DRIVER_SRC = os.path.join(HERE, "driver", "fwi_main.c")
+DRIVER_BIN = os.path.join(HERE, "fwi_main")
+
+
+def build_driver() -> str:
+    """The C driver (src/main.c's control flow on the C ABI), linked against libfwi.so."""
+    cc = shutil.which("gcc") or "cc"
+    tmp = DRIVER_BIN + ".tmp"
+    cmd = [cc, "-std=c99", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-o", tmp, "-L", HERE, "-lfwi",
+           "-L/opt/rocm/lib", "-lamdhip64", "-lm", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER_BIN)
+    return DRIVER_BIN
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -66,8 +84,9 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DRIVER_BIN}")
     return LIB
 
 

@@ -378,8 +378,9 @@ struct fwi_plan {
     return 0;
   }
 
-  int gradient_batch(const Batch& b, const ModelArgs& m, const float* obs, double* grad, int first_batch,
-                     float* syn_out, hipStream_t st) {
+  // obs != null: residual = syn - obs and misfit; adj_src != null: the residual is adj_src itself.
+  int gradient_batch(const Batch& b, const ModelArgs& m, const float* obs, const float* adj_src, double* grad,
+                     int first_batch, float* syn_out, hipStream_t st) {
     const int qm = opt.grad_mode == FWI_GRAD_EXACT ? 1 : 2;
     const int lag = opt.grad_mode == FWI_GRAD_EXACT ? 1 : 2;
     float* gather = syn_out ? syn_out + b.gather_off : gathers();
@@ -398,7 +399,11 @@ struct fwi_plan {
       }
       if (int r = fwd_step(b, m, n, n >= slast ? qm : 0, qa, qb, gather, st)) return r;
     }
-    if (int r = residual(b, gather, obs, resid(), d_misfit, first_batch ? 0 : 1, st)) return r;
+    if (adj_src) {
+      CK(hipMemcpyAsync(resid(), adj_src + b.gather_off, b.ngather * sizeof(float), hipMemcpyDeviceToDevice, st));
+    } else {
+      if (int r = residual(b, gather, obs, resid(), d_misfit, first_batch ? 0 : 1, st)) return r;
+    }
     // adjoint
     CK(hipMemsetAsync(adj(0), 0, 12 * (size_t)cap_cell * sizeof(float), st));
     CK(hipMemsetAsync(sgrad(0), 0, 3 * (size_t)cap_cell * sizeof(float), st));
@@ -712,7 +717,7 @@ int fwi_gradient(fwi_plan* p, const float* vh, const float* vp, const float* vn,
   if (e != hipSuccess) return p->fail(e, "zero grad");
   bool first = true;
   for (auto& b : p->batches) {
-    if (int r = p->gradient_batch(b, m, obs, grad, first ? 1 : 0, syn, st)) return r;
+    if (int r = p->gradient_batch(b, m, obs, nullptr, grad, first ? 1 : 0, syn, st)) return r;
     first = false;
   }
   if (misfit) {
@@ -754,6 +759,19 @@ long long fwi_plan_cells(const fwi_plan* p, long long* strip_cells) {
 
 int fwi_plan_ckpt_interval(const fwi_plan* p) { return p->K; }
 int fwi_plan_rows_per_item(const fwi_plan* p) { return p->tz; }
+
+int fwi_gradient_src(fwi_plan* p, const float* vh, const float* vp, const float* vn, const float* adj_src,
+                     double* grad, float* syn, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  ModelArgs m{vh, vp, vn, p->nx};
+  if (int r = p->tune(m, st)) return r;
+  hipError_t e = hipMemsetAsync(grad, 0, 3 * (size_t)p->nx * p->nz * sizeof(double), st);
+  if (e != hipSuccess) return p->fail(e, "zero grad");
+  for (auto& b : p->batches)
+    if (int r = p->gradient_batch(b, m, nullptr, adj_src, grad, 0, syn, st)) return r;
+  p->timing_collect();
+  return 0;
+}
 
 int fwi_chain_rule(const float* vp, const float* eps, const float* dl, const double* g, double* out, long long n,
                    void* stream) {

@@ -183,6 +183,20 @@ class Plan:
         return m.value, grad, syn
 
 
+    def gradient_src(self, model, adj_src: torch.Tensor, want_syn: bool = False, stream=None):
+        """Gradient of <adj_src, gathers(model)> with respect to (v_hor, v_po, v_nmo): (grad float64, syn)."""
+        vh, vp, vn = self._model(model)
+        adj_src = adj_src.contiguous().float()
+        assert adj_src.numel() == self.gather_size
+        grad = torch.empty((3, self.nz, self.nx), dtype=torch.float64, device=vh.device)
+        syn = torch.empty(self.gather_size, dtype=torch.float32, device=vh.device) if want_syn else None
+        _lib.check(_lib.load().fwi_gradient_src(self._h, vh.data_ptr(), vp.data_ptr(), vn.data_ptr(),
+                                                adj_src.data_ptr(), grad.data_ptr(),
+                                                syn.data_ptr() if syn is not None else None, _stream_ptr(stream)),
+                   "fwi_gradient_src")
+        return grad, syn
+
+
 def chain_rule(vp: torch.Tensor, eps: torch.Tensor, delta: torch.Tensor, grad: torch.Tensor, stream=None):
     """dJ/d(vp, eps, delta) [3, nz, nx] float64 from dJ/d(v_hor, v_po, v_nmo) (fwi_chain_rule)."""
     vp, eps, delta = (t.contiguous().float() for t in (vp, eps, delta))

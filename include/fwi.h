@@ -95,6 +95,13 @@ int fwi_misfit(fwi_plan* plan, const float* v_hor, const float* v_po, const floa
 int fwi_gradient(fwi_plan* plan, const float* v_hor, const float* v_po, const float* v_nmo, const float* obs,
                  double* grad, double* misfit, float* syn, void* stream);
 
+/* Gradient of <adj_src, gathers(model)> (the adjoint driven by an arbitrary adjoint source instead of the
+ * residual): adj_src has the survey gather layout.  With adj_src = dL/dsyn this is dL/d(v_hor, v_po, v_nmo)
+ * for any loss L of the gathers; fwi_gradient is the case adj_src = syn - obs.  No counterpart in the
+ * reference, whose adjoint source is always the residual (parallelcalls.cu:159-163). */
+int fwi_gradient_src(fwi_plan* plan, const float* v_hor, const float* v_po, const float* v_nmo, const float* adj_src,
+                     double* grad, float* syn, void* stream);
+
 /* Chain rule to (vp, epsilon, delta) for v_po = vp^2, v_hor = vp^2(1+2 eps), v_nmo = vp^2(1+2 delta):
  * out = [dJ/dvp, dJ/deps, dJ/ddelta] (FP64, 3*n) from grad = [dJ/dv_hor, dJ/dv_po, dJ/dv_nmo]. */
 int fwi_chain_rule(const float* vp, const float* eps, const float* delta, const double* grad, double* out, long long n,
