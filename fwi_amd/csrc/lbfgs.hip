@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(256) k_backup_model(const float* __restrict__ 
 }
 
 // Model update on rows iz > layer (parallelcalls.cu:273-286 / :343-404 with flags 1,1,1,0,0):
-// m_k = base_k - alpha_k * dir_k * scale_k, then v_hor >= v_nmo.  base may alias m.
+// m_k = base_k - alpha_k * dir_k * scale_k, then v_hor >= v_nmo.  base holds three contiguous blocks of n.
 __global__ void __launch_bounds__(256) k_update3(float* __restrict__ vh, float* __restrict__ vp, float* __restrict__ vn,
                                                  const float* __restrict__ base, const double* __restrict__ dir,
                                                  Coef3 alpha, Coef3 scale, int nx, int nz) {
@@ -226,7 +226,9 @@ int fwi_lbfgs_sd_step(fwi_lbfgs* h, float* vh, float* vp, float* vn, const doubl
   Coef3 a{{alpha[0], alpha[1], alpha[2]}};
   Coef3 sc{{0, 0, 0}};
   for (int k = 0; k < 3; ++k) sc.c[k] = nn[k] > 0 ? 1.0 / std::sqrt(nn[k]) : 0.0;
-  hipLaunchKernelGGL(k_update3, dim3(h->blocks(h->n)), dim3(256), 0, st, vh, vp, vn, vh, grad, a, sc, h->nx, h->nz);
+  // base = the backup just taken: three contiguous blocks (the model fields themselves are separate arrays)
+  hipLaunchKernelGGL(k_update3, dim3(h->blocks(h->n)), dim3(256), 0, st, vh, vp, vn, h->mback, grad, a, sc, h->nx,
+                     h->nz);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : h->fail(e, "sd_step");
 }
