@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--nt", type=int, default=0)
     ap.add_argument("--mode", choices=["exact", "refcompat"], default="exact")
     ap.add_argument("--ckpt", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")
+    ap.add_argument("--order", type=int, default=-1, help="work-item order (-1 = tuned)")
     ap.add_argument("--plain-steps", type=int, default=400)
     ap.add_argument("--cpu-steps", type=int, default=4000)
     ap.add_argument("--no-plain", action="store_true")
@@ -172,7 +174,8 @@ def main():
     true_sq = pre_variables(*(dev(x) for x in w.true_model))
     init_sq = pre_variables(*(dev(x) for x in w.init_model))
     grad_mode = 1 if args.mode == "exact" else 0
-    plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt)
+    plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
+                rows_per_item=args.rows, item_order=args.order)
     log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, workspace {plan.workspace_bytes / 1e9:.1f} GB")
     t0 = time.perf_counter()
     obs = plan.forward(true_sq)
@@ -262,6 +265,7 @@ def main():
             "ckpt_interval": plan.ckpt_interval,
             "batch_shots": plan.batch_size,
             "rows_per_item": plan.rows_per_item,
+            "item_order": plan.item_order,
             "s_per_gradient_batched": round(sec, 4),
             "s_per_gradient_plain": round(plain, 3) if plain else None,
             "speedup_vs_plain": round(plain / sec, 2) if plain else None,

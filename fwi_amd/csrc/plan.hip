@@ -60,6 +60,7 @@ struct fwi_plan {
   fwi_options opt{};
   int K = 1, J = 1;  // checkpoint interval and number of segments
   int tz = 32;       // rows per work item
+  int order = 0;     // work-item order (build_items)
   bool tuned = false;
   std::vector<Batch> batches;
   long long gather_total = 0;
@@ -150,18 +151,38 @@ struct fwi_plan {
   }
 
   // ---------------------------------------------------------------------------------------------
-  // Work items ordered row run -> band -> shot, so that the four waves of a workgroup are four shots on
-  // the same tile (their model rows are shared through the L1) and the items of one XCD form a
-  // contiguous range of tiles (their halo rows stay in that XCD's L2).
-  int build_items(Batch& b, int rows) {
+  // Work items in one of three orders (the tuner picks one per plan):
+  //   0  row run -> band -> shot: the four waves of a workgroup are four shots on one tile (model rows
+  //      shared through the L1), x-neighbour tiles are a few workgroups apart;
+  //   1  shot -> row run -> band: a workgroup is four adjacent bands of one shot (x halos shared through
+  //      the L1), z-neighbour row runs one band row apart;
+  //   2  band -> shot group of 4 -> row run -> shot: consecutive workgroups walk down a column of tiles,
+  //      so z halos are reused from the L2 while the neighbour is still resident.
+  // Workgroup ids are remapped so each XCD receives a contiguous range of this order.
+  int build_items(Batch& b, int rows, int order) {
     std::vector<int4> items;
     int maxnx = 0;
     for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
-    for (int z0 = kRim; z0 < nz - kRim; z0 += rows) {
-      const int z1 = std::min(z0 + rows, nz - kRim);
-      for (int x0 = 0; x0 < maxnx; x0 += 64)
+    std::vector<std::pair<int, int>> runs;
+    for (int z0 = kRim; z0 < nz - kRim; z0 += rows) runs.push_back({z0, std::min(z0 + rows, nz - kRim)});
+    if (order == 1) {
+      for (int s = 0; s < b.count; ++s)
+        for (auto& r : runs)
+          for (int x0 = 0; x0 < b.h_shots[s].nx; x0 += 64) items.push_back(make_int4(s, x0, r.first, r.second));
+    } else if (order == 2) {
+      for (int x0 = 0; x0 < maxnx; x0 += 64) {
+        std::vector<int> sh;
         for (int s = 0; s < b.count; ++s)
-          if (x0 < b.h_shots[s].nx) items.push_back(make_int4(s, x0, z0, z1));
+          if (x0 < b.h_shots[s].nx) sh.push_back(s);
+        for (size_t g = 0; g < sh.size(); g += 4)
+          for (auto& r : runs)
+            for (size_t k = g; k < std::min(sh.size(), g + 4); ++k) items.push_back(make_int4(sh[k], x0, r.first, r.second));
+      }
+    } else {
+      for (auto& r : runs)
+        for (int x0 = 0; x0 < maxnx; x0 += 64)
+          for (int s = 0; s < b.count; ++s)
+            if (x0 < b.h_shots[s].nx) items.push_back(make_int4(s, x0, r.first, r.second));
     }
     (void)hipFree(b.d_items);
     b.d_items = nullptr;
@@ -439,46 +460,55 @@ struct fwi_plan {
   }
 
   // ---------------------------------------------------------------------------------------------
-  // Pick the rows per work item by timing the fused forward and adjoint steps of the largest batch.
+  // Pick the work-item order and rows per item by timing the fused forward and adjoint steps of the
+  // largest batch (a few steps per candidate on the plan's own buffers).
   int tune(const ModelArgs& m, hipStream_t st) {
-    if (opt.rows_per_item > 0 || tuned) {
-      tuned = true;
-      return 0;
-    }
-    const int cands[] = {16, 24, 32, 48, 64, 96};
+    if (tuned) return 0;
+    std::vector<int> rows_c = {16, 24, 32, 48, 64, 96};
+    std::vector<int> order_c = {0, 1, 2};
+    if (opt.rows_per_item > 0) rows_c = {opt.rows_per_item};
+    if (opt.item_order >= 0) order_c = {opt.item_order};
     Batch& b = batches[0];
     hipEvent_t e0, e1;
     CK(hipEventCreate(&e0));
     CK(hipEventCreate(&e1));
     float best = 1e30f;
-    int best_tz = tz;
-    for (int c : cands) {
-      if (c > nz - 8 && c != cands[0]) continue;
-      if (int r = build_items(b, c)) return r;
-      if (int r = zero_fwd(b, st)) return r;
-      CK(hipMemsetAsync(adj(0), 0, 12 * (size_t)cap_cell * sizeof(float), st));
-      for (int w = 0; w < 2; ++w) {
-        if (int r = fwd_step(b, m, w, 0, nullptr, nullptr, nullptr, st)) return r;
+    int best_tz = tz, best_order = order;
+    const bool single = rows_c.size() == 1 && order_c.size() == 1;
+    for (int o : order_c)
+      for (int c : rows_c) {
+        if (c > nz - 8 && c != rows_c[0]) continue;
+        if (int r = build_items(b, c, o)) return r;
+        if (single) {
+          best_tz = c;
+          best_order = o;
+          break;
+        }
+        if (int r = zero_fwd(b, st)) return r;
+        CK(hipMemsetAsync(adj(0), 0, 12 * (size_t)cap_cell * sizeof(float), st));
+        for (int w = 0; w < 2; ++w)
+          if (int r = fwd_step(b, m, w, 0, nullptr, nullptr, nullptr, st)) return r;
+        CK(hipEventRecord(e0, st));
+        for (int n = 0; n < 6; ++n) {
+          if (int r = fwd_step(b, m, n, 0, nullptr, nullptr, nullptr, st)) return r;
+          if (int r = adj_step(b, m, nt - n, nullptr, nullptr, st)) return r;
+        }
+        CK(hipEventRecord(e1, st));
+        CK(hipEventSynchronize(e1));
+        float ms = 0;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          best_tz = c;
+          best_order = o;
+        }
       }
-      CK(hipEventRecord(e0, st));
-      for (int n = 0; n < 6; ++n) {
-        if (int r = fwd_step(b, m, n, 0, nullptr, nullptr, nullptr, st)) return r;
-        if (int r = adj_step(b, m, nt - n, nullptr, nullptr, st)) return r;
-      }
-      CK(hipEventRecord(e1, st));
-      CK(hipEventSynchronize(e1));
-      float ms = 0;
-      CK(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        best_tz = c;
-      }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     tz = best_tz;
+    order = best_order;
     for (auto& bb : batches)
-      if (int r = build_items(bb, tz)) return r;
+      if (int r = build_items(bb, tz, order)) return r;
     tuned = true;
     return 0;
   }
@@ -517,6 +547,7 @@ void fwi_options_default(fwi_options* o) {
   std::memset(o, 0, sizeof *o);
   o->grad_mode = FWI_GRAD_EXACT;
   o->arith = FWI_ARITH_FP32;
+  o->item_order = -1;
 }
 
 int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out) {
@@ -651,13 +682,12 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
         ck(hipMemcpy(b.d_sblocks, b.h_sblocks.data(), b.h_sblocks.size() * sizeof(int2), hipMemcpyHostToDevice),
            "copy sblocks");
     }
-    if (rc == 0) rc = p->build_items(b, p->tz);
+    if (rc == 0) rc = p->build_items(b, p->tz, p->opt.item_order > 0 ? p->opt.item_order : 0);
   }
   if (rc) {
     delete p;
     return rc;
   }
-  if (p->opt.rows_per_item > 0) p->tuned = true;
   *out = p;
   return 0;
 }
@@ -759,6 +789,7 @@ long long fwi_plan_cells(const fwi_plan* p, long long* strip_cells) {
 
 int fwi_plan_ckpt_interval(const fwi_plan* p) { return p->K; }
 int fwi_plan_rows_per_item(const fwi_plan* p) { return p->tz; }
+int fwi_plan_item_order(const fwi_plan* p) { return p->order; }
 
 int fwi_gradient_src(fwi_plan* p, const float* vh, const float* vp, const float* vn, const float* adj_src,
                      double* grad, float* syn, void* stream) {

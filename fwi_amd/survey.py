@@ -59,7 +59,8 @@ class Plan:
 
     def __init__(self, nx: int, nz: int, nt: int, dh: float, dt: float, cpml: int, shots: Sequence[Shot],
                  wavelet: np.ndarray, cpml_ab, grad_mode: int = GRAD_EXACT, arith: int = ARITH_FP32,
-                 ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0):
+                 ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0,
+                 item_order: int = -1):
         L = _lib.load()
         self.nx, self.nz, self.nt, self.dh, self.dt, self.cpml = nx, nz, nt, dh, dt, cpml
         self.shots = list(shots)
@@ -81,6 +82,7 @@ class Plan:
         opt.max_batch = max_batch
         opt.mem_budget = mem_budget
         opt.rows_per_item = rows_per_item
+        opt.item_order = item_order
         h = ctypes.c_void_p()
         _lib.check(L.fwi_plan_create(ctypes.byref(sv), ctypes.byref(opt), ctypes.byref(h)), "fwi_plan_create")
         self._h = h
@@ -115,6 +117,10 @@ class Plan:
     @property
     def rows_per_item(self) -> int:
         return int(_lib.load().fwi_plan_rows_per_item(self._h))
+
+    @property
+    def item_order(self) -> int:
+        return int(_lib.load().fwi_plan_item_order(self._h))
 
     def cells(self):
         """(sum of nx_s*nz over shots, C-PML cells of all windows)."""
