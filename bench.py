@@ -11,7 +11,7 @@ Beside it, on the same workload:
     steps and scaled to the full survey (it is linear in shots and steps);
   * roofline of the dominant kernel (the adjoint step): algorithmic bytes per launch over its average
     duration, measured with HIP events on the stream the kernel runs on during the timed region;
-  * cpu_baseline: the CPU oracle (f32, exact gradient, OpenMP) on one shot for a bounded number of steps.
+  * cpu_baseline: the CPU oracle (f32i, exact gradient, OpenMP) on one shot for a bounded number of steps.
 """
 from __future__ import annotations
 
@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--nt", type=int, default=0)
     ap.add_argument("--mode", choices=["exact", "refcompat"], default="exact")
     ap.add_argument("--ckpt", type=int, default=0)
+    ap.add_argument("--arith", type=int, default=3,
+                    help="0 FP32 two-level, 1 mixed (reference), 2 FP32 difference stencil, 3 FP32 increment form")
     ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")
     ap.add_argument("--order", type=int, default=-1, help="work-item order (-1 = tuned)")
     ap.add_argument("--plain-steps", type=int, default=400)
@@ -126,7 +128,7 @@ def plain_path_seconds(w, model_sq_dev, src_dev, ab, nsteps):
 
 
 def cpu_baseline(w, nsteps):
-    """Oracle (f32 arithmetic, exact gradient, OpenMP over all host threads) on one shot of the workload
+    """Oracle (f32i arithmetic, exact gradient, OpenMP over all host threads) on one shot of the workload
     for nsteps steps: forward + adjoint + imaging.  Returns (Gcell-updates/s, threads, seconds)."""
     import numpy as np
 
@@ -140,7 +142,7 @@ def cpu_baseline(w, nsteps):
     sq = O.pre_variables(*w.init_model)
     obs = np.zeros((nsteps, shot.nrec), np.float32)
     t0 = time.perf_counter()
-    O.gradient("f32", True, sq, tab, src, obs, shot.nx, w.nz, nsteps, w.cpml, w.dh, w.dt, shot)
+    O.gradient("f32i", True, sq, tab, src, obs, shot.nx, w.nz, nsteps, w.cpml, w.dh, w.dt, shot)
     sec = time.perf_counter() - t0
     return 2.0 * shot.nx * w.nz * nsteps / sec / 1e9, threads, sec
 
@@ -175,7 +177,7 @@ def main():
     init_sq = pre_variables(*(dev(x) for x in w.init_model))
     grad_mode = 1 if args.mode == "exact" else 0
     plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
-                rows_per_item=args.rows, item_order=args.order)
+                rows_per_item=args.rows, item_order=args.order, arith=args.arith)
     log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, workspace {plan.workspace_bytes / 1e9:.1f} GB")
     t0 = time.perf_counter()
     obs = plan.forward(true_sq)
@@ -241,7 +243,7 @@ def main():
         ns = min(args.cpu_steps, nt)
         cg, threads, csec = cpu_baseline(w, ns)
         cpu = {"value": round(cg, 4), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
-               "sample": f"oracle f32 exact gradient, 1 shot of {w.name} ({w.nx}x{w.nz}), {ns} of {nt} steps, "
+               "sample": f"oracle f32i exact gradient, 1 shot of {w.name} ({w.nx}x{w.nz}), {ns} of {nt} steps, "
                          f"fwd+adj, {csec:.1f} s"}
         log(f"cpu baseline: {cg:.3f} Gcell/s on {threads} threads ({csec:.1f} s)")
 
@@ -262,6 +264,7 @@ def main():
             "workload": f"{w.name}: {w.nx}x{w.nz} dh={w.dh} m, {len(w.shots)} shots, {w.shots[0].nrec} receivers, "
                         f"nt={nt}, dt={w.dt} s, {w.f} Hz Ricker, cpml={w.cpml}",
             "grad_mode": args.mode,
+            "arith": {0: "fp32", 1: "mixed", 2: "fp32-diff", 3: "fp32-increment"}[args.arith],
             "ckpt_interval": plan.ckpt_interval,
             "batch_shots": plan.batch_size,
             "rows_per_item": plan.rows_per_item,

@@ -54,7 +54,8 @@ struct GridArgs {
   const float* tb;  // C-PML b
 };
 
-// Forward step.  P/F "cur" hold step n, "old" hold n-1 and receive n+1.
+// Forward step.  P/F "cur" hold step n, "old" hold n-1 and receive n+1.  In the increment form "old"
+// only receives n+1 and Q, R (the increments P^n - P^{n-1}, F^n - F^{n-1}) are updated in place.
 struct FwdArgs {
   GridArgs g;
   ModelArgs m;
@@ -70,6 +71,8 @@ struct FwdArgs {
   float* psz;
   float* zx;
   float* zz;
+  float* Qc;      // increment form: Q, R
+  float* Rc;
   float* gather;  // batch gathers or null
   float* qa;      // per-step store or null
   float* qb;

@@ -52,7 +52,12 @@ __global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __r
 // One forward step of every shot (parallelcalls.cu:120-130 SecondDerivate_P, :65-86 ZETA_F, :88-117
 // propagator_F).  QM selects what is kept for the adjoint pass: 0 nothing, 1 the model-multiplied terms
 // a = p2 + aten_p, b = r2 + aten_f (exact gradient), 2 p2, r2 (reference-compatible gradient).
-template <int QM, class S>
+// INC: increment form of the same leapfrog.  The state is (P^n, Q^n = P^n - P^{n-1}) and the update is
+// Q^{n+1} = Q^n + d, P^{n+1} = P^n + Q^{n+1}, which equals P^{n+1} = 2P^n - P^{n-1} + d in exact
+// arithmetic.  In FP32 the stored Q keeps its own relative precision, whereas the difference of two
+// stored levels loses it: with s samples per period the rounding of the two-level form is amplified
+// about (s/2pi)^2 times (about 600 at 8 Hz, dt = 0.8 ms), that of the increment form about s/2pi.
+template <int QM, class S, bool INC = false>
 __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
   __shared__ float xbuf[4][72];
   const int wave = wave_id(), lane = threadIdx.x & 63;
@@ -93,7 +98,9 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
     fq[8] = ldu(F, o + 4u * nx);
     const float c = ldu(P, o);
     const float h = ldu(P, (unsigned)(hx + nx * z));
-    const float pold = ldu(Po, o), fold = ldu(Fo, o);
+    // two-level form: P^{n-1}, F^{n-1}; increment form: Q^n, R^n
+    const float pold = INC ? ldu(a.Qc + sd.off, o) : ldu(Po, o);
+    const float fold = INC ? ldu(a.Rc + sd.off, o) : ldu(Fo, o);
     const float vhv = ldu(vh, mo), vpv = ldu(vp, mo), vnv = ldu(vn, mo);
     float u[9];
     xrow9(buf, lane, c, h, u);
@@ -120,11 +127,26 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
       }
       const float tp = dh2 * (p2 + atp);
       const float tf = dh2 * (r2 + atf);
-      float pn = 2.f * c - pold + vhv * C * tp + vpv * C * tf;
-      float fn = 2.f * fq[4] - fold + vnv * C * tp + vpv * C * tf;
-      if (ix == sd.isx && z == sd.isz) {
-        pn += srcv;
-        fn += srcv;
+      float pn, fn;
+      if (INC) {
+        float dq = vhv * C * tp + vpv * C * tf;
+        float dr = vnv * C * tp + vpv * C * tf;
+        if (ix == sd.isx && z == sd.isz) {
+          dq += srcv;
+          dr += srcv;
+        }
+        const float qn = pold + dq, rn = fold + dr;
+        pn = c + qn;
+        fn = fq[4] + rn;
+        stu(a.Qc + sd.off, o, qn);
+        stu(a.Rc + sd.off, o, rn);
+      } else {
+        pn = 2.f * c - pold + vhv * C * tp + vpv * C * tf;
+        fn = 2.f * fq[4] - fold + vnv * C * tp + vpv * C * tf;
+        if (ix == sd.isx && z == sd.isz) {
+          pn += srcv;
+          fn += srcv;
+        }
       }
       stu(Po, o, pn);
       stu(Fo, o, fn);
@@ -559,12 +581,19 @@ __global__ void __launch_bounds__(256) k_strip_copy(StripArgs a, int dir, float*
   }
 }
 
-// res = syn - obs (parallelcalls.cu:133-142) and per-block FP64 partial sums of res^2.
-__global__ void __launch_bounds__(256) k_residual(const float* __restrict__ syn, const float* __restrict__ obs,
-                                                  float* __restrict__ res, long long n, double* __restrict__ partial) {
+
+// Residual res = syn - obs of every shot of a batch and per-(shot, block) FP64 partial sums of res^2
+// (parallelcalls.cu:133-142, main.c:448-454).  Block (b, s) covers a fixed slice of shot s's gather, so
+// the misfit of a shot does not depend on which other shots share its batch.
+__global__ void __launch_bounds__(256) k_residual(const ShotDev* __restrict__ shots, int nt,
+                                                  const float* __restrict__ syn, const float* __restrict__ obs,
+                                                  float* __restrict__ res, double* __restrict__ partial) {
   __shared__ double red[4];
+  const ShotDev sd = shots[blockIdx.y];
+  const long long n = (long long)nt * sd.nrec;
   double acc = 0.0;
-  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+  for (long long j = blockIdx.x * 256LL + threadIdx.x; j < n; j += (long long)gridDim.x * 256) {
+    const long long i = sd.goff + j;
     const float r = syn[i] - obs[i];
     if (res) res[i] = r;
     acc += (double)r * (double)r;
@@ -572,22 +601,28 @@ __global__ void __launch_bounds__(256) k_residual(const float* __restrict__ syn,
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (threadIdx.x == 0) partial[blockIdx.y * (long long)gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// Sum of n doubles by one workgroup in a fixed order (deterministic).  out[0] (+)= scale * sum.
-__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partial, int n, double scale,
-                                                      double* __restrict__ out, int accumulate) {
+// Per-shot misfits scale * sum(partials of the shot) in a fixed tree, then added to out[0] shot by shot
+// in batch order (accumulate = 0 starts from zero).  shot_out (may be null) receives each shot's misfit.
+__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partial, int per_shot, int nshots,
+                                                      double scale, double* __restrict__ out, int accumulate,
+                                                      double* __restrict__ shot_out) {
   __shared__ double red[4];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) acc += partial[i];
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double s = scale * ((red[0] + red[1]) + (red[2] + red[3]));
-    out[0] = accumulate ? out[0] + s : s;
+  double total = accumulate ? out[0] : 0.0;
+  for (int s = 0; s < nshots; ++s) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < per_shot; i += 256) acc += partial[(long long)s * per_shot + i];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    const double v = scale * ((red[0] + red[1]) + (red[2] + red[3]));
+    total += v;
+    if (threadIdx.x == 0 && shot_out) shot_out[s] = v;
+    __syncthreads();
   }
+  if (threadIdx.x == 0) out[0] = total;
 }
 
 // FP64 cross-shot sum of the batch's FP32 per-shot gradients into the model gradient (3 fields).

@@ -29,7 +29,7 @@ using namespace fwi;
 
 namespace {
 
-constexpr int kResidualBlocks = 1024;
+constexpr int kResidualBlocks = 64;  // per shot
 
 struct Batch {
   int s0 = 0, count = 0;
@@ -71,8 +71,9 @@ struct fwi_plan {
   float* d_work = nullptr;  // all per-cell workspace of the largest batch
   size_t work_bytes = 0;
   long long cap_cell = 0, cap_strip = 0, cap_gather = 0;
-  double* d_partial = nullptr;
+  double* d_partial = nullptr;   // kResidualBlocks per shot of the largest batch
   double* d_misfit = nullptr;
+  double* d_shot_misfit = nullptr;  // per shot of the survey, from the last misfit or gradient
   // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute),
   // 2 = adjoint step; every `timing_stride`-th launch of a kind is bracketed by events on its stream
   int timing_stride = 0;
@@ -111,15 +112,27 @@ struct fwi_plan {
   }
 
   // workspace views (sized for cap_cell)
-  float* fwd(int k) const { return d_work + (size_t)k * cap_cell; }                 // 0..7
-  float* adj(int k) const { return d_work + (size_t)(8 + k) * cap_cell; }           // 0..11
-  float* sgrad(int k) const { return d_work + (size_t)(20 + k) * cap_cell; }         // 0..2
-  float* qbuf() const { return d_work + (size_t)23 * cap_cell; }                    // K * 2 * cap_cell
-  float* ckpt(int j) const {                                                        // j = 1..J-1
-    return d_work + (size_t)(23 + 2 * (size_t)K) * cap_cell + (size_t)(j - 1) * (4 * cap_cell + 4 * cap_strip);
+  // forward: 0,1 P ping-pong, 2,3 F ping-pong, 4 psx, 5 psz, 6 zx, 7 zz, 8 Q = P^n - P^{n-1},
+  // 9 R = F^n - F^{n-1} (increment form only)
+  static constexpr int kFwdF = 10, kAdjF = 12, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
+  float* fwd(int k) const { return d_work + (size_t)k * cap_cell; }
+  float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..11
+  float* sgrad(int k) const { return d_work + (size_t)(kFwdF + kAdjF + k) * cap_cell; }      // 0..2
+  float* qbuf() const { return d_work + (size_t)kFixedF * cap_cell; }                      // K * 2 * cap_cell
+  float* ckpt(int j) const {                                                               // j = 1..J-1
+    return d_work + (size_t)(kFixedF + 2 * (size_t)K) * cap_cell + (size_t)(j - 1) * (4 * cap_cell + 4 * cap_strip);
   }
   float* gathers() const {
-    return d_work + (size_t)(23 + 2 * (size_t)K) * cap_cell + (size_t)(J - 1) * (4 * cap_cell + 4 * cap_strip);
+    return d_work + (size_t)(kFixedF + 2 * (size_t)K) * cap_cell + (size_t)(J - 1) * (4 * cap_cell + 4 * cap_strip);
+  }
+  bool inc() const { return opt.arith == FWI_ARITH_FP32I; }
+  // the four cell fields a checkpoint holds before step n: both levels of P and F, or in the increment
+  // form the current P, F and their increments Q, R
+  int ckpt_field(int f, int n) const {
+    if (!inc()) return f;
+    const int cur = n & 1;
+    const int map[4] = {cur, 2 + cur, 8, 9};
+    return map[f];
   }
   float* resid() const { return gathers() + cap_gather; }
   std::string err;
@@ -147,6 +160,7 @@ struct fwi_plan {
     (void)hipFree(d_work);
     (void)hipFree(d_partial);
     (void)hipFree(d_misfit);
+    (void)hipFree(d_shot_misfit);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
   }
 
@@ -238,6 +252,8 @@ struct fwi_plan {
     a.psz = fwd(5);
     a.zx = fwd(6);
     a.zz = fwd(7);
+    a.Qc = fwd(8);
+    a.Rc = fwd(9);
     a.gather = gather;
     a.qa = qa;
     a.qb = qb;
@@ -245,7 +261,12 @@ struct fwi_plan {
     dim3 grid((b.nitems + 3) / 4);
     const int tev = timed_begin(qm == 0 ? 0 : 1, st);
 #define FWD_LAUNCH(QMV, SV) hipLaunchKernelGGL((k_fwd_main<QMV, SV>), grid, dim3(256), 0, st, a)
-    if (am == FWI_ARITH_MIXED) {
+#define FWD_LAUNCH_INC(QMV) hipLaunchKernelGGL((k_fwd_main<QMV, float, true>), grid, dim3(256), 0, st, a)
+    if (am == FWI_ARITH_FP32I) {
+      if (qm == 0) FWD_LAUNCH_INC(0);
+      else if (qm == 1) FWD_LAUNCH_INC(1);
+      else FWD_LAUNCH_INC(2);
+    } else if (am == FWI_ARITH_MIXED) {
       if (qm == 0) FWD_LAUNCH(0, double);
       else if (qm == 1) FWD_LAUNCH(1, double);
       else FWD_LAUNCH(2, double);
@@ -259,6 +280,7 @@ struct fwi_plan {
       else FWD_LAUNCH(2, float);
     }
 #undef FWD_LAUNCH
+#undef FWD_LAUNCH_INC
     timed_end(tev, st);
     CK(hipGetLastError());
     return 0;
@@ -364,7 +386,7 @@ struct fwi_plan {
   }
 
   int zero_fwd(const Batch& b, hipStream_t st) {
-    CK(hipMemsetAsync(fwd(0), 0, 8 * (size_t)cap_cell * sizeof(float), st));
+    CK(hipMemsetAsync(fwd(0), 0, kFwdF * (size_t)cap_cell * sizeof(float), st));
     (void)b;
     return 0;
   }
@@ -372,21 +394,24 @@ struct fwi_plan {
   int save_ckpt(const Batch& b, int j, hipStream_t st) {
     float* c = ckpt(j);
     for (int f = 0; f < 4; ++f)
-      CK(hipMemcpyAsync(c + (size_t)f * b.ncell, fwd(f), b.ncell * sizeof(float), hipMemcpyDeviceToDevice, st));
+      CK(hipMemcpyAsync(c + (size_t)f * b.ncell, fwd(ckpt_field(f, j * K)), b.ncell * sizeof(float),
+                        hipMemcpyDeviceToDevice, st));
     return strip_copy(b, 0, c + 4 * (size_t)b.ncell, st);
   }
   int load_ckpt(const Batch& b, int j, hipStream_t st) {
     float* c = ckpt(j);
     for (int f = 0; f < 4; ++f)
-      CK(hipMemcpyAsync(fwd(f), c + (size_t)f * b.ncell, b.ncell * sizeof(float), hipMemcpyDeviceToDevice, st));
+      CK(hipMemcpyAsync(fwd(ckpt_field(f, j * K)), c + (size_t)f * b.ncell, b.ncell * sizeof(float),
+                        hipMemcpyDeviceToDevice, st));
     return strip_copy(b, 1, c + 4 * (size_t)b.ncell, st);
   }
 
   int residual(const Batch& b, const float* syn, const float* obs, float* res, double* out, int accumulate,
                hipStream_t st) {
-    hipLaunchKernelGGL(k_residual, dim3(kResidualBlocks), dim3(256), 0, st, syn, obs + b.gather_off, res, b.ngather,
-                       d_partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, d_partial, kResidualBlocks, 0.5, out, accumulate);
+    hipLaunchKernelGGL(k_residual, dim3(kResidualBlocks, b.count), dim3(256), 0, st, b.d_shots, nt, syn,
+                       obs + b.gather_off, res, d_partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, d_partial, kResidualBlocks, b.count, 0.5, out,
+                       accumulate, d_shot_misfit + b.s0);
     CK(hipGetLastError());
     return 0;
   }
@@ -546,7 +571,7 @@ extern "C" {
 void fwi_options_default(fwi_options* o) {
   std::memset(o, 0, sizeof *o);
   o->grad_mode = FWI_GRAD_EXACT;
-  o->arith = FWI_ARITH_FP32;
+  o->arith = FWI_ARITH_FP32I;
   o->item_order = -1;
 }
 
@@ -577,7 +602,8 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     set_message("unknown grad_mode");
     return 1;
   }
-  if (p->opt.arith != FWI_ARITH_FP32 && p->opt.arith != FWI_ARITH_MIXED && p->opt.arith != FWI_ARITH_FP32D) {
+  if (p->opt.arith != FWI_ARITH_FP32 && p->opt.arith != FWI_ARITH_MIXED && p->opt.arith != FWI_ARITH_FP32D &&
+      p->opt.arith != FWI_ARITH_FP32I) {
     delete p;
     set_message("unknown arith");
     return 1;
@@ -595,8 +621,8 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     strips[i] = strip_count(p->shots[i].nx, p->nz, p->cpml);
     gath[i] = (long long)nt * p->shots[i].nrec;
   }
-  // per-cell floats: 8 fwd + 12 adj + 3 grad + 2K imaging terms + 4 per checkpoint
-  const double per_cell = 23.0 + 2.0 * p->K + 4.0 * (p->J - 1);
+  // per-cell floats: 10 fwd + 12 adj + 3 grad + 2K imaging terms + 4 per checkpoint
+  const double per_cell = (double)fwi_plan::kFixedF + 2.0 * p->K + 4.0 * (p->J - 1);
   const double per_strip = 4.0 * (p->J - 1);
   size_t freeb = 0, totalb = 0;
   if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
@@ -642,8 +668,11 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
   ck(hipMalloc(&p->d_src, nt * sizeof(float)), "hipMalloc(src)");
   ck(hipMalloc(&p->d_ta, p->cpml * sizeof(float)), "hipMalloc(ta)");
   ck(hipMalloc(&p->d_tb, p->cpml * sizeof(float)), "hipMalloc(tb)");
-  ck(hipMalloc(&p->d_partial, kResidualBlocks * sizeof(double)), "hipMalloc(partial)");
+  int max_count = 0;
+  for (auto& b : p->batches) max_count = std::max(max_count, b.count);
+  ck(hipMalloc(&p->d_partial, (size_t)kResidualBlocks * max_count * sizeof(double)), "hipMalloc(partial)");
   ck(hipMalloc(&p->d_misfit, sizeof(double)), "hipMalloc(misfit)");
+  ck(hipMalloc(&p->d_shot_misfit, (size_t)ns * sizeof(double)), "hipMalloc(shot misfit)");
   if (rc == 0) ck(hipMemcpy(p->d_src, p->wavelet.data(), nt * sizeof(float), hipMemcpyHostToDevice), "copy src");
   if (rc == 0) ck(hipMemcpy(p->d_ta, p->ta.data(), p->cpml * sizeof(float), hipMemcpyHostToDevice), "copy ta");
   if (rc == 0) ck(hipMemcpy(p->d_tb, p->tb.data(), p->cpml * sizeof(float), hipMemcpyHostToDevice), "copy tb");
@@ -757,6 +786,13 @@ int fwi_gradient(fwi_plan* p, const float* vh, const float* vp, const float* vn,
   }
   p->timing_collect();
   return 0;
+}
+
+int fwi_plan_shot_misfits(fwi_plan* p, double* out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(out, p->d_shot_misfit, p->shots.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? 0 : p->fail(e, "shot misfit readback");
 }
 
 void fwi_plan_set_timing(fwi_plan* p, int stride) {

@@ -20,6 +20,7 @@ GRAD_EXACT = 1
 ARITH_FP32 = 0
 ARITH_MIXED = 1
 ARITH_FP32D = 2
+ARITH_FP32I = 3  # default: FP32 with the forward leapfrog in increment form
 
 
 def cpml_table(p: CpmlParams):
@@ -58,7 +59,7 @@ class Plan:
     """
 
     def __init__(self, nx: int, nz: int, nt: int, dh: float, dt: float, cpml: int, shots: Sequence[Shot],
-                 wavelet: np.ndarray, cpml_ab, grad_mode: int = GRAD_EXACT, arith: int = ARITH_FP32,
+                 wavelet: np.ndarray, cpml_ab, grad_mode: int = GRAD_EXACT, arith: int = ARITH_FP32I,
                  ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0,
                  item_order: int = -1):
         L = _lib.load()
@@ -127,6 +128,13 @@ class Plan:
         s = ctypes.c_longlong()
         c = _lib.load().fwi_plan_cells(self._h, ctypes.byref(s))
         return int(c), int(s.value)
+
+    def shot_misfits(self) -> np.ndarray:
+        """Per-shot FP64 misfits of the last misfit or gradient call (they sum, in shot order, to the total)."""
+        out = np.zeros(len(self.shots), np.float64)
+        _lib.check(_lib.load().fwi_plan_shot_misfits(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                      None), "fwi_plan_shot_misfits")
+        return out
 
     def set_timing(self, stride: int):
         _lib.load().fwi_plan_set_timing(self._h, stride)

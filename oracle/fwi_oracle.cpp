@@ -16,6 +16,9 @@
 //   f32d: T=float,  S=F32D   -- float, second derivatives in difference form
 //         sum_k c_k((u_{+k}-u_0)+(u_{-k}-u_0)) (this project's FP32 arithmetic; the weights sum to zero,
 //         so it is the same operator without the cancellation of the O(|u|) terms).
+//   f32i: T=float,  S=F32I   -- float, forward leapfrog in increment form: the state is (P^n, Q^n =
+//         P^n - P^{n-1}) and Q^{n+1} = Q^n + d, P^{n+1} = P^n + Q^{n+1} (this project's default FP32
+//         arithmetic); the same recursion as f32 in exact arithmetic.
 // Build with -ffp-contract=off so the rounding of every expression is the one written here.
 //
 // Indexing: idx = ix + nx*iz, x fastest (include/parallelcalls.h:5).  Omega = {4<=ix<=nx-5, 4<=iz<=nz-5}
@@ -27,7 +30,8 @@
 //   cpml_table          : closed form of Komatitsch & Martin C-PML, kappa=1 (test_oracle_pins.py)
 //   ricker              : closed form                                     (test_oracle_pins.py)
 //   forward (all modes) : isotropic reduction (eps=delta=0 => P==F), analytic 2-D Green's function
-//                         before the first reflection, linearity, reciprocity (test_oracle_pins.py)
+//                         before the first reflection, linearity, reciprocity (test_oracle_pins.py);
+//                         f32i against f64 at 1e-5 and ahead of f32 on a long, finely sampled run
 //   exact adjoint       : dot-product test of the one-step transpose, FD directional derivatives
 //   ref-compat gradient : restated formula; relation to the exact gradient (lag + 1/dt) on tiny grids
 //   L-BFGS two-loop     : dense BFGS inverse update with H0 = gamma*I (Nocedal & Wright Alg. 7.4)
@@ -58,12 +62,17 @@ struct Geo {
 };
 
 struct F32D {};  // tag: float arithmetic, difference-form second derivative
+struct F32I {};  // tag: float arithmetic, forward leapfrog in increment form (state P^n, Q^n = P^n - P^{n-1})
 template <class S>
 struct SumType {
   using type = S;
 };
 template <>
 struct SumType<F32D> {
+  using type = float;
+};
+template <>
+struct SumType<F32I> {
   using type = float;
 };
 
@@ -354,10 +363,12 @@ template <class T, class S>
 struct FwdState {
   Geo g;
   long n;
-  Fields<T> f;  // 0 P_pre 1 P_past 2 F_pre 3 F_past 4 p2 5 r2 6 psx 7 psz 8 zx 9 zz 10 atp 11 atf
+  // 0 P_pre 1 P_past 2 F_pre 3 F_past 4 p2 5 r2 6 psx 7 psz 8 zx 9 zz 10 atp 11 atf 12 Q 13 R
+  // (Q = P^n - P^{n-1}, R = F^n - F^{n-1}: increment form only)
+  Fields<T> f;
   std::vector<T> shotbuf;
   int cur = 0;  // parity of the ping-pong
-  FwdState(const Geo& g_) : g(g_), n((long)g_.nx * g_.nz), f(n, 12) {}
+  FwdState(const Geo& g_) : g(g_), n((long)g_.nx * g_.nz), f(n, 14) {}
   T* Pcur() { return cur == 0 ? f[0] : f[1]; }
   T* Pold() { return cur == 0 ? f[1] : f[0]; }
   T* Fcur() { return cur == 0 ? f[2] : f[3]; }
@@ -381,18 +392,42 @@ void fwd_step(FwdState<T, S>& st, const T* vh, const T* vp, const T* vn, const T
   const T* r2 = st.f[5];
   const T* atp = st.f[10];
   const T* atf = st.f[11];
-#pragma omp parallel for
-  for (int iz = kRim; iz < nz - kRim; ++iz)
-    for (int ix = kRim; ix < nx - kRim; ++ix) {
-      long i = ix + (long)nx * iz;
-      T tp = dh2 * (p2[i] + atp[i]);
-      T tf = dh2 * (r2[i] + atf[i]);
-      Pm[i] = (T)2 * P[i] - Pm[i] + vh[i] * C * tp + vp[i] * C * tf;
-      Fm[i] = (T)2 * F[i] - Fm[i] + vn[i] * C * tp + vp[i] * C * tf;
-    }
   long is = sg.isx + (long)nx * sg.isz;
-  Pm[is] += src[it];
-  Fm[is] += src[it];
+  if constexpr (std::is_same<S, F32I>::value) {
+    // increment form (the batched kernel's FP32I): Q += d (+ source), P^{n+1} = P^n + Q; the source
+    // cell is interior (the batched API validates it)
+    T* Q = st.f[12];
+    T* R = st.f[13];
+#pragma omp parallel for
+    for (int iz = kRim; iz < nz - kRim; ++iz)
+      for (int ix = kRim; ix < nx - kRim; ++ix) {
+        long i = ix + (long)nx * iz;
+        T tp = dh2 * (p2[i] + atp[i]);
+        T tf = dh2 * (r2[i] + atf[i]);
+        T dq = vh[i] * C * tp + vp[i] * C * tf;
+        T dr = vn[i] * C * tp + vp[i] * C * tf;
+        if (i == is) {
+          dq += src[it];
+          dr += src[it];
+        }
+        Q[i] = Q[i] + dq;
+        R[i] = R[i] + dr;
+        Pm[i] = P[i] + Q[i];
+        Fm[i] = F[i] + R[i];
+      }
+  } else {
+#pragma omp parallel for
+    for (int iz = kRim; iz < nz - kRim; ++iz)
+      for (int ix = kRim; ix < nx - kRim; ++ix) {
+        long i = ix + (long)nx * iz;
+        T tp = dh2 * (p2[i] + atp[i]);
+        T tf = dh2 * (r2[i] + atf[i]);
+        Pm[i] = (T)2 * P[i] - Pm[i] + vh[i] * C * tp + vp[i] * C * tf;
+        Fm[i] = (T)2 * F[i] - Fm[i] + vn[i] * C * tp + vp[i] * C * tf;
+      }
+    Pm[is] += src[it];
+    Fm[is] += src[it];
+  }
   if (gather)
     for (int r = 0; r < sg.nrec; ++r) {
       long i = sg.ir0 + r + (long)nx * sg.irz;
@@ -899,6 +934,7 @@ ORC_FWD(mix, float, double)
 ORC_FWD(f32, float, float)
 ORC_FWD(f64, double, double)
 ORC_FWD(f32d, float, F32D)
+ORC_FWD(f32i, float, F32I)
 
 // One shot's gradient.  mode_exact=0: reference-compatible (serialcalls.cu:68-90 semantics: one-step
 // snapshot lag, dt scaling, forward-recursion C-PML on the adjoint); 1: exact discrete gradient.
@@ -915,6 +951,7 @@ ORC_GRAD(mix, float, double)
 ORC_GRAD(f32, float, float)
 ORC_GRAD(f64, double, double)
 ORC_GRAD(f32d, float, F32D)
+ORC_GRAD(f32i, float, F32I)
 
 // One linear forward state step without source (for the dot-product test), FP64.
 // state = 8 fields of nx*nz: P, P_old, F, F_old, psx, zx, psz, zz  ->  out: P', P, F', F, psx', zx', psz', zz'

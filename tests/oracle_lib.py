@@ -61,7 +61,7 @@ def _declare(L):
     L.orc_compute_ri.argtypes = [_F] * 6 + [_f, _f, _f, _i, _i]
     L.orc_update_r.argtypes = [_F] * 6 + [_f] * 6 + [_i, _i, _i]
     L.orc_update_LBFGS.argtypes = [_F] * 9 + [_i, _i, _f, _f, _f] + [_i] * 5
-    for suf, P in (("mix", _F), ("f32", _F), ("f64", _D), ("f32d", _F)):
+    for suf, P in (("mix", _F), ("f32", _F), ("f64", _D), ("f32d", _F), ("f32i", _F)):
         fn = getattr(L, f"orc_forward_{suf}")
         fn.argtypes = [P] * 6 + [_i, _i, _i, _i, _d, _d, _i, _i, _i, _i, _i, P]
         fn.restype = None
