@@ -16,6 +16,9 @@ struct ShotDev {
   long long off;  // offset of the shot's nx*nz fields in the batch field arrays (cells)
   long long soff; // offset of the shot's strip cells in packed strip arrays
   long long goff; // offset of the shot's gather [nt][nrec] in the batch gather arrays
+  long long gslot;  // offset of the gradient slot the shot accumulates into (its group leader's window)
+  int lead;         // 1 if the shot leads its gradient group (its slot is summed into the model gradient)
+  int pad_;
 };
 
 // C-PML cells of a window: the side strips Omega & (ix < cpml | ix > nx-cpml-1) for rows 4..nz-5, then

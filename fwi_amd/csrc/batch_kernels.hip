@@ -318,9 +318,9 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
       stu(Mo, o, mn);
       if (a.qa) {
         const float p2 = ldu(a.qa + sd.off, o), r2 = ldu(a.qb + sd.off, o);
-        float* gcx = a.gcx + sd.off;
-        float* gcz = a.gcz + sd.off;
-        float* gdx = a.gdx + sd.off;
+        float* gcx = a.gcx + sd.gslot;
+        float* gcz = a.gcz + sd.gslot;
+        float* gdx = a.gdx + sd.gslot;
         stu(gcx, o, ldu(gcx, o) + a.dt * p2 * ln);
         stu(gcz, o, ldu(gcz, o) + a.dt * r2 * (ln + mn));
         stu(gdx, o, ldu(gdx, o) + a.dt * p2 * mn);
@@ -434,13 +434,21 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
 
 // Main exact adjoint step with the exact imaging condition
 //   g_hor += dt^2 mP^k a^{k-1}, g_nmo += dt^2 mF^k a^{k-1}, g_po += dt^2 (mP^k + mF^k) b^{k-1}.
-template <class S>
+// GROUP: the four waves of a workgroup are up to four shots of one gradient group (identical windows) on
+// the same tile; a wave whose item has shot -(leader+1) is a placeholder that loads nothing.  Per row the
+// waves' contributions meet in LDS and are added in wave order, ((c0 + c1) + c2) + c3, then waves 0, 1, 2
+// read-modify-write g_hor, g_nmo, g_po of the group's slot once: 6 B of gradient traffic per cell and shot
+// instead of 24.  Items are padded to whole workgroups, so every wave reaches every barrier.
+template <class S, bool GROUP = false>
 __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   __shared__ float xbuf[4][72];
+  __shared__ float gsum[GROUP ? 2 : 1][4][3][64];
   const int wave = wave_id(), lane = threadIdx.x & 63;
   const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
   if (item >= a.nitems) return;
-  const int4 itm = a.items[item];
+  int4 itm = a.items[item];
+  const bool ghost = GROUP && itm.x < 0;
+  if (ghost) itm.x = -itm.x - 1;
   const ShotDev sd = a.g.shots[itm.x];
   const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
@@ -469,22 +477,26 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const int zc = nz - cpml - 4;  // rows at or below this need the C-PML terms in z
   float gq[9];         // gbar_z along z (rows z-4..z+4)
   float lq[5], mq[5];  // mP, mF at rows z..z+4
+  if (!ghost) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int r = z0 - 4 + k;
-    const unsigned ir = (unsigned)(cx + nx * r);
-    const float l = ldu(L, ir);
-    const float m = ldu(M, ir);
-    float v = dt2 * (ldu(vp, (unsigned)(cx + nxm * r)) * (l + m));
-    if (r >= zb) v = v + coef_z(a.g.ta, r, nz, cpml) * ldu(ztz, ir);
-    gq[k] = (act && r >= kRim) ? v : 0.f;
-    if (k >= 4) {
-      lq[k - 4] = l;
-      mq[k - 4] = m;
+    for (int k = 0; k < 8; ++k) {
+      const int r = z0 - 4 + k;
+      const unsigned ir = (unsigned)(cx + nx * r);
+      const float l = ldu(L, ir);
+      const float m = ldu(M, ir);
+      float v = dt2 * (ldu(vp, (unsigned)(cx + nxm * r)) * (l + m));
+      if (r >= zb) v = v + coef_z(a.g.ta, r, nz, cpml) * ldu(ztz, ir);
+      gq[k] = (act && r >= kRim) ? v : 0.f;
+      if (k >= 4) {
+        lq[k - 4] = l;
+        mq[k - 4] = m;
+      }
     }
   }
   for (int z = z0; z < z1; ++z) {
     const unsigned o = (unsigned)(cx + nx * z);
+    float ch = 0.f, cn = 0.f, cp = 0.f;  // this shot's imaging contributions (GROUP)
+    if (!ghost) {
     const unsigned mo = (unsigned)(cx + nxm * z);
     const unsigned ho = (unsigned)(hx + nx * z), hmo = (unsigned)(hx + nxm * z);
     {
@@ -535,12 +547,18 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       stu(Mo, o, mn);
       if (a.qa) {
         const float qa = ldu(a.qa + sd.off, o), qb = ldu(a.qb + sd.off, o);
-        float* gh_ = a.gh + sd.off;
-        float* gn_ = a.gn + sd.off;
-        float* gp_ = a.gp + sd.off;
-        stu(gh_, o, ldu(gh_, o) + dt2 * ln * qa);
-        stu(gn_, o, ldu(gn_, o) + dt2 * mn * qa);
-        stu(gp_, o, ldu(gp_, o) + dt2 * (ln + mn) * qb);
+        if (GROUP) {
+          ch = dt2 * ln * qa;
+          cn = dt2 * mn * qa;
+          cp = dt2 * (ln + mn) * qb;
+        } else {
+          float* gh_ = a.gh + sd.gslot;
+          float* gn_ = a.gn + sd.gslot;
+          float* gp_ = a.gp + sd.gslot;
+          stu(gh_, o, ldu(gh_, o) + dt2 * ln * qa);
+          stu(gn_, o, ldu(gn_, o) + dt2 * mn * qa);
+          stu(gp_, o, ldu(gp_, o) + dt2 * (ln + mn) * qb);
+        }
       }
     }
 #pragma unroll
@@ -549,6 +567,21 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     for (int k = 0; k < 4; ++k) {
       lq[k] = lq[k + 1];
       mq[k] = mq[k + 1];
+    }
+    }  // !ghost
+    if (GROUP && a.qa) {
+      // double-buffered by row parity: one barrier per row suffices, since a wave can only overwrite this
+      // buffer two rows later, after every wave has passed the next row's barrier
+      float(*gs)[3][64] = gsum[GROUP ? (z & 1) : 0];
+      gs[wave][0][lane] = ch;
+      gs[wave][1][lane] = cn;
+      gs[wave][2][lane] = cp;
+      __syncthreads();
+      if (wave < 3 && act) {
+        const float sum = ((gs[0][wave][lane] + gs[1][wave][lane]) + gs[2][wave][lane]) + gs[3][wave][lane];
+        float* g = (wave == 0 ? a.gh : wave == 1 ? a.gn : a.gp) + sd.gslot;
+        stu(g, o, ldu(g, o) + sum);
+      }
     }
   }
 }
@@ -625,8 +658,8 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__
   if (threadIdx.x == 0) out[0] = total;
 }
 
-// FP64 cross-shot sum of the batch's FP32 per-shot gradients into the model gradient (3 fields).
-// Shots are added in batch order for a deterministic result.
+// FP64 cross-shot sum of the batch's FP32 gradient slots (one per gradient group, held by the group's
+// leader) into the model gradient (3 fields).  Slots are added in batch order for a deterministic result.
 __global__ void __launch_bounds__(256) k_cross_shot_sum(const ShotDev* __restrict__ shots, int nshots, int nxm, int nz,
                                                         const float* __restrict__ g0, const float* __restrict__ g1,
                                                         const float* __restrict__ g2, double* __restrict__ out,
@@ -638,8 +671,8 @@ __global__ void __launch_bounds__(256) k_cross_shot_sum(const ShotDev* __restric
   for (int s = 0; s < nshots; ++s) {
     const ShotDev sd = shots[s];
     const int lx = x - sd.x0;
-    if (lx < 0 || lx >= sd.nx) continue;
-    const long long i = sd.off + lx + (long long)sd.nx * z;
+    if (!sd.lead || lx < 0 || lx >= sd.nx) continue;
+    const long long i = sd.gslot + lx + (long long)sd.nx * z;
     a0 += (double)g0[i];
     a1 += (double)g1[i];
     a2 += (double)g2[i];

@@ -35,8 +35,12 @@ struct Batch {
   int s0 = 0, count = 0;
   long long ncell = 0, nstrip = 0, ngather = 0, gather_off = 0;
   ShotDev* d_shots = nullptr;
-  int4* d_items = nullptr;
+  int4* d_items = nullptr;      // forward work items
   int nitems = 0;
+  int4* d_items_adj = nullptr;  // adjoint work items (grouped: whole workgroups per group and tile)
+  int nitems_adj = 0;
+  std::vector<int2> groups;     // gradient groups (first shot in the batch, count)
+  bool grouped = false;         // some group has more than one shot: the adjoint uses the grouped kernel
   int2* d_sblocks = nullptr;
   int nsblocks = 0;
   std::vector<ShotDev> h_shots;
@@ -59,8 +63,10 @@ struct fwi_plan {
   std::vector<float> wavelet, ta, tb;
   fwi_options opt{};
   int K = 1, J = 1;  // checkpoint interval and number of segments
-  int tz = 32;       // rows per work item
+  int tz = 32;       // rows per work item (forward kernels)
   int order = 0;     // work-item order (build_items)
+  int tz_adj = 32;   // rows per work item and order of the adjoint kernels
+  int order_adj = 0;
   bool tuned = false;
   std::vector<Batch> batches;
   long long gather_total = 0;
@@ -152,6 +158,7 @@ struct fwi_plan {
     for (auto& b : batches) {
       (void)hipFree(b.d_shots);
       (void)hipFree(b.d_items);
+      (void)hipFree(b.d_items_adj);
       (void)hipFree(b.d_sblocks);
     }
     (void)hipFree(d_src);
@@ -165,15 +172,8 @@ struct fwi_plan {
   }
 
   // ---------------------------------------------------------------------------------------------
-  // Work items in one of three orders (the tuner picks one per plan):
-  //   0  row run -> band -> shot: the four waves of a workgroup are four shots on one tile (model rows
-  //      shared through the L1), x-neighbour tiles are a few workgroups apart;
-  //   1  shot -> row run -> band: a workgroup is four adjacent bands of one shot (x halos shared through
-  //      the L1), z-neighbour row runs one band row apart;
-  //   2  band -> shot group of 4 -> row run -> shot: consecutive workgroups walk down a column of tiles,
-  //      so z halos are reused from the L2 while the neighbour is still resident.
-  // Workgroup ids are remapped so each XCD receives a contiguous range of this order.
-  int build_items(Batch& b, int rows, int order) {
+  // Work items: (shot, 64-column band, first row, end row) per wave, four per workgroup.
+  static std::vector<int4> ungrouped_items(const Batch& b, int nz, int rows, int order) {
     std::vector<int4> items;
     int maxnx = 0;
     for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
@@ -198,12 +198,58 @@ struct fwi_plan {
           for (int s = 0; s < b.count; ++s)
             if (x0 < b.h_shots[s].nx) items.push_back(make_int4(s, x0, r.first, r.second));
     }
-    (void)hipFree(b.d_items);
-    b.d_items = nullptr;
-    b.nitems = (int)items.size();
-    CK(hipMalloc(&b.d_items, items.size() * sizeof(int4)));
-    CK(hipMemcpy(b.d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    return items;
+  }
+
+  // Grouped adjoint items: every workgroup is one gradient group on one tile, member k on wave k, and
+  // -(leader+1) (a placeholder wave) where the group has fewer than 4 shots.  Orders: 0 row run -> band ->
+  // group; 2 band -> group -> row run.
+  static std::vector<int4> grouped_items(const Batch& b, int nz, int rows, int order) {
+    std::vector<int4> items;
+    std::vector<std::pair<int, int>> runs;
+    for (int z0 = kRim; z0 < nz - kRim; z0 += rows) runs.push_back({z0, std::min(z0 + rows, nz - kRim)});
+    int maxnx = 0;
+    for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
+    auto push = [&](const int2& g, int x0, const std::pair<int, int>& r) {
+      for (int k = 0; k < 4; ++k)
+        items.push_back(make_int4(k < g.y ? g.x + k : -(g.x + 1), x0, r.first, r.second));
+    };
+    if (order == 2) {
+      for (int x0 = 0; x0 < maxnx; x0 += 64)
+        for (auto& g : b.groups)
+          if (x0 < b.h_shots[g.x].nx)
+            for (auto& r : runs) push(g, x0, r);
+    } else {
+      for (auto& r : runs)
+        for (int x0 = 0; x0 < maxnx; x0 += 64)
+          for (auto& g : b.groups)
+            if (x0 < b.h_shots[g.x].nx) push(g, x0, r);
+    }
+    return items;
+  }
+
+  int upload_items(int4*& dst, int& n, const std::vector<int4>& items) {
+    (void)hipFree(dst);
+    dst = nullptr;
+    n = (int)items.size();
+    CK(hipMalloc(&dst, items.size() * sizeof(int4)));
+    CK(hipMemcpy(dst, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
     return 0;
+  }
+
+  // Work items of the forward kernels in one of three orders (the tuner picks one per plan):
+  //   0  row run -> band -> shot: the four waves of a workgroup are four shots on one tile (model rows
+  //      shared through the L1), x-neighbour tiles are a few workgroups apart;
+  //   1  shot -> row run -> band: a workgroup is four adjacent bands of one shot (x halos shared through
+  //      the L1), z-neighbour row runs one band row apart;
+  //   2  band -> shot group of 4 -> row run -> shot: consecutive workgroups walk down a column of tiles,
+  //      so z halos are reused from the L2 while the neighbour is still resident.
+  // Workgroup ids are remapped so each XCD receives a contiguous range of this order.
+  int build_items(Batch& b, int rows, int order) { return upload_items(b.d_items, b.nitems, ungrouped_items(b, nz, rows, order)); }
+  // Work items of the adjoint kernels: grouped (orders 0 and 2) when the batch has gradient groups.
+  int build_adj_items(Batch& b, int rows, int order) {
+    return upload_items(b.d_items_adj, b.nitems_adj,
+                        b.grouped ? grouped_items(b, nz, rows, order) : ungrouped_items(b, nz, rows, order));
   }
 
   GridArgs grid_args(const Batch& b) const {
@@ -295,7 +341,7 @@ struct fwi_plan {
     float* Mo = adj(2 + (cur ^ 1));
     const int am = opt.arith;
     StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
-    dim3 grid((b.nitems + 3) / 4);
+    dim3 grid((b.nitems_adj + 3) / 4);
 #define ARITH_LAUNCH(KERNEL, GRID, ...)                                                     \
   do {                                                                                      \
     if (am == FWI_ARITH_MIXED)                                                              \
@@ -310,8 +356,8 @@ struct fwi_plan {
       AdjRefArgs a;
       a.g = grid_args(b);
       a.m = m;
-      a.items = b.d_items;
-      a.nitems = b.nitems;
+      a.items = b.d_items_adj;
+      a.nitems = b.nitems_adj;
       a.dh2C = (dh * dh) * ((dt * dt) / (dh * dh));
       a.dt = dt;
       a.Lc = Lc;
@@ -340,8 +386,8 @@ struct fwi_plan {
       AdjExArgs a;
       a.g = grid_args(b);
       a.m = m;
-      a.items = b.d_items;
-      a.nitems = b.nitems;
+      a.items = b.d_items_adj;
+      a.nitems = b.nitems_adj;
       a.dt2 = dt * dt;
       a.Lc = Lc;
       a.Lo = Lo;
@@ -368,7 +414,16 @@ struct fwi_plan {
         ARITH_LAUNCH(k_adj_ex_strip2, dim3(b.nsblocks), sa, a);
       }
       const int tev = timed_begin(2, st);
-      ARITH_LAUNCH(k_adj_ex_main, grid, a);
+      if (b.grouped) {
+        if (am == FWI_ARITH_MIXED)
+          hipLaunchKernelGGL((k_adj_ex_main<double, true>), grid, dim3(256), 0, st, a);
+        else if (am == FWI_ARITH_FP32D)
+          hipLaunchKernelGGL((k_adj_ex_main<F32D, true>), grid, dim3(256), 0, st, a);
+        else
+          hipLaunchKernelGGL((k_adj_ex_main<float, true>), grid, dim3(256), 0, st, a);
+      } else {
+        ARITH_LAUNCH(k_adj_ex_main, grid, a);
+      }
       timed_end(tev, st);
     }
 #undef ARITH_LAUNCH
@@ -485,55 +540,93 @@ struct fwi_plan {
   }
 
   // ---------------------------------------------------------------------------------------------
-  // Pick the work-item order and rows per item by timing the fused forward and adjoint steps of the
-  // largest batch (a few steps per candidate on the plan's own buffers).
+  // Pick the work-item order and rows per item of the forward kernels, then of the adjoint kernels (with
+  // the imaging terms, so the gradient update is part of what is timed), by timing a few steps of each
+  // candidate on the plan's own buffers and its largest batch.
   int tune(const ModelArgs& m, hipStream_t st) {
     if (tuned) return 0;
     std::vector<int> rows_c = {16, 24, 32, 48, 64, 96};
-    std::vector<int> order_c = {0, 1, 2};
     if (opt.rows_per_item > 0) rows_c = {opt.rows_per_item};
-    if (opt.item_order >= 0) order_c = {opt.item_order};
     Batch& b = batches[0];
+    std::vector<int> order_f = {0, 1, 2};
+    std::vector<int> order_a = b.grouped ? std::vector<int>{0, 2} : std::vector<int>{0, 1, 2};
+    if (opt.item_order >= 0) {
+      order_f = {opt.item_order};
+      order_a = {b.grouped && opt.item_order == 1 ? 0 : opt.item_order};
+    }
     hipEvent_t e0, e1;
     CK(hipEventCreate(&e0));
     CK(hipEventCreate(&e1));
+    auto timed = [&](auto&& body, float& ms) -> int {
+      CK(hipEventRecord(e0, st));
+      if (int r = body()) return r;
+      CK(hipEventRecord(e1, st));
+      CK(hipEventSynchronize(e1));
+      CK(hipEventElapsedTime(&ms, e0, e1));
+      return 0;
+    };
+    // forward
     float best = 1e30f;
-    int best_tz = tz, best_order = order;
-    const bool single = rows_c.size() == 1 && order_c.size() == 1;
-    for (int o : order_c)
+    for (int o : order_f)
       for (int c : rows_c) {
         if (c > nz - 8 && c != rows_c[0]) continue;
         if (int r = build_items(b, c, o)) return r;
-        if (single) {
-          best_tz = c;
-          best_order = o;
+        if (rows_c.size() == 1 && order_f.size() == 1) {
+          tz = c;
+          order = o;
           break;
         }
         if (int r = zero_fwd(b, st)) return r;
-        CK(hipMemsetAsync(adj(0), 0, 12 * (size_t)cap_cell * sizeof(float), st));
         for (int w = 0; w < 2; ++w)
           if (int r = fwd_step(b, m, w, 0, nullptr, nullptr, nullptr, st)) return r;
-        CK(hipEventRecord(e0, st));
-        for (int n = 0; n < 6; ++n) {
-          if (int r = fwd_step(b, m, n, 0, nullptr, nullptr, nullptr, st)) return r;
-          if (int r = adj_step(b, m, nt - n, nullptr, nullptr, st)) return r;
-        }
-        CK(hipEventRecord(e1, st));
-        CK(hipEventSynchronize(e1));
         float ms = 0;
-        CK(hipEventElapsedTime(&ms, e0, e1));
+        if (int r = timed([&]() -> int {
+              for (int n = 2; n < 8; ++n)
+                if (int r2 = fwd_step(b, m, n, 0, nullptr, nullptr, nullptr, st)) return r2;
+              return 0;
+            }, ms))
+          return r;
         if (ms < best) {
           best = ms;
-          best_tz = c;
-          best_order = o;
+          tz = c;
+          order = o;
+        }
+      }
+    // adjoint, imaging into zeroed scratch slots
+    float* qa = qbuf();
+    CK(hipMemsetAsync(qa, 0, 2 * (size_t)cap_cell * sizeof(float), st));
+    best = 1e30f;
+    for (int o : order_a)
+      for (int c : rows_c) {
+        if (c > nz - 8 && c != rows_c[0]) continue;
+        if (int r = build_adj_items(b, c, o)) return r;
+        if (rows_c.size() == 1 && order_a.size() == 1) {
+          tz_adj = c;
+          order_adj = o;
+          break;
+        }
+        CK(hipMemsetAsync(adj(0), 0, kAdjF * (size_t)cap_cell * sizeof(float), st));
+        for (int w = 0; w < 2; ++w)
+          if (int r = adj_step(b, m, nt - w, qa, qa + cap_cell, st)) return r;
+        float ms = 0;
+        if (int r = timed([&]() -> int {
+              for (int n = 2; n < 8; ++n)
+                if (int r2 = adj_step(b, m, nt - n, qa, qa + cap_cell, st)) return r2;
+              return 0;
+            }, ms))
+          return r;
+        if (ms < best) {
+          best = ms;
+          tz_adj = c;
+          order_adj = o;
         }
       }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    tz = best_tz;
-    order = best_order;
-    for (auto& bb : batches)
+    for (auto& bb : batches) {
       if (int r = build_items(bb, tz, order)) return r;
+      if (int r = build_adj_items(bb, tz_adj, order_adj)) return r;
+    }
     tuned = true;
     return 0;
   }
@@ -608,10 +701,15 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     set_message("unknown arith");
     return 1;
   }
+  if (p->opt.grad_group < 0 || p->opt.grad_group > 4) {
+    delete p;
+    set_message("grad_group must be 0..4");
+    return 1;
+  }
   const int nt = p->nt;
   p->K = p->opt.ckpt_interval > 0 ? std::min(p->opt.ckpt_interval, nt) : (int)std::ceil(std::sqrt((double)nt));
   p->J = (nt + p->K - 1) / p->K;
-  p->tz = p->opt.rows_per_item > 0 ? p->opt.rows_per_item : 32;
+  p->tz = p->tz_adj = p->opt.rows_per_item > 0 ? p->opt.rows_per_item : 32;
 
   // per-shot sizes
   const int ns = s->nshots;
@@ -695,6 +793,9 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
       d.off = off;
       d.soff = soff;
       d.goff = goff;
+      d.gslot = off;
+      d.lead = 1;
+      d.pad_ = 0;
       off += (long long)h.nx * p->nz;
       soff += d.nstrip;
       goff += (long long)nt * h.nrec;
@@ -702,6 +803,21 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
       for (int k0 = 0; k0 < d.nstrip; k0 += 256) b.h_sblocks.push_back(make_int2(k, k0));
     }
     b.nsblocks = (int)b.h_sblocks.size();
+    // gradient groups: consecutive shots with identical windows, up to grad_group (exact mode only)
+    const int gmax = p->opt.grad_mode == FWI_GRAD_EXACT ? (p->opt.grad_group == 0 ? 4 : p->opt.grad_group) : 1;
+    for (int k = 0; k < b.count;) {
+      int n = 1;
+      while (n < gmax && k + n < b.count && b.h_shots[k + n].nx == b.h_shots[k].nx &&
+             b.h_shots[k + n].x0 == b.h_shots[k].x0)
+        ++n;
+      b.groups.push_back(make_int2(k, n));
+      for (int j = 0; j < n; ++j) {
+        b.h_shots[k + j].gslot = b.h_shots[k].off;
+        b.h_shots[k + j].lead = j == 0;
+      }
+      if (n > 1) b.grouped = true;
+      k += n;
+    }
     ck(hipMalloc(&b.d_shots, b.h_shots.size() * sizeof(ShotDev)), "hipMalloc(shots)");
     if (rc == 0)
       ck(hipMemcpy(b.d_shots, b.h_shots.data(), b.h_shots.size() * sizeof(ShotDev), hipMemcpyHostToDevice), "copy shots");
@@ -712,6 +828,7 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
            "copy sblocks");
     }
     if (rc == 0) rc = p->build_items(b, p->tz, p->opt.item_order > 0 ? p->opt.item_order : 0);
+    if (rc == 0) rc = p->build_adj_items(b, p->tz, p->opt.item_order == 2 ? 2 : 0);
   }
   if (rc) {
     delete p;
@@ -826,6 +943,12 @@ long long fwi_plan_cells(const fwi_plan* p, long long* strip_cells) {
 int fwi_plan_ckpt_interval(const fwi_plan* p) { return p->K; }
 int fwi_plan_rows_per_item(const fwi_plan* p) { return p->tz; }
 int fwi_plan_item_order(const fwi_plan* p) { return p->order; }
+int fwi_plan_adj_tiling(const fwi_plan* p, int* rows, int* order, int* grouped) {
+  if (rows) *rows = p->tz_adj;
+  if (order) *order = p->order_adj;
+  if (grouped) *grouped = p->batches.empty() ? 0 : (int)p->batches[0].grouped;
+  return 0;
+}
 
 int fwi_gradient_src(fwi_plan* p, const float* vh, const float* vp, const float* vn, const float* adj_src,
                      double* grad, float* syn, void* stream) {

@@ -61,7 +61,7 @@ class Plan:
     def __init__(self, nx: int, nz: int, nt: int, dh: float, dt: float, cpml: int, shots: Sequence[Shot],
                  wavelet: np.ndarray, cpml_ab, grad_mode: int = GRAD_EXACT, arith: int = ARITH_FP32I,
                  ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0,
-                 item_order: int = -1):
+                 item_order: int = -1, grad_group: int = 0):
         L = _lib.load()
         self.nx, self.nz, self.nt, self.dh, self.dt, self.cpml = nx, nz, nt, dh, dt, cpml
         self.shots = list(shots)
@@ -84,6 +84,7 @@ class Plan:
         opt.mem_budget = mem_budget
         opt.rows_per_item = rows_per_item
         opt.item_order = item_order
+        opt.grad_group = grad_group
         h = ctypes.c_void_p()
         _lib.check(L.fwi_plan_create(ctypes.byref(sv), ctypes.byref(opt), ctypes.byref(h)), "fwi_plan_create")
         self._h = h
@@ -122,6 +123,13 @@ class Plan:
     @property
     def item_order(self) -> int:
         return int(_lib.load().fwi_plan_item_order(self._h))
+
+    @property
+    def adj_tiling(self):
+        """(rows per item, item order, grouped) of the adjoint kernels."""
+        r, o, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.load().fwi_plan_adj_tiling(self._h, ctypes.byref(r), ctypes.byref(o), ctypes.byref(g))
+        return r.value, o.value, bool(g.value)
 
     def cells(self):
         """(sum of nx_s*nz over shots, C-PML cells of all windows)."""

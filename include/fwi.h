@@ -62,11 +62,15 @@ typedef struct fwi_options {
   double mem_budget;   /* bytes of device memory the plan may use, 0 = 85 % of free memory */
   int rows_per_item;   /* z rows per wave in the stencil kernels, 0 = chosen by the tuner */
   int item_order;      /* work-item order 0..2 (see plan.hip build_items), -1 = chosen by the tuner */
+  int grad_group;      /* exact mode: up to this many consecutive shots with identical windows share one
+                          FP32 gradient slot, their per-step imaging terms added in shot order before one
+                          read-modify-write (0 = 4, the default; 1 = one slot per shot) */
 } fwi_options;
 
 typedef struct fwi_plan fwi_plan;
 
-/* Fill *opt with defaults (exact gradient, FP32, K = ceil(sqrt(nt)), automatic batch, tile and order). */
+/* Fill *opt with defaults (exact gradient, FP32 increment form, K = ceil(sqrt(nt)), automatic batch, tile and
+ * order, gradient groups of 4). */
 void fwi_options_default(fwi_options* opt);
 
 /* Validate the survey, size the batches, allocate the device workspace. */
@@ -154,10 +158,13 @@ void fwi_plan_set_timing(fwi_plan* plan, int stride);
 int fwi_plan_kernel_time(fwi_plan* plan, int kind, double* total_ms, long long* count);
 /* Cells of all shot windows (nx_s*nz summed over shots) and, in *strip_cells, their C-PML cells. */
 long long fwi_plan_cells(const fwi_plan* plan, long long* strip_cells);
-/* Checkpoint interval K, z rows per work item and work-item order in use (after the first call). */
+/* Checkpoint interval K, z rows per work item and work-item order in use (after the first call): the
+ * forward kernels' and, through fwi_plan_adj_tiling, the adjoint kernels' (grouped = 1 when the first
+ * batch accumulates gradients per group of shots). */
 int fwi_plan_ckpt_interval(const fwi_plan* plan);
 int fwi_plan_rows_per_item(const fwi_plan* plan);
 int fwi_plan_item_order(const fwi_plan* plan);
+int fwi_plan_adj_tiling(const fwi_plan* plan, int* rows_per_item, int* item_order, int* grouped);
 
 /* Host helpers. */
 void fwi_cpml_table(int cpml, float vmax, float dt, float dh, float f, float d0, float L, float* a, float* b);

@@ -916,6 +916,86 @@ double shot_gradient(int mode_exact, const T* vh, const T* vp, const T* vn, cons
   return phi;
 }
 
+// Exact gradient of a group of shots that share one window (the batched path's gradient groups): every
+// shot runs its own forward and exact adjoint (shot_gradient, adjoint_exact_shot); per step and cell the
+// shots' imaging terms c_s are added in shot order, with zeros for absent members of a group of fewer than
+// four, before one update of the shared accumulators: g += ((c_0 + c_1) + c_2) + c_3.  With one shot this
+// is adjoint_exact_shot's update.  geo: (isx, isz, irz, ir0, nrec) per shot; obs/syn: shot-major gathers.
+template <class T, class S>
+void group_gradient(const T* vh, const T* vp, const T* vn, const T* tab_a, const T* tab_b, const T* src, int nx,
+                    int nz, int nt, int cpml, double dh_, double dt_, int ns, const int* geo, const T* obs, T* syn,
+                    T* g_hor, T* g_po, T* g_nmo, double* misfits) {
+  Geo g{nx, nz, cpml};
+  Tables<T> tb(tab_a, tab_b, cpml, nx, nz);
+  Scal sc{dh_, dt_};
+  const long n = (long)nx * nz;
+  const T dh = (T)dh_, dt = (T)dt_, dt2 = dt * dt;
+  std::vector<ShotGeo> sg(ns);
+  std::vector<std::vector<T>> q(ns), res(ns);
+  std::vector<long> goff(ns + 1, 0);
+  for (int s = 0; s < ns; ++s) {
+    sg[s] = ShotGeo{geo[5 * s], geo[5 * s + 1], geo[5 * s + 2], geo[5 * s + 3], geo[5 * s + 4]};
+    goff[s + 1] = goff[s] + (long)nt * sg[s].nrec;
+  }
+  for (int s = 0; s < ns; ++s) {
+    T* sy = syn + goff[s];
+    forward_shot<T, S>(vh, vp, vn, tb, src, g, sg[s], nt, sc, sy, 1, &q[s]);
+    res[s].resize((size_t)nt * sg[s].nrec);
+    double phi = 0;
+    for (size_t i = 0; i < res[s].size(); ++i) {
+      res[s][i] = sy[i] - obs[goff[s] + i];
+      phi += 0.5 * (double)res[s][i] * (double)res[s][i];
+    }
+    misfits[s] = phi;
+  }
+  std::vector<AdjOp<T, S>*> op(ns);
+  std::vector<Fields<T>*> f(ns);
+  struct Ptr { T *muP, *muF, *shP, *shF, *sx, *zx, *sz, *zz, *nP, *nF, *oP, *oF; };
+  std::vector<Ptr> pt(ns);
+  for (int s = 0; s < ns; ++s) {
+    op[s] = new AdjOp<T, S>(g, tb, vh, vp, vn, dh, dt);
+    f[s] = new Fields<T>(n, 12);
+    Fields<T>& F = *f[s];
+    pt[s] = Ptr{F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8], F[9], F[10], F[11]};
+  }
+  for (int k = nt; k >= 1; --k) {
+    for (int s = 0; s < ns; ++s) {
+      Ptr& p = pt[s];
+      op[s]->apply(p.muP, p.muF, p.shP, p.shF, p.sx, p.zx, p.sz, p.zz, p.nP, p.nF, p.oP, p.oF);
+      for (int r = 0; r < sg[s].nrec; ++r) {
+        long i = sg[s].ir0 + r + (long)nx * sg[s].irz;
+        T v = res[s][(size_t)(k - 1) * sg[s].nrec + r];
+        p.nP[i] += v;
+        p.nF[i] += v;
+      }
+      std::swap(p.muP, p.nP);
+      std::swap(p.muF, p.nF);
+      std::swap(p.shP, p.oP);
+      std::swap(p.shF, p.oF);
+    }
+#pragma omp parallel for
+    for (int iz = kRim; iz < nz - kRim; ++iz)
+      for (int ix = kRim; ix < nx - kRim; ++ix) {
+        long i = ix + (long)nx * iz;
+        T ch[4] = {0, 0, 0, 0}, cn[4] = {0, 0, 0, 0}, cp[4] = {0, 0, 0, 0};
+        for (int s = 0; s < ns && s < 4; ++s) {
+          const T qa = q[s][(size_t)(k - 1) * 2 * n + i], qb = q[s][(size_t)(k - 1) * 2 * n + n + i];
+          const T lp = pt[s].muP[i], lf = pt[s].muF[i];
+          ch[s] = dt2 * lp * qa;
+          cn[s] = dt2 * lf * qa;
+          cp[s] = dt2 * (lp + lf) * qb;
+        }
+        g_hor[i] += ((ch[0] + ch[1]) + ch[2]) + ch[3];
+        g_nmo[i] += ((cn[0] + cn[1]) + cn[2]) + cn[3];
+        g_po[i] += ((cp[0] + cp[1]) + cp[2]) + cp[3];
+      }
+  }
+  for (int s = 0; s < ns; ++s) {
+    delete op[s];
+    delete f[s];
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -952,6 +1032,22 @@ ORC_GRAD(f32, float, float)
 ORC_GRAD(f64, double, double)
 ORC_GRAD(f32d, float, F32D)
 ORC_GRAD(f32i, float, F32I)
+
+// Exact gradient of a gradient group (1..4 shots sharing the window), accumulated into g_*; per-shot FP64
+// misfits into misfits[ns].
+#define ORC_GROUP(SUFFIX, T, S)                                                                                    \
+  void orc_group_gradient_##SUFFIX(const T* vh, const T* vp, const T* vn, const T* tab_a, const T* tab_b,           \
+                                   const T* src, int nx, int nz, int nt, int cpml, double dh, double dt, int ns,    \
+                                   const int* geo, const T* obs, T* syn, T* g_hor, T* g_po, T* g_nmo,               \
+                                   double* misfits) {                                                               \
+    group_gradient<T, S>(vh, vp, vn, tab_a, tab_b, src, nx, nz, nt, cpml, dh, dt, ns, geo, obs, syn, g_hor, g_po,   \
+                         g_nmo, misfits);                                                                           \
+  }
+ORC_GROUP(mix, float, double)
+ORC_GROUP(f32, float, float)
+ORC_GROUP(f64, double, double)
+ORC_GROUP(f32d, float, F32D)
+ORC_GROUP(f32i, float, F32I)
 
 // One linear forward state step without source (for the dot-product test), FP64.
 // state = 8 fields of nx*nz: P, P_old, F, F_old, psx, zx, psz, zz  ->  out: P', P, F', F, psx', zx', psz', zz'

@@ -54,12 +54,43 @@ def oracle_gathers(w, mode, sq, tab, src):
     return out
 
 
-def oracle_gradient(w, mode, exact, sq, tab, src, obs_list):
-    """Sum over shots (FP64) of the oracle gradients, placed into model columns; total FP64 misfit."""
+def gradient_groups(shots, group=4, max_batch=0):
+    """The batched path's gradient groups: consecutive shots of a batch with identical windows, up to
+    `group` per group (batches of `max_batch` shots, 0 = one batch)."""
+    out, b0 = [], 0
+    n = len(shots)
+    while b0 < n:
+        b1 = n if max_batch <= 0 else min(n, b0 + max_batch)
+        k = b0
+        while k < b1:
+            m = 1
+            while m < group and k + m < b1 and (shots[k + m].x0, shots[k + m].nx) == (shots[k].x0, shots[k].nx):
+                m += 1
+            out.append(list(range(k, k + m)))
+            k += m
+        b0 = b1
+    return out
+
+
+def oracle_gradient(w, mode, exact, sq, tab, src, obs_list, group=4, max_batch=0):
+    """Sum over shots (FP64) of the oracle gradients, placed into model columns; total FP64 misfit.  In exact
+    mode shots are accumulated in the batched path's gradient groups (group=1: one accumulator per shot)."""
     T = np.float64 if mode == "f64" else np.float32
     G = np.zeros((3, w.nz, w.nx))
     phi = 0.0
     syns = []
+    if exact and group > 1:
+        for members in gradient_groups(w.shots, group, max_batch):
+            s0 = w.shots[members[0]]
+            mis, sy, gh, gp, gn = O.group_gradient(mode, window_model(sq, s0), tab, src.astype(T),
+                                                   [obs_list[i] for i in members], s0.nx, w.nz, w.nt, w.cpml, w.dh,
+                                                   w.dt, [w.shots[i] for i in members])
+            for v in mis:
+                phi += v
+            syns.extend(sy)
+            for k, g in enumerate((gh, gp, gn)):
+                G[k][:, s0.x0:s0.x0 + s0.nx] += g.astype(np.float64)
+        return phi, G, syns
     for s, obs in zip(w.shots, obs_list):
         ph, _, syn, gh, gp, gn = O.gradient(mode, exact, window_model(sq, s), tab, src.astype(T), obs, s.nx, w.nz, w.nt,
                                             w.cpml, w.dh, w.dt, s)

@@ -68,6 +68,9 @@ def _declare(L):
         fn = getattr(L, f"orc_gradient_{suf}")
         fn.argtypes = [_i] + [P] * 7 + [_i, _i, _i, _i, _d, _d, _i, _i, _i, _i, _i, P, P, P, P, _D]
         fn.restype = _d
+        fn = getattr(L, f"orc_group_gradient_{suf}")
+        fn.argtypes = [P] * 6 + [_i, _i, _i, _i, _d, _d, _i, ctypes.c_void_p, P, P, P, P, P, _D]
+        fn.restype = None
     L.orc_step_fwd_f64.argtypes = [_D] * 5 + [_i, _i, _i, _d, _d, _D, _D]
     L.orc_step_adj_f64.argtypes = [_D] * 5 + [_i, _i, _i, _d, _d, _D, _D]
 
@@ -146,6 +149,28 @@ def gradient(mode, exact, model_sq, tab, src, obs, nx, nz, nt, cpml, dh, dt, sho
                                                  shot.irz, shot.ir0, shot.nrec, ptr(syn), ptr(gh), ptr(gp), ptr(gn),
                                                  ptr(m32))
     return phi, float(m32[0]), syn, gh, gp, gn
+
+
+def group_gradient(mode, model_sq, tab, src, obs_list, nx, nz, nt, cpml, dh, dt, shots):
+    """Exact gradient of a gradient group (1..4 shots sharing one window, imaging terms added in shot order
+    per step).  Returns (per-shot FP64 misfits, syn list [nt,nrec], g_hor, g_po, g_nmo)."""
+    T = _dtype_of(mode)
+    vh, vp, vn = (np.ascontiguousarray(x, T) for x in model_sq)
+    ta, tb = (np.ascontiguousarray(x, T) for x in tab)
+    s = np.ascontiguousarray(src, T)
+    geo = np.array([[sh.isx, sh.isz, sh.irz, sh.ir0, sh.nrec] for sh in shots], np.int32).reshape(-1)
+    obs = np.ascontiguousarray(np.concatenate([np.asarray(o, T).reshape(-1) for o in obs_list]), T)
+    syn = np.zeros_like(obs)
+    gh, gp, gn = (np.zeros((nz, nx), T) for _ in range(3))
+    mis = np.zeros(len(shots))
+    getattr(lib(), f"orc_group_gradient_{mode}")(ptr(vh), ptr(vp), ptr(vn), ptr(ta), ptr(tb), ptr(s), nx, nz, nt, cpml,
+                                                 dh, dt, len(shots), geo.ctypes.data, ptr(obs), ptr(syn), ptr(gh),
+                                                 ptr(gp), ptr(gn), ptr(mis))
+    syns, o = [], 0
+    for sh in shots:
+        syns.append(syn[o:o + nt * sh.nrec].reshape(nt, sh.nrec))
+        o += nt * sh.nrec
+    return mis, syns, gh, gp, gn
 
 
 def chain_rule(vp, eps, dl, g_hor, g_po, g_nmo):

@@ -79,31 +79,67 @@ def test_gradient_matches_oracle(torch, exact, arith, mode):
 
 def test_checkpoint_interval_and_batching_invariance(torch):
     """Recomputation from checkpoints replays the same kernels: every interval K (including storing all
-    steps) and every batch split gives bit-identical gradients and misfits (per-shot reductions added in
-    shot order), in the two-level and the increment form."""
+    steps) gives bit-identical gradients, in the two-level and the increment form; batch splits and group
+    sizes that form the same gradient groups give bit-identical gradients, and the misfit (per-shot
+    reductions added in shot order) is identical for every split."""
+    from helpers import gradient_groups
+
     w = small_workload(nshots=3, nt=200)
-    say("checkpoint interval / batch split invariance small nt=200")
+    say("checkpoint interval / batch split / gradient group invariance small nt=200")
     tab, src, sq_true, sq_init = oracle_setup(w, "f32")
     obs = torch.from_numpy(_cat(oracle_gathers(w, "f32", sq_true, tab, src))).cuda()
     model = _dev_model(torch, sq_init)
-    ref = None
-    for K, mb in ((0, 0), (1, 0), (7, 0), (200, 0), (13, 1), (0, 2)):
+    ref, phis = {}, {}
+    cases = [(0, 0, 0), (1, 0, 0), (7, 0, 0), (200, 0, 0), (13, 1, 0), (0, 2, 0), (0, 0, 1), (7, 0, 2), (0, 2, 1)]
+    for K, mb, grp in cases:
         for exact in (1, 0):
-          for arith in (3, 0):
-            plan = _plan(w, tab, src, ckpt_interval=K, max_batch=mb, grad_mode=exact, arith=arith)
-            phi, G, _ = plan.gradient(model, obs)
-            torch.cuda.synchronize()
-            G = G.cpu().numpy()
-            sm = plan.shot_misfits()
-            assert phi == sum(sm.tolist())  # per-shot misfits added in shot order
-            key = (exact, arith)
-            if ref is None:
-                ref = {}
-            if key not in ref:
-                ref[key] = (phi, G)
-            else:
-                assert phi == ref[key][0], (K, mb, exact, arith)
-                assert np.array_equal(G, ref[key][1]), (K, mb, exact, arith)
+            for arith in (3, 0):
+                plan = _plan(w, tab, src, ckpt_interval=K, max_batch=mb, grad_mode=exact, arith=arith,
+                             grad_group=grp)
+                phi, G, _ = plan.gradient(model, obs)
+                torch.cuda.synchronize()
+                G = G.cpu().numpy()
+                sm = plan.shot_misfits()
+                assert phi == sum(sm.tolist())  # per-shot misfits added in shot order
+                groups = tuple(map(tuple, gradient_groups(w.shots, (grp or 4) if exact else 1, mb)))
+                key = (exact, arith, groups)
+                phis.setdefault((exact, arith), phi)
+                assert phi == phis[(exact, arith)], (K, mb, grp, exact, arith)
+                if key not in ref:
+                    ref[key] = G
+                else:
+                    assert np.array_equal(G, ref[key]), (K, mb, grp, exact, arith)
+    # the exact mode met each of the three group structures at least twice
+    assert sum(1 for k in ref if k[0] == 1 and k[1] == 3) == 3
+
+
+@pytest.mark.parametrize("layout", ["six_full", "mixed_windows", "per_shot"])
+def test_gradient_groups_match_oracle(torch, layout):
+    """Exact gradient with gradient groups: six full-width shots form groups of 4 and 2 (two placeholder
+    waves); with one narrower window in the middle the groups are 3, 1, 2 in one grouped launch; grad_group=1
+    keeps one accumulator per shot.  Each against the oracle with the same groups (bit-level bar)."""
+    from helpers import gradient_groups
+
+    w = small_workload(nshots=6, nt=160)
+    grp = 1 if layout == "per_shot" else 4
+    if layout == "mixed_windows":
+        s = w.shots[3]
+        w.shots[3] = M.Shot(x0=8, nx=w.nx - 8, isx=s.isx - 8, isz=s.isz, irz=s.irz, ir0=12, nrec=w.nx - 8 - 24)
+    say(f"gradient groups {layout}: {gradient_groups(w.shots, grp)}")
+    tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+    obs = oracle_gathers(w, "f32i", sq_true, tab, src)
+    phi_ref, G_ref, _ = oracle_gradient(w, "f32i", True, sq_init, tab, src, obs, group=grp)
+    plan = _plan(w, tab, src, grad_group=grp)
+    phi, G, _ = plan.gradient(_dev_model(torch, sq_init), torch.from_numpy(_cat(obs)).cuda())
+    torch.cuda.synchronize()
+    print("adjoint tiling", plan.adj_tiling)
+    assert plan.adj_tiling[2] == (layout != "per_shot")
+    G = G.cpu().numpy()
+    assert abs(phi - phi_ref) <= 1e-9 * phi_ref
+    for k in range(3):
+        e = rel_l2(G[k], G_ref[k])
+        print(f"groups {layout} grad[{k}] rel-L2 {e:.3e}")
+        assert e <= TOL
 
 
 def test_ragged_windows_match_oracle(torch):

@@ -361,3 +361,33 @@ def test_f32i_increment_form_tracks_fp64():
     assert max(err["f32i"][1]) < 1e-4
     for k in range(3):
         assert err["f32i"][1][k] * 5 < err["f32"][1][k]
+
+
+def test_group_gradient_equals_per_shot_sum():
+    """Gradient groups (shots sharing a window whose per-step imaging terms are added before one update)
+    compute the same sum as one accumulator per shot: equal to 1e-14 in FP64 and to FP32 rounding in f32i;
+    a group of one is the per-shot function bit for bit."""
+    from helpers import oracle_gathers, oracle_gradient, oracle_setup, rel_l2, small_workload
+
+    w = small_workload(nshots=3, nt=120)
+    for mode, tol in (("f64", 1e-14), ("f32i", 1e-6)):
+        tab, src, st, si = oracle_setup(w, mode)
+        obs = oracle_gathers(w, mode, st, tab, src)
+        p1, G1, s1 = oracle_gradient(w, mode, True, si, tab, src, obs, group=1)
+        p4, G4, s4 = oracle_gradient(w, mode, True, si, tab, src, obs, group=4)
+        assert p1 == p4
+        for a, b in zip(s1, s4):
+            assert np.array_equal(a, b)
+        for k in range(3):
+            assert rel_l2(G4[k], G1[k]) < tol
+        # a planted mistake: one member's terms counted twice moves the result by O(1/3)
+        G_bad = G4 + G1 / 3.0
+        assert rel_l2(G_bad[0], G1[0]) > 0.1
+    tab, src, st, si = oracle_setup(w, "f32i")
+    obs = oracle_gathers(w, "f32i", st, tab, src)
+    s0 = w.shots[0]
+    mis, _, gh, gp, gn = O.group_gradient("f32i", tuple(np.ascontiguousarray(x[:, s0.x0:s0.x0 + s0.nx]) for x in si),
+                                          tab, src, [obs[0]], s0.nx, w.nz, w.nt, w.cpml, w.dh, w.dt, [s0])
+    ph, _, _, h1, p1_, n1 = O.gradient("f32i", True, tuple(np.ascontiguousarray(x[:, s0.x0:s0.x0 + s0.nx]) for x in si),
+                                       tab, src, obs[0], s0.nx, w.nz, w.nt, w.cpml, w.dh, w.dt, s0)
+    assert mis[0] == ph and np.array_equal(gh, h1) and np.array_equal(gp, p1_) and np.array_equal(gn, n1)
