@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--nt", type=int, default=0)
     ap.add_argument("--mode", choices=["exact", "refcompat"], default="exact")
     ap.add_argument("--ckpt", type=int, default=0)
+    ap.add_argument("--ckpt-format", type=int, default=0, help="checkpoints: 0 FP32, 1 BF16, 2 FP16 (scaled)")
     ap.add_argument("--arith", type=int, default=3,
                     help="0 FP32 two-level, 1 mixed (reference), 2 FP32 difference stencil, 3 FP32 increment form")
     ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")
@@ -177,7 +178,8 @@ def main():
     init_sq = pre_variables(*(dev(x) for x in w.init_model))
     grad_mode = 1 if args.mode == "exact" else 0
     plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
-                rows_per_item=args.rows, item_order=args.order, arith=args.arith)
+                rows_per_item=args.rows, item_order=args.order, arith=args.arith,
+                ckpt_format=args.ckpt_format)
     log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, workspace {plan.workspace_bytes / 1e9:.1f} GB")
     t0 = time.perf_counter()
     obs = plan.forward(true_sq)
@@ -266,6 +268,8 @@ def main():
             "grad_mode": args.mode,
             "arith": {0: "fp32", 1: "mixed", 2: "fp32-diff", 3: "fp32-increment"}[args.arith],
             "ckpt_interval": plan.ckpt_interval,
+            "ckpt_format": {0: "fp32", 1: "bf16-scaled", 2: "fp16-scaled"}[args.ckpt_format],
+            "workspace_gb": round(plan.workspace_bytes / 1e9, 2),
             "batch_shots": plan.batch_size,
             "rows_per_item": plan.rows_per_item,
             "item_order": plan.item_order,

@@ -615,6 +615,147 @@ __global__ void __launch_bounds__(256) k_strip_copy(StripArgs a, int dir, float*
 }
 
 
+// ---------------------------------------------------------------------------------------------
+// 16-bit checkpoints with per-snapshot scaling.  Each of a shot's eight checkpoint fields (four cell
+// fields, four C-PML memory variables on the strip cells) is stored in 16 bits after an exact
+// power-of-two scaling by its max-abs over that shot at that step (2^e <= max|x| < 2^(e+1); e = 0 for an
+// all-zero field), rounded to nearest even:
+//   FMT 1 (BF16): bf16(x * 2^-e), so a restored value is x (1 + d), |d| <= 2^-9;
+//   FMT 2 (FP16): fp16(x * 2^(14-e)): the field's maximum lands in [2^14, 2^15), |d| <= 2^-11 down to
+//   2^-29 of the maximum (FP16 subnormals below that).
+// ---------------------------------------------------------------------------------------------
+
+__device__ inline unsigned short bf16_rne(float v) {
+  unsigned u = __float_as_uint(v);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf16_to_float(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+// scale exponent of a field from the bits of its max-abs
+__device__ inline int ck_exponent(unsigned amax_bits) {
+  const int be = (int)((amax_bits >> 23) & 0xFFu);
+  return amax_bits == 0u ? 0 : (be == 0 ? -126 : be - 127);
+}
+template <int FMT>
+__device__ inline unsigned short ck_pack(float v, int e) {
+  if (FMT == 1) return bf16_rne(ldexpf(v, -e));
+  const _Float16 h = (_Float16)ldexpf(v, 14 - e);
+  return __builtin_bit_cast(unsigned short, h);
+}
+template <int FMT>
+__device__ inline float ck_unpack(unsigned short h, int e) {
+  if (FMT == 1) return ldexpf(bf16_to_float(h), e);
+  return ldexpf((float)__builtin_bit_cast(_Float16, h), e - 14);
+}
+
+__device__ inline void wave_max_atomic(float m, unsigned* dst) {
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(dst, __float_as_uint(m));  // non-negative floats order as their bits
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(unsigned* __restrict__ p, unsigned v, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// max|x| of four cell fields per shot: amax[shot * 8 + k], k = 0..3.  Grid (blocks per shot, shots).
+__global__ void __launch_bounds__(256) k_ck_absmax_cells(const ShotDev* __restrict__ shots, int nz, const float* f0,
+                                                         const float* f1, const float* f2, const float* f3,
+                                                         unsigned* __restrict__ amax) {
+  const ShotDev sd = shots[blockIdx.y];
+  const long long n = (long long)sd.nx * nz;
+  float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const long long c = sd.off + i;
+    m0 = fmaxf(m0, fabsf(f0[c]));
+    m1 = fmaxf(m1, fabsf(f1[c]));
+    m2 = fmaxf(m2, fabsf(f2[c]));
+    m3 = fmaxf(m3, fabsf(f3[c]));
+  }
+  unsigned* a = amax + blockIdx.y * 8;
+  wave_max_atomic(m0, a + 0);
+  wave_max_atomic(m1, a + 1);
+  wave_max_atomic(m2, a + 2);
+  wave_max_atomic(m3, a + 3);
+}
+
+// max|x| of four C-PML fields over each shot's strip cells: amax[shot * 8 + 4 + k].
+__global__ void __launch_bounds__(256) k_ck_absmax_strips(StripArgs a, const float* f0, const float* f1,
+                                                          const float* f2, const float* f3, unsigned* __restrict__ amax) {
+  const int2 blk = a.blocks[blockIdx.x];
+  const ShotDev sd = a.g.shots[blk.x];
+  const int k = blk.y + threadIdx.x;
+  float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+  if (k < sd.nstrip) {
+    int ix, iz;
+    strip_cell(k, sd.nx, a.g.nz, a.g.cpml, ix, iz);
+    const long long i = sd.off + ix + (long long)sd.nx * iz;
+    m0 = fabsf(f0[i]);
+    m1 = fabsf(f1[i]);
+    m2 = fabsf(f2[i]);
+    m3 = fabsf(f3[i]);
+  }
+  unsigned* d = amax + blk.x * 8 + 4;
+  wave_max_atomic(m0, d + 0);
+  wave_max_atomic(m1, d + 1);
+  wave_max_atomic(m2, d + 2);
+  wave_max_atomic(m3, d + 3);
+}
+
+// Pack (dir 0) or unpack (dir 1) four cell fields: packed[k * ncell_total + off + i].
+template <int FMT>
+__global__ void __launch_bounds__(256) k_ck_pack_cells(const ShotDev* __restrict__ shots, int nz, int dir, float* f0,
+                                                       float* f1, float* f2, float* f3,
+                                                       unsigned short* __restrict__ packed, long long ncell_total,
+                                                       const unsigned* __restrict__ amax) {
+  const ShotDev sd = shots[blockIdx.y];
+  const long long n = (long long)sd.nx * nz;
+  const unsigned* a = amax + blockIdx.y * 8;
+  const int e0 = ck_exponent(a[0]), e1 = ck_exponent(a[1]), e2 = ck_exponent(a[2]), e3 = ck_exponent(a[3]);
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const long long c = sd.off + i;
+    if (dir == 0) {
+      packed[c] = ck_pack<FMT>(f0[c], e0);
+      packed[c + ncell_total] = ck_pack<FMT>(f1[c], e1);
+      packed[c + 2 * ncell_total] = ck_pack<FMT>(f2[c], e2);
+      packed[c + 3 * ncell_total] = ck_pack<FMT>(f3[c], e3);
+    } else {
+      f0[c] = ck_unpack<FMT>(packed[c], e0);
+      f1[c] = ck_unpack<FMT>(packed[c + ncell_total], e1);
+      f2[c] = ck_unpack<FMT>(packed[c + 2 * ncell_total], e2);
+      f3[c] = ck_unpack<FMT>(packed[c + 3 * ncell_total], e3);
+    }
+  }
+}
+
+// Pack / unpack four C-PML fields at the strip cells: packed[k * nstrip_total + soff + cell].
+template <int FMT>
+__global__ void __launch_bounds__(256) k_ck_pack_strips(StripArgs a, int dir, float* f0, float* f1, float* f2, float* f3,
+                                                        unsigned short* __restrict__ packed, long long nstrip_total,
+                                                        const unsigned* __restrict__ amax) {
+  const int2 blk = a.blocks[blockIdx.x];
+  const ShotDev sd = a.g.shots[blk.x];
+  const int k = blk.y + threadIdx.x;
+  if (k >= sd.nstrip) return;
+  const unsigned* am = amax + blk.x * 8 + 4;
+  const int e0 = ck_exponent(am[0]), e1 = ck_exponent(am[1]), e2 = ck_exponent(am[2]), e3 = ck_exponent(am[3]);
+  int ix, iz;
+  strip_cell(k, sd.nx, a.g.nz, a.g.cpml, ix, iz);
+  const long long i = sd.off + ix + (long long)sd.nx * iz;
+  const long long j = sd.soff + k;
+  if (dir == 0) {
+    packed[j] = ck_pack<FMT>(f0[i], e0);
+    packed[j + nstrip_total] = ck_pack<FMT>(f1[i], e1);
+    packed[j + 2 * nstrip_total] = ck_pack<FMT>(f2[i], e2);
+    packed[j + 3 * nstrip_total] = ck_pack<FMT>(f3[i], e3);
+  } else {
+    f0[i] = ck_unpack<FMT>(packed[j], e0);
+    f1[i] = ck_unpack<FMT>(packed[j + nstrip_total], e1);
+    f2[i] = ck_unpack<FMT>(packed[j + 2 * nstrip_total], e2);
+    f3[i] = ck_unpack<FMT>(packed[j + 3 * nstrip_total], e3);
+  }
+}
+
 // Residual res = syn - obs of every shot of a batch and per-(shot, block) FP64 partial sums of res^2
 // (parallelcalls.cu:133-142, main.c:448-454).  Block (b, s) covers a fixed slice of shot s's gather, so
 // the misfit of a shot does not depend on which other shots share its batch.

@@ -30,6 +30,7 @@ using namespace fwi;
 namespace {
 
 constexpr int kResidualBlocks = 64;  // per shot
+constexpr int kCkBlocks = 64;        // per shot, checkpoint packing
 
 struct Batch {
   int s0 = 0, count = 0;
@@ -80,6 +81,9 @@ struct fwi_plan {
   double* d_partial = nullptr;   // kResidualBlocks per shot of the largest batch
   double* d_misfit = nullptr;
   double* d_shot_misfit = nullptr;  // per shot of the survey, from the last misfit or gradient
+  size_t ck_esize = 4;              // bytes per checkpoint value (4 FP32, 2 BF16 / FP16)
+  int max_count = 0;                // shots in the largest batch
+  unsigned* d_ck_amax = nullptr;    // 16-bit checkpoints: max-abs bits per (checkpoint, shot, field)
   // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute),
   // 2 = adjoint step; every `timing_stride`-th launch of a kind is bracketed by events on its stream
   int timing_stride = 0;
@@ -125,12 +129,15 @@ struct fwi_plan {
   float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..11
   float* sgrad(int k) const { return d_work + (size_t)(kFwdF + kAdjF + k) * cap_cell; }      // 0..2
   float* qbuf() const { return d_work + (size_t)kFixedF * cap_cell; }                      // K * 2 * cap_cell
-  float* ckpt(int j) const {                                                               // j = 1..J-1
-    return d_work + (size_t)(kFixedF + 2 * (size_t)K) * cap_cell + (size_t)(j - 1) * (4 * cap_cell + 4 * cap_strip);
+  // checkpoints j = 1..J-1: 4 cell fields then 4 strip fields, FP32 or 16-bit (ck_esize bytes per value)
+  size_t ck_bytes() const { return (4 * (size_t)cap_cell + 4 * (size_t)cap_strip) * ck_esize; }
+  char* ckpt(int j) const {
+    return (char*)(d_work + (size_t)(kFixedF + 2 * (size_t)K) * cap_cell) + (size_t)(j - 1) * ck_bytes();
   }
   float* gathers() const {
-    return d_work + (size_t)(kFixedF + 2 * (size_t)K) * cap_cell + (size_t)(J - 1) * (4 * cap_cell + 4 * cap_strip);
+    return (float*)((char*)(d_work + (size_t)(kFixedF + 2 * (size_t)K) * cap_cell) + (size_t)(J - 1) * ck_bytes());
   }
+  float* resid() const { return gathers() + cap_gather; }
   bool inc() const { return opt.arith == FWI_ARITH_FP32I; }
   // the four cell fields a checkpoint holds before step n: both levels of P and F, or in the increment
   // form the current P, F and their increments Q, R
@@ -140,7 +147,6 @@ struct fwi_plan {
     const int map[4] = {cur, 2 + cur, 8, 9};
     return map[f];
   }
-  float* resid() const { return gathers() + cap_gather; }
   std::string err;
 
   int fail(hipError_t e, const char* what) {
@@ -168,6 +174,7 @@ struct fwi_plan {
     (void)hipFree(d_partial);
     (void)hipFree(d_misfit);
     (void)hipFree(d_shot_misfit);
+    (void)hipFree(d_ck_amax);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
   }
 
@@ -447,18 +454,72 @@ struct fwi_plan {
   }
 
   int save_ckpt(const Batch& b, int j, hipStream_t st) {
-    float* c = ckpt(j);
+    char* c = ckpt(j);
+    const int n = j * K;
+    if (opt.ckpt_format != FWI_CKPT_FP32) {
+      const bool h = opt.ckpt_format == FWI_CKPT_FP16;
+      unsigned* am = d_ck_amax + (size_t)j * max_count * 8;
+      hipLaunchKernelGGL(k_fill_u32, dim3((b.count * 8 + 255) / 256), dim3(256), 0, st, am, 0u, b.count * 8);
+      hipLaunchKernelGGL(k_ck_absmax_cells, dim3(kCkBlocks, b.count), dim3(256), 0, st, b.d_shots, nz,
+                         fwd(ckpt_field(0, n)), fwd(ckpt_field(1, n)), fwd(ckpt_field(2, n)), fwd(ckpt_field(3, n)), am);
+      if (h)
+        hipLaunchKernelGGL(k_ck_pack_cells<2>, dim3(kCkBlocks, b.count), dim3(256), 0, st,
+                         b.d_shots, nz, 0, fwd(ckpt_field(0, n)), fwd(ckpt_field(1, n)), fwd(ckpt_field(2, n)),
+                         fwd(ckpt_field(3, n)), (unsigned short*)c, b.ncell, am);
+      else
+        hipLaunchKernelGGL(k_ck_pack_cells<1>, dim3(kCkBlocks, b.count), dim3(256), 0, st,
+                         b.d_shots, nz, 0, fwd(ckpt_field(0, n)), fwd(ckpt_field(1, n)), fwd(ckpt_field(2, n)),
+                         fwd(ckpt_field(3, n)), (unsigned short*)c, b.ncell, am);
+      if (b.nsblocks > 0) {
+        StripArgs sa{grid_args(b), ModelArgs{nullptr, nullptr, nullptr, nx}, b.d_sblocks, b.nsblocks};
+        hipLaunchKernelGGL(k_ck_absmax_strips, dim3(b.nsblocks), dim3(256), 0, st, sa, fwd(4), fwd(5), fwd(6), fwd(7), am);
+        if (h)
+          hipLaunchKernelGGL(k_ck_pack_strips<2>, dim3(b.nsblocks), dim3(256), 0, st, sa, 0, fwd(4), fwd(5), fwd(6),
+                             fwd(7), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
+        else
+          hipLaunchKernelGGL(k_ck_pack_strips<1>, dim3(b.nsblocks), dim3(256), 0, st, sa, 0, fwd(4), fwd(5), fwd(6),
+                             fwd(7), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
+      }
+      CK(hipGetLastError());
+      return 0;
+    }
+    float* cf = (float*)c;
     for (int f = 0; f < 4; ++f)
-      CK(hipMemcpyAsync(c + (size_t)f * b.ncell, fwd(ckpt_field(f, j * K)), b.ncell * sizeof(float),
+      CK(hipMemcpyAsync(cf + (size_t)f * b.ncell, fwd(ckpt_field(f, n)), b.ncell * sizeof(float),
                         hipMemcpyDeviceToDevice, st));
-    return strip_copy(b, 0, c + 4 * (size_t)b.ncell, st);
+    return strip_copy(b, 0, cf + 4 * (size_t)b.ncell, st);
   }
   int load_ckpt(const Batch& b, int j, hipStream_t st) {
-    float* c = ckpt(j);
+    char* c = ckpt(j);
+    const int n = j * K;
+    if (opt.ckpt_format != FWI_CKPT_FP32) {
+      const bool h = opt.ckpt_format == FWI_CKPT_FP16;
+      const unsigned* am = d_ck_amax + (size_t)j * max_count * 8;
+      if (h)
+        hipLaunchKernelGGL(k_ck_pack_cells<2>, dim3(kCkBlocks, b.count), dim3(256), 0, st,
+                         b.d_shots, nz, 1, fwd(ckpt_field(0, n)), fwd(ckpt_field(1, n)), fwd(ckpt_field(2, n)),
+                         fwd(ckpt_field(3, n)), (unsigned short*)c, b.ncell, am);
+      else
+        hipLaunchKernelGGL(k_ck_pack_cells<1>, dim3(kCkBlocks, b.count), dim3(256), 0, st,
+                         b.d_shots, nz, 1, fwd(ckpt_field(0, n)), fwd(ckpt_field(1, n)), fwd(ckpt_field(2, n)),
+                         fwd(ckpt_field(3, n)), (unsigned short*)c, b.ncell, am);
+      if (b.nsblocks > 0) {
+        StripArgs sa{grid_args(b), ModelArgs{nullptr, nullptr, nullptr, nx}, b.d_sblocks, b.nsblocks};
+        if (h)
+          hipLaunchKernelGGL(k_ck_pack_strips<2>, dim3(b.nsblocks), dim3(256), 0, st, sa, 1, fwd(4), fwd(5), fwd(6),
+                             fwd(7), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
+        else
+          hipLaunchKernelGGL(k_ck_pack_strips<1>, dim3(b.nsblocks), dim3(256), 0, st, sa, 1, fwd(4), fwd(5), fwd(6),
+                             fwd(7), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
+      }
+      CK(hipGetLastError());
+      return 0;
+    }
+    float* cf = (float*)c;
     for (int f = 0; f < 4; ++f)
-      CK(hipMemcpyAsync(fwd(ckpt_field(f, j * K)), c + (size_t)f * b.ncell, b.ncell * sizeof(float),
+      CK(hipMemcpyAsync(fwd(ckpt_field(f, n)), cf + (size_t)f * b.ncell, b.ncell * sizeof(float),
                         hipMemcpyDeviceToDevice, st));
-    return strip_copy(b, 1, c + 4 * (size_t)b.ncell, st);
+    return strip_copy(b, 1, cf + 4 * (size_t)b.ncell, st);
   }
 
   int residual(const Batch& b, const float* syn, const float* obs, float* res, double* out, int accumulate,
@@ -701,6 +762,19 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     set_message("unknown arith");
     return 1;
   }
+  if (p->opt.ckpt_format != FWI_CKPT_FP32 && p->opt.ckpt_format != FWI_CKPT_BF16 &&
+      p->opt.ckpt_format != FWI_CKPT_FP16) {
+    delete p;
+    set_message("unknown ckpt_format");
+    return 1;
+  }
+  if (p->opt.ckpt_format != FWI_CKPT_FP32 && p->opt.arith != FWI_ARITH_FP32I) {
+    delete p;
+    set_message("16-bit checkpoints need the increment form (FWI_ARITH_FP32I): two time levels stored in 16 bits "
+                "lose their difference");
+    return 1;
+  }
+  p->ck_esize = p->opt.ckpt_format == FWI_CKPT_FP32 ? 4 : 2;
   if (p->opt.grad_group < 0 || p->opt.grad_group > 4) {
     delete p;
     set_message("grad_group must be 0..4");
@@ -719,9 +793,10 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     strips[i] = strip_count(p->shots[i].nx, p->nz, p->cpml);
     gath[i] = (long long)nt * p->shots[i].nrec;
   }
-  // per-cell floats: 10 fwd + 12 adj + 3 grad + 2K imaging terms + 4 per checkpoint
-  const double per_cell = (double)fwi_plan::kFixedF + 2.0 * p->K + 4.0 * (p->J - 1);
-  const double per_strip = 4.0 * (p->J - 1);
+  // per-cell floats: 10 fwd + 12 adj + 3 grad + 2K imaging terms + 4 per checkpoint (2 in 16 bits)
+  const double ckf = (double)p->ck_esize / 4.0;  // checkpoint values per float
+  const double per_cell = (double)fwi_plan::kFixedF + 2.0 * p->K + 4.0 * ckf * (p->J - 1);
+  const double per_strip = 4.0 * ckf * (p->J - 1);
   size_t freeb = 0, totalb = 0;
   if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
     delete p;
@@ -756,7 +831,8 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     p->cap_strip = std::max(p->cap_strip, b.nstrip);
     p->cap_gather = std::max(p->cap_gather, b.ngather);
   }
-  const size_t nfloat = (size_t)(per_cell * p->cap_cell) + (size_t)(per_strip * p->cap_strip) + 2 * p->cap_gather;
+  const size_t nfloat = (size_t)(fwi_plan::kFixedF + 2 * (size_t)p->K) * p->cap_cell +
+                        (p->ck_bytes() * (size_t)(p->J - 1) + 3) / 4 + 2 * p->cap_gather;
   p->work_bytes = nfloat * sizeof(float);
   int rc = 0;
   auto ck = [&](hipError_t e, const char* w) {
@@ -766,9 +842,11 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
   ck(hipMalloc(&p->d_src, nt * sizeof(float)), "hipMalloc(src)");
   ck(hipMalloc(&p->d_ta, p->cpml * sizeof(float)), "hipMalloc(ta)");
   ck(hipMalloc(&p->d_tb, p->cpml * sizeof(float)), "hipMalloc(tb)");
-  int max_count = 0;
-  for (auto& b : p->batches) max_count = std::max(max_count, b.count);
+  for (auto& b : p->batches) p->max_count = std::max(p->max_count, b.count);
+  const int max_count = p->max_count;
   ck(hipMalloc(&p->d_partial, (size_t)kResidualBlocks * max_count * sizeof(double)), "hipMalloc(partial)");
+  if (p->opt.ckpt_format != FWI_CKPT_FP32)
+    ck(hipMalloc(&p->d_ck_amax, (size_t)p->J * max_count * 8 * sizeof(unsigned)), "hipMalloc(ckpt scales)");
   ck(hipMalloc(&p->d_misfit, sizeof(double)), "hipMalloc(misfit)");
   ck(hipMalloc(&p->d_shot_misfit, (size_t)ns * sizeof(double)), "hipMalloc(shot misfit)");
   if (rc == 0) ck(hipMemcpy(p->d_src, p->wavelet.data(), nt * sizeof(float), hipMemcpyHostToDevice), "copy src");

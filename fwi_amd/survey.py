@@ -21,6 +21,9 @@ ARITH_FP32 = 0
 ARITH_MIXED = 1
 ARITH_FP32D = 2
 ARITH_FP32I = 3  # default: FP32 with the forward leapfrog in increment form
+CKPT_FP32 = 0
+CKPT_BF16 = 1  # 16-bit checkpoints, power-of-two scale per field, shot and snapshot
+CKPT_FP16 = 2
 
 
 def cpml_table(p: CpmlParams):
@@ -61,7 +64,7 @@ class Plan:
     def __init__(self, nx: int, nz: int, nt: int, dh: float, dt: float, cpml: int, shots: Sequence[Shot],
                  wavelet: np.ndarray, cpml_ab, grad_mode: int = GRAD_EXACT, arith: int = ARITH_FP32I,
                  ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0,
-                 item_order: int = -1, grad_group: int = 0):
+                 item_order: int = -1, grad_group: int = 0, ckpt_format: int = 0):
         L = _lib.load()
         self.nx, self.nz, self.nt, self.dh, self.dt, self.cpml = nx, nz, nt, dh, dt, cpml
         self.shots = list(shots)
@@ -85,6 +88,7 @@ class Plan:
         opt.rows_per_item = rows_per_item
         opt.item_order = item_order
         opt.grad_group = grad_group
+        opt.ckpt_format = ckpt_format
         h = ctypes.c_void_p()
         _lib.check(L.fwi_plan_create(ctypes.byref(sv), ctypes.byref(opt), ctypes.byref(h)), "fwi_plan_create")
         self._h = h

@@ -54,6 +54,12 @@ enum { FWI_GRAD_REFCOMPAT = 0, FWI_GRAD_EXACT = 1 };
  * 20-50x for 8 more bytes per cell on the forward and recomputation steps. */
 enum { FWI_ARITH_FP32 = 0, FWI_ARITH_MIXED = 1, FWI_ARITH_FP32D = 2, FWI_ARITH_FP32I = 3 };
 
+/* Checkpoint storage: FP32, or 16 bits after an exact power-of-two scale per field, shot and snapshot (half
+ * the checkpoint memory; needs FWI_ARITH_FP32I): BF16 (8-bit mantissa) or FP16 with the field maximum
+ * scaled to [2^14, 2^15) (11-bit mantissa).  Recomputed segments then start from rounded states: the
+ * gradient moves by ~1e-2 (BF16) or ~1e-3 (FP16) relative. */
+enum { FWI_CKPT_FP32 = 0, FWI_CKPT_BF16 = 1, FWI_CKPT_FP16 = 2 };
+
 typedef struct fwi_options {
   int grad_mode;       /* FWI_GRAD_EXACT (true discrete gradient) or FWI_GRAD_REFCOMPAT (the reference's) */
   int arith;           /* FWI_ARITH_FP32I (default), FWI_ARITH_FP32, FWI_ARITH_FP32D or FWI_ARITH_MIXED */
@@ -65,6 +71,7 @@ typedef struct fwi_options {
   int grad_group;      /* exact mode: up to this many consecutive shots with identical windows share one
                           FP32 gradient slot, their per-step imaging terms added in shot order before one
                           read-modify-write (0 = 4, the default; 1 = one slot per shot) */
+  int ckpt_format;     /* FWI_CKPT_FP32 (default), FWI_CKPT_BF16 or FWI_CKPT_FP16 */
 } fwi_options;
 
 typedef struct fwi_plan fwi_plan;

@@ -916,6 +916,95 @@ double shot_gradient(int mode_exact, const T* vh, const T* vp, const T* vn, cons
   return phi;
 }
 
+// BF16 checkpoints (the batched path's FWI_CKPT_BF16): each checkpoint field is stored as
+// bf16(x * 2^-e), 2^e <= max|x| < 2^(e+1) over the field (e = 0 if it is all zero), rounded to nearest even
+// on the top 16 bits, and restored as bf16 * 2^e.
+inline int ck_exponent(float amax) {
+  uint32_t b;
+  std::memcpy(&b, &amax, 4);
+  const int be = (int)((b >> 23) & 0xFFu);
+  return b == 0u ? 0 : (be == 0 ? -126 : be - 127);
+}
+inline float ck_round_bf16(float x, int e) {
+  float v = std::ldexp(x, -e);
+  uint32_t u;
+  std::memcpy(&u, &v, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  u &= 0xFFFF0000u;
+  std::memcpy(&v, &u, 4);
+  return std::ldexp(v, e);
+}
+// FP16 round to nearest even of a float with |v| < 2^15: on the FP16 grid (ulp 2^(floor(log2|v|)-10) for
+// |v| >= 2^-14, 2^-24 below), computed exactly in double; nearbyint rounds ties to even.
+inline float ck_round_fp16(float x, int e) {
+  const double v = std::ldexp((double)x, 14 - e);
+  if (v == 0.0) return std::ldexp(0.0f * x, 0);
+  int ex;
+  std::frexp(v, &ex);  // |v| in [2^(ex-1), 2^ex)
+  const int ue = ex - 1 >= -14 ? ex - 11 : -24;
+  const double r = std::nearbyint(std::ldexp(v, -ue));
+  return (float)std::ldexp(std::ldexp(r, ue), e - 14);
+}
+inline void ck_round_field(float* f, long n, int fmt) {
+  float m = 0.f;
+  for (long i = 0; i < n; ++i) m = std::max(m, std::fabs(f[i]));
+  const int e = ck_exponent(m);
+  for (long i = 0; i < n; ++i) f[i] = fmt == 2 ? ck_round_fp16(f[i], e) : ck_round_bf16(f[i], e);
+}
+
+// Forward of one shot storing the imaging terms of every step (qmode 1) as the batched path ends up using
+// them with checkpoints every K steps: the last segment's terms come from the forward itself, every other
+// segment's from a recomputation that starts at its checkpoint (segment 0 at the zero state).  fmt 0: FP32
+// checkpoints, which reproduce the forward exactly; fmt 1 / 2: BF16 / FP16 checkpoints of the
+// increment-form state (P, F, Q, R and the four C-PML memory variables), so segments 1..J-2 start from
+// rounded states.
+template <class T, class S>
+void forward_shot_ck(const T* vh, const T* vp, const T* vn, const Tables<T>& tb, const T* src, const Geo& g,
+                     const ShotGeo& sg, int nt, Scal sc, T* gather, std::vector<T>* q, int K, int fmt) {
+  forward_shot<T, S>(vh, vp, vn, tb, src, g, sg, nt, sc, gather, 1, q);
+  if (fmt == 0) return;
+  static_assert(std::is_same<T, float>::value, "BF16 checkpoints of the float state");
+  const T dh = (T)sc.dh, dt = (T)sc.dt, dh2 = dh * dh, C = (dt * dt) / dh2;
+  const long n = (long)g.nx * g.nz;
+  const int J = (nt + K - 1) / K;
+  FwdState<T, S> st(g);
+  std::vector<std::vector<T>> snap((size_t)J);
+  auto fields = [](FwdState<T, S>& s) {
+    return std::vector<T*>{s.Pcur(), s.Fcur(), s.f[12], s.f[13], s.f[6], s.f[7], s.f[8], s.f[9]};
+  };
+  for (int it = 0; it < nt; ++it) {
+    if (it > 0 && it % K == 0 && it / K <= J - 2) {
+      std::vector<T>& c = snap[(size_t)(it / K)];
+      c.resize((size_t)8 * n);
+      auto fs = fields(st);
+      for (int k = 0; k < 8; ++k) {
+        std::memcpy(c.data() + (size_t)k * n, fs[k], n * sizeof(T));
+        ck_round_field(c.data() + (size_t)k * n, n, fmt);
+      }
+    }
+    fwd_step<T, S>(st, vh, vp, vn, tb.ax.data(), tb.bx.data(), tb.az.data(), tb.bz.data(), src, C, dh, dh2, sg, it,
+                   nullptr, nt);
+  }
+  for (int seg = 1; seg <= J - 2; ++seg) {
+    FwdState<T, S> r(g);
+    r.cur = (seg * K) & 1;
+    auto fs = fields(r);
+    for (int k = 0; k < 8; ++k) std::memcpy(fs[k], snap[(size_t)seg].data() + (size_t)k * n, n * sizeof(T));
+    for (int nn = seg * K; nn < std::min((seg + 1) * K, nt); ++nn) {
+      fwd_step<T, S>(r, vh, vp, vn, tb.ax.data(), tb.bx.data(), tb.az.data(), tb.bz.data(), src, C, dh, dh2, sg, nn,
+                     nullptr, nt);
+      T* qa = q->data() + (size_t)nn * 2 * n;
+      T* qb = qa + n;
+      for (int iz = kRim; iz < g.nz - kRim; ++iz)
+        for (int ix = kRim; ix < g.nx - kRim; ++ix) {
+          long i = ix + (long)g.nx * iz;
+          qa[i] = r.f[4][i] + r.f[10][i];
+          qb[i] = r.f[5][i] + r.f[11][i];
+        }
+    }
+  }
+}
+
 // Exact gradient of a group of shots that share one window (the batched path's gradient groups): every
 // shot runs its own forward and exact adjoint (shot_gradient, adjoint_exact_shot); per step and cell the
 // shots' imaging terms c_s are added in shot order, with zeros for absent members of a group of fewer than
@@ -924,7 +1013,7 @@ double shot_gradient(int mode_exact, const T* vh, const T* vp, const T* vn, cons
 template <class T, class S>
 void group_gradient(const T* vh, const T* vp, const T* vn, const T* tab_a, const T* tab_b, const T* src, int nx,
                     int nz, int nt, int cpml, double dh_, double dt_, int ns, const int* geo, const T* obs, T* syn,
-                    T* g_hor, T* g_po, T* g_nmo, double* misfits) {
+                    T* g_hor, T* g_po, T* g_nmo, double* misfits, int K = 0, int ckfmt = 0) {
   Geo g{nx, nz, cpml};
   Tables<T> tb(tab_a, tab_b, cpml, nx, nz);
   Scal sc{dh_, dt_};
@@ -939,7 +1028,12 @@ void group_gradient(const T* vh, const T* vp, const T* vn, const T* tab_a, const
   }
   for (int s = 0; s < ns; ++s) {
     T* sy = syn + goff[s];
-    forward_shot<T, S>(vh, vp, vn, tb, src, g, sg[s], nt, sc, sy, 1, &q[s]);
+    if (ckfmt) {
+      if constexpr (std::is_same<S, F32I>::value)
+        forward_shot_ck<T, S>(vh, vp, vn, tb, src, g, sg[s], nt, sc, sy, &q[s], K, ckfmt);
+    } else {
+      forward_shot<T, S>(vh, vp, vn, tb, src, g, sg[s], nt, sc, sy, 1, &q[s]);
+    }
     res[s].resize((size_t)nt * sg[s].nrec);
     double phi = 0;
     for (size_t i = 0; i < res[s].size(); ++i) {
@@ -1048,6 +1142,20 @@ ORC_GROUP(f32, float, float)
 ORC_GROUP(f64, double, double)
 ORC_GROUP(f32d, float, F32D)
 ORC_GROUP(f32i, float, F32I)
+
+// One checkpoint field rounded as a 16-bit checkpoint stores it (fmt 1 BF16, 2 FP16; scale from the
+// field's max-abs).
+void orc_ck_round(float* f, long n, int fmt) { ck_round_field(f, n, fmt); }
+
+// The f32i group gradient with the batched path's checkpoints every K steps in format ckfmt (0 FP32,
+// 1 BF16, 2 FP16, both with per-field power-of-two scaling; only the 16-bit formats change the result).
+void orc_group_gradient_ck_f32i(const float* vh, const float* vp, const float* vn, const float* tab_a,
+                                const float* tab_b, const float* src, int nx, int nz, int nt, int cpml, double dh,
+                                double dt, int ns, const int* geo, const float* obs, float* syn, float* g_hor,
+                                float* g_po, float* g_nmo, double* misfits, int K, int ckfmt) {
+  group_gradient<float, F32I>(vh, vp, vn, tab_a, tab_b, src, nx, nz, nt, cpml, dh, dt, ns, geo, obs, syn, g_hor, g_po,
+                              g_nmo, misfits, K, ckfmt);
+}
 
 // One linear forward state step without source (for the dot-product test), FP64.
 // state = 8 fields of nx*nz: P, P_old, F, F_old, psx, zx, psz, zz  ->  out: P', P, F', F, psx', zx', psz', zz'

@@ -72,19 +72,20 @@ def gradient_groups(shots, group=4, max_batch=0):
     return out
 
 
-def oracle_gradient(w, mode, exact, sq, tab, src, obs_list, group=4, max_batch=0):
+def oracle_gradient(w, mode, exact, sq, tab, src, obs_list, group=4, max_batch=0, ckpt=None):
     """Sum over shots (FP64) of the oracle gradients, placed into model columns; total FP64 misfit.  In exact
-    mode shots are accumulated in the batched path's gradient groups (group=1: one accumulator per shot)."""
+    mode shots are accumulated in the batched path's gradient groups (group=1: one accumulator per shot);
+    ckpt=(K, 1) emulates BF16 checkpoints every K steps."""
     T = np.float64 if mode == "f64" else np.float32
     G = np.zeros((3, w.nz, w.nx))
     phi = 0.0
     syns = []
-    if exact and group > 1:
+    if exact and (group > 1 or ckpt is not None):
         for members in gradient_groups(w.shots, group, max_batch):
             s0 = w.shots[members[0]]
             mis, sy, gh, gp, gn = O.group_gradient(mode, window_model(sq, s0), tab, src.astype(T),
                                                    [obs_list[i] for i in members], s0.nx, w.nz, w.nt, w.cpml, w.dh,
-                                                   w.dt, [w.shots[i] for i in members])
+                                                   w.dt, [w.shots[i] for i in members], ckpt=ckpt)
             for v in mis:
                 phi += v
             syns.extend(sy)

@@ -71,6 +71,11 @@ def _declare(L):
         fn = getattr(L, f"orc_group_gradient_{suf}")
         fn.argtypes = [P] * 6 + [_i, _i, _i, _i, _d, _d, _i, ctypes.c_void_p, P, P, P, P, P, _D]
         fn.restype = None
+    L.orc_group_gradient_ck_f32i.argtypes = [_F] * 6 + [_i, _i, _i, _i, _d, _d, _i, ctypes.c_void_p] + [_F] * 5 + [
+        _D, _i, _i]
+    L.orc_group_gradient_ck_f32i.restype = None
+    L.orc_ck_round.argtypes = [_F, ctypes.c_long, _i]
+    L.orc_ck_round.restype = None
     L.orc_step_fwd_f64.argtypes = [_D] * 5 + [_i, _i, _i, _d, _d, _D, _D]
     L.orc_step_adj_f64.argtypes = [_D] * 5 + [_i, _i, _i, _d, _d, _D, _D]
 
@@ -151,7 +156,7 @@ def gradient(mode, exact, model_sq, tab, src, obs, nx, nz, nt, cpml, dh, dt, sho
     return phi, float(m32[0]), syn, gh, gp, gn
 
 
-def group_gradient(mode, model_sq, tab, src, obs_list, nx, nz, nt, cpml, dh, dt, shots):
+def group_gradient(mode, model_sq, tab, src, obs_list, nx, nz, nt, cpml, dh, dt, shots, ckpt=None):
     """Exact gradient of a gradient group (1..4 shots sharing one window, imaging terms added in shot order
     per step).  Returns (per-shot FP64 misfits, syn list [nt,nrec], g_hor, g_po, g_nmo)."""
     T = _dtype_of(mode)
@@ -163,9 +168,16 @@ def group_gradient(mode, model_sq, tab, src, obs_list, nx, nz, nt, cpml, dh, dt,
     syn = np.zeros_like(obs)
     gh, gp, gn = (np.zeros((nz, nx), T) for _ in range(3))
     mis = np.zeros(len(shots))
-    getattr(lib(), f"orc_group_gradient_{mode}")(ptr(vh), ptr(vp), ptr(vn), ptr(ta), ptr(tb), ptr(s), nx, nz, nt, cpml,
-                                                 dh, dt, len(shots), geo.ctypes.data, ptr(obs), ptr(syn), ptr(gh),
-                                                 ptr(gp), ptr(gn), ptr(mis))
+    if ckpt is not None and ckpt[1] != 0:
+        # checkpoints every ckpt[0] steps in format ckpt[1] (1 = BF16, 2 = FP16), f32i only
+        assert mode == "f32i"
+        lib().orc_group_gradient_ck_f32i(ptr(vh), ptr(vp), ptr(vn), ptr(ta), ptr(tb), ptr(s), nx, nz, nt, cpml, dh, dt,
+                                         len(shots), geo.ctypes.data, ptr(obs), ptr(syn), ptr(gh), ptr(gp), ptr(gn),
+                                         ptr(mis), int(ckpt[0]), int(ckpt[1]))
+    else:
+        getattr(lib(), f"orc_group_gradient_{mode}")(ptr(vh), ptr(vp), ptr(vn), ptr(ta), ptr(tb), ptr(s), nx, nz, nt,
+                                                     cpml, dh, dt, len(shots), geo.ctypes.data, ptr(obs), ptr(syn),
+                                                     ptr(gh), ptr(gp), ptr(gn), ptr(mis))
     syns, o = [], 0
     for sh in shots:
         syns.append(syn[o:o + nt * sh.nrec].reshape(nt, sh.nrec))

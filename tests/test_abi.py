@@ -59,3 +59,23 @@ def test_cpml_table_library_matches_oracle():
     ta, tb = O.cpml_table(p)
     np.testing.assert_array_equal(a, ta.astype(np.float32))
     np.testing.assert_array_equal(b, tb.astype(np.float32))
+
+
+@pytest.mark.parametrize("kw,msg", [({"ckpt_format": 1, "arith": 0}, "increment form"),
+                                    ({"ckpt_format": 3}, "ckpt_format"), ({"grad_group": 5}, "grad_group"),
+                                    ({"arith": 7}, "arith")])
+def test_plan_rejects_bad_options_before_touching_the_gpu(kw, msg):
+    """Option errors come back from fwi_plan_create with a message, before any device call."""
+    import numpy as np
+
+    from fwi_amd import _lib
+    from fwi_amd import models as M
+    from fwi_amd.survey import Plan
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libfwi.so not built")
+    nx, nz, cpml = 64, 48, 10
+    shots = [M.reference_shot(nx, cpml, 30, 6)]
+    tab = (np.zeros(cpml, np.float32), np.ones(cpml, np.float32))
+    with pytest.raises(RuntimeError, match=msg):
+        Plan(nx, nz, 50, 10.0, 1e-3, cpml, shots, np.zeros(50, np.float32), tab, **kw)

@@ -142,6 +142,35 @@ def test_gradient_groups_match_oracle(torch, layout):
         assert e <= TOL
 
 
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_16bit_checkpoints_match_oracle(torch, fmt):
+    """BF16 (fmt 1) and scaled FP16 (fmt 2) checkpoints every 16 steps: the recomputed segments start from
+    the rounded states exactly as the oracle's emulation has them (bit-level bar), with a narrower window
+    in the batch so the strip packing covers two window widths; the FP32-checkpoint gradient differs by the
+    rounding's effect, which is printed."""
+    w = small_workload(nshots=3, nt=160)
+    s = w.shots[2]
+    w.shots[2] = M.Shot(x0=8, nx=w.nx - 8, isx=s.isx - 8, isz=s.isz, irz=s.irz, ir0=12, nrec=w.nx - 8 - 24)
+    say(f"16-bit checkpoints fmt={fmt} K=16 nt=160, 3 shots")
+    tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+    obs = oracle_gathers(w, "f32i", sq_true, tab, src)
+    phi_ref, G_ref, _ = oracle_gradient(w, "f32i", True, sq_init, tab, src, obs, ckpt=(16, fmt))
+    obs_d = torch.from_numpy(_cat(obs)).cuda()
+    model = _dev_model(torch, sq_init)
+    plan = _plan(w, tab, src, ckpt_interval=16, ckpt_format=fmt)
+    phi, G, _ = plan.gradient(model, obs_d)
+    plan32 = _plan(w, tab, src, ckpt_interval=16)
+    _, G32, _ = plan32.gradient(model, obs_d)
+    torch.cuda.synchronize()
+    G, G32 = G.cpu().numpy(), G32.cpu().numpy()
+    assert plan.workspace_bytes < plan32.workspace_bytes
+    assert abs(phi - phi_ref) <= 1e-9 * phi_ref
+    for k in range(3):
+        e = rel_l2(G[k], G_ref[k])
+        print(f"ckpt fmt {fmt} grad[{k}] vs oracle rel-L2 {e:.3e}, vs FP32 checkpoints {rel_l2(G[k], G32[k]):.3e}")
+        assert e <= TOL
+
+
 def test_ragged_windows_match_oracle(torch):
     """Shots with their own windowed subdomains and receiver spreads in one launch."""
     nx, nz, nt, cpml = 220, 72, 200, 12

@@ -391,3 +391,53 @@ def test_group_gradient_equals_per_shot_sum():
     ph, _, _, h1, p1_, n1 = O.gradient("f32i", True, tuple(np.ascontiguousarray(x[:, s0.x0:s0.x0 + s0.nx]) for x in si),
                                        tab, src, obs[0], s0.nx, w.nz, w.nt, w.cpml, w.dh, w.dt, s0)
     assert mis[0] == ph and np.array_equal(gh, h1) and np.array_equal(gp, p1_) and np.array_equal(gn, n1)
+
+
+def test_16bit_checkpoint_rounding_matches_torch_and_numpy():
+    """The oracle's 16-bit checkpoint rounding (exact power-of-two scale from the max-abs, then round to
+    nearest even) equals torch's float32 -> bfloat16 and numpy's float32 -> float16 conversions of the scaled
+    field, for fields of very different magnitudes, with zeros, ties and FP16 subnormals."""
+    import torch
+
+    rng = np.random.default_rng(3)
+    for scale in (1e-30, 1e-3, 1.0, 7.5e4, 3e30):
+        x = (rng.standard_normal(20000) * scale).astype(np.float32)
+        x *= np.float32(10.0) ** rng.integers(-9, 1, x.size).astype(np.float32)  # down into FP16 subnormals
+        x[:10] = 0.0
+        x[10:20] = np.float32(scale) * np.float32(1 + 2 ** -8)   # BF16 ties
+        x[20:30] = np.float32(scale) * np.float32(1 + 2 ** -11)  # FP16 ties
+        m = np.abs(x).max()
+        e = max(int(np.floor(np.log2(m))), -126)
+        y = x.copy()
+        O.lib().orc_ck_round(O.ptr(y), y.size, 1)
+        ref = torch.from_numpy(np.ldexp(x, -e).astype(np.float32)).to(torch.bfloat16).float().numpy()
+        assert np.array_equal(y, np.ldexp(ref, e).astype(np.float32)), scale
+        nz = x != 0
+        assert np.max(np.abs(y[nz] - x[nz]) / np.abs(x[nz])) <= 2.0 ** -8
+        y = x.copy()
+        O.lib().orc_ck_round(O.ptr(y), y.size, 2)
+        ref = np.ldexp(np.ldexp(x, 14 - e).astype(np.float32).astype(np.float16).astype(np.float32), e - 14)
+        assert np.array_equal(y, ref.astype(np.float32)), scale
+        big = np.abs(x) >= m * 2.0 ** -28
+        assert np.max(np.abs(y[big] - x[big]) / np.abs(x[big])) <= 2.0 ** -10
+
+
+def test_16bit_checkpoint_gradient_close_to_fp32_checkpoints():
+    """Emulated 16-bit checkpoints move the exact gradient (BF16 about 1e-2, FP16 about 8x less) and
+    change nothing when there is no intermediate checkpoint (K >= nt)."""
+    from helpers import oracle_gathers, oracle_gradient, oracle_setup, rel_l2, small_workload
+
+    w = small_workload(nshots=2, nt=160)
+    tab, src, st, si = oracle_setup(w, "f32i")
+    obs = oracle_gathers(w, "f32i", st, tab, src)
+    p0, G0, _ = oracle_gradient(w, "f32i", True, si, tab, src, obs)
+    errs = {}
+    for fmt in (1, 2):
+        p1, G1, _ = oracle_gradient(w, "f32i", True, si, tab, src, obs, ckpt=(16, fmt))
+        p2, G2, _ = oracle_gradient(w, "f32i", True, si, tab, src, obs, ckpt=(160, fmt))
+        assert p0 == p1 == p2  # the forward itself never reads a checkpoint
+        errs[fmt] = [rel_l2(G1[k], G0[k]) for k in range(3)]
+        assert all(np.array_equal(G2[k], G0[k]) for k in range(3))
+    print("16-bit checkpoints vs fp32 checkpoints (bf16, fp16):", errs)
+    assert 0 < max(errs[1]) < 3e-2 and 0 < max(errs[2]) < 5e-3
+    assert max(errs[2]) * 4 < max(errs[1])
