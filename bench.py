@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--mode", choices=["exact", "refcompat"], default="exact")
     ap.add_argument("--ckpt", type=int, default=0)
     ap.add_argument("--ckpt-format", type=int, default=0, help="checkpoints: 0 FP32, 1 BF16, 2 FP16 (scaled)")
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying captured graphs")
     ap.add_argument("--arith", type=int, default=3,
                     help="0 FP32 two-level, 1 mixed (reference), 2 FP32 difference stencil, 3 FP32 increment form")
     ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")
@@ -179,7 +180,7 @@ def main():
     grad_mode = 1 if args.mode == "exact" else 0
     plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
                 rows_per_item=args.rows, item_order=args.order, arith=args.arith,
-                ckpt_format=args.ckpt_format)
+                ckpt_format=args.ckpt_format, use_graph=0 if args.no_graph else 1)
     log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, workspace {plan.workspace_bytes / 1e9:.1f} GB")
     t0 = time.perf_counter()
     obs = plan.forward(true_sq)
@@ -190,7 +191,6 @@ def main():
         plan.gradient(init_sq, obs)
         torch.cuda.synchronize()
         log(f"warmup gradient {i}: {time.perf_counter() - t0:.3f} s")
-    plan.set_timing(args.timing_stride)
     torch.cuda.synchronize()
     times = []
     phi = None
@@ -207,6 +207,19 @@ def main():
     work = 2.0 * ncell * nt
     gcells = work / sec / 1e9
     assert np.isfinite(phi) and torch.isfinite(grad).all().item()
+    graphs = plan.graph_stats()
+    log(f"graph captures {graphs[0]}, replays {graphs[1]}")
+    # kernel durations: one more gradient with every timing_stride-th launch bracketed by HIP events (the
+    # instrumented call launches eagerly; its kernels are the ones the timed calls replayed)
+    sec_instr = None
+    if args.timing_stride > 0:
+        plan.set_timing(args.timing_stride)
+        t0 = time.perf_counter()
+        phi_i, _, _ = plan.gradient(init_sq, obs)
+        torch.cuda.synchronize()
+        sec_instr = time.perf_counter() - t0
+        assert phi_i == phi
+        log(f"instrumented eager gradient: {sec_instr:.3f} s")
 
     # roofline of the adjoint step kernel (dominant): algorithmic bytes per launch / mean event time
     adj_ms, adj_n = plan.kernel_time(2)
@@ -274,6 +287,8 @@ def main():
             "rows_per_item": plan.rows_per_item,
             "item_order": plan.item_order,
             "s_per_gradient_batched": round(sec, 4),
+            "graphs": {"captures": graphs[0], "replays": graphs[1]},
+            "s_per_gradient_eager_instrumented": round(sec_instr, 4) if sec_instr else None,
             "s_per_gradient_plain": round(plain, 3) if plain else None,
             "speedup_vs_plain": round(plain / sec, 2) if plain else None,
             "misfit": phi,

@@ -756,6 +756,15 @@ __global__ void __launch_bounds__(256) k_ck_pack_strips(StripArgs a, int dir, fl
   }
 }
 
+// Clears and copies of the batched path as kernels: they can be captured into a graph, where memset
+// nodes are avoided.
+__global__ void __launch_bounds__(256) k_zero_f32(float* __restrict__ p, long long n) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = 0.f;
+}
+__global__ void __launch_bounds__(256) k_copy_f32(float* __restrict__ d, const float* __restrict__ s, long long n) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = s[i];
+}
+
 // Residual res = syn - obs of every shot of a batch and per-(shot, block) FP64 partial sums of res^2
 // (parallelcalls.cu:133-142, main.c:448-454).  Block (b, s) covers a fixed slice of shot s's gather, so
 // the misfit of a shot does not depend on which other shots share its batch.

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "../../include/fwi.h"
@@ -160,7 +161,70 @@ struct fwi_plan {
     return 1;
   }
 
+  int zero(float* p, size_t n, hipStream_t st) {
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    if (n) hipLaunchKernelGGL(k_zero_f32, dim3(g), dim3(256), 0, st, p, (long long)n);
+    CK(hipGetLastError());
+    return 0;
+  }
+  int copy(float* d, const float* src, size_t n, hipStream_t st) {
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    if (n) hipLaunchKernelGGL(k_copy_f32, dim3(g), dim3(256), 0, st, d, src, (long long)n);
+    CK(hipGetLastError());
+    return 0;
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  // Graphs: the device work of one batch of one entry point is captured once, on the plan's own stream,
+  // and its executable graph is cached under the entry point, the batch and the caller's buffers; later
+  // calls with the same buffers (an inversion updates its model in place) replay it on the caller's
+  // stream.  Not used while kernel timing is on.
+  struct GraphEntry {
+    std::array<const void*, 10> key;
+    hipGraphExec_t exec;
+  };
+  static constexpr size_t kMaxGraphs = 8;
+  std::vector<GraphEntry> graphs;
+  hipStream_t cap_stream = nullptr;
+  long long graph_captures = 0, graph_replays = 0;
+
+  template <class F>
+  int run_graph(const std::array<const void*, 10>& key, hipStream_t st, F&& body) {
+    if (!opt.use_graph || timing_stride > 0) return body(st);
+    for (size_t i = 0; i < graphs.size(); ++i)
+      if (graphs[i].key == key) {
+        std::rotate(graphs.begin() + i, graphs.begin() + i + 1, graphs.end());  // most recently used last
+        CK(hipGraphLaunch(graphs.back().exec, st));
+        ++graph_replays;
+        return 0;
+      }
+    if (!cap_stream) CK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+    CK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+    const int r = body(cap_stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(cap_stream, &graph);
+    if (r) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    if (e != hipSuccess) return fail(e, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(ei, "hipGraphInstantiate");
+    if (graphs.size() >= kMaxGraphs) {  // least recently used first
+      (void)hipGraphExecDestroy(graphs.front().exec);
+      graphs.erase(graphs.begin());
+    }
+    graphs.push_back({key, exec});
+    ++graph_captures;
+    CK(hipGraphLaunch(exec, st));
+    return 0;
+  }
+
   ~fwi_plan() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (auto& b : batches) {
       (void)hipFree(b.d_shots);
       (void)hipFree(b.d_items);
@@ -448,7 +512,7 @@ struct fwi_plan {
   }
 
   int zero_fwd(const Batch& b, hipStream_t st) {
-    CK(hipMemsetAsync(fwd(0), 0, kFwdF * (size_t)cap_cell * sizeof(float), st));
+    if (int r = zero(fwd(0), kFwdF * (size_t)cap_cell, st)) return r;
     (void)b;
     return 0;
   }
@@ -485,8 +549,7 @@ struct fwi_plan {
     }
     float* cf = (float*)c;
     for (int f = 0; f < 4; ++f)
-      CK(hipMemcpyAsync(cf + (size_t)f * b.ncell, fwd(ckpt_field(f, n)), b.ncell * sizeof(float),
-                        hipMemcpyDeviceToDevice, st));
+      if (int r = copy(cf + (size_t)f * b.ncell, fwd(ckpt_field(f, n)), b.ncell, st)) return r;
     return strip_copy(b, 0, cf + 4 * (size_t)b.ncell, st);
   }
   int load_ckpt(const Batch& b, int j, hipStream_t st) {
@@ -517,8 +580,7 @@ struct fwi_plan {
     }
     float* cf = (float*)c;
     for (int f = 0; f < 4; ++f)
-      CK(hipMemcpyAsync(fwd(ckpt_field(f, n)), cf + (size_t)f * b.ncell, b.ncell * sizeof(float),
-                        hipMemcpyDeviceToDevice, st));
+      if (int r = copy(fwd(ckpt_field(f, n)), cf + (size_t)f * b.ncell, b.ncell, st)) return r;
     return strip_copy(b, 1, cf + 4 * (size_t)b.ncell, st);
   }
 
@@ -562,13 +624,13 @@ struct fwi_plan {
       if (int r = fwd_step(b, m, n, n >= slast ? qm : 0, qa, qb, gather, st)) return r;
     }
     if (adj_src) {
-      CK(hipMemcpyAsync(resid(), adj_src + b.gather_off, b.ngather * sizeof(float), hipMemcpyDeviceToDevice, st));
+      if (int r = copy(resid(), adj_src + b.gather_off, b.ngather, st)) return r;
     } else {
       if (int r = residual(b, gather, obs, resid(), d_misfit, first_batch ? 0 : 1, st)) return r;
     }
     // adjoint
-    CK(hipMemsetAsync(adj(0), 0, 12 * (size_t)cap_cell * sizeof(float), st));
-    CK(hipMemsetAsync(sgrad(0), 0, 3 * (size_t)cap_cell * sizeof(float), st));
+    if (int r = zero(adj(0), kAdjF * (size_t)cap_cell, st)) return r;
+    if (int r = zero(sgrad(0), kGradF * (size_t)cap_cell, st)) return r;
     int seg = J - 1;
     for (int k = nt; k >= 1; --k) {
       const int n = k - lag;
@@ -727,6 +789,7 @@ void fwi_options_default(fwi_options* o) {
   o->grad_mode = FWI_GRAD_EXACT;
   o->arith = FWI_ARITH_FP32I;
   o->item_order = -1;
+  o->use_graph = 1;
 }
 
 int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out) {
@@ -940,8 +1003,12 @@ int fwi_forward(fwi_plan* p, const float* vh, const float* vp, const float* vn, 
   hipStream_t st = (hipStream_t)stream;
   ModelArgs m{vh, vp, vn, p->nx};
   if (int r = p->tune(m, st)) return r;
-  for (auto& b : p->batches)
-    if (int r = p->forward_batch(b, m, gathers + b.gather_off, st)) return r;
+  for (size_t i = 0; i < p->batches.size(); ++i) {
+    Batch& b = p->batches[i];
+    const std::array<const void*, 10> key{(const void*)1, (const void*)i, vh, vp, vn, gathers};
+    if (int r = p->run_graph(key, st, [&](hipStream_t s) { return p->forward_batch(b, m, gathers + b.gather_off, s); }))
+      return r;
+  }
   p->timing_collect();
   return 0;
 }
@@ -951,11 +1018,15 @@ int fwi_misfit(fwi_plan* p, const float* vh, const float* vp, const float* vn, c
   hipStream_t st = (hipStream_t)stream;
   ModelArgs m{vh, vp, vn, p->nx};
   if (int r = p->tune(m, st)) return r;
-  bool first = true;
-  for (auto& b : p->batches) {
-    if (int r = p->forward_batch(b, m, p->gathers(), st)) return r;
-    if (int r = p->residual(b, p->gathers(), obs, nullptr, p->d_misfit, first ? 0 : 1, st)) return r;
-    first = false;
+  for (size_t i = 0; i < p->batches.size(); ++i) {
+    Batch& b = p->batches[i];
+    const int acc = i > 0;
+    const std::array<const void*, 10> key{(const void*)2, (const void*)i, vh, vp, vn, obs};
+    if (int r = p->run_graph(key, st, [&](hipStream_t s) {
+          if (int r2 = p->forward_batch(b, m, p->gathers(), s)) return r2;
+          return p->residual(b, p->gathers(), obs, nullptr, p->d_misfit, acc, s);
+        }))
+      return r;
   }
   hipError_t e = hipMemcpyAsync(misfit, p->d_misfit, sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -969,10 +1040,14 @@ int fwi_gradient(fwi_plan* p, const float* vh, const float* vp, const float* vn,
   if (int r = p->tune(m, st)) return r;
   hipError_t e = hipMemsetAsync(grad, 0, 3 * (size_t)p->nx * p->nz * sizeof(double), st);
   if (e != hipSuccess) return p->fail(e, "zero grad");
-  bool first = true;
-  for (auto& b : p->batches) {
-    if (int r = p->gradient_batch(b, m, obs, nullptr, grad, first ? 1 : 0, syn, st)) return r;
-    first = false;
+  for (size_t i = 0; i < p->batches.size(); ++i) {
+    Batch& b = p->batches[i];
+    const int first = i == 0;
+    const std::array<const void*, 10> key{(const void*)3, (const void*)i, vh, vp, vn, obs, grad, syn};
+    if (int r = p->run_graph(key, st, [&](hipStream_t s) {
+          return p->gradient_batch(b, m, obs, nullptr, grad, first, syn, s);
+        }))
+      return r;
   }
   if (misfit) {
     e = hipMemcpyAsync(misfit, p->d_misfit, sizeof(double), hipMemcpyDeviceToHost, st);
@@ -1021,6 +1096,11 @@ long long fwi_plan_cells(const fwi_plan* p, long long* strip_cells) {
 int fwi_plan_ckpt_interval(const fwi_plan* p) { return p->K; }
 int fwi_plan_rows_per_item(const fwi_plan* p) { return p->tz; }
 int fwi_plan_item_order(const fwi_plan* p) { return p->order; }
+int fwi_plan_graph_stats(const fwi_plan* p, long long* captures, long long* replays) {
+  if (captures) *captures = p->graph_captures;
+  if (replays) *replays = p->graph_replays;
+  return 0;
+}
 int fwi_plan_adj_tiling(const fwi_plan* p, int* rows, int* order, int* grouped) {
   if (rows) *rows = p->tz_adj;
   if (order) *order = p->order_adj;
@@ -1035,8 +1115,14 @@ int fwi_gradient_src(fwi_plan* p, const float* vh, const float* vp, const float*
   if (int r = p->tune(m, st)) return r;
   hipError_t e = hipMemsetAsync(grad, 0, 3 * (size_t)p->nx * p->nz * sizeof(double), st);
   if (e != hipSuccess) return p->fail(e, "zero grad");
-  for (auto& b : p->batches)
-    if (int r = p->gradient_batch(b, m, nullptr, adj_src, grad, 0, syn, st)) return r;
+  for (size_t i = 0; i < p->batches.size(); ++i) {
+    Batch& b = p->batches[i];
+    const std::array<const void*, 10> key{(const void*)4, (const void*)i, vh, vp, vn, adj_src, grad, syn};
+    if (int r = p->run_graph(key, st, [&](hipStream_t s) {
+          return p->gradient_batch(b, m, nullptr, adj_src, grad, 0, syn, s);
+        }))
+      return r;
+  }
   p->timing_collect();
   return 0;
 }

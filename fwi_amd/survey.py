@@ -64,7 +64,7 @@ class Plan:
     def __init__(self, nx: int, nz: int, nt: int, dh: float, dt: float, cpml: int, shots: Sequence[Shot],
                  wavelet: np.ndarray, cpml_ab, grad_mode: int = GRAD_EXACT, arith: int = ARITH_FP32I,
                  ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0,
-                 item_order: int = -1, grad_group: int = 0, ckpt_format: int = 0):
+                 item_order: int = -1, grad_group: int = 0, ckpt_format: int = 0, use_graph: int = 1):
         L = _lib.load()
         self.nx, self.nz, self.nt, self.dh, self.dt, self.cpml = nx, nz, nt, dh, dt, cpml
         self.shots = list(shots)
@@ -89,9 +89,11 @@ class Plan:
         opt.item_order = item_order
         opt.grad_group = grad_group
         opt.ckpt_format = ckpt_format
+        opt.use_graph = use_graph
         h = ctypes.c_void_p()
         _lib.check(L.fwi_plan_create(ctypes.byref(sv), ctypes.byref(opt), ctypes.byref(h)), "fwi_plan_create")
         self._h = h
+        self._bufs = {}
         self.grad_mode = grad_mode
         self.gather_size = int(L.fwi_plan_gather_size(h))
         offs = np.cumsum([0] + [nt * s.nrec for s in self.shots])
@@ -134,6 +136,12 @@ class Plan:
         r, o, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _lib.load().fwi_plan_adj_tiling(self._h, ctypes.byref(r), ctypes.byref(o), ctypes.byref(g))
         return r.value, o.value, bool(g.value)
+
+    def graph_stats(self):
+        """(graph captures, graph replays) so far."""
+        c, r = ctypes.c_longlong(), ctypes.c_longlong()
+        _lib.load().fwi_plan_graph_stats(self._h, ctypes.byref(c), ctypes.byref(r))
+        return c.value, r.value
 
     def cells(self):
         """(sum of nx_s*nz over shots, C-PML cells of all windows)."""
@@ -181,9 +189,20 @@ class Plan:
             assert t.is_cuda and t.dtype == torch.float32 and t.shape == (self.nz, self.nx)
         return vh, vp, vn
 
-    def forward(self, model, stream=None) -> torch.Tensor:
+    def _persistent(self, name, shape, dtype, device):
+        """A plan-owned output buffer (same address on every call, so graph replays apply)."""
+        buf = self._bufs.get(name)
+        if buf is None or buf.device != device:
+            buf = torch.empty(shape, dtype=dtype, device=device)
+            self._bufs[name] = buf
+        return buf
+
+    def forward(self, model, stream=None, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Gathers of every shot; `out` (float32, gather_size) is filled in place when given."""
         vh, vp, vn = self._model(model)
-        out = torch.empty(self.gather_size, dtype=torch.float32, device=vh.device)
+        if out is None:
+            out = torch.empty(self.gather_size, dtype=torch.float32, device=vh.device)
+        assert out.is_contiguous() and out.dtype == torch.float32 and out.numel() == self.gather_size
         _lib.check(_lib.load().fwi_forward(self._h, vh.data_ptr(), vp.data_ptr(), vn.data_ptr(), out.data_ptr(),
                                            _stream_ptr(stream)), "fwi_forward")
         return out
@@ -196,17 +215,20 @@ class Plan:
                                           ctypes.byref(m), _stream_ptr(stream)), "fwi_misfit")
         return m.value
 
-    def gradient(self, model, obs: torch.Tensor, want_syn: bool = False, stream=None):
-        """Returns (misfit, grad[3, nz, nx] float64 for (v_hor, v_po, v_nmo), syn or None)."""
+    def gradient(self, model, obs: torch.Tensor, want_syn: bool = False, stream=None, out: torch.Tensor | None = None):
+        """Returns (misfit, grad[3, nz, nx] float64 for (v_hor, v_po, v_nmo), syn or None).  The gradient is
+        written into `out` when given (and returned), else into a plan-owned buffer that is returned cloned;
+        either way the library sees the same buffers from call to call and replays its captured graphs."""
         vh, vp, vn = self._model(model)
         obs = obs.contiguous()
-        grad = torch.empty((3, self.nz, self.nx), dtype=torch.float64, device=vh.device)
-        syn = torch.empty(self.gather_size, dtype=torch.float32, device=vh.device) if want_syn else None
+        grad = out if out is not None else self._persistent("grad", (3, self.nz, self.nx), torch.float64, vh.device)
+        assert grad.is_contiguous() and grad.dtype == torch.float64 and grad.numel() == 3 * self.nz * self.nx
+        syn = self._persistent("syn", (self.gather_size,), torch.float32, vh.device) if want_syn else None
         m = ctypes.c_double()
         _lib.check(_lib.load().fwi_gradient(self._h, vh.data_ptr(), vp.data_ptr(), vn.data_ptr(), obs.data_ptr(),
                                             grad.data_ptr(), ctypes.byref(m), syn.data_ptr() if syn is not None else None,
                                             _stream_ptr(stream)), "fwi_gradient")
-        return m.value, grad, syn
+        return m.value, (grad if out is not None else grad.clone()), (syn.clone() if syn is not None else None)
 
 
     def gradient_src(self, model, adj_src: torch.Tensor, want_syn: bool = False, stream=None):
@@ -214,13 +236,13 @@ class Plan:
         vh, vp, vn = self._model(model)
         adj_src = adj_src.contiguous().float()
         assert adj_src.numel() == self.gather_size
-        grad = torch.empty((3, self.nz, self.nx), dtype=torch.float64, device=vh.device)
-        syn = torch.empty(self.gather_size, dtype=torch.float32, device=vh.device) if want_syn else None
+        grad = self._persistent("grad", (3, self.nz, self.nx), torch.float64, vh.device)
+        syn = self._persistent("syn", (self.gather_size,), torch.float32, vh.device) if want_syn else None
         _lib.check(_lib.load().fwi_gradient_src(self._h, vh.data_ptr(), vp.data_ptr(), vn.data_ptr(),
                                                 adj_src.data_ptr(), grad.data_ptr(),
                                                 syn.data_ptr() if syn is not None else None, _stream_ptr(stream)),
                    "fwi_gradient_src")
-        return grad, syn
+        return grad.clone(), (syn.clone() if syn is not None else None)
 
 
 def chain_rule(vp: torch.Tensor, eps: torch.Tensor, delta: torch.Tensor, grad: torch.Tensor, stream=None):

@@ -72,6 +72,9 @@ typedef struct fwi_options {
                           FP32 gradient slot, their per-step imaging terms added in shot order before one
                           read-modify-write (0 = 4, the default; 1 = one slot per shot) */
   int ckpt_format;     /* FWI_CKPT_FP32 (default), FWI_CKPT_BF16 or FWI_CKPT_FP16 */
+  int use_graph;       /* 1 (default): each batch's device work of fwi_forward / fwi_misfit / fwi_gradient /
+                          fwi_gradient_src is captured once as a HIP graph per set of caller buffers and
+                          replayed on later calls with the same buffers; 0: launched eagerly */
 } fwi_options;
 
 typedef struct fwi_plan fwi_plan;
@@ -172,6 +175,8 @@ int fwi_plan_ckpt_interval(const fwi_plan* plan);
 int fwi_plan_rows_per_item(const fwi_plan* plan);
 int fwi_plan_item_order(const fwi_plan* plan);
 int fwi_plan_adj_tiling(const fwi_plan* plan, int* rows_per_item, int* item_order, int* grouped);
+/* Graph captures and replays so far (use_graph). */
+int fwi_plan_graph_stats(const fwi_plan* plan, long long* captures, long long* replays);
 
 /* Host helpers. */
 void fwi_cpml_table(int cpml, float vmax, float dt, float dh, float f, float d0, float L, float* a, float* b);
