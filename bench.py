@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="survey1600")
-    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--shots", type=int, default=0, help="shots (0 = the workload's own count)")
     ap.add_argument("--nt", type=int, default=0)
     ap.add_argument("--mode", choices=["exact", "refcompat"], default="exact")
     ap.add_argument("--ckpt", type=int, default=0)
@@ -141,7 +141,7 @@ def cpu_baseline(w, nsteps):
     shot = w.shots[len(w.shots) // 2]
     tab = O.cpml_table(w.cpml_params)
     src = O.ricker(nsteps, w.f, w.dt)
-    sq = O.pre_variables(*w.init_model)
+    sq = O.pre_variables(*[np.ascontiguousarray(x[:, shot.x0:shot.x0 + shot.nx]) for x in w.init_model])
     obs = np.zeros((nsteps, shot.nrec), np.float32)
     t0 = time.perf_counter()
     O.gradient("f32i", True, sq, tab, src, obs, shot.nx, w.nz, nsteps, w.cpml, w.dh, w.dt, shot)
@@ -169,7 +169,7 @@ def main():
         torch.cuda.synchronize()
         log("calibration copy: 1 GiB read + 1 GiB written")
         del a, b
-    w = M.workload(args.workload, nshots=args.shots, nt=args.nt or None)
+    w = M.workload(args.workload, nshots=args.shots or None, nt=args.nt or None)
     nt = w.nt
     log(f"workload {w.name}: {w.nx}x{w.nz}, {len(w.shots)} shots, nt={nt}, dt={w.dt}, f={w.f}")
     ab = cpml_table(w.cpml_params)
@@ -262,6 +262,10 @@ def main():
                          f"fwd+adj, {csec:.1f} s"}
         log(f"cpu baseline: {cg:.3f} Gcell/s on {threads} threads ({csec:.1f} s)")
 
+    nrecs = [sh.nrec for sh in w.shots]
+    widths = [sh.nx for sh in w.shots]
+    spread = (f"{nrecs[0]} receivers" if min(nrecs) == max(nrecs) else f"{min(nrecs)}-{max(nrecs)} receivers") + (
+        "" if min(widths) == max(widths) == w.nx else f", windows of {min(widths)}-{max(widths)} columns")
     line = {
         "metric": "Gcell-updates/s fwd+adj; s per FWI gradient, batched vs plain, 1 MI355X",
         "value": round(gcells, 3),
@@ -276,7 +280,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded layered VTI model with faults, Ricker source)",
         "config": {
-            "workload": f"{w.name}: {w.nx}x{w.nz} dh={w.dh} m, {len(w.shots)} shots, {w.shots[0].nrec} receivers, "
+            "workload": f"{w.name}: {w.nx}x{w.nz} dh={w.dh} m, {len(w.shots)} shots, {spread}, "
                         f"nt={nt}, dt={w.dt} s, {w.f} Hz Ricker, cpml={w.cpml}",
             "grad_mode": args.mode,
             "arith": {0: "fp32", 1: "mixed", 2: "fp32-diff", 3: "fp32-increment"}[args.arith],

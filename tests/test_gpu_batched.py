@@ -226,3 +226,44 @@ def test_fp32_error_against_fp64_oracle(torch, arith, bound, fine):
     errs = [rel_l2(G[k], G64[k]) for k in range(3)]
     print(f"arith {arith} vs FP64: misfit rel {abs(phi - phi64) / phi64:.3e}, grad rel-L2 {errs}")
     assert max(errs) < bound
+
+
+def test_adjoint128_checkpoint28_fp32(torch):
+    """The adjoint-check configuration on the GPU (128x128, 2 shots, nt = 800, 12 Hz, FP32 increment form,
+    checkpoints every 28 steps + C-PML strips): the gradient equals the oracle's at the bit level, its error
+    against the FP64 oracle is stated and bounded, and it agrees with central differences of the GPU's own
+    FP32 misfit along a smooth perturbation of (vp, eps, delta)."""
+    from fwi_amd.survey import chain_rule, pre_variables
+
+    w = M.workload("adjoint128")
+    say("adjoint128: K=28 FP32 vs oracle f32i, FP64 and FD")
+    tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+    obs = oracle_gathers(w, "f32i", sq_true, tab, src)
+    phi_ref, G_ref, _ = oracle_gradient(w, "f32i", True, sq_init, tab, src, obs)
+    plan = _plan(w, tab, src, ckpt_interval=28)
+    assert plan.ckpt_interval == 28
+    obs_d = torch.from_numpy(_cat(obs)).cuda()
+    phi, G, _ = plan.gradient(_dev_model(torch, sq_init), obs_d)
+    torch.cuda.synchronize()
+    Gn = G.cpu().numpy()
+    for k in range(3):
+        assert rel_l2(Gn[k], G_ref[k]) <= TOL
+    # FP64 oracle on the same observed data
+    tab64, src64, _, si64 = oracle_setup(w, "f64")
+    _, G64, _ = oracle_gradient(w, "f64", True, si64, tab64, src64.astype(np.float64), [o.astype(np.float64) for o in obs])
+    e64 = [rel_l2(Gn[k], G64[k]) for k in range(3)]
+    print(f"adjoint128 FP32 (K=28) vs FP64 gradient rel-L2 (v_hor, v_po, v_nmo): {e64}")
+    assert max(e64) < 3e-4
+    # central differences of the FP32 misfit, h = 1e-1 .. 1e-3 of a perturbation of unit size
+    m0 = [torch.from_numpy(np.ascontiguousarray(x, np.float32)).cuda() for x in w.init_model]
+    dm = M.smooth_perturbation(w.nx, w.nz, seed=11, sigma=6.0)
+    d = [torch.from_numpy(np.ascontiguousarray(x * s, np.float32)).cuda() for x, s in zip(dm, (50.0, 0.01, 0.01))]
+    gv = chain_rule(*m0, G)
+    pred = sum(torch.sum(g * dd.double()).item() for g, dd in zip(gv, d))
+    errs = []
+    for h in (1e-1, 3e-2, 1e-2):
+        Jp = plan.misfit(pre_variables(*[a + h * b for a, b in zip(m0, d)]), obs_d)
+        Jm = plan.misfit(pre_variables(*[a - h * b for a, b in zip(m0, d)]), obs_d)
+        errs.append(abs((Jp - Jm) / (2 * h) - pred) / abs(pred))
+    print(f"adjoint128 FP32 FD relative errors (h = 1e-1, 3e-2, 1e-2): {errs}")
+    assert min(errs) < 2e-3

@@ -441,3 +441,34 @@ def test_16bit_checkpoint_gradient_close_to_fp32_checkpoints():
     print("16-bit checkpoints vs fp32 checkpoints (bf16, fp16):", errs)
     assert 0 < max(errs[1]) < 3e-2 and 0 < max(errs[2]) < 5e-3
     assert max(errs[2]) * 4 < max(errs[1])
+
+
+def test_adjoint128_fd_fp64_over_h_range():
+    """The adjoint-check configuration (128x128, dh 10 m, 2 shots, 88 live receivers each, nt = 800, dt 1 ms,
+    12 Hz): central finite differences of the FP64 misfit along a seeded smooth perturbation of vp, epsilon and
+    delta at once, h = 1e-2 ... 1e-6, against <grad, dm> of the exact FP64 gradient (both shots)."""
+    from helpers import oracle_gathers, oracle_gradient, oracle_setup
+
+    w = M.workload("adjoint128")
+    assert (w.nx, w.nz, w.nt, w.f, len(w.shots)) == (128, 128, 800, 12.0, 2) and w.shots[0].nrec == 88
+    tab, src, sq_true, sq_init = oracle_setup(w, "f64")
+    src = src.astype(np.float64)
+    obs = oracle_gathers(w, "f64", sq_true, tab, src)
+    m0 = [x.astype(np.float64) for x in w.init_model]
+    dm = M.smooth_perturbation(w.nx, w.nz, seed=11, sigma=6.0)
+    d = [dm[0] * 50.0, dm[1] * 0.01, dm[2] * 0.01]
+
+    def J(mm):
+        sq = O.pre_variables(*mm, dtype=np.float64)
+        return sum(0.5 * np.sum((g - o) ** 2) for g, o in zip(oracle_gathers(w, "f64", sq, tab, src), obs))
+
+    phi, G, _ = oracle_gradient(w, "f64", True, O.pre_variables(*m0, dtype=np.float64), tab, src, obs)
+    gv = O.chain_rule(*m0, G[0], G[1], G[2])
+    pred = sum(np.sum(g * dd) for g, dd in zip(gv, d))
+    errs = []
+    for h in (1e-2, 1e-3, 1e-4, 1e-5, 1e-6):
+        fd = (J([a + h * b for a, b in zip(m0, d)]) - J([a - h * b for a, b in zip(m0, d)])) / (2 * h)
+        errs.append(abs(fd - pred) / abs(pred))
+    print("adjoint128 FP64 FD relative errors, h = 1e-2..1e-6:", errs)
+    assert errs[0] / errs[1] > 50 and errs[1] / errs[2] > 50  # O(h^2) until round-off
+    assert min(errs) < 1e-9
