@@ -99,8 +99,8 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
     const float c = ldu(P, o);
     const float h = ldu(P, (unsigned)(hx + nx * z));
     // two-level form: P^{n-1}, F^{n-1}; increment form: Q^n, R^n
-    const float pold = INC ? ldu(a.Qc + sd.off, o) : ldu(Po, o);
-    const float fold = INC ? ldu(a.Rc + sd.off, o) : ldu(Fo, o);
+    const float pold = INC ? ldn(a.Qc + sd.off, o) : ldn(Po, o);
+    const float fold = INC ? ldn(a.Rc + sd.off, o) : ldn(Fo, o);
     const float vhv = ldu(vh, mo), vpv = ldu(vp, mo), vnv = ldu(vn, mo);
     float u[9];
     xrow9(buf, lane, c, h, u);
@@ -138,8 +138,8 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
         const float qn = pold + dq, rn = fold + dr;
         pn = c + qn;
         fn = fq[4] + rn;
-        stu(a.Qc + sd.off, o, qn);
-        stu(a.Rc + sd.off, o, rn);
+        stn(a.Qc + sd.off, o, qn);
+        stn(a.Rc + sd.off, o, rn);
       } else {
         pn = 2.f * c - pold + vhv * C * tp + vpv * C * tf;
         fn = 2.f * fq[4] - fold + vnv * C * tp + vpv * C * tf;
@@ -148,16 +148,16 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
           fn += srcv;
         }
       }
-      stu(Po, o, pn);
-      stu(Fo, o, fn);
+      stn(Po, o, pn);
+      stn(Fo, o, fn);
       if (a.gather && z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec)
         a.gather[sd.goff + (long)a.it * sd.nrec + (ix - sd.ir0)] = pn + fn;
       if (QM == 1) {
-        stu(a.qa + sd.off, o, p2 + atp);
-        stu(a.qb + sd.off, o, r2 + atf);
+        stn(a.qa + sd.off, o, p2 + atp);
+        stn(a.qb + sd.off, o, r2 + atf);
       } else if (QM == 2) {
-        stu(a.qa + sd.off, o, p2);
-        stu(a.qb + sd.off, o, r2);
+        stn(a.qa + sd.off, o, p2);
+        stn(a.qb + sd.off, o, r2);
       }
     }
 #pragma unroll
@@ -519,7 +519,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     }
     go = act ? go : 0.f;
     gh = hact ? gh : 0.f;
-    const float lold = ldu(Lo, o), mold = ldu(Mo, o);
+    const float lold = ldn(Lo, o), mold = ldn(Mo, o);
     float gx[9];
     xrow9(buf, lane, go, gh, gx);
     if (act) {
@@ -543,10 +543,10 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
         ln += r;
         mn += r;
       }
-      stu(Lo, o, ln);
-      stu(Mo, o, mn);
+      stn(Lo, o, ln);
+      stn(Mo, o, mn);
       if (a.qa) {
-        const float qa = ldu(a.qa + sd.off, o), qb = ldu(a.qb + sd.off, o);
+        const float qa = ldn(a.qa + sd.off, o), qb = ldn(a.qb + sd.off, o);
         if (GROUP) {
           ch = dt2 * ln * qa;
           cn = dt2 * mn * qa;
@@ -555,9 +555,9 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
           float* gh_ = a.gh + sd.gslot;
           float* gn_ = a.gn + sd.gslot;
           float* gp_ = a.gp + sd.gslot;
-          stu(gh_, o, ldu(gh_, o) + dt2 * ln * qa);
-          stu(gn_, o, ldu(gn_, o) + dt2 * mn * qa);
-          stu(gp_, o, ldu(gp_, o) + dt2 * (ln + mn) * qb);
+          stn(gh_, o, ldn(gh_, o) + dt2 * ln * qa);
+          stn(gn_, o, ldn(gn_, o) + dt2 * mn * qa);
+          stn(gp_, o, ldn(gp_, o) + dt2 * (ln + mn) * qb);
         }
       }
     }
@@ -580,7 +580,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       if (wave < 3 && act) {
         const float sum = ((gs[0][wave][lane] + gs[1][wave][lane]) + gs[2][wave][lane]) + gs[3][wave][lane];
         float* g = (wave == 0 ? a.gh : wave == 1 ? a.gn : a.gp) + sd.gslot;
-        stu(g, o, ldu(g, o) + sum);
+        stn(g, o, ldn(g, o) + sum);
       }
     }
   }

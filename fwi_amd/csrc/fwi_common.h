@@ -114,6 +114,14 @@ __device__ __forceinline__ float ldu(const float* base, unsigned idx) {
 __device__ __forceinline__ void stu(float* base, unsigned idx, float v) {
   *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (idx << 2)) = v;
 }
+// Streaming (non-temporal) variants for values touched once per launch (centre-only state, imaging terms,
+// gradient slots), so the L2 keeps the rows that neighbouring work items read again (stencil halos, model).
+__device__ __forceinline__ float ldn(const float* base, unsigned idx) {
+  return __builtin_nontemporal_load(reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (idx << 2)));
+}
+__device__ __forceinline__ void stn(float* base, unsigned idx, float v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (idx << 2)));
+}
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 // Wave index within the workgroup as a wave-uniform (scalar) value.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
