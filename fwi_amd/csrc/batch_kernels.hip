@@ -118,10 +118,10 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
         const float tpx = p2 + tx;
         const float tfz = r2 + tz;
         const float azv = coef_z(a.g.ta, z, nz, cpml), bzv = coef_z(a.g.tb, z, nz, cpml);
-        const float zxn = bxv * ldu(zx, o) + axv * tpx;
-        const float zzn = bzv * ldu(zz, o) + azv * tfz;
-        stu(zx, o, zxn);
-        stu(zz, o, zzn);
+        const float zxn = bxv * ldn(zx, o) + axv * tpx;
+        const float zzn = bzv * ldn(zz, o) + azv * tfz;
+        stn(zx, o, zxn);
+        stn(zz, o, zzn);
         atp = tx + zxn;
         atf = tz + zzn;
       }
@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
     const float u3o = ldu(vn, mo) * mq[0];
     const float u1h = ldu(vh, hmo) * ldu(L, ho);
     const float u3h = ldu(vn, hmo) * ldu(M, ho);
-    const float lold = ldu(Lo, o), mold = ldu(Mo, o);
+    const float lold = ldn(Lo, o), mold = ldn(Mo, o);
     float u1[9], u3[9];
     xrow9(buf, lane, u1o, u1h, u1);
     xrow9(buf, lane, u3o, u3h, u3);
@@ -314,16 +314,16 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
         ln += r;
         mn += r;
       }
-      stu(Lo, o, ln);
-      stu(Mo, o, mn);
+      stn(Lo, o, ln);
+      stn(Mo, o, mn);
       if (a.qa) {
-        const float p2 = ldu(a.qa + sd.off, o), r2 = ldu(a.qb + sd.off, o);
+        const float p2 = ldn(a.qa + sd.off, o), r2 = ldn(a.qb + sd.off, o);
         float* gcx = a.gcx + sd.gslot;
         float* gcz = a.gcz + sd.gslot;
         float* gdx = a.gdx + sd.gslot;
-        stu(gcx, o, ldu(gcx, o) + a.dt * p2 * ln);
-        stu(gcz, o, ldu(gcz, o) + a.dt * r2 * (ln + mn));
-        stu(gdx, o, ldu(gdx, o) + a.dt * p2 * mn);
+        stn(gcx, o, ldn(gcx, o) + a.dt * p2 * ln);
+        stn(gcz, o, ldn(gcz, o) + a.dt * r2 * (ln + mn));
+        stn(gdx, o, ldn(gdx, o) + a.dt * p2 * mn);
       }
     }
 #pragma unroll
@@ -762,7 +762,8 @@ __global__ void __launch_bounds__(256) k_zero_f32(float* __restrict__ p, long lo
   for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = 0.f;
 }
 __global__ void __launch_bounds__(256) k_copy_f32(float* __restrict__ d, const float* __restrict__ s, long long n) {
-  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = s[i];
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(s + i), d + i);
 }
 
 // Residual res = syn - obs of every shot of a batch and per-(shot, block) FP64 partial sums of res^2
