@@ -181,7 +181,8 @@ def main():
     plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
                 rows_per_item=args.rows, item_order=args.order, arith=args.arith,
                 ckpt_format=args.ckpt_format, use_graph=0 if args.no_graph else 1)
-    log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, workspace {plan.workspace_bytes / 1e9:.1f} GB")
+    log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, {plan.kept_steps} steps kept from the forward, "
+        f"workspace {plan.workspace_bytes / 1e9:.1f} GB")
     t0 = time.perf_counter()
     obs = plan.forward(true_sq)
     torch.cuda.synchronize()
@@ -285,6 +286,7 @@ def main():
             "grad_mode": args.mode,
             "arith": {0: "fp32", 1: "mixed", 2: "fp32-diff", 3: "fp32-increment"}[args.arith],
             "ckpt_interval": plan.ckpt_interval,
+            "kept_steps": plan.kept_steps,
             "ckpt_format": {0: "fp32", 1: "bf16-scaled", 2: "fp16-scaled"}[args.ckpt_format],
             "workspace_gb": round(plan.workspace_bytes / 1e9, 2),
             "batch_shots": plan.batch_size,

@@ -65,6 +65,10 @@ struct fwi_plan {
   std::vector<float> wavelet, ta, tb;
   fwi_options opt{};
   int K = 1, J = 1;  // checkpoint interval and number of segments
+  int S = 1;         // trailing segments whose imaging terms the forward keeps (no checkpoint, no recomputation)
+  int sfirst() const { return (J - S) * K; }                       // first step kept from the forward
+  int qslots() const { return std::max(nt - sfirst(), K); }         // imaging-term slots (2 fields each)
+  int nckpt() const { return std::max(J - S - 1, 0); }              // checkpoints 1..J-S-1
   int tz = 32;       // rows per work item (forward kernels)
   int order = 0;     // work-item order (build_items)
   int tz_adj = 32;   // rows per work item and order of the adjoint kernels
@@ -129,14 +133,22 @@ struct fwi_plan {
   float* fwd(int k) const { return d_work + (size_t)k * cap_cell; }
   float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..11
   float* sgrad(int k) const { return d_work + (size_t)(kFwdF + kAdjF + k) * cap_cell; }      // 0..2
-  float* qbuf() const { return d_work + (size_t)kFixedF * cap_cell; }                      // K * 2 * cap_cell
+  float* qbuf() const { return d_work + (size_t)kFixedF * cap_cell; }                      // qslots * 2 * cap_cell
   // checkpoints j = 1..J-1: 4 cell fields then 4 strip fields, FP32 or 16-bit (ck_esize bytes per value)
   size_t ck_bytes() const { return (4 * (size_t)cap_cell + 4 * (size_t)cap_strip) * ck_esize; }
   char* ckpt(int j) const {
-    return (char*)(d_work + (size_t)(kFixedF + 2 * (size_t)K) * cap_cell) + (size_t)(j - 1) * ck_bytes();
+    return (char*)(d_work + (size_t)(kFixedF + 2 * (size_t)qslots()) * cap_cell) + (size_t)(j - 1) * ck_bytes();
   }
   float* gathers() const {
-    return (float*)((char*)(d_work + (size_t)(kFixedF + 2 * (size_t)K) * cap_cell) + (size_t)(J - 1) * ck_bytes());
+    return (float*)((char*)(d_work + (size_t)(kFixedF + 2 * (size_t)qslots()) * cap_cell) + (size_t)nckpt() * ck_bytes());
+  }
+  size_t workspace_floats(int S_) const {
+    const int keep = S;
+    const_cast<fwi_plan*>(this)->S = S_;
+    const size_t n = (size_t)(kFixedF + 2 * (size_t)qslots()) * cap_cell + (ck_bytes() * (size_t)nckpt() + 3) / 4 +
+                     2 * (size_t)cap_gather;
+    const_cast<fwi_plan*>(this)->S = keep;
+    return n;
   }
   float* resid() const { return gathers() + cap_gather; }
   bool inc() const { return opt.arith == FWI_ARITH_FP32I; }
@@ -610,18 +622,18 @@ struct fwi_plan {
     float* gather = syn_out ? syn_out + b.gather_off : gathers();
     const size_t slot = 2 * (size_t)cap_cell;
     if (int r = zero_fwd(b, st)) return r;
-    // forward with checkpoints; the last segment keeps its imaging terms
-    const int slast = (J - 1) * K;
+    // forward with checkpoints before segments 1..J-S-1; the last S segments keep their imaging terms
+    const int s0 = sfirst();
     for (int n = 0; n < nt; ++n) {
-      if (n > 0 && n % K == 0)
+      if (n > 0 && n % K == 0 && n / K <= nckpt())
         if (int r = save_ckpt(b, n / K, st)) return r;
       float* qa = nullptr;
       float* qb = nullptr;
-      if (n >= slast) {
-        qa = qbuf() + (size_t)(n - slast) * slot;
+      if (n >= s0) {
+        qa = qbuf() + (size_t)(n - s0) * slot;
         qb = qa + cap_cell;
       }
-      if (int r = fwd_step(b, m, n, n >= slast ? qm : 0, qa, qb, gather, st)) return r;
+      if (int r = fwd_step(b, m, n, n >= s0 ? qm : 0, qa, qb, gather, st)) return r;
     }
     if (adj_src) {
       if (int r = copy(resid(), adj_src + b.gather_off, b.ngather, st)) return r;
@@ -631,12 +643,17 @@ struct fwi_plan {
     // adjoint
     if (int r = zero(adj(0), kAdjF * (size_t)cap_cell, st)) return r;
     if (int r = zero(sgrad(0), kGradF * (size_t)cap_cell, st)) return r;
-    int seg = J - 1;
+    // segments below s0 are recomputed from their checkpoints into slots 0..K-1, whose kept terms the
+    // adjoint has consumed by then (it walks down from nt)
+    int seg = J - S;
     for (int k = nt; k >= 1; --k) {
       const int n = k - lag;
       const float* qa = nullptr;
       const float* qb = nullptr;
-      if (n >= 0) {
+      if (n >= s0) {
+        qa = qbuf() + (size_t)(n - s0) * slot;
+        qb = qa + cap_cell;
+      } else if (n >= 0) {
         while (n < seg * K) {
           --seg;
           if (seg > 0) {
@@ -894,8 +911,18 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     p->cap_strip = std::max(p->cap_strip, b.nstrip);
     p->cap_gather = std::max(p->cap_gather, b.ngather);
   }
-  const size_t nfloat = (size_t)(fwi_plan::kFixedF + 2 * (size_t)p->K) * p->cap_cell +
-                        (p->ck_bytes() * (size_t)(p->J - 1) + 3) / 4 + 2 * p->cap_gather;
+  // trailing segments kept from the forward: as many as the budget allows with FP32 checkpoints (recomputation
+  // reproduces them exactly, so the result does not change), one with 16-bit checkpoints (whose recomputed
+  // segments start from rounded states), or exactly store_segments
+  if (p->opt.store_segments > 0) {
+    p->S = std::min(p->opt.store_segments, p->J);
+  } else if (p->opt.ckpt_format == FWI_CKPT_FP32) {
+    p->S = 1;
+    while (p->S < p->J && 4.0 * (double)p->workspace_floats(p->S + 1) <= budget) ++p->S;
+  } else {
+    p->S = 1;
+  }
+  const size_t nfloat = p->workspace_floats(p->S);
   p->work_bytes = nfloat * sizeof(float);
   int rc = 0;
   auto ck = [&](hipError_t e, const char* w) {
@@ -1094,6 +1121,7 @@ long long fwi_plan_cells(const fwi_plan* p, long long* strip_cells) {
 }
 
 int fwi_plan_ckpt_interval(const fwi_plan* p) { return p->K; }
+int fwi_plan_kept_steps(const fwi_plan* p) { return p->nt - p->sfirst(); }
 int fwi_plan_rows_per_item(const fwi_plan* p) { return p->tz; }
 int fwi_plan_item_order(const fwi_plan* p) { return p->order; }
 int fwi_plan_graph_stats(const fwi_plan* p, long long* captures, long long* replays) {

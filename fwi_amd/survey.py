@@ -64,7 +64,8 @@ class Plan:
     def __init__(self, nx: int, nz: int, nt: int, dh: float, dt: float, cpml: int, shots: Sequence[Shot],
                  wavelet: np.ndarray, cpml_ab, grad_mode: int = GRAD_EXACT, arith: int = ARITH_FP32I,
                  ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0,
-                 item_order: int = -1, grad_group: int = 0, ckpt_format: int = 0, use_graph: int = 1):
+                 item_order: int = -1, grad_group: int = 0, ckpt_format: int = 0, use_graph: int = 1,
+                 store_segments: int = 0):
         L = _lib.load()
         self.nx, self.nz, self.nt, self.dh, self.dt, self.cpml = nx, nz, nt, dh, dt, cpml
         self.shots = list(shots)
@@ -90,6 +91,7 @@ class Plan:
         opt.grad_group = grad_group
         opt.ckpt_format = ckpt_format
         opt.use_graph = use_graph
+        opt.store_segments = store_segments
         h = ctypes.c_void_p()
         _lib.check(L.fwi_plan_create(ctypes.byref(sv), ctypes.byref(opt), ctypes.byref(h)), "fwi_plan_create")
         self._h = h
@@ -121,6 +123,11 @@ class Plan:
     @property
     def ckpt_interval(self) -> int:
         return int(_lib.load().fwi_plan_ckpt_interval(self._h))
+
+    @property
+    def kept_steps(self) -> int:
+        """Steps whose imaging terms the forward keeps (not recomputed)."""
+        return int(_lib.load().fwi_plan_kept_steps(self._h))
 
     @property
     def rows_per_item(self) -> int:

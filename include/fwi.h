@@ -72,6 +72,9 @@ typedef struct fwi_options {
                           FP32 gradient slot, their per-step imaging terms added in shot order before one
                           read-modify-write (0 = 4, the default; 1 = one slot per shot) */
   int ckpt_format;     /* FWI_CKPT_FP32 (default), FWI_CKPT_BF16 or FWI_CKPT_FP16 */
+  int store_segments;  /* trailing checkpoint segments whose imaging terms the forward pass keeps (they are
+                          neither checkpointed nor recomputed): 0 = as many as mem_budget allows with FP32
+                          checkpoints (the result is the same for any count), 1 with 16-bit checkpoints */
   int use_graph;       /* 1 (default): each batch's device work of fwi_forward / fwi_misfit / fwi_gradient /
                           fwi_gradient_src is captured once as a HIP graph per set of caller buffers and
                           replayed on later calls with the same buffers; 0: launched eagerly */
@@ -172,6 +175,8 @@ long long fwi_plan_cells(const fwi_plan* plan, long long* strip_cells);
  * forward kernels' and, through fwi_plan_adj_tiling, the adjoint kernels' (grouped = 1 when the first
  * batch accumulates gradients per group of shots). */
 int fwi_plan_ckpt_interval(const fwi_plan* plan);
+/* Time steps whose imaging terms the forward pass keeps (the rest are recomputed from checkpoints). */
+int fwi_plan_kept_steps(const fwi_plan* plan);
 int fwi_plan_rows_per_item(const fwi_plan* plan);
 int fwi_plan_item_order(const fwi_plan* plan);
 int fwi_plan_adj_tiling(const fwi_plan* plan, int* rows_per_item, int* item_order, int* grouped);

@@ -90,12 +90,17 @@ def test_checkpoint_interval_and_batching_invariance(torch):
     obs = torch.from_numpy(_cat(oracle_gathers(w, "f32", sq_true, tab, src))).cuda()
     model = _dev_model(torch, sq_init)
     ref, phis = {}, {}
-    cases = [(0, 0, 0), (1, 0, 0), (7, 0, 0), (200, 0, 0), (13, 1, 0), (0, 2, 0), (0, 0, 1), (7, 0, 2), (0, 2, 1)]
-    for K, mb, grp in cases:
+    # (K, max_batch, grad_group, store_segments); store_segments 1 recomputes every segment but the last,
+    # 0 keeps as many as memory allows (here all of them)
+    cases = [(0, 0, 0, 1), (1, 0, 0, 1), (7, 0, 0, 1), (200, 0, 0, 1), (13, 1, 0, 1), (0, 2, 0, 1), (0, 0, 1, 1),
+             (7, 0, 2, 1), (0, 2, 1, 1), (7, 0, 0, 0), (7, 0, 0, 3), (13, 1, 0, 2)]
+    for K, mb, grp, keep in cases:
         for exact in (1, 0):
             for arith in (3, 0):
                 plan = _plan(w, tab, src, ckpt_interval=K, max_batch=mb, grad_mode=exact, arith=arith,
-                             grad_group=grp)
+                             grad_group=grp, store_segments=keep)
+                if keep == 1 and K in (7, 13):
+                    assert plan.kept_steps < w.nt // 2
                 phi, G, _ = plan.gradient(model, obs)
                 torch.cuda.synchronize()
                 G = G.cpu().numpy()
@@ -104,11 +109,11 @@ def test_checkpoint_interval_and_batching_invariance(torch):
                 groups = tuple(map(tuple, gradient_groups(w.shots, (grp or 4) if exact else 1, mb)))
                 key = (exact, arith, groups)
                 phis.setdefault((exact, arith), phi)
-                assert phi == phis[(exact, arith)], (K, mb, grp, exact, arith)
+                assert phi == phis[(exact, arith)], (K, mb, grp, keep, exact, arith)
                 if key not in ref:
                     ref[key] = G
                 else:
-                    assert np.array_equal(G, ref[key]), (K, mb, grp, exact, arith)
+                    assert np.array_equal(G, ref[key]), (K, mb, grp, keep, exact, arith)
     # the exact mode met each of the three group structures at least twice
     assert sum(1 for k in ref if k[0] == 1 and k[1] == 3) == 3
 
@@ -158,8 +163,9 @@ def test_16bit_checkpoints_match_oracle(torch, fmt):
     obs_d = torch.from_numpy(_cat(obs)).cuda()
     model = _dev_model(torch, sq_init)
     plan = _plan(w, tab, src, ckpt_interval=16, ckpt_format=fmt)
+    assert plan.kept_steps == 16
     phi, G, _ = plan.gradient(model, obs_d)
-    plan32 = _plan(w, tab, src, ckpt_interval=16)
+    plan32 = _plan(w, tab, src, ckpt_interval=16, store_segments=1)
     _, G32, _ = plan32.gradient(model, obs_d)
     torch.cuda.synchronize()
     G, G32 = G.cpu().numpy(), G32.cpu().numpy()
@@ -240,8 +246,8 @@ def test_adjoint128_checkpoint28_fp32(torch):
     tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
     obs = oracle_gathers(w, "f32i", sq_true, tab, src)
     phi_ref, G_ref, _ = oracle_gradient(w, "f32i", True, sq_init, tab, src, obs)
-    plan = _plan(w, tab, src, ckpt_interval=28)
-    assert plan.ckpt_interval == 28
+    plan = _plan(w, tab, src, ckpt_interval=28, store_segments=1)
+    assert plan.ckpt_interval == 28 and plan.kept_steps == 800 - 28 * 28
     obs_d = torch.from_numpy(_cat(obs)).cuda()
     phi, G, _ = plan.gradient(_dev_model(torch, sq_init), obs_d)
     torch.cuda.synchronize()

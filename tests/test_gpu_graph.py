@@ -23,7 +23,7 @@ def test_graph_replays_match_eager():
     tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
     obs = torch.from_numpy(np.concatenate([o.reshape(-1) for o in oracle_gathers(w, "f32i", sq_true, tab, src)])).cuda()
     mk = lambda g: Plan(w.nx, w.nz, w.nt, w.dh, w.dt, w.cpml, w.shots, src, tab, ckpt_interval=16, max_batch=2,
-                        use_graph=g)
+                        use_graph=g, store_segments=1)
     pg, pe = mk(1), mk(0)
     model = [torch.from_numpy(np.ascontiguousarray(x, np.float32)).cuda() for x in sq_init]
     out = torch.empty((3, w.nz, w.nx), dtype=torch.float64, device="cuda")
