@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--ckpt", type=int, default=0)
     ap.add_argument("--ckpt-format", type=int, default=0, help="checkpoints: 0 FP32, 1 BF16, 2 FP16 (scaled)")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying captured graphs")
+    ap.add_argument("--keep", type=int, default=0, help="trailing segments kept from the forward (0 = memory allows)")
+    ap.add_argument("--grad-group", type=int, default=0, help="shots per gradient group (0 = 4)")
     ap.add_argument("--arith", type=int, default=3,
                     help="0 FP32 two-level, 1 mixed (reference), 2 FP32 difference stencil, 3 FP32 increment form")
     ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")
@@ -180,7 +182,8 @@ def main():
     grad_mode = 1 if args.mode == "exact" else 0
     plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
                 rows_per_item=args.rows, item_order=args.order, arith=args.arith,
-                ckpt_format=args.ckpt_format, use_graph=0 if args.no_graph else 1)
+                ckpt_format=args.ckpt_format, use_graph=0 if args.no_graph else 1, store_segments=args.keep,
+                grad_group=args.grad_group)
     log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, {plan.kept_steps} steps kept from the forward, "
         f"workspace {plan.workspace_bytes / 1e9:.1f} GB")
     t0 = time.perf_counter()

@@ -8,17 +8,19 @@ namespace fwi {
 // One shot of a batch as the kernels see it.
 struct ShotDev {
   int nx;         // window width (cells)
+  int ps;         // row pitch of the shot's fields: nx rounded up to whole 128-byte lines (32 floats)
   int x0;         // model column of window column 0
   int isx, isz;   // source cell
   int irz, ir0;   // receiver row and first receiver column
   int nrec;       // receivers
   int nstrip;     // C-PML cells of the window (chi_x U chi_z)
-  long long off;  // offset of the shot's nx*nz fields in the batch field arrays (cells)
+  long long off;  // offset of the shot's ps*nz fields in the batch field arrays (cells)
   long long soff; // offset of the shot's strip cells in packed strip arrays
   long long goff; // offset of the shot's gather [nt][nrec] in the batch gather arrays
-  long long gslot;  // offset of the gradient slot the shot accumulates into (its group leader's window)
+  long long gslot;  // offset of the FP32 gradient slot of the shot's gradient group
+  int gx0, gnx;     // the slot covers model columns [gx0, gx0 + gnx) (the union of the group's windows)
+  int gps;          // row pitch of the slot (gnx rounded up to 32)
   int lead;         // 1 if the shot leads its gradient group (its slot is summed into the model gradient)
-  int pad_;
 };
 
 // C-PML cells of a window: the side strips Omega & (ix < cpml | ix > nx-cpml-1) for rows 4..nz-5, then

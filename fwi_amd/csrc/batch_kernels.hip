@@ -35,16 +35,16 @@ __global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __r
   const ShotDev sd = a.g.shots[blk.x];
   const int k = blk.y + threadIdx.x;
   if (k >= sd.nstrip) return;
-  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
+  const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml;
   int ix, iz;
   strip_cell(k, nx, nz, cpml, ix, iz);
-  const long i = sd.off + ix + (long)nx * iz;
+  const long i = sd.off + ix + (long)ps * iz;
   if (ix < cpml || ix > nx - cpml - 1) {
     const float t = d1_at<S>(Pc, i, 1, a.g.dh);
     psx[i] = coef_x(a.g.tb, ix, nx, cpml) * psx[i] + coef_x(a.g.ta, ix, nx, cpml) * t;
   }
   if (iz > nz - cpml - 1) {
-    const float t = d1_at<S>(Fc, i, nx, a.g.dh);
+    const float t = d1_at<S>(Fc, i, ps, a.g.dh);
     psz[i] = coef_z(a.g.tb, iz, nz, cpml) * psz[i] + coef_z(a.g.ta, iz, nz, cpml) * t;
   }
 }
@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
   if (item >= a.nitems) return;
   const int4 itm = a.items[item];
   const ShotDev sd = a.g.shots[itm.x];
-  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
+  const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
   const bool act = ix >= kRim && ix < nx - kRim;
   // every lane loads from a valid column; values of columns outside the shot are never consumed
@@ -91,13 +91,13 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
   const int z0 = itm.z, z1 = itm.w;
   float fq[9];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) fq[k] = ldu(F, (unsigned)(cx + nx * (z0 - 4 + k)));
+  for (int k = 0; k < 8; ++k) fq[k] = ldu(F, (unsigned)(cx + ps * (z0 - 4 + k)));
   for (int z = z0; z < z1; ++z) {
-    const unsigned o = (unsigned)(cx + nx * z);
+    const unsigned o = (unsigned)(cx + ps * z);
     const unsigned mo = (unsigned)(cx + nxm * z);
-    fq[8] = ldu(F, o + 4u * nx);
+    fq[8] = ldu(F, o + 4u * ps);
     const float c = ldu(P, o);
-    const float h = ldu(P, (unsigned)(hx + nx * z));
+    const float h = ldu(P, (unsigned)(hx + ps * z));
     // two-level form: P^{n-1}, F^{n-1}; increment form: Q^n, R^n
     const float pold = INC ? ldn(a.Qc + sd.off, o) : ldn(Po, o);
     const float fold = INC ? ldn(a.Rc + sd.off, o) : ldn(Fo, o);
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
       const float r2 = d2s<S>(fq[0], fq[1], fq[2], fq[3], fq[4], fq[5], fq[6], fq[7], fq[8], dh2);
       float atp = 0.f, atf = 0.f;
       if (xstrip || z > nz - cpml - 1) {
-        const unsigned snx = (unsigned)nx;
+        const unsigned snx = (unsigned)ps;
         const float tx = d1s<S>(ldu(psx, o - 4), ldu(psx, o - 3), ldu(psx, o - 2), ldu(psx, o - 1), ldu(psx, o + 1),
                                 ldu(psx, o + 2), ldu(psx, o + 3), ldu(psx, o + 4), dh);
         const float tz = d1s<S>(ldu(psz, o - 4 * snx), ldu(psz, o - 3 * snx), ldu(psz, o - 2 * snx),
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(256) k_adj_ref_strip(StripArgs a, const float*
   const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   int ix, iz;
   strip_cell(k, nx, nz, cpml, ix, iz);
-  const long i = sd.off + ix + (long)nx * iz;
+  const long i = sd.off + ix + (long)sd.ps * iz;
   const long mi = sd.x0 + ix + (long)nxm * iz;
   const float dh = a.g.dh;
   if (ix < cpml || ix > nx - cpml - 1) {
@@ -204,8 +204,8 @@ __global__ void __launch_bounds__(256) k_adj_ref_strip(StripArgs a, const float*
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
       const float v = a.m.vp[mi + (long)(j - 4) * nxm];
-      u2[j] = v * Lc[i + (long)(j - 4) * nx];
-      u4[j] = v * Mc[i + (long)(j - 4) * nx];
+      u2[j] = v * Lc[i + (long)(j - 4) * sd.ps];
+      u4[j] = v * Mc[i + (long)(j - 4) * sd.ps];
     }
     const float p1cz = d1s<S>(u2[0], u2[1], u2[2], u2[3], u2[5], u2[6], u2[7], u2[8], dh);
     const float r1cz = d1s<S>(u4[0], u4[1], u4[2], u4[3], u4[5], u4[6], u4[7], u4[8], dh);
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
   if (item >= a.nitems) return;
   const int4 itm = a.items[item];
   const ShotDev sd = a.g.shots[itm.x];
-  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
+  const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
   const bool act = ix >= kRim && ix < nx - kRim;
   const int cx = clampi(ix, 0, nx - 1);
@@ -249,8 +249,8 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int r = z0 - 4 + k;
-    const float l = ldu(L, (unsigned)(cx + nx * r));
-    const float m = ldu(M, (unsigned)(cx + nx * r));
+    const float l = ldu(L, (unsigned)(cx + ps * r));
+    const float m = ldu(M, (unsigned)(cx + ps * r));
     const float v = ldu(vp, (unsigned)(cx + nxm * r));
     q2[k] = v * l;
     q4[k] = v * m;
@@ -260,12 +260,12 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
     }
   }
   for (int z = z0; z < z1; ++z) {
-    const unsigned o = (unsigned)(cx + nx * z);
+    const unsigned o = (unsigned)(cx + ps * z);
     const unsigned mo = (unsigned)(cx + nxm * z);
-    const unsigned ho = (unsigned)(hx + nx * z), hmo = (unsigned)(hx + nxm * z);
+    const unsigned ho = (unsigned)(hx + ps * z), hmo = (unsigned)(hx + nxm * z);
     {
-      const float l = ldu(L, o + 4u * nx);
-      const float m = ldu(M, o + 4u * nx);
+      const float l = ldu(L, o + 4u * ps);
+      const float m = ldu(M, o + 4u * ps);
       const float v = ldu(vp, mo + 4u * nxm);
       lq[4] = l;
       mq[4] = m;
@@ -290,9 +290,9 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
       if (xstrip || z > nz - cpml - 1) {
         const long g = sd.off + o;
         const float d1 = d1_at<S>(a.s1, g, 1, dh);
-        const float d2 = d1_at<S>(a.s2, g, nx, dh);
+        const float d2 = d1_at<S>(a.s2, g, ps, dh);
         const float d3 = d1_at<S>(a.s3, g, 1, dh);
-        const float d4 = d1_at<S>(a.s4, g, nx, dh);
+        const float d4 = d1_at<S>(a.s4, g, ps, dh);
         const float azv = coef_z(a.g.ta, z, nz, cpml), bzv = coef_z(a.g.tb, z, nz, cpml);
         const float n1 = bxv * a.z1[g] + axv * (d1 + p2cx);
         const float n2 = bzv * a.z2[g] + azv * (d2 + p2cz);
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip1(StripArgs a, float dt2, c
   const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
   int ix, iz;
   strip_cell(k, nx, nz, cpml, ix, iz);
-  const long i = sd.off + ix + (long)nx * iz;
+  const long i = sd.off + ix + (long)sd.ps * iz;
   const long mi = sd.x0 + ix + (long)a.m.nxm * iz;
   const float mP = Lc[i], mF = Mc[i];
   if (ix < cpml || ix > nx - cpml - 1) {
@@ -383,7 +383,7 @@ __device__ __forceinline__ float gbar_x(const AdjExArgs& a, const ShotDev& sd, c
   if (jx < kRim || jx >= nx - kRim || jz < kRim || jz >= nz - kRim) return 0.f;
   const bool gam = jx < cpml || jx > nx - cpml - 1 || jz > nz - cpml - 1;
   if (gamma_only && !gam) return 0.f;
-  const long i = jx + (long)nx * jz;
+  const long i = jx + (long)sd.ps * jz;
   const long mi = sd.x0 + jx + (long)a.m.nxm * jz;
   const float ab = a.dt2 * (a.m.vh[mi] * L[i] + a.m.vn[mi] * M[i]);
   return ab + coef_x(a.g.ta, jx, nx, cpml) * a.ztx[sd.off + i];
@@ -394,7 +394,7 @@ __device__ __forceinline__ float gbar_z(const AdjExArgs& a, const ShotDev& sd, c
   if (jx < kRim || jx >= nx - kRim || jz < kRim || jz >= nz - kRim) return 0.f;
   const bool gam = jx < cpml || jx > nx - cpml - 1 || jz > nz - cpml - 1;
   if (gamma_only && !gam) return 0.f;
-  const long i = jx + (long)nx * jz;
+  const long i = jx + (long)sd.ps * jz;
   const long mi = sd.x0 + jx + (long)a.m.nxm * jz;
   const float bb = a.dt2 * (a.m.vp[mi] * (L[i] + M[i]));
   return bb + coef_z(a.g.ta, jz, nz, cpml) * a.ztz[sd.off + i];
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
   const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
   int ix, iz;
   strip_cell(k, nx, nz, cpml, ix, iz);
-  const long i = sd.off + ix + (long)nx * iz;
+  const long i = sd.off + ix + (long)sd.ps * iz;
   const float* L = a.Lc + sd.off;
   const float* M = a.Mc + sd.off;
   const float dh = a.g.dh;
@@ -434,11 +434,13 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
 
 // Main exact adjoint step with the exact imaging condition
 //   g_hor += dt^2 mP^k a^{k-1}, g_nmo += dt^2 mF^k a^{k-1}, g_po += dt^2 (mP^k + mF^k) b^{k-1}.
-// GROUP: the four waves of a workgroup are up to four shots of one gradient group (identical windows) on
-// the same tile; a wave whose item has shot -(leader+1) is a placeholder that loads nothing.  Per row the
-// waves' contributions meet in LDS and are added in wave order, ((c0 + c1) + c2) + c3, then waves 0, 1, 2
-// read-modify-write g_hor, g_nmo, g_po of the group's slot once: 6 B of gradient traffic per cell and shot
-// instead of 24.  Items are padded to whole workgroups, so every wave reaches every barrier.
+// GROUP: the four waves of a workgroup are up to four shots of one gradient group on the same 64 model
+// columns (each wave at its own window's local offset); a wave whose item has shot -(leader+1) is a
+// placeholder that loads nothing.  Per row the waves' contributions meet in LDS and are added in wave order,
+// ((c0 + c1) + c2) + c3 (zero where a shot's window does not reach), then waves 0, 1, 2 read-modify-write
+// g_hor, g_nmo, g_po of the group's slot, which spans the union of its windows: 6 B of gradient traffic per
+// cell and shot instead of 24 when the windows coincide.  Items are padded to whole workgroups, so every
+// wave reaches every barrier.
 template <class S, bool GROUP = false>
 __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   __shared__ float xbuf[4][72];
@@ -450,7 +452,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const bool ghost = GROUP && itm.x < 0;
   if (ghost) itm.x = -itm.x - 1;
   const ShotDev sd = a.g.shots[itm.x];
-  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
+  const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
   const bool act = ix >= kRim && ix < nx - kRim;
   const int cx = clampi(ix, 0, nx - 1);
@@ -481,7 +483,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int r = z0 - 4 + k;
-      const unsigned ir = (unsigned)(cx + nx * r);
+      const unsigned ir = (unsigned)(cx + ps * r);
       const float l = ldu(L, ir);
       const float m = ldu(M, ir);
       float v = dt2 * (ldu(vp, (unsigned)(cx + nxm * r)) * (l + m));
@@ -494,14 +496,14 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     }
   }
   for (int z = z0; z < z1; ++z) {
-    const unsigned o = (unsigned)(cx + nx * z);
+    const unsigned o = (unsigned)(cx + ps * z);
     float ch = 0.f, cn = 0.f, cp = 0.f;  // this shot's imaging contributions (GROUP)
     if (!ghost) {
     const unsigned mo = (unsigned)(cx + nxm * z);
-    const unsigned ho = (unsigned)(hx + nx * z), hmo = (unsigned)(hx + nxm * z);
+    const unsigned ho = (unsigned)(hx + ps * z), hmo = (unsigned)(hx + nxm * z);
     {
       const int r = z + 4;
-      const unsigned ir = o + 4u * nx;
+      const unsigned ir = o + 4u * ps;
       const float l = ldu(L, ir);
       const float m = ldu(M, ir);
       lq[4] = l;
@@ -532,7 +534,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       }
       if (z >= zc) {
         const float* pt = a.ptz + sd.off;
-        const unsigned snx = (unsigned)nx;
+        const unsigned snx = (unsigned)ps;
         af = af - d1s<S>(ldu(pt, o - 4 * snx), ldu(pt, o - 3 * snx), ldu(pt, o - 2 * snx), ldu(pt, o - snx),
                          ldu(pt, o + snx), ldu(pt, o + 2 * snx), ldu(pt, o + 3 * snx), ldu(pt, o + 4 * snx), dh);
       }
@@ -577,10 +579,12 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       gs[wave][1][lane] = cn;
       gs[wave][2][lane] = cp;
       __syncthreads();
-      if (wave < 3 && act) {
+      const int mc = sd.x0 + ix;  // model column of the lane, the same in every wave of the workgroup
+      if (wave < 3 && mc >= sd.gx0 && mc < sd.gx0 + sd.gnx) {
         const float sum = ((gs[0][wave][lane] + gs[1][wave][lane]) + gs[2][wave][lane]) + gs[3][wave][lane];
         float* g = (wave == 0 ? a.gh : wave == 1 ? a.gn : a.gp) + sd.gslot;
-        stn(g, o, ldn(g, o) + sum);
+        const unsigned go = (unsigned)(mc - sd.gx0 + sd.gps * z);
+        stn(g, go, ldn(g, go) + sum);
       }
     }
   }
@@ -599,7 +603,7 @@ __global__ void __launch_bounds__(256) k_strip_copy(StripArgs a, int dir, float*
   if (k >= sd.nstrip) return;
   int ix, iz;
   strip_cell(k, sd.nx, a.g.nz, a.g.cpml, ix, iz);
-  const long i = sd.off + ix + (long)sd.nx * iz;
+  const long i = sd.off + ix + (long)sd.ps * iz;
   const long j = sd.soff + k;
   if (dir == 0) {
     packed[j] = f0[i];
@@ -663,7 +667,7 @@ __global__ void __launch_bounds__(256) k_ck_absmax_cells(const ShotDev* __restri
                                                          const float* f1, const float* f2, const float* f3,
                                                          unsigned* __restrict__ amax) {
   const ShotDev sd = shots[blockIdx.y];
-  const long long n = (long long)sd.nx * nz;
+  const long long n = (long long)sd.ps * nz;  // padded columns included (they hold zeros)
   float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
   for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const long long c = sd.off + i;
@@ -689,7 +693,7 @@ __global__ void __launch_bounds__(256) k_ck_absmax_strips(StripArgs a, const flo
   if (k < sd.nstrip) {
     int ix, iz;
     strip_cell(k, sd.nx, a.g.nz, a.g.cpml, ix, iz);
-    const long long i = sd.off + ix + (long long)sd.nx * iz;
+    const long long i = sd.off + ix + (long long)sd.ps * iz;
     m0 = fabsf(f0[i]);
     m1 = fabsf(f1[i]);
     m2 = fabsf(f2[i]);
@@ -709,7 +713,7 @@ __global__ void __launch_bounds__(256) k_ck_pack_cells(const ShotDev* __restrict
                                                        unsigned short* __restrict__ packed, long long ncell_total,
                                                        const unsigned* __restrict__ amax) {
   const ShotDev sd = shots[blockIdx.y];
-  const long long n = (long long)sd.nx * nz;
+  const long long n = (long long)sd.ps * nz;  // padded columns included (they hold zeros)
   const unsigned* a = amax + blockIdx.y * 8;
   const int e0 = ck_exponent(a[0]), e1 = ck_exponent(a[1]), e2 = ck_exponent(a[2]), e3 = ck_exponent(a[3]);
   for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
@@ -741,7 +745,7 @@ __global__ void __launch_bounds__(256) k_ck_pack_strips(StripArgs a, int dir, fl
   const int e0 = ck_exponent(am[0]), e1 = ck_exponent(am[1]), e2 = ck_exponent(am[2]), e3 = ck_exponent(am[3]);
   int ix, iz;
   strip_cell(k, sd.nx, a.g.nz, a.g.cpml, ix, iz);
-  const long long i = sd.off + ix + (long long)sd.nx * iz;
+  const long long i = sd.off + ix + (long long)sd.ps * iz;
   const long long j = sd.soff + k;
   if (dir == 0) {
     packed[j] = ck_pack<FMT>(f0[i], e0);
@@ -809,8 +813,9 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__
   if (threadIdx.x == 0) out[0] = total;
 }
 
-// FP64 cross-shot sum of the batch's FP32 gradient slots (one per gradient group, held by the group's
-// leader) into the model gradient (3 fields).  Slots are added in batch order for a deterministic result.
+// FP64 cross-shot sum of the batch's FP32 gradient slots (one per gradient group, over the union of its
+// windows, flagged on the group's leader) into the model gradient (3 fields).  Slots are added in the batch
+// order of their leaders for a deterministic result.
 __global__ void __launch_bounds__(256) k_cross_shot_sum(const ShotDev* __restrict__ shots, int nshots, int nxm, int nz,
                                                         const float* __restrict__ g0, const float* __restrict__ g1,
                                                         const float* __restrict__ g2, double* __restrict__ out,
@@ -821,9 +826,9 @@ __global__ void __launch_bounds__(256) k_cross_shot_sum(const ShotDev* __restric
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   for (int s = 0; s < nshots; ++s) {
     const ShotDev sd = shots[s];
-    const int lx = x - sd.x0;
-    if (!sd.lead || lx < 0 || lx >= sd.nx) continue;
-    const long long i = sd.gslot + lx + (long long)sd.nx * z;
+    const int lx = x - sd.gx0;
+    if (!sd.lead || lx < 0 || lx >= sd.gnx) continue;
+    const long long i = sd.gslot + lx + (long long)sd.gps * z;
     a0 += (double)g0[i];
     a1 += (double)g1[i];
     a2 += (double)g2[i];

@@ -32,16 +32,25 @@ namespace {
 
 constexpr int kResidualBlocks = 64;  // per shot
 constexpr int kCkBlocks = 64;        // per shot, checkpoint packing
+// Row pitch of a shot's fields: whole 128-byte lines, so every row starts on a line and streaming
+// stores cover whole lines.
+inline int row_pitch(int nx) { return (nx + 31) & ~31; }
 
 struct Batch {
   int s0 = 0, count = 0;
   long long ncell = 0, nstrip = 0, ngather = 0, gather_off = 0;
+  long long nreal = 0;          // cells of the windows (ncell counts the padded rows)
   ShotDev* d_shots = nullptr;
   int4* d_items = nullptr;      // forward work items
   int nitems = 0;
   int4* d_items_adj = nullptr;  // adjoint work items (grouped: whole workgroups per group and tile)
   int nitems_adj = 0;
-  std::vector<int2> groups;     // gradient groups (first shot in the batch, count)
+  struct Group {
+    int n = 0;
+    int m[4] = {0, 0, 0, 0};  // members (batch indices) in wave order; m[0] leads
+    int ux0 = 0, unx = 0;     // union of the members' windows (model columns)
+  };
+  std::vector<Group> groups;    // gradient groups
   bool grouped = false;         // some group has more than one shot: the adjoint uses the grouped kernel
   int2* d_sblocks = nullptr;
   int nsblocks = 0;
@@ -293,21 +302,32 @@ struct fwi_plan {
     for (int z0 = kRim; z0 < nz - kRim; z0 += rows) runs.push_back({z0, std::min(z0 + rows, nz - kRim)});
     int maxnx = 0;
     for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
-    auto push = [&](const int2& g, int x0, const std::pair<int, int>& r) {
-      for (int k = 0; k < 4; ++k)
-        items.push_back(make_int4(k < g.y ? g.x + k : -(g.x + 1), x0, r.first, r.second));
+    // one workgroup: the group's members on the 64 model columns from bm, each at its window's offset
+    auto push = [&](const Batch::Group& g, int bm, const std::pair<int, int>& r) {
+      for (int k = 0; k < 4; ++k) {
+        bool member = k < g.n;
+        if (member) {
+          const ShotDev& d = b.h_shots[g.m[k]];
+          member = bm < d.x0 + d.nx && bm + 64 > d.x0;
+        }
+        const int s = member ? g.m[k] : g.m[0];
+        items.push_back(make_int4(member ? s : -(s + 1), bm - b.h_shots[s].x0, r.first, r.second));
+      }
     };
+    int maxunx = 0;
+    for (auto& g : b.groups) maxunx = std::max(maxunx, g.unx);
     if (order == 2) {
-      for (int x0 = 0; x0 < maxnx; x0 += 64)
+      for (int x = 0; x < maxunx; x += 64)
         for (auto& g : b.groups)
-          if (x0 < b.h_shots[g.x].nx)
-            for (auto& r : runs) push(g, x0, r);
+          if (x < g.unx)
+            for (auto& r : runs) push(g, g.ux0 + x, r);
     } else {
       for (auto& r : runs)
-        for (int x0 = 0; x0 < maxnx; x0 += 64)
+        for (int x = 0; x < maxunx; x += 64)
           for (auto& g : b.groups)
-            if (x0 < b.h_shots[g.x].nx) push(g, x0, r);
+            if (x < g.unx) push(g, g.ux0 + x, r);
     }
+    (void)maxnx;
     return items;
   }
 
@@ -869,7 +889,7 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
   const int ns = s->nshots;
   std::vector<long long> cells(ns), strips(ns), gath(ns);
   for (int i = 0; i < ns; ++i) {
-    cells[i] = (long long)p->shots[i].nx * p->nz;
+    cells[i] = (long long)row_pitch(p->shots[i].nx) * p->nz;
     strips[i] = strip_count(p->shots[i].nx, p->nz, p->cpml);
     gath[i] = (long long)nt * p->shots[i].nrec;
   }
@@ -951,6 +971,7 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
       const fwi_shot& h = p->shots[b.s0 + k];
       ShotDev d;
       d.nx = h.nx;
+      d.ps = row_pitch(h.nx);
       d.x0 = h.x0;
       d.isx = h.isx;
       d.isz = h.isz;
@@ -962,29 +983,52 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
       d.soff = soff;
       d.goff = goff;
       d.gslot = off;
+      d.gx0 = h.x0;
+      d.gnx = h.nx;
+      d.gps = d.ps;
       d.lead = 1;
-      d.pad_ = 0;
-      off += (long long)h.nx * p->nz;
+      off += (long long)d.ps * p->nz;
+      b.nreal += (long long)h.nx * p->nz;
       soff += d.nstrip;
       goff += (long long)nt * h.nrec;
       b.h_shots.push_back(d);
       for (int k0 = 0; k0 < d.nstrip; k0 += 256) b.h_sblocks.push_back(make_int2(k, k0));
     }
     b.nsblocks = (int)b.h_sblocks.size();
-    // gradient groups: consecutive shots with identical windows, up to grad_group (exact mode only)
+    // gradient groups (exact mode): the batch's shots ordered by window (x0, width, then batch order) in runs
+    // of up to grad_group; each group's FP32 slot spans the union of its windows, slots back to back (their
+    // total is at most the batch's cells)
     const int gmax = p->opt.grad_mode == FWI_GRAD_EXACT ? (p->opt.grad_group == 0 ? 4 : p->opt.grad_group) : 1;
-    for (int k = 0; k < b.count;) {
-      int n = 1;
-      while (n < gmax && k + n < b.count && b.h_shots[k + n].nx == b.h_shots[k].nx &&
-             b.h_shots[k + n].x0 == b.h_shots[k].x0)
-        ++n;
-      b.groups.push_back(make_int2(k, n));
-      for (int j = 0; j < n; ++j) {
-        b.h_shots[k + j].gslot = b.h_shots[k].off;
-        b.h_shots[k + j].lead = j == 0;
+    std::vector<int> ord(b.count);
+    for (int k = 0; k < b.count; ++k) ord[k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) {
+      const ShotDev &a = b.h_shots[u], &c = b.h_shots[v];
+      return a.x0 != c.x0 ? a.x0 < c.x0 : a.nx < c.nx;
+    });
+    long long slot = 0;
+    for (int k = 0; k < b.count; k += gmax) {
+      Batch::Group g;
+      g.n = std::min(gmax, b.count - k);
+      int ux1 = 0;
+      g.ux0 = 1 << 30;
+      for (int j = 0; j < g.n; ++j) {
+        g.m[j] = ord[k + j];
+        const ShotDev& d = b.h_shots[g.m[j]];
+        g.ux0 = std::min(g.ux0, d.x0);
+        ux1 = std::max(ux1, d.x0 + d.nx);
       }
-      if (n > 1) b.grouped = true;
-      k += n;
+      g.unx = ux1 - g.ux0;
+      for (int j = 0; j < g.n; ++j) {
+        ShotDev& d = b.h_shots[g.m[j]];
+        d.gslot = slot;
+        d.gx0 = g.ux0;
+        d.gnx = g.unx;
+        d.gps = row_pitch(g.unx);
+        d.lead = j == 0;
+      }
+      slot += (long long)row_pitch(g.unx) * p->nz;
+      if (g.n > 1) b.grouped = true;
+      b.groups.push_back(g);
     }
     ck(hipMalloc(&b.d_shots, b.h_shots.size() * sizeof(ShotDev)), "hipMalloc(shots)");
     if (rc == 0)
@@ -1113,7 +1157,7 @@ int fwi_plan_kernel_time(fwi_plan* p, int kind, double* total_ms, long long* cou
 long long fwi_plan_cells(const fwi_plan* p, long long* strip_cells) {
   long long c = 0, s = 0;
   for (auto& b : p->batches) {
-    c += b.ncell;
+    c += b.nreal;
     s += b.nstrip;
   }
   if (strip_cells) *strip_cells = s;

@@ -1005,88 +1005,112 @@ void forward_shot_ck(const T* vh, const T* vp, const T* vn, const Tables<T>& tb,
   }
 }
 
-// Exact gradient of a group of shots that share one window (the batched path's gradient groups): every
-// shot runs its own forward and exact adjoint (shot_gradient, adjoint_exact_shot); per step and cell the
-// shots' imaging terms c_s are added in shot order, with zeros for absent members of a group of fewer than
-// four, before one update of the shared accumulators: g += ((c_0 + c_1) + c_2) + c_3.  With one shot this
-// is adjoint_exact_shot's update.  geo: (isx, isz, irz, ir0, nrec) per shot; obs/syn: shot-major gathers.
+// Exact gradient of a gradient group of the batched path: up to four shots, each with its own window
+// (x0, nx) of the full model.  Every shot runs its own forward and exact adjoint (shot_gradient,
+// adjoint_exact_shot); per step and cell of the union of the windows the shots' imaging terms c_s (zero where
+// a window's interior does not reach) are added in member order, ((c_0 + c_1) + c_2) + c_3, before one update
+// of the group's accumulators g (union width x nz).  With one shot this is adjoint_exact_shot's update.
+// geo: (x0, nx, isx, isz, irz, ir0, nrec) per member; obs/syn: member-major gathers.
 template <class T, class S>
-void group_gradient(const T* vh, const T* vp, const T* vn, const T* tab_a, const T* tab_b, const T* src, int nx,
+void group_gradient(const T* vh_m, const T* vp_m, const T* vn_m, const T* tab_a, const T* tab_b, const T* src, int nxm,
                     int nz, int nt, int cpml, double dh_, double dt_, int ns, const int* geo, const T* obs, T* syn,
                     T* g_hor, T* g_po, T* g_nmo, double* misfits, int K = 0, int ckfmt = 0) {
-  Geo g{nx, nz, cpml};
-  Tables<T> tb(tab_a, tab_b, cpml, nx, nz);
   Scal sc{dh_, dt_};
-  const long n = (long)nx * nz;
   const T dh = (T)dh_, dt = (T)dt_, dt2 = dt * dt;
-  std::vector<ShotGeo> sg(ns);
-  std::vector<std::vector<T>> q(ns), res(ns);
-  std::vector<long> goff(ns + 1, 0);
+  struct Member {
+    int x0, nx;
+    ShotGeo sg;
+    long n;
+    std::vector<T> vh, vp, vn, q, res;
+    Tables<T>* tb;
+    AdjOp<T, S>* op;
+    Fields<T>* f;
+    T *muP, *muF, *shP, *shF, *sx, *zx, *sz, *zz, *nP, *nF, *oP, *oF;
+  };
+  std::vector<Member> M((size_t)ns);
+  int ux0 = 1 << 30, ux1 = 0;
+  long goff = 0;
   for (int s = 0; s < ns; ++s) {
-    sg[s] = ShotGeo{geo[5 * s], geo[5 * s + 1], geo[5 * s + 2], geo[5 * s + 3], geo[5 * s + 4]};
-    goff[s + 1] = goff[s] + (long)nt * sg[s].nrec;
-  }
-  for (int s = 0; s < ns; ++s) {
-    T* sy = syn + goff[s];
+    Member& m = M[(size_t)s];
+    const int* gg = geo + 7 * s;
+    m.x0 = gg[0];
+    m.nx = gg[1];
+    m.sg = ShotGeo{gg[2], gg[3], gg[4], gg[5], gg[6]};
+    m.n = (long)m.nx * nz;
+    ux0 = std::min(ux0, m.x0);
+    ux1 = std::max(ux1, m.x0 + m.nx);
+    for (auto* v : {&m.vh, &m.vp, &m.vn}) v->resize((size_t)m.n);
+    for (int iz = 0; iz < nz; ++iz)
+      for (int ix = 0; ix < m.nx; ++ix) {
+        const long i = ix + (long)m.nx * iz, mi = m.x0 + ix + (long)nxm * iz;
+        m.vh[(size_t)i] = vh_m[mi];
+        m.vp[(size_t)i] = vp_m[mi];
+        m.vn[(size_t)i] = vn_m[mi];
+      }
+    Geo g{m.nx, nz, cpml};
+    m.tb = new Tables<T>(tab_a, tab_b, cpml, m.nx, nz);
+    T* sy = syn + goff;
     if (ckfmt) {
       if constexpr (std::is_same<S, F32I>::value)
-        forward_shot_ck<T, S>(vh, vp, vn, tb, src, g, sg[s], nt, sc, sy, &q[s], K, ckfmt);
+        forward_shot_ck<T, S>(m.vh.data(), m.vp.data(), m.vn.data(), *m.tb, src, g, m.sg, nt, sc, sy, &m.q, K, ckfmt);
     } else {
-      forward_shot<T, S>(vh, vp, vn, tb, src, g, sg[s], nt, sc, sy, 1, &q[s]);
+      forward_shot<T, S>(m.vh.data(), m.vp.data(), m.vn.data(), *m.tb, src, g, m.sg, nt, sc, sy, 1, &m.q);
     }
-    res[s].resize((size_t)nt * sg[s].nrec);
+    m.res.resize((size_t)nt * m.sg.nrec);
     double phi = 0;
-    for (size_t i = 0; i < res[s].size(); ++i) {
-      res[s][i] = sy[i] - obs[goff[s] + i];
-      phi += 0.5 * (double)res[s][i] * (double)res[s][i];
+    for (size_t i = 0; i < m.res.size(); ++i) {
+      m.res[i] = sy[i] - obs[goff + (long)i];
+      phi += 0.5 * (double)m.res[i] * (double)m.res[i];
     }
     misfits[s] = phi;
+    goff += (long)nt * m.sg.nrec;
+    m.op = new AdjOp<T, S>(g, *m.tb, m.vh.data(), m.vp.data(), m.vn.data(), dh, dt);
+    m.f = new Fields<T>(m.n, 12);
+    Fields<T>& F = *m.f;
+    m.muP = F[0]; m.muF = F[1]; m.shP = F[2]; m.shF = F[3];
+    m.sx = F[4]; m.zx = F[5]; m.sz = F[6]; m.zz = F[7];
+    m.nP = F[8]; m.nF = F[9]; m.oP = F[10]; m.oF = F[11];
   }
-  std::vector<AdjOp<T, S>*> op(ns);
-  std::vector<Fields<T>*> f(ns);
-  struct Ptr { T *muP, *muF, *shP, *shF, *sx, *zx, *sz, *zz, *nP, *nF, *oP, *oF; };
-  std::vector<Ptr> pt(ns);
-  for (int s = 0; s < ns; ++s) {
-    op[s] = new AdjOp<T, S>(g, tb, vh, vp, vn, dh, dt);
-    f[s] = new Fields<T>(n, 12);
-    Fields<T>& F = *f[s];
-    pt[s] = Ptr{F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8], F[9], F[10], F[11]};
-  }
+  const int unx = ux1 - ux0;
   for (int k = nt; k >= 1; --k) {
-    for (int s = 0; s < ns; ++s) {
-      Ptr& p = pt[s];
-      op[s]->apply(p.muP, p.muF, p.shP, p.shF, p.sx, p.zx, p.sz, p.zz, p.nP, p.nF, p.oP, p.oF);
-      for (int r = 0; r < sg[s].nrec; ++r) {
-        long i = sg[s].ir0 + r + (long)nx * sg[s].irz;
-        T v = res[s][(size_t)(k - 1) * sg[s].nrec + r];
-        p.nP[i] += v;
-        p.nF[i] += v;
+    for (auto& m : M) {
+      m.op->apply(m.muP, m.muF, m.shP, m.shF, m.sx, m.zx, m.sz, m.zz, m.nP, m.nF, m.oP, m.oF);
+      for (int r = 0; r < m.sg.nrec; ++r) {
+        long i = m.sg.ir0 + r + (long)m.nx * m.sg.irz;
+        T v = m.res[(size_t)(k - 1) * m.sg.nrec + r];
+        m.nP[i] += v;
+        m.nF[i] += v;
       }
-      std::swap(p.muP, p.nP);
-      std::swap(p.muF, p.nF);
-      std::swap(p.shP, p.oP);
-      std::swap(p.shF, p.oF);
+      std::swap(m.muP, m.nP);
+      std::swap(m.muF, m.nF);
+      std::swap(m.shP, m.oP);
+      std::swap(m.shF, m.oF);
     }
 #pragma omp parallel for
     for (int iz = kRim; iz < nz - kRim; ++iz)
-      for (int ix = kRim; ix < nx - kRim; ++ix) {
-        long i = ix + (long)nx * iz;
+      for (int u = 0; u < unx; ++u) {
         T ch[4] = {0, 0, 0, 0}, cn[4] = {0, 0, 0, 0}, cp[4] = {0, 0, 0, 0};
         for (int s = 0; s < ns && s < 4; ++s) {
-          const T qa = q[s][(size_t)(k - 1) * 2 * n + i], qb = q[s][(size_t)(k - 1) * 2 * n + n + i];
-          const T lp = pt[s].muP[i], lf = pt[s].muF[i];
+          const Member& m = M[(size_t)s];
+          const int ix = ux0 + u - m.x0;
+          if (ix < kRim || ix >= m.nx - kRim) continue;
+          const long i = ix + (long)m.nx * iz;
+          const T qa = m.q[(size_t)(k - 1) * 2 * m.n + i], qb = m.q[(size_t)(k - 1) * 2 * m.n + m.n + i];
+          const T lp = m.muP[i], lf = m.muF[i];
           ch[s] = dt2 * lp * qa;
           cn[s] = dt2 * lf * qa;
           cp[s] = dt2 * (lp + lf) * qb;
         }
-        g_hor[i] += ((ch[0] + ch[1]) + ch[2]) + ch[3];
-        g_nmo[i] += ((cn[0] + cn[1]) + cn[2]) + cn[3];
-        g_po[i] += ((cp[0] + cp[1]) + cp[2]) + cp[3];
+        const long o = u + (long)unx * iz;
+        g_hor[o] += ((ch[0] + ch[1]) + ch[2]) + ch[3];
+        g_nmo[o] += ((cn[0] + cn[1]) + cn[2]) + cn[3];
+        g_po[o] += ((cp[0] + cp[1]) + cp[2]) + cp[3];
       }
   }
-  for (int s = 0; s < ns; ++s) {
-    delete op[s];
-    delete f[s];
+  for (auto& m : M) {
+    delete m.op;
+    delete m.f;
+    delete m.tb;
   }
 }
 
@@ -1127,14 +1151,14 @@ ORC_GRAD(f64, double, double)
 ORC_GRAD(f32d, float, F32D)
 ORC_GRAD(f32i, float, F32I)
 
-// Exact gradient of a gradient group (1..4 shots sharing the window), accumulated into g_*; per-shot FP64
-// misfits into misfits[ns].
+// Exact gradient of a gradient group (1..4 shots with their own windows of the nxm-wide model), accumulated
+// into g_* over the union of the windows (union width x nz); per-shot FP64 misfits into misfits[ns].
 #define ORC_GROUP(SUFFIX, T, S)                                                                                    \
   void orc_group_gradient_##SUFFIX(const T* vh, const T* vp, const T* vn, const T* tab_a, const T* tab_b,           \
-                                   const T* src, int nx, int nz, int nt, int cpml, double dh, double dt, int ns,    \
+                                   const T* src, int nxm, int nz, int nt, int cpml, double dh, double dt, int ns,   \
                                    const int* geo, const T* obs, T* syn, T* g_hor, T* g_po, T* g_nmo,               \
                                    double* misfits) {                                                               \
-    group_gradient<T, S>(vh, vp, vn, tab_a, tab_b, src, nx, nz, nt, cpml, dh, dt, ns, geo, obs, syn, g_hor, g_po,   \
+    group_gradient<T, S>(vh, vp, vn, tab_a, tab_b, src, nxm, nz, nt, cpml, dh, dt, ns, geo, obs, syn, g_hor, g_po,  \
                          g_nmo, misfits);                                                                           \
   }
 ORC_GROUP(mix, float, double)
@@ -1150,10 +1174,10 @@ void orc_ck_round(float* f, long n, int fmt) { ck_round_field(f, n, fmt); }
 // The f32i group gradient with the batched path's checkpoints every K steps in format ckfmt (0 FP32,
 // 1 BF16, 2 FP16, both with per-field power-of-two scaling; only the 16-bit formats change the result).
 void orc_group_gradient_ck_f32i(const float* vh, const float* vp, const float* vn, const float* tab_a,
-                                const float* tab_b, const float* src, int nx, int nz, int nt, int cpml, double dh,
+                                const float* tab_b, const float* src, int nxm, int nz, int nt, int cpml, double dh,
                                 double dt, int ns, const int* geo, const float* obs, float* syn, float* g_hor,
                                 float* g_po, float* g_nmo, double* misfits, int K, int ckfmt) {
-  group_gradient<float, F32I>(vh, vp, vn, tab_a, tab_b, src, nx, nz, nt, cpml, dh, dt, ns, geo, obs, syn, g_hor, g_po,
+  group_gradient<float, F32I>(vh, vp, vn, tab_a, tab_b, src, nxm, nz, nt, cpml, dh, dt, ns, geo, obs, syn, g_hor, g_po,
                               g_nmo, misfits, K, ckfmt);
 }
 

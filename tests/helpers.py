@@ -55,43 +55,54 @@ def oracle_gathers(w, mode, sq, tab, src):
 
 
 def gradient_groups(shots, group=4, max_batch=0):
-    """The batched path's gradient groups: consecutive shots of a batch with identical windows, up to
-    `group` per group (batches of `max_batch` shots, 0 = one batch)."""
+    """The batched path's gradient groups: within each batch (of `max_batch` shots, 0 = one batch) the shots
+    ordered by window (x0, width, then index) in runs of up to `group`; member lists in wave order."""
     out, b0 = [], 0
     n = len(shots)
     while b0 < n:
         b1 = n if max_batch <= 0 else min(n, b0 + max_batch)
-        k = b0
-        while k < b1:
-            m = 1
-            while m < group and k + m < b1 and (shots[k + m].x0, shots[k + m].nx) == (shots[k].x0, shots[k].nx):
-                m += 1
-            out.append(list(range(k, k + m)))
-            k += m
+        order = sorted(range(b0, b1), key=lambda i: (shots[i].x0, shots[i].nx, i))
+        out.extend(order[k:k + group] for k in range(0, len(order), group))
         b0 = b1
     return out
 
 
 def oracle_gradient(w, mode, exact, sq, tab, src, obs_list, group=4, max_batch=0, ckpt=None):
     """Sum over shots (FP64) of the oracle gradients, placed into model columns; total FP64 misfit.  In exact
-    mode shots are accumulated in the batched path's gradient groups (group=1: one accumulator per shot);
-    ckpt=(K, 1) emulates BF16 checkpoints every K steps."""
+    mode shots are accumulated in the batched path's gradient groups (group=1: one accumulator per shot), whose
+    slots are added in the order of their leaders within each batch; ckpt=(K, fmt) emulates 16-bit checkpoints
+    every K steps (fmt 1 BF16, 2 FP16)."""
     T = np.float64 if mode == "f64" else np.float32
     G = np.zeros((3, w.nz, w.nx))
     phi = 0.0
-    syns = []
     if exact and (group > 1 or ckpt is not None):
-        for members in gradient_groups(w.shots, group, max_batch):
-            s0 = w.shots[members[0]]
-            mis, sy, gh, gp, gn = O.group_gradient(mode, window_model(sq, s0), tab, src.astype(T),
-                                                   [obs_list[i] for i in members], s0.nx, w.nz, w.nt, w.cpml, w.dh,
-                                                   w.dt, [w.shots[i] for i in members], ckpt=ckpt)
-            for v in mis:
-                phi += v
-            syns.extend(sy)
-            for k, g in enumerate((gh, gp, gn)):
-                G[k][:, s0.x0:s0.x0 + s0.nx] += g.astype(np.float64)
+        groups = gradient_groups(w.shots, group, max_batch)
+        results = {}
+        mis_all = {}
+        syns = [None] * len(w.shots)
+        for members in groups:
+            mis, sy, ux0, gh, gp, gn = O.group_gradient(mode, sq, tab, src.astype(T), [obs_list[i] for i in members],
+                                                        w.nx, w.nz, w.nt, w.cpml, w.dh, w.dt,
+                                                        [w.shots[i] for i in members], ckpt=ckpt)
+            results[members[0]] = (ux0, (gh, gp, gn))
+            for i, v, y in zip(members, mis, sy):
+                mis_all[i] = v
+                syns[i] = y
+        for i in range(len(w.shots)):  # misfits in shot order, slots in leader order (per batch)
+            phi += mis_all[i]
+        b0 = 0
+        while b0 < len(w.shots):  # each batch's slots summed first, then added (the device's order)
+            b1 = len(w.shots) if max_batch <= 0 else min(len(w.shots), b0 + max_batch)
+            Gb = np.zeros_like(G)
+            for lead in range(b0, b1):
+                if lead in results:
+                    ux0, gs = results[lead]
+                    for k, g in enumerate(gs):
+                        Gb[k][:, ux0:ux0 + g.shape[1]] += g.astype(np.float64)
+            G += Gb
+            b0 = b1
         return phi, G, syns
+    syns = []
     for s, obs in zip(w.shots, obs_list):
         ph, _, syn, gh, gp, gn = O.gradient(mode, exact, window_model(sq, s), tab, src.astype(T), obs, s.nx, w.nz, w.nt,
                                             w.cpml, w.dh, w.dt, s)

@@ -156,33 +156,37 @@ def gradient(mode, exact, model_sq, tab, src, obs, nx, nz, nt, cpml, dh, dt, sho
     return phi, float(m32[0]), syn, gh, gp, gn
 
 
-def group_gradient(mode, model_sq, tab, src, obs_list, nx, nz, nt, cpml, dh, dt, shots, ckpt=None):
-    """Exact gradient of a gradient group (1..4 shots sharing one window, imaging terms added in shot order
-    per step).  Returns (per-shot FP64 misfits, syn list [nt,nrec], g_hor, g_po, g_nmo)."""
+def group_gradient(mode, model_sq, tab, src, obs_list, nxm, nz, nt, cpml, dh, dt, shots, ckpt=None):
+    """Exact gradient of a gradient group (1..4 shots, each with its own window of the nxm-wide model; imaging
+    terms added in member order per step over the union of the windows).  Returns (per-shot FP64 misfits,
+    syn list [nt,nrec], union x0, g_hor, g_po, g_nmo over [nz, union width])."""
     T = _dtype_of(mode)
     vh, vp, vn = (np.ascontiguousarray(x, T) for x in model_sq)
+    assert vh.shape == (nz, nxm)
     ta, tb = (np.ascontiguousarray(x, T) for x in tab)
     s = np.ascontiguousarray(src, T)
-    geo = np.array([[sh.isx, sh.isz, sh.irz, sh.ir0, sh.nrec] for sh in shots], np.int32).reshape(-1)
+    geo = np.array([[sh.x0, sh.nx, sh.isx, sh.isz, sh.irz, sh.ir0, sh.nrec] for sh in shots], np.int32).reshape(-1)
     obs = np.ascontiguousarray(np.concatenate([np.asarray(o, T).reshape(-1) for o in obs_list]), T)
     syn = np.zeros_like(obs)
-    gh, gp, gn = (np.zeros((nz, nx), T) for _ in range(3))
+    ux0 = min(sh.x0 for sh in shots)
+    unx = max(sh.x0 + sh.nx for sh in shots) - ux0
+    gh, gp, gn = (np.zeros((nz, unx), T) for _ in range(3))
     mis = np.zeros(len(shots))
     if ckpt is not None and ckpt[1] != 0:
         # checkpoints every ckpt[0] steps in format ckpt[1] (1 = BF16, 2 = FP16), f32i only
         assert mode == "f32i"
-        lib().orc_group_gradient_ck_f32i(ptr(vh), ptr(vp), ptr(vn), ptr(ta), ptr(tb), ptr(s), nx, nz, nt, cpml, dh, dt,
-                                         len(shots), geo.ctypes.data, ptr(obs), ptr(syn), ptr(gh), ptr(gp), ptr(gn),
+        lib().orc_group_gradient_ck_f32i(ptr(vh), ptr(vp), ptr(vn), ptr(ta), ptr(tb), ptr(s), nxm, nz, nt, cpml, dh,
+                                         dt, len(shots), geo.ctypes.data, ptr(obs), ptr(syn), ptr(gh), ptr(gp), ptr(gn),
                                          ptr(mis), int(ckpt[0]), int(ckpt[1]))
     else:
-        getattr(lib(), f"orc_group_gradient_{mode}")(ptr(vh), ptr(vp), ptr(vn), ptr(ta), ptr(tb), ptr(s), nx, nz, nt,
+        getattr(lib(), f"orc_group_gradient_{mode}")(ptr(vh), ptr(vp), ptr(vn), ptr(ta), ptr(tb), ptr(s), nxm, nz, nt,
                                                      cpml, dh, dt, len(shots), geo.ctypes.data, ptr(obs), ptr(syn),
                                                      ptr(gh), ptr(gp), ptr(gn), ptr(mis))
     syns, o = [], 0
     for sh in shots:
         syns.append(syn[o:o + nt * sh.nrec].reshape(nt, sh.nrec))
         o += nt * sh.nrec
-    return mis, syns, gh, gp, gn
+    return mis, syns, ux0, gh, gp, gn
 
 
 def chain_rule(vp, eps, dl, g_hor, g_po, g_nmo):

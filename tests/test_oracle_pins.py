@@ -364,12 +364,14 @@ def test_f32i_increment_form_tracks_fp64():
 
 
 def test_group_gradient_equals_per_shot_sum():
-    """Gradient groups (shots sharing a window whose per-step imaging terms are added before one update)
-    compute the same sum as one accumulator per shot: equal to 1e-14 in FP64 and to FP32 rounding in f32i;
-    a group of one is the per-shot function bit for bit."""
+    """Gradient groups (shots whose per-step imaging terms are added over the union of their windows before
+    one update; here one window is narrower) compute the same sum as one accumulator per shot: equal to 1e-14
+    in FP64 and to FP32 rounding in f32i; a group of one is the per-shot function bit for bit."""
     from helpers import oracle_gathers, oracle_gradient, oracle_setup, rel_l2, small_workload
 
     w = small_workload(nshots=3, nt=120)
+    s = w.shots[1]  # one narrower window: the group's slot spans the union of the windows
+    w.shots[1] = M.Shot(x0=8, nx=w.nx - 20, isx=s.isx - 8, isz=s.isz, irz=s.irz, ir0=12, nrec=w.nx - 20 - 24)
     for mode, tol in (("f64", 1e-14), ("f32i", 1e-6)):
         tab, src, st, si = oracle_setup(w, mode)
         obs = oracle_gathers(w, mode, st, tab, src)
@@ -386,8 +388,9 @@ def test_group_gradient_equals_per_shot_sum():
     tab, src, st, si = oracle_setup(w, "f32i")
     obs = oracle_gathers(w, "f32i", st, tab, src)
     s0 = w.shots[0]
-    mis, _, gh, gp, gn = O.group_gradient("f32i", tuple(np.ascontiguousarray(x[:, s0.x0:s0.x0 + s0.nx]) for x in si),
-                                          tab, src, [obs[0]], s0.nx, w.nz, w.nt, w.cpml, w.dh, w.dt, [s0])
+    mis, _, ux0, gh, gp, gn = O.group_gradient("f32i", si, tab, src, [obs[0]], w.nx, w.nz, w.nt, w.cpml, w.dh, w.dt,
+                                               [s0])
+    assert ux0 == s0.x0 and gh.shape == (w.nz, s0.nx)
     ph, _, _, h1, p1_, n1 = O.gradient("f32i", True, tuple(np.ascontiguousarray(x[:, s0.x0:s0.x0 + s0.nx]) for x in si),
                                        tab, src, obs[0], s0.nx, w.nz, w.nt, w.cpml, w.dh, w.dt, s0)
     assert mis[0] == ph and np.array_equal(gh, h1) and np.array_equal(gp, p1_) and np.array_equal(gn, n1)
