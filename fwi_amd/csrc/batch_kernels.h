@@ -60,7 +60,8 @@ struct GridArgs {
 };
 
 // Forward step.  P/F "cur" hold step n, "old" hold n-1 and receive n+1.  In the increment form "old"
-// only receives n+1 and Q, R (the increments P^n - P^{n-1}, F^n - F^{n-1}) are updated in place.
+// only receives n+1, and Q, R (the increments P^n - P^{n-1}, F^n - F^{n-1}) are read from Qc, Rc and
+// written to Qo, Ro.
 struct FwdArgs {
   GridArgs g;
   ModelArgs m;
@@ -76,8 +77,10 @@ struct FwdArgs {
   float* psz;
   float* zx;
   float* zz;
-  float* Qc;      // increment form: Q, R
-  float* Rc;
+  const float* Qc;  // increment form: Q, R of step n (read) and of step n+1 (written)
+  const float* Rc;
+  float* Qo;
+  float* Ro;
   float* gather;  // batch gathers or null
   float* qa;      // per-step store or null
   float* qb;

@@ -138,8 +138,8 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
         const float qn = pold + dq, rn = fold + dr;
         pn = c + qn;
         fn = fq[4] + rn;
-        stn(a.Qc + sd.off, o, qn);
-        stn(a.Rc + sd.off, o, rn);
+        stn(a.Qo + sd.off, o, qn);
+        stn(a.Ro + sd.off, o, rn);
       } else {
         pn = 2.f * c - pold + vhv * C * tp + vpv * C * tf;
         fn = 2.f * fq[4] - fold + vnv * C * tp + vpv * C * tf;

@@ -136,9 +136,19 @@ struct fwi_plan {
   }
 
   // workspace views (sized for cap_cell)
-  // forward: 0,1 P ping-pong, 2,3 F ping-pong, 4 psx, 5 psz, 6 zx, 7 zz, 8 Q = P^n - P^{n-1},
-  // 9 R = F^n - F^{n-1} (increment form only)
-  static constexpr int kFwdF = 10, kAdjF = 12, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
+  // forward fields: the state of step n in set n mod 3 in the increment form (P 0..2, F 3..5, Q = P^n - P^{n-1}
+  // 6..8, R = F^n - F^{n-1} 9..11; three sets so that fused step pairs can keep step n while writing n+1 and
+  // n+2), in set n mod 2 of P 0..1 and F 3..4 in the two-level form; C-PML memory psx 12, psz 13, zx 14, zz 15
+  static constexpr int kFwdF = 16, kAdjF = 12, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
+  int fset(int n) const { return inc() ? n % 3 : (n & 1); }
+  float* fP(int n) const { return fwd(fset(n)); }
+  float* fF(int n) const { return fwd(3 + fset(n)); }
+  float* fQ(int n) const { return fwd(6 + n % 3); }
+  float* fR(int n) const { return fwd(9 + n % 3); }
+  float* fpsx() const { return fwd(12); }
+  float* fpsz() const { return fwd(13); }
+  float* fzx() const { return fwd(14); }
+  float* fzz() const { return fwd(15); }
   float* fwd(int k) const { return d_work + (size_t)k * cap_cell; }
   float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..11
   float* sgrad(int k) const { return d_work + (size_t)(kFwdF + kAdjF + k) * cap_cell; }      // 0..2
@@ -162,11 +172,14 @@ struct fwi_plan {
   float* resid() const { return gathers() + cap_gather; }
   bool inc() const { return opt.arith == FWI_ARITH_FP32I; }
   // the four cell fields a checkpoint holds before step n: both levels of P and F, or in the increment
-  // form the current P, F and their increments Q, R
+  // form the set of step n (P, F and their increments Q, R)
   int ckpt_field(int f, int n) const {
-    if (!inc()) return f;
-    const int cur = n & 1;
-    const int map[4] = {cur, 2 + cur, 8, 9};
+    if (!inc()) {
+      const int map[4] = {0, 1, 3, 4};
+      return map[f];
+    }
+    const int m = n % 3;
+    const int map[4] = {m, 3 + m, 6 + m, 9 + m};
     return map[f];
   }
   std::string err;
@@ -371,20 +384,19 @@ struct fwi_plan {
   // one forward step of a batch
   int fwd_step(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
                hipStream_t st) {
-    const int cur = n & 1;
-    float* Pc = fwd(cur);
-    float* Po = fwd(cur ^ 1);
-    float* Fc = fwd(2 + cur);
-    float* Fo = fwd(2 + (cur ^ 1));
+    float* Pc = fP(n);
+    float* Po = fP(n + 1);
+    float* Fc = fF(n);
+    float* Fo = fF(n + 1);
     StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
     const int am = opt.arith;
     if (b.nsblocks > 0) {
       if (am == FWI_ARITH_MIXED)
-        hipLaunchKernelGGL(k_fwd_strip<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fwd(4), fwd(5));
+        hipLaunchKernelGGL(k_fwd_strip<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fpsx(), fpsz());
       else if (am == FWI_ARITH_FP32D)
-        hipLaunchKernelGGL(k_fwd_strip<F32D>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fwd(4), fwd(5));
+        hipLaunchKernelGGL(k_fwd_strip<F32D>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fpsx(), fpsz());
       else
-        hipLaunchKernelGGL(k_fwd_strip<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fwd(4), fwd(5));
+        hipLaunchKernelGGL(k_fwd_strip<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fpsx(), fpsz());
     }
     FwdArgs a;
     a.g = grid_args(b);
@@ -397,12 +409,14 @@ struct fwi_plan {
     a.Po = Po;
     a.Fc = Fc;
     a.Fo = Fo;
-    a.psx = fwd(4);
-    a.psz = fwd(5);
-    a.zx = fwd(6);
-    a.zz = fwd(7);
-    a.Qc = fwd(8);
-    a.Rc = fwd(9);
+    a.psx = fpsx();
+    a.psz = fpsz();
+    a.zx = fzx();
+    a.zz = fzz();
+    a.Qc = fQ(n);
+    a.Rc = fR(n);
+    a.Qo = fQ(n + 1);
+    a.Ro = fR(n + 1);
     a.gather = gather;
     a.qa = qa;
     a.qb = qb;
@@ -537,7 +551,7 @@ struct fwi_plan {
   int strip_copy(const Batch& b, int dir, float* packed, hipStream_t st) {
     if (b.nsblocks == 0) return 0;
     StripArgs sa{grid_args(b), ModelArgs{nullptr, nullptr, nullptr, nx}, b.d_sblocks, b.nsblocks};
-    hipLaunchKernelGGL(k_strip_copy, dim3(b.nsblocks), dim3(256), 0, st, sa, dir, fwd(4), fwd(5), fwd(6), fwd(7),
+    hipLaunchKernelGGL(k_strip_copy, dim3(b.nsblocks), dim3(256), 0, st, sa, dir, fpsx(), fpsz(), fzx(), fzz(),
                        packed, b.nstrip);
     CK(hipGetLastError());
     return 0;
@@ -568,13 +582,13 @@ struct fwi_plan {
                          fwd(ckpt_field(3, n)), (unsigned short*)c, b.ncell, am);
       if (b.nsblocks > 0) {
         StripArgs sa{grid_args(b), ModelArgs{nullptr, nullptr, nullptr, nx}, b.d_sblocks, b.nsblocks};
-        hipLaunchKernelGGL(k_ck_absmax_strips, dim3(b.nsblocks), dim3(256), 0, st, sa, fwd(4), fwd(5), fwd(6), fwd(7), am);
+        hipLaunchKernelGGL(k_ck_absmax_strips, dim3(b.nsblocks), dim3(256), 0, st, sa, fpsx(), fpsz(), fzx(), fzz(), am);
         if (h)
-          hipLaunchKernelGGL(k_ck_pack_strips<2>, dim3(b.nsblocks), dim3(256), 0, st, sa, 0, fwd(4), fwd(5), fwd(6),
-                             fwd(7), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
+          hipLaunchKernelGGL(k_ck_pack_strips<2>, dim3(b.nsblocks), dim3(256), 0, st, sa, 0, fpsx(), fpsz(), fzx(),
+                             fzz(), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
         else
-          hipLaunchKernelGGL(k_ck_pack_strips<1>, dim3(b.nsblocks), dim3(256), 0, st, sa, 0, fwd(4), fwd(5), fwd(6),
-                             fwd(7), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
+          hipLaunchKernelGGL(k_ck_pack_strips<1>, dim3(b.nsblocks), dim3(256), 0, st, sa, 0, fpsx(), fpsz(), fzx(),
+                             fzz(), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
       }
       CK(hipGetLastError());
       return 0;
@@ -601,11 +615,11 @@ struct fwi_plan {
       if (b.nsblocks > 0) {
         StripArgs sa{grid_args(b), ModelArgs{nullptr, nullptr, nullptr, nx}, b.d_sblocks, b.nsblocks};
         if (h)
-          hipLaunchKernelGGL(k_ck_pack_strips<2>, dim3(b.nsblocks), dim3(256), 0, st, sa, 1, fwd(4), fwd(5), fwd(6),
-                             fwd(7), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
+          hipLaunchKernelGGL(k_ck_pack_strips<2>, dim3(b.nsblocks), dim3(256), 0, st, sa, 1, fpsx(), fpsz(), fzx(),
+                             fzz(), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
         else
-          hipLaunchKernelGGL(k_ck_pack_strips<1>, dim3(b.nsblocks), dim3(256), 0, st, sa, 1, fwd(4), fwd(5), fwd(6),
-                             fwd(7), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
+          hipLaunchKernelGGL(k_ck_pack_strips<1>, dim3(b.nsblocks), dim3(256), 0, st, sa, 1, fpsx(), fpsz(), fzx(),
+                             fzz(), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
       }
       CK(hipGetLastError());
       return 0;
