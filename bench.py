@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying captured graphs")
     ap.add_argument("--keep", type=int, default=0, help="trailing segments kept from the forward (0 = memory allows)")
     ap.add_argument("--grad-group", type=int, default=0, help="shots per gradient group (0 = 4)")
+    ap.add_argument("--pairs", action="store_true", help="fused core step pairs (increment form)")
     ap.add_argument("--arith", type=int, default=3,
                     help="0 FP32 two-level, 1 mixed (reference), 2 FP32 difference stencil, 3 FP32 increment form")
     ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")
@@ -183,7 +184,7 @@ def main():
     plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
                 rows_per_item=args.rows, item_order=args.order, arith=args.arith,
                 ckpt_format=args.ckpt_format, use_graph=0 if args.no_graph else 1, store_segments=args.keep,
-                grad_group=args.grad_group)
+                grad_group=args.grad_group, step_pairs=1 if args.pairs else 0)
     log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, {plan.kept_steps} steps kept from the forward, "
         f"workspace {plan.workspace_bytes / 1e9:.1f} GB")
     t0 = time.perf_counter()
@@ -237,6 +238,11 @@ def main():
     fwd_avg = fwd_ms / max(fwd_n, 1) / 1e3
     if adj_n:
         log(f"adjoint step: {adj_n} timed launches, mean {adj_avg * 1e6:.1f} us -> {achieved:.0f} GB/s algorithmic")
+    pair_ms, pair_n = plan.kernel_time(3)
+    pairs_ms, pairs_n = plan.kernel_time(4)
+    if pair_n or pairs_n:
+        log(f"fused core pair: {pair_n} timed launches, mean {pair_ms / max(pair_n, 1) * 1e3:.1f} us; storing pair "
+            f"{pairs_n} launches, mean {pairs_ms / max(pairs_n, 1) * 1e3:.1f} us (rows per item {plan.pair_rows})")
     if fwd_n:
         log(f"forward step: {fwd_n} timed launches, mean {fwd_avg * 1e6:.1f} us -> "
             f"{fwd_bytes / max(fwd_avg, 1e-12) / 1e9:.0f} GB/s; storing step mean "
@@ -290,6 +296,7 @@ def main():
             "arith": {0: "fp32", 1: "mixed", 2: "fp32-diff", 3: "fp32-increment"}[args.arith],
             "ckpt_interval": plan.ckpt_interval,
             "kept_steps": plan.kept_steps,
+            "step_pair_rows": plan.pair_rows,
             "ckpt_format": {0: "fp32", 1: "bf16-scaled", 2: "fp16-scaled"}[args.ckpt_format],
             "workspace_gb": round(plan.workspace_bytes / 1e9, 2),
             "batch_shots": plan.batch_size,

@@ -33,7 +33,7 @@ class FwiOptions(ctypes.Structure):
     _fields_ = [("grad_mode", c_int), ("arith", c_int), ("ckpt_interval", c_int), ("max_batch", c_int),
                 ("mem_budget", c_double), ("rows_per_item", c_int), ("item_order", c_int),
                 ("grad_group", c_int), ("ckpt_format", c_int), ("store_segments", c_int),
-                ("use_graph", c_int)]
+                ("use_graph", c_int), ("step_pairs", c_int)]
 
 
 # Every exported symbol with its signature (argument types, return type).
@@ -94,6 +94,7 @@ BATCHED = {
     "fwi_plan_rows_per_item": ([vp_], c_int),
     "fwi_plan_item_order": ([vp_], c_int),
     "fwi_plan_kept_steps": ([vp_], c_int),
+    "fwi_plan_pair_rows": ([vp_], c_int),
     "fwi_plan_graph_stats": ([vp_, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)], c_int),
     "fwi_plan_adj_tiling": ([vp_, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)], c_int),
 }

@@ -85,6 +85,36 @@ struct FwdArgs {
   float* qa;      // per-step store or null
   float* qb;
   int it;
+  const int2* xr; // per-item column range [x, y) the item updates, or null (its whole band)
+};
+
+// Two forward steps n, n+1 of the increment form on the core of every shot (the cells at least 4 columns and
+// rows away from the C-PML cells, where no C-PML term enters either step).  Items: (shot, first of 56 output
+// columns, z0, z1).  Reads the set of step n, writes the set of step n+2 and, on the 4-cell layer next to the
+// shell, P and F of step n+1 (which the shell's single steps read).
+struct FwdPairArgs {
+  GridArgs g;
+  ModelArgs m;
+  const int4* items;
+  int nitems;
+  float C;
+  const float* src;
+  const float* P0;
+  const float* F0;
+  const float* Q0;
+  const float* R0;
+  float* P1;
+  float* F1;
+  float* P2;
+  float* F2;
+  float* Q2;
+  float* R2;
+  float* gather;     // batch gathers or null
+  float* qa0;        // imaging terms of step n (QM 1: a, b; QM 2: p2, r2) or null
+  float* qb0;
+  float* qa1;        // of step n+1
+  float* qb1;
+  int it;            // n
 };
 
 // Strip (C-PML) blocks: (shot, first strip cell) per 256-thread block.

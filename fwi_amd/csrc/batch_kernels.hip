@@ -67,7 +67,11 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
   const ShotDev sd = a.g.shots[itm.x];
   const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
-  const bool act = ix >= kRim && ix < nx - kRim;
+  bool act = ix >= kRim && ix < nx - kRim;
+  if (a.xr) {  // shell items of the step-pair scheme update only their column range
+    const int2 r = a.xr[item];
+    act = act && ix >= r.x && ix < r.y;
+  }
   // every lane loads from a valid column; values of columns outside the shot are never consumed
   const int cx = clampi(ix, 0, nx - 1);
   const int hx = lane < 8 ? clampi(itm.y + halo_col(lane), 0, nx - 1) : cx;
@@ -162,6 +166,151 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) fq[k] = fq[k + 1];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Two increment-form forward steps per launch on the core of every shot: columns [cpml+4, nx-cpml-4) and
+// rows [4, nz-cpml-4), where neither step nor its 4-cell halo touches a C-PML cell, so no C-PML term enters.
+// A wave owns 56 output columns [x0, x0+56) and rows [z0, z1).  Its 64 lanes compute step n on columns
+// [x0-4, x0+60) and rows [z0-4, z1+4) (the halo that step n+1 needs), marching down: at row r it finishes
+// step n at row r and step n+1 at row r-4.  D2x acts on P only and D2z on F only, so step n+1 takes the x
+// neighbours of P^{n+1} from the wave's own lanes (LDS row) and the z neighbours of F^{n+1} from a register
+// queue.  Per output cell the state crosses HBM once for two steps.  Every cell is computed with the
+// single-step kernel's expressions in the same order, so results are identical to two single steps.
+// ---------------------------------------------------------------------------------------------
+template <int QM>
+__global__ void __launch_bounds__(256) k_fwd_pair(FwdPairArgs a) {
+  __shared__ float xbuf[4][72];
+  const int wave = wave_id(), lane = threadIdx.x & 63;
+  const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
+  if (item >= a.nitems) return;
+  const int4 itm = a.items[item];
+  const ShotDev sd = a.g.shots[itm.x];
+  const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
+  const int cx0 = cpml + 4, cx1 = nx - cpml - 4, zc1 = nz - cpml - 4;  // the core
+  const int x0 = itm.y;
+  const int col = x0 - 4 + lane;                                  // this lane's column for step n
+  const int cc = clampi(col, 0, nx - 1);
+  const int hc = lane < 8 ? clampi(x0 - 4 + halo_col(lane), 0, nx - 1) : cc;
+  const bool own = lane >= 4 && lane < 60 && col < cx1;           // output columns of this wave
+  const bool thin = col < cx0 + 4 || col >= cx1 - 4;              // next to the side shell
+  const float dh2 = a.g.dh2, C = a.C;
+  const float* P0 = a.P0 + sd.off;
+  const float* F0 = a.F0 + sd.off;
+  const float* Q0 = a.Q0 + sd.off;
+  const float* R0 = a.R0 + sd.off;
+  const float* vh = a.m.vh + sd.x0;
+  const float* vp = a.m.vp + sd.x0;
+  const float* vn = a.m.vn + sd.x0;
+  float* buf = xbuf[wave];
+  const float src0 = a.src[a.it], src1 = a.src[a.it + 1];
+  const int z0 = itm.z, z1 = itm.w;
+  const bool rec = a.gather && col >= sd.ir0 && col < sd.ir0 + sd.nrec;
+  float f0q[9];                    // F^n at rows r-4..r+4
+  float f1q[9];                    // F^{n+1} at rows r-8..r
+  float p1q[5], q1q[5], r1q[5];    // P^{n+1}, Q^{n+1}, R^{n+1} at rows r-4..r
+#pragma unroll
+  for (int k = 0; k < 8; ++k) f0q[k] = ldu(F0, (unsigned)(cc + ps * max(z0 - 8 + k, 0)));
+#pragma unroll
+  for (int k = 0; k < 9; ++k) f1q[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) p1q[k] = q1q[k] = r1q[k] = 0.f;
+  for (int r = z0 - 4; r < z1 + 4; ++r) {
+    // ---- step n at row r (rows above 4 are the rim: zero at every step)
+    const unsigned o = (unsigned)(cc + ps * max(r, 0));
+    f0q[8] = ldu(F0, (unsigned)(cc + ps * max(r + 4, 0)));
+    const float c = ldu(P0, o);
+    const float h = ldu(P0, (unsigned)(hc + ps * max(r, 0)));
+    const float q0 = ldn(Q0, o), r0 = ldn(R0, o);
+    const unsigned mo = (unsigned)(cc + nxm * max(r, 0));
+    const float vhv = ldu(vh, mo), vpv = ldu(vp, mo), vnv = ldu(vn, mo);
+    float u[9];
+    xrow9(buf, lane, c, h, u);
+    float p1 = 0.f, f1 = 0.f, q1 = 0.f, r1 = 0.f;
+    if (r >= kRim) {
+      const float p2 = d2s<float>(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8], dh2);
+      const float r2 = d2s<float>(f0q[0], f0q[1], f0q[2], f0q[3], f0q[4], f0q[5], f0q[6], f0q[7], f0q[8], dh2);
+      const float atp = 0.f, atf = 0.f;  // no C-PML term in the core and its halo
+      const float tp = dh2 * (p2 + atp);
+      const float tf = dh2 * (r2 + atf);
+      float dq = vhv * C * tp + vpv * C * tf;
+      float dr = vnv * C * tp + vpv * C * tf;
+      if (col == sd.isx && r == sd.isz) {
+        dq += src0;
+        dr += src0;
+      }
+      q1 = q0 + dq;
+      r1 = r0 + dr;
+      p1 = c + q1;
+      f1 = f0q[4] + r1;
+      if (own && r >= z0 && r < z1) {
+        const unsigned oo = (unsigned)(col + ps * r);
+        if (thin || r >= zc1 - 4) {
+          stn(a.P1 + sd.off, oo, p1);
+          stn(a.F1 + sd.off, oo, f1);
+        }
+        if (rec && r == sd.irz) a.gather[sd.goff + (long)a.it * sd.nrec + (col - sd.ir0)] = p1 + f1;
+        if (QM == 1) {
+          stn(a.qa0 + sd.off, oo, p2 + atp);
+          stn(a.qb0 + sd.off, oo, r2 + atf);
+        } else if (QM == 2) {
+          stn(a.qa0 + sd.off, oo, p2);
+          stn(a.qb0 + sd.off, oo, r2);
+        }
+      }
+    }
+    f1q[8] = f1;
+    p1q[4] = p1;
+    q1q[4] = q1;
+    r1q[4] = r1;
+    // ---- step n+1 at row rb = r - 4
+    const int rb = r - 4;
+    if (rb >= z0) {
+      float w[9];
+      xrow9(buf, lane, p1q[0], p1q[0], w);  // x neighbours of P^{n+1}: valid on lanes 4..59
+      const unsigned mb = (unsigned)(cc + nxm * rb);
+      const float vhb = ldu(vh, mb), vpb = ldu(vp, mb), vnb = ldu(vn, mb);
+      if (own) {
+        const float p2 = d2s<float>(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], dh2);
+        const float r2 = d2s<float>(f1q[0], f1q[1], f1q[2], f1q[3], f1q[4], f1q[5], f1q[6], f1q[7], f1q[8], dh2);
+        const float atp = 0.f, atf = 0.f;
+        const float tp = dh2 * (p2 + atp);
+        const float tf = dh2 * (r2 + atf);
+        float dq = vhb * C * tp + vpb * C * tf;
+        float dr = vnb * C * tp + vpb * C * tf;
+        if (col == sd.isx && rb == sd.isz) {
+          dq += src1;
+          dr += src1;
+        }
+        const float q2 = q1q[0] + dq, r2n = r1q[0] + dr;
+        const float pn = p1q[0] + q2, fn = f1q[4] + r2n;
+        const unsigned ob = (unsigned)(col + ps * rb);
+        stn(a.P2 + sd.off, ob, pn);
+        stn(a.F2 + sd.off, ob, fn);
+        stn(a.Q2 + sd.off, ob, q2);
+        stn(a.R2 + sd.off, ob, r2n);
+        if (rec && rb == sd.irz) a.gather[sd.goff + (long)(a.it + 1) * sd.nrec + (col - sd.ir0)] = pn + fn;
+        if (QM == 1) {
+          stn(a.qa1 + sd.off, ob, p2 + atp);
+          stn(a.qb1 + sd.off, ob, r2 + atf);
+        } else if (QM == 2) {
+          stn(a.qa1 + sd.off, ob, p2);
+          stn(a.qb1 + sd.off, ob, r2);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      f0q[k] = f0q[k + 1];
+      f1q[k] = f1q[k + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      p1q[k] = p1q[k + 1];
+      q1q[k] = q1q[k + 1];
+      r1q[k] = r1q[k + 1];
+    }
   }
 }
 

@@ -45,6 +45,11 @@ struct Batch {
   int nitems = 0;
   int4* d_items_adj = nullptr;  // adjoint work items (grouped: whole workgroups per group and tile)
   int nitems_adj = 0;
+  int4* d_items_core = nullptr;  // step pairs: fused core items (56 output columns each)
+  int nitems_core = 0;
+  int4* d_items_shell = nullptr; // step pairs: single-step items of the shell, with their column ranges
+  int2* d_xr_shell = nullptr;
+  int nitems_shell = 0;
   struct Group {
     int n = 0;
     int m[4] = {0, 0, 0, 0};  // members (batch indices) in wave order; m[0] leads
@@ -82,6 +87,7 @@ struct fwi_plan {
   int order = 0;     // work-item order (build_items)
   int tz_adj = 32;   // rows per work item and order of the adjoint kernels
   int order_adj = 0;
+  int tz_pair = 32;  // rows per fused core item (step pairs)
   bool tuned = false;
   std::vector<Batch> batches;
   long long gather_total = 0;
@@ -99,13 +105,14 @@ struct fwi_plan {
   int max_count = 0;                // shots in the largest batch
   unsigned* d_ck_amax = nullptr;    // 16-bit checkpoints: max-abs bits per (checkpoint, shot, field)
   // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute),
-  // 2 = adjoint step; every `timing_stride`-th launch of a kind is bracketed by events on its stream
+  // 2 = adjoint step, 3 / 4 = fused core step pair without / with store; every `timing_stride`-th launch of
+  // a kind is bracketed by events on its stream
   int timing_stride = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_open;  // (kind, event index of the start; end = start + 1)
-  double kt_ms[3] = {0, 0, 0};
-  long long kt_count[3] = {0, 0, 0};
-  long long launch_no[3] = {0, 0, 0};
+  double kt_ms[5] = {0, 0, 0, 0, 0};
+  long long kt_count[5] = {0, 0, 0, 0, 0};
+  long long launch_no[5] = {0, 0, 0, 0, 0};
 
   int timed_begin(int kind, hipStream_t st) {
     if (timing_stride <= 0 || (launch_no[kind]++ % timing_stride) != 0) return -1;
@@ -263,6 +270,9 @@ struct fwi_plan {
       (void)hipFree(b.d_shots);
       (void)hipFree(b.d_items);
       (void)hipFree(b.d_items_adj);
+      (void)hipFree(b.d_items_core);
+      (void)hipFree(b.d_items_shell);
+      (void)hipFree(b.d_xr_shell);
       (void)hipFree(b.d_sblocks);
     }
     (void)hipFree(d_src);
@@ -348,6 +358,7 @@ struct fwi_plan {
     (void)hipFree(dst);
     dst = nullptr;
     n = (int)items.size();
+    if (items.empty()) return 0;
     CK(hipMalloc(&dst, items.size() * sizeof(int4)));
     CK(hipMemcpy(dst, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
     return 0;
@@ -362,6 +373,57 @@ struct fwi_plan {
   //      so z halos are reused from the L2 while the neighbour is still resident.
   // Workgroup ids are remapped so each XCD receives a contiguous range of this order.
   int build_items(Batch& b, int rows, int order) { return upload_items(b.d_items, b.nitems, ungrouped_items(b, nz, rows, order)); }
+  // Step-pair items: the fused core items (56 output columns, runs of `rows`, row run -> band -> shot) and the
+  // shell's single-step items with the column range each one updates.  The core of a window is columns
+  // [cpml+4, nx-cpml-4) and rows [4, nz-cpml-4); a window without a core is all shell.
+  int build_pair_items(Batch& b, int rows) {
+    std::vector<int4> core, shell;
+    std::vector<int2> xr;
+    const int zc1 = nz - cpml - 4;
+    auto runs_of = [&](int za, int zb) {
+      std::vector<std::pair<int, int>> r;
+      for (int z0 = za; z0 < zb; z0 += rows) r.push_back({z0, std::min(z0 + rows, zb)});
+      return r;
+    };
+    const auto core_runs = runs_of(kRim, zc1);
+    int maxnx = 0;
+    for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
+    for (auto& r : core_runs)
+      for (int xo = cpml + 4; xo < maxnx; xo += 56)
+        for (int s = 0; s < b.count; ++s) {
+          const int cx1 = b.h_shots[s].nx - cpml - 4;
+          if (xo < cx1 && zc1 > kRim) core.push_back(make_int4(s, xo, r.first, r.second));
+        }
+    const auto all_runs = runs_of(kRim, nz - kRim);
+    const auto bottom_runs = runs_of(std::max(zc1, kRim), nz - kRim);
+    for (int s = 0; s < b.count; ++s) {
+      const int nxs = b.h_shots[s].nx, cx0 = cpml + 4, cx1 = nxs - cpml - 4;
+      auto band = [&](int x, int x1, const std::vector<std::pair<int, int>>& rr, int lo, int hi) {
+        for (; x < x1; x += 64)
+          for (auto& r : rr) {
+            shell.push_back(make_int4(s, x, r.first, r.second));
+            xr.push_back(make_int2(lo, hi));
+          }
+      };
+      if (cx1 <= cx0 || zc1 <= kRim) {  // no core: the whole window is shell
+        band(0, nxs, all_runs, 0, nxs);
+        continue;
+      }
+      band(0, cx0, all_runs, 0, cx0);             // left side
+      band(cx1, nxs, all_runs, cx1, nxs);         // right side
+      band(cx0, cx1, bottom_runs, cx0, cx1);      // bottom between the sides
+    }
+    if (int r = upload_items(b.d_items_core, b.nitems_core, core)) return r;
+    if (int r = upload_items(b.d_items_shell, b.nitems_shell, shell)) return r;
+    (void)hipFree(b.d_xr_shell);
+    b.d_xr_shell = nullptr;
+    if (!xr.empty()) {
+      CK(hipMalloc(&b.d_xr_shell, xr.size() * sizeof(int2)));
+      CK(hipMemcpy(b.d_xr_shell, xr.data(), xr.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    return 0;
+  }
+
   // Work items of the adjoint kernels: grouped (orders 0 and 2) when the batch has gradient groups.
   int build_adj_items(Batch& b, int rows, int order) {
     return upload_items(b.d_items_adj, b.nitems_adj,
@@ -382,33 +444,35 @@ struct fwi_plan {
 
   // ---------------------------------------------------------------------------------------------
   // one forward step of a batch
-  int fwd_step(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
-               hipStream_t st) {
-    float* Pc = fP(n);
-    float* Po = fP(n + 1);
-    float* Fc = fF(n);
-    float* Fo = fF(n + 1);
+  int fwd_strip(const Batch& b, const ModelArgs& m, int n, hipStream_t st) {
+    if (b.nsblocks == 0) return 0;
     StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
     const int am = opt.arith;
-    if (b.nsblocks > 0) {
-      if (am == FWI_ARITH_MIXED)
-        hipLaunchKernelGGL(k_fwd_strip<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fpsx(), fpsz());
-      else if (am == FWI_ARITH_FP32D)
-        hipLaunchKernelGGL(k_fwd_strip<F32D>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fpsx(), fpsz());
-      else
-        hipLaunchKernelGGL(k_fwd_strip<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, Pc, Fc, fpsx(), fpsz());
-    }
+    if (am == FWI_ARITH_MIXED)
+      hipLaunchKernelGGL(k_fwd_strip<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, fP(n), fF(n), fpsx(), fpsz());
+    else if (am == FWI_ARITH_FP32D)
+      hipLaunchKernelGGL(k_fwd_strip<F32D>, dim3(b.nsblocks), dim3(256), 0, st, sa, fP(n), fF(n), fpsx(), fpsz());
+    else
+      hipLaunchKernelGGL(k_fwd_strip<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, fP(n), fF(n), fpsx(), fpsz());
+    CK(hipGetLastError());
+    return 0;
+  }
+
+  // single-step main kernel over an item list (all items, or the shell's with column ranges)
+  int fwd_main(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
+               const int4* items, int nitems, const int2* xr, int kind, hipStream_t st) {
+    if (nitems == 0) return 0;
     FwdArgs a;
     a.g = grid_args(b);
     a.m = m;
-    a.items = b.d_items;
-    a.nitems = b.nitems;
+    a.items = items;
+    a.nitems = nitems;
     a.C = (dt * dt) / (dh * dh);
     a.src = d_src;
-    a.Pc = Pc;
-    a.Po = Po;
-    a.Fc = Fc;
-    a.Fo = Fo;
+    a.Pc = fP(n);
+    a.Po = fP(n + 1);
+    a.Fc = fF(n);
+    a.Fo = fF(n + 1);
     a.psx = fpsx();
     a.psz = fpsz();
     a.zx = fzx();
@@ -421,8 +485,10 @@ struct fwi_plan {
     a.qa = qa;
     a.qb = qb;
     a.it = n;
-    dim3 grid((b.nitems + 3) / 4);
-    const int tev = timed_begin(qm == 0 ? 0 : 1, st);
+    a.xr = xr;
+    dim3 grid((nitems + 3) / 4);
+    const int am = opt.arith;
+    const int tev = kind >= 0 ? timed_begin(kind, st) : -1;
 #define FWD_LAUNCH(QMV, SV) hipLaunchKernelGGL((k_fwd_main<QMV, SV>), grid, dim3(256), 0, st, a)
 #define FWD_LAUNCH_INC(QMV) hipLaunchKernelGGL((k_fwd_main<QMV, float, true>), grid, dim3(256), 0, st, a)
     if (am == FWI_ARITH_FP32I) {
@@ -448,6 +514,60 @@ struct fwi_plan {
     CK(hipGetLastError());
     return 0;
   }
+
+  // one forward step of a batch
+  int fwd_step(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
+               hipStream_t st) {
+    if (int r = fwd_strip(b, m, n, st)) return r;
+    return fwd_main(b, m, n, qm, qa, qb, gather, b.d_items, b.nitems, nullptr, qm == 0 ? 0 : 1, st);
+  }
+
+  // Two forward steps n, n+1 (increment form): the shell by single steps around the fused core pair.  The
+  // shell's step n and the core pair read only the set of step n and write disjoint cells; the C-PML update
+  // and the shell's step n+1 then read step n+1, which the core leaves on its 4-cell layer next to the shell.
+  bool pairs(const Batch& b) const { return inc() && opt.step_pairs != 0 && b.nitems_core > 0; }
+  int fwd_pair(const Batch& b, const ModelArgs& m, int n, int qm, float* qa0, float* qb0, float* qa1, float* qb1,
+               float* gather, hipStream_t st) {
+    if (int r = fwd_strip(b, m, n, st)) return r;
+    if (int r = fwd_main(b, m, n, qm, qa0, qb0, gather, b.d_items_shell, b.nitems_shell, b.d_xr_shell, -1, st))
+      return r;
+    FwdPairArgs a;
+    a.g = grid_args(b);
+    a.m = m;
+    a.items = b.d_items_core;
+    a.nitems = b.nitems_core;
+    a.C = (dt * dt) / (dh * dh);
+    a.src = d_src;
+    a.P0 = fP(n);
+    a.F0 = fF(n);
+    a.Q0 = fQ(n);
+    a.R0 = fR(n);
+    a.P1 = fP(n + 1);
+    a.F1 = fF(n + 1);
+    a.P2 = fP(n + 2);
+    a.F2 = fF(n + 2);
+    a.Q2 = fQ(n + 2);
+    a.R2 = fR(n + 2);
+    a.gather = gather;
+    a.qa0 = qa0;
+    a.qb0 = qb0;
+    a.qa1 = qa1;
+    a.qb1 = qb1;
+    a.it = n;
+    dim3 grid((b.nitems_core + 3) / 4);
+    const int tev = timed_begin(qm == 0 ? 3 : 4, st);
+    if (qm == 0)
+      hipLaunchKernelGGL(k_fwd_pair<0>, grid, dim3(256), 0, st, a);
+    else if (qm == 1)
+      hipLaunchKernelGGL(k_fwd_pair<1>, grid, dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL(k_fwd_pair<2>, grid, dim3(256), 0, st, a);
+    timed_end(tev, st);
+    CK(hipGetLastError());
+    if (int r = fwd_strip(b, m, n + 1, st)) return r;
+    return fwd_main(b, m, n + 1, qm, qa1, qb1, gather, b.d_items_shell, b.nitems_shell, b.d_xr_shell, -1, st);
+  }
+
 
   // one adjoint step k (produces m^k from m^{k+1}, m^{k+2}); q slot pointers may be null
   int adj_step(const Batch& b, const ModelArgs& m, int k, const float* qa, const float* qb, hipStream_t st) {
@@ -643,8 +763,15 @@ struct fwi_plan {
   // ---------------------------------------------------------------------------------------------
   int forward_batch(const Batch& b, const ModelArgs& m, float* gather, hipStream_t st) {
     if (int r = zero_fwd(b, st)) return r;
-    for (int n = 0; n < nt; ++n)
-      if (int r = fwd_step(b, m, n, 0, nullptr, nullptr, gather, st)) return r;
+    for (int n = 0; n < nt;) {
+      if (pairs(b) && n + 1 < nt) {
+        if (int r = fwd_pair(b, m, n, 0, nullptr, nullptr, nullptr, nullptr, gather, st)) return r;
+        n += 2;
+      } else {
+        if (int r = fwd_step(b, m, n, 0, nullptr, nullptr, gather, st)) return r;
+        n += 1;
+      }
+    }
     return 0;
   }
 
@@ -658,16 +785,24 @@ struct fwi_plan {
     if (int r = zero_fwd(b, st)) return r;
     // forward with checkpoints before segments 1..J-S-1; the last S segments keep their imaging terms
     const int s0 = sfirst();
-    for (int n = 0; n < nt; ++n) {
-      if (n > 0 && n % K == 0 && n / K <= nckpt())
+    auto ckpt_at = [&](int n) { return n > 0 && n % K == 0 && n / K <= nckpt(); };
+    auto qslot = [&](int n) { return n >= s0 ? qbuf() + (size_t)(n - s0) * slot : nullptr; };
+    for (int n = 0; n < nt;) {
+      if (ckpt_at(n))
         if (int r = save_ckpt(b, n / K, st)) return r;
-      float* qa = nullptr;
-      float* qb = nullptr;
-      if (n >= s0) {
-        qa = qbuf() + (size_t)(n - s0) * slot;
-        qb = qa + cap_cell;
+      // a pair may not straddle a checkpoint or the first kept step
+      if (pairs(b) && n + 1 < nt && !ckpt_at(n + 1) && (n >= s0) == (n + 1 >= s0)) {
+        float* q0 = qslot(n);
+        float* q1 = qslot(n + 1);
+        if (int r = fwd_pair(b, m, n, n >= s0 ? qm : 0, q0, q0 ? q0 + cap_cell : nullptr, q1,
+                             q1 ? q1 + cap_cell : nullptr, gather, st))
+          return r;
+        n += 2;
+      } else {
+        float* q0 = qslot(n);
+        if (int r = fwd_step(b, m, n, n >= s0 ? qm : 0, q0, q0 ? q0 + cap_cell : nullptr, gather, st)) return r;
+        n += 1;
       }
-      if (int r = fwd_step(b, m, n, n >= s0 ? qm : 0, qa, qb, gather, st)) return r;
     }
     if (adj_src) {
       if (int r = copy(resid(), adj_src + b.gather_off, b.ngather, st)) return r;
@@ -696,9 +831,16 @@ struct fwi_plan {
             if (int r = zero_fwd(b, st)) return r;
           }
           const int e = std::min((seg + 1) * K, nt);
-          for (int nn = seg * K; nn < e; ++nn) {
+          for (int nn = seg * K; nn < e;) {
             float* wa = qbuf() + (size_t)(nn - seg * K) * slot;
-            if (int r = fwd_step(b, m, nn, qm, wa, wa + cap_cell, nullptr, st)) return r;
+            if (pairs(b) && nn + 1 < e) {
+              if (int r = fwd_pair(b, m, nn, qm, wa, wa + cap_cell, wa + slot, wa + slot + cap_cell, nullptr, st))
+                return r;
+              nn += 2;
+            } else {
+              if (int r = fwd_step(b, m, nn, qm, wa, wa + cap_cell, nullptr, st)) return r;
+              nn += 1;
+            }
           }
         }
         qa = qbuf() + (size_t)(n - seg * K) * slot;
@@ -795,11 +937,40 @@ struct fwi_plan {
           order_adj = o;
         }
       }
+    // fused core step pairs: rows per item
+    if (inc() && opt.step_pairs != 0) {
+      std::vector<int> rows_p = {16, 32, 48, 64};
+      if (opt.rows_per_item > 0) rows_p = {opt.rows_per_item};
+      best = 1e30f;
+      for (int c : rows_p) {
+        if (c > nz - 8 && c != rows_p[0]) continue;
+        if (int r = build_pair_items(b, c)) return r;
+        if (rows_p.size() == 1 || b.nitems_core == 0) {
+          tz_pair = c;
+          break;
+        }
+        if (int r = zero_fwd(b, st)) return r;
+        if (int r = fwd_pair(b, m, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, st)) return r;
+        float ms = 0;
+        if (int r = timed([&]() -> int {
+              for (int n = 2; n < 8; n += 2)
+                if (int r2 = fwd_pair(b, m, n, 0, nullptr, nullptr, nullptr, nullptr, nullptr, st)) return r2;
+              return 0;
+            }, ms))
+          return r;
+        if (ms < best) {
+          best = ms;
+          tz_pair = c;
+        }
+      }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     for (auto& bb : batches) {
       if (int r = build_items(bb, tz, order)) return r;
       if (int r = build_adj_items(bb, tz_adj, order_adj)) return r;
+      if (inc() && opt.step_pairs != 0)
+        if (int r = build_pair_items(bb, tz_pair)) return r;
     }
     tuned = true;
     return 0;
@@ -841,6 +1012,7 @@ void fwi_options_default(fwi_options* o) {
   o->arith = FWI_ARITH_FP32I;
   o->item_order = -1;
   o->use_graph = 1;
+  o->step_pairs = 0;
 }
 
 int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out) {
@@ -1055,6 +1227,7 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     }
     if (rc == 0) rc = p->build_items(b, p->tz, p->opt.item_order > 0 ? p->opt.item_order : 0);
     if (rc == 0) rc = p->build_adj_items(b, p->tz, p->opt.item_order == 2 ? 2 : 0);
+    if (rc == 0 && p->inc() && p->opt.step_pairs != 0) rc = p->build_pair_items(b, p->tz_pair);
   }
   if (rc) {
     delete p;
@@ -1153,7 +1326,7 @@ int fwi_plan_shot_misfits(fwi_plan* p, double* out, void* stream) {
 void fwi_plan_set_timing(fwi_plan* p, int stride) {
   p->timing_collect();
   p->timing_stride = stride;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 5; ++k) {
     p->kt_ms[k] = 0;
     p->kt_count[k] = 0;
     p->launch_no[k] = 0;
@@ -1161,7 +1334,7 @@ void fwi_plan_set_timing(fwi_plan* p, int stride) {
 }
 
 int fwi_plan_kernel_time(fwi_plan* p, int kind, double* total_ms, long long* count) {
-  if (kind < 0 || kind > 2) return 1;
+  if (kind < 0 || kind > 4) return 1;
   p->timing_collect();
   *total_ms = p->kt_ms[kind];
   *count = p->kt_count[kind];
@@ -1186,6 +1359,9 @@ int fwi_plan_graph_stats(const fwi_plan* p, long long* captures, long long* repl
   if (captures) *captures = p->graph_captures;
   if (replays) *replays = p->graph_replays;
   return 0;
+}
+int fwi_plan_pair_rows(const fwi_plan* p) {
+  return (p->inc() && p->opt.step_pairs != 0 && !p->batches.empty() && p->batches[0].nitems_core > 0) ? p->tz_pair : 0;
 }
 int fwi_plan_adj_tiling(const fwi_plan* p, int* rows, int* order, int* grouped) {
   if (rows) *rows = p->tz_adj;

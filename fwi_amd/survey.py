@@ -65,7 +65,7 @@ class Plan:
                  wavelet: np.ndarray, cpml_ab, grad_mode: int = GRAD_EXACT, arith: int = ARITH_FP32I,
                  ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0,
                  item_order: int = -1, grad_group: int = 0, ckpt_format: int = 0, use_graph: int = 1,
-                 store_segments: int = 0):
+                 store_segments: int = 0, step_pairs: int = 0):
         L = _lib.load()
         self.nx, self.nz, self.nt, self.dh, self.dt, self.cpml = nx, nz, nt, dh, dt, cpml
         self.shots = list(shots)
@@ -92,6 +92,7 @@ class Plan:
         opt.ckpt_format = ckpt_format
         opt.use_graph = use_graph
         opt.store_segments = store_segments
+        opt.step_pairs = step_pairs
         h = ctypes.c_void_p()
         _lib.check(L.fwi_plan_create(ctypes.byref(sv), ctypes.byref(opt), ctypes.byref(h)), "fwi_plan_create")
         self._h = h
@@ -123,6 +124,11 @@ class Plan:
     @property
     def ckpt_interval(self) -> int:
         return int(_lib.load().fwi_plan_ckpt_interval(self._h))
+
+    @property
+    def pair_rows(self) -> int:
+        """Rows per fused core item of the step pairs (0: single steps only)."""
+        return int(_lib.load().fwi_plan_pair_rows(self._h))
 
     @property
     def kept_steps(self) -> int:

@@ -78,6 +78,10 @@ typedef struct fwi_options {
   int use_graph;       /* 1 (default): each batch's device work of fwi_forward / fwi_misfit / fwi_gradient /
                           fwi_gradient_src is captured once as a HIP graph per set of caller buffers and
                           replayed on later calls with the same buffers; 0: launched eagerly */
+  int step_pairs;      /* increment form: 1 advances the core of every window (4+ cells from the C-PML) two steps
+                          per launch and the rest by single steps; 0 (default): single steps everywhere.  The
+                          result is the same either way; on survey1600 the pairs are not faster yet (the fused
+                          kernel gains ~10 % per cell, the separate shell launches lose more). */
 } fwi_options;
 
 typedef struct fwi_plan fwi_plan;
@@ -166,7 +170,8 @@ int fwi_dot3(const double* a, const double* b, long long n, double* out, void* s
  * main.c:323,368-372).  With stride > 0, every stride-th launch of each fused step kernel is bracketed by
  * HIP events on the stream it runs on; fwi_plan_kernel_time returns the summed event time and the
  * number of timed launches of a kind (0 forward step, 1 forward step storing imaging terms, 2 adjoint
- * step) since the last fwi_plan_set_timing call. */
+ * step, 3 / 4 fused core step pair without / with stored imaging terms) since the last fwi_plan_set_timing
+ * call. */
 void fwi_plan_set_timing(fwi_plan* plan, int stride);
 int fwi_plan_kernel_time(fwi_plan* plan, int kind, double* total_ms, long long* count);
 /* Cells of all shot windows (nx_s*nz summed over shots) and, in *strip_cells, their C-PML cells. */
@@ -180,6 +185,8 @@ int fwi_plan_kept_steps(const fwi_plan* plan);
 int fwi_plan_rows_per_item(const fwi_plan* plan);
 int fwi_plan_item_order(const fwi_plan* plan);
 int fwi_plan_adj_tiling(const fwi_plan* plan, int* rows_per_item, int* item_order, int* grouped);
+/* Rows per fused core item of the step pairs in use (0 when the plan takes single steps only). */
+int fwi_plan_pair_rows(const fwi_plan* plan);
 /* Graph captures and replays so far (use_graph). */
 int fwi_plan_graph_stats(const fwi_plan* plan, long long* captures, long long* replays);
 

@@ -273,3 +273,31 @@ def test_adjoint128_checkpoint28_fp32(torch):
         errs.append(abs((Jp - Jm) / (2 * h) - pred) / abs(pred))
     print(f"adjoint128 FP32 FD relative errors (h = 1e-1, 3e-2, 1e-2): {errs}")
     assert min(errs) < 2e-3
+
+
+def test_step_pairs_identical_to_single_steps(torch):
+    """Fused core step pairs (an option) against single steps everywhere (the default): gathers, misfit and exact and
+    reference-compatible gradients bit for bit, with an odd checkpoint interval (pairs stop at checkpoints and
+    at the first kept step), recomputed segments of odd length, and ragged windows, one of them too narrow to
+    have a core."""
+    w = small_workload(nx=200, nz=72, nt=150, cpml=12, nshots=3)
+    w.shots = [M.Shot(x0=0, nx=120, isx=60, isz=6, irz=6, ir0=20, nrec=70),
+               M.Shot(x0=30, nx=170, isx=90, isz=7, irz=6, ir0=12, nrec=170 - 24),
+               M.Shot(x0=150, nx=44, isx=20, isz=6, irz=8, ir0=14, nrec=16)]
+    say("step pairs vs single steps: 3 ragged shots (one without core), K=7, 1 kept segment")
+    tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+    obs = torch.from_numpy(_cat(oracle_gathers(w, "f32i", sq_true, tab, src))).cuda()
+    model = _dev_model(torch, sq_init)
+    for exact in (1, 0):
+        res = []
+        for pairs in (1, 0):
+            plan = _plan(w, tab, src, ckpt_interval=7, store_segments=1, grad_mode=exact, step_pairs=pairs)
+            assert (plan.pair_rows > 0) == bool(pairs)
+            g = plan.forward(model)
+            phi, G, syn = plan.gradient(model, obs, want_syn=True)
+            torch.cuda.synchronize()
+            res.append((g.cpu().numpy(), phi, G.cpu().numpy(), syn.cpu().numpy()))
+        (g1, p1, G1, s1), (g0, p0, G0, s0) = res
+        assert np.array_equal(g1, g0) and np.array_equal(s1, s0)
+        assert p1 == p0
+        assert np.array_equal(G1, G0), exact
