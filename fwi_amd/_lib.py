@@ -85,6 +85,7 @@ BATCHED = {
     "fwi_lbfgs_restore": ([vp_, _F, _F, _F, vp_], c_int),
     "fwi_lbfgs_get_direction": ([vp_, vp_, vp_], c_int),
     "fwi_lbfgs_pairs": ([vp_], c_int),
+    "fwi_model_clamp": ([vp_, vp_, vp_, c_ll, ctypes.c_float, ctypes.c_float, vp_], c_int),
     "fwi_dot3": ([vp_, vp_, c_ll, ctypes.POINTER(c_double), vp_], c_int),
     "fwi_plan_set_timing": ([vp_, c_int], None),
     "fwi_plan_kernel_time": ([vp_, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll)], c_int),

@@ -117,6 +117,18 @@ __global__ void __launch_bounds__(256) k_update3(float* __restrict__ vh, float* 
   vn[i] = nf;
 }
 
+// Physical bounds: each squared velocity clamped to [lo, hi], then v_hor >= v_nmo (the pseudo-acoustic
+// system is stable for v_hor >= v_nmo >= 0 and v_po >= 0 within the time-step limit).
+__global__ void __launch_bounds__(256) k_clamp3(float* __restrict__ vh, float* __restrict__ vp, float* __restrict__ vn,
+                                                long long n, float lo, float hi) {
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= n) return;
+  const float h = fminf(fmaxf(vh[i], lo), hi), p = fminf(fmaxf(vp[i], lo), hi), m = fminf(fmaxf(vn[i], lo), hi);
+  vh[i] = h < m ? m : h;
+  vp[i] = p;
+  vn[i] = m;
+}
+
 }  // namespace
 }  // namespace fwi
 
@@ -334,3 +346,19 @@ int fwi_dot3(const double* a, const double* b, long long n, double* out, void* s
 }
 
 }  // extern "C"
+
+int fwi_model_clamp(float* vh, float* vp, float* vn, long long n, float lo, float hi, void* stream) {
+  if (!(lo >= 0.f) || !(hi >= lo)) {
+    set_message("fwi_model_clamp: need 0 <= lo <= hi");
+    return 1;
+  }
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_clamp3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, vh, vp, vn, n, lo,
+                     hi);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_message(std::string("fwi_model_clamp: ") + hipGetErrorString(e));
+    return (int)e;
+  }
+  return 0;
+}

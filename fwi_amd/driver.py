@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import models as M
-from .optim import LBFGS, SD_STEP, LineSearch
+from .optim import LBFGS, SD_STEP, LineSearch, clamp_model
 from .survey import Plan, cpml_table, pre_variables
 
 
@@ -38,9 +38,12 @@ def _dev(a):
 
 
 def run(w: M.Workload, stages=(None,), iterations: int = 100, memory: int = 5, it_lbfgs: int = 1,
-        grad_mode: int = 1, verbose: bool = True, **plan_kw):
+        grad_mode: int = 1, verbose: bool = True, bounds=None, **plan_kw):
     """Invert `w` from its initial model; stages are Ricker frequencies (None = the workload's).
+    bounds = (vmin, vmax) in m/s clamps every updated model (fwi_model_clamp); None keeps the reference's
+    update, whose only bound is v_hor >= v_nmo.
     Returns (final (v_hor, v_po, v_nmo) device tensors, list of StageLog)."""
+    bound = (lambda: clamp_model(model, *bounds)) if bounds else (lambda: None)
     true_sq = pre_variables(*(_dev(x) for x in w.true_model))
     model = list(pre_variables(*(_dev(x) for x in w.init_model)))
     logs = []
@@ -63,11 +66,13 @@ def run(w: M.Workload, stages=(None,), iterations: int = 100, memory: int = 5, i
                 print(json.dumps({"stage_f": f, "iteration": n, "misfit": phi}), flush=True)
             if n < it_lbfgs:
                 opt.sd_step(model, grad, SD_STEP)
+                bound()
                 continue
             opt.direction(model, grad)
 
             def evaluate(alpha):
                 opt.trial(model, alpha)
+                bound()
                 return plan.misfit(model, obs)
 
             acc, nt_ = ls.run(phi, evaluate, lambda: opt.restore(model))
@@ -90,10 +95,12 @@ def main():
     ap.add_argument("--shots", type=int, default=0)
     ap.add_argument("--nt", type=int, default=0)
     ap.add_argument("--out", default="")
+    ap.add_argument("--bounds", default="", help="vmin,vmax in m/s: clamp every updated model")
     args = ap.parse_args()
     w = M.workload(args.workload, nshots=args.shots or None, nt=args.nt or None)
     stages = tuple(float(x) for x in args.stages.split(",")) if args.stages else (None,)
-    model, logs = run(w, stages=stages, iterations=args.iterations)
+    bounds = tuple(float(x) for x in args.bounds.split(",")) if args.bounds else None
+    model, logs = run(w, stages=stages, iterations=args.iterations, bounds=bounds)
     for lg in logs:
         print(json.dumps(dataclasses.asdict(lg)))
     if args.out:

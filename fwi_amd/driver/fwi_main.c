@@ -11,7 +11,9 @@
  * line-search rules are the reference's (main.c:38-62, 260-262, 275-333, 539-806).
  *
  * Usage: fwi_main --dir DIR [--nx 841 --nz 270 --dh 20 --dt 2e-3 --tend 5 --cpml 20 --nshots 15 --f 3
- *                  --stages 2 --iterations 100 --mode exact|refcompat]
+ *                  --stages 2 --iterations 100 --mode exact|refcompat --vmin V --vmax V]
+ * --vmin/--vmax (m/s, optional) clamp every updated model with fwi_model_clamp; without them the update is
+ * the reference's, whose only bound is v_hor >= v_nmo.
  * DIR holds vp_ori.bin vh_ori.bin vn_ori.bin vp_ini.bin vh_ini.bin vn_ini.bin (raw float32, x fastest:
  * vp, epsilon, delta); results are written to DIR/v_hor.bin, v_po.bin, v_nmo.bin.
  */
@@ -81,6 +83,7 @@ int main(int argc, char** argv) {
   int CPML = 20, nx = 841, nz = 270, nshots = 15, stages = 2, iterations = 100, exact = 1;
   float dh = 20, dt = 2e-3f, tend = 5, f = 3;
   const char* dir = NULL;
+  float vmin = -1, vmax = -1;
   for (int i = 1; i + 1 < argc; i += 2) {
     const char* k = argv[i];
     const char* v = argv[i + 1];
@@ -96,6 +99,8 @@ int main(int argc, char** argv) {
     else if (!strcmp(k, "--stages")) stages = atoi(v);
     else if (!strcmp(k, "--iterations")) iterations = atoi(v);
     else if (!strcmp(k, "--mode")) exact = strcmp(v, "refcompat") != 0;
+    else if (!strcmp(k, "--vmin")) vmin = (float)atof(v);
+    else if (!strcmp(k, "--vmax")) vmax = (float)atof(v);
     else {
       fprintf(stderr, "fwi_main: unknown option %s\n", k);
       return 2;
@@ -181,6 +186,8 @@ int main(int argc, char** argv) {
       if (it_n < 1) { /* main.c:539-568 */
         const double sd[3] = {3e5, 2e5, 2.5e5};
         if (fwi_lbfgs_sd_step(opt, v_hor, v_po, v_nmo, grad, sd, NULL)) die("fwi_lbfgs_sd_step");
+        if (vmax > 0 && fwi_model_clamp(v_hor, v_po, v_nmo, n, vmin > 0 ? vmin * vmin : 0.f, vmax * vmax, NULL))
+          die("fwi_model_clamp");
         continue;
       }
       if (fwi_lbfgs_direction(opt, v_hor, v_po, v_nmo, grad, NULL)) die("fwi_lbfgs_direction");
@@ -189,6 +196,8 @@ int main(int argc, char** argv) {
       int stop = 0;
       while (trial > phi) {
         if (fwi_lbfgs_trial(opt, v_hor, v_po, v_nmo, alpha, NULL)) die("fwi_lbfgs_trial");
+        if (vmax > 0 && fwi_model_clamp(v_hor, v_po, v_nmo, n, vmin > 0 ? vmin * vmin : 0.f, vmax * vmax, NULL))
+          die("fwi_model_clamp");
         if (fwi_misfit(plan, v_hor, v_po, v_nmo, obs, &trial, NULL)) die("fwi_misfit");
         if (isnan(trial)) {
           trial = 1e30;

@@ -22,6 +22,15 @@ from .survey import _stream_ptr
 SD_STEP = (3e5, 2e5, 2.5e5)  # v_hor, v_po, v_nmo step lengths of the first iteration (main.c:567)
 
 
+def clamp_model(model, vmin: float, vmax: float, stream=None):
+    """Physical bounds on the squared velocities (v_hor, v_po, v_nmo): each to [vmin^2, vmax^2], then
+    v_hor >= v_nmo (fwi_model_clamp; the reference has only the last bound)."""
+    for t in model:
+        assert t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+    _lib.check(_lib.load().fwi_model_clamp(*[t.data_ptr() for t in model], model[0].numel(), float(vmin) ** 2,
+                                           float(vmax) ** 2, _stream_ptr(stream)), "fwi_model_clamp")
+
+
 class LBFGS:
     """Device optimizer state for the blocks (v_hor, v_po, v_nmo) of an nz x nx model, memory m."""
 

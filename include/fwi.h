@@ -163,6 +163,11 @@ int fwi_lbfgs_restore(fwi_lbfgs* h, float* v_hor, float* v_po, float* v_nmo, voi
 /* Copy the current direction r (FP64 3*n) and the number of stored pairs. */
 int fwi_lbfgs_get_direction(fwi_lbfgs* h, double* out, void* stream);
 int fwi_lbfgs_pairs(const fwi_lbfgs* h);
+/* Physical bounds for a driver (no counterpart in the reference, whose only bound is v_hor >= v_nmo): clamp
+ * every squared velocity of the three fields (n cells each) to [lo, hi], then v_hor >= v_nmo.  An L-BFGS step
+ * from few curvature pairs can otherwise drive a squared velocity negative, and the coupled system then
+ * grows without bound. */
+int fwi_model_clamp(float* v_hor, float* v_po, float* v_nmo, long long n, float lo, float hi, void* stream);
 /* Deterministic FP64 dot products of three blocks of n: out (host double[3]). */
 int fwi_dot3(const double* a, const double* b, long long n, double* out, void* stream);
 

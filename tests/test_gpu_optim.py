@@ -141,3 +141,19 @@ def test_c_driver_runs_reference_flow(torch, tmp_path):
     for name in ("v_hor.bin", "v_po.bin", "v_nmo.bin"):
         a = fio.read_raw(os.path.join(tmp_path, name), 120, 80)
         assert np.isfinite(a).all()
+
+
+def test_model_clamp_matches_numpy(torch):
+    """fwi_model_clamp: each squared velocity to [vmin^2, vmax^2], then v_hor >= v_nmo."""
+    from fwi_amd.optim import clamp_model
+
+    rng = np.random.default_rng(2)
+    m = [(rng.standard_normal((30, 40)) * 1e7 + 5e6).astype(np.float32) for _ in range(3)]
+    dev = [torch.from_numpy(x.copy()).cuda() for x in m]
+    clamp_model(dev, 1400.0, 5000.0)
+    torch.cuda.synchronize()
+    lo, hi = np.float32(1400.0 ** 2), np.float32(5000.0 ** 2)
+    c = [np.minimum(np.maximum(x, lo), hi) for x in m]
+    c[0] = np.where(c[0] < c[2], c[2], c[0])
+    for k in range(3):
+        np.testing.assert_array_equal(dev[k].cpu().numpy(), c[k])
