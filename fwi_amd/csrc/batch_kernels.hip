@@ -590,7 +590,11 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
 // g_hor, g_nmo, g_po of the group's slot, which spans the union of its windows: 6 B of gradient traffic per
 // cell and shot instead of 24 when the windows coincide.  Items are padded to whole workgroups, so every
 // wave reaches every barrier.
-template <class S, bool GROUP = false>
+//
+// EDGE: the launch over the work items that need C-PML terms (bands within reach of a side strip, rows from
+// nz-cpml-4 down); the other launch runs the interior items, whose rows are software-pipelined.  Registers are
+// allocated per kernel, so the interior kernel does not pay for the C-PML paths.
+template <class S, bool GROUP, bool EDGE>
 __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   __shared__ float xbuf[4][72];
   __shared__ float gsum[GROUP ? 2 : 1][4][3][64];
@@ -603,6 +607,32 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const ShotDev sd = a.g.shots[itm.x];
   const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
+  const int z0 = itm.z, z1 = itm.w;
+  const bool img = a.qa != nullptr;
+  // GROUP: field `wave` of the group's slot, read-modify-written by waves 0..2 on the model columns of the
+  // union window; other lanes (and wave 3) address past the resource, so their accesses drop out
+  const int mc = sd.x0 + ix;  // model column of the lane, the same in every wave of the workgroup
+  const bool gown = GROUP && img && wave < 3 && mc >= sd.gx0 && mc < sd.gx0 + sd.gnx;
+  const brsrc grs = make_rsrc((wave == 0 ? a.gh : wave == 1 ? a.gn : a.gp) + sd.gslot, 4u * sd.gps * nz);
+  const unsigned gcol = gown ? 4u * (unsigned)(mc - sd.gx0) : kOutOfRange;
+  const unsigned grow = 4u * (unsigned)sd.gps;
+  if (GROUP && ghost) {
+    // placeholder wave: contributes zeros, takes part in the row barriers and in the slot's update
+    if (!img) return;
+    float gv = bld<kBufNT>(grs, gcol + grow * z0);
+    for (int z = z0; z < z1; ++z) {
+      const float g0 = gv;
+      gv = bld<kBufNT>(grs, gcol + grow * min(z + 1, z1 - 1));
+      float(*gs)[3][64] = gsum[GROUP ? (z & 1) : 0];
+      gs[wave][0][lane] = 0.f;
+      gs[wave][1][lane] = 0.f;
+      gs[wave][2][lane] = 0.f;
+      __syncthreads();
+      const float sum = ((gs[0][wave][lane] + gs[1][wave][lane]) + gs[2][wave][lane]) + gs[3][wave][lane];
+      bst<kBufNT>(grs, gcol + grow * z, g0 + sum);
+    }
+    return;
+  }
   const bool act = ix >= kRim && ix < nx - kRim;
   const int cx = clampi(ix, 0, nx - 1);
   const int hxr = itm.y + halo_col(lane);
@@ -622,105 +652,149 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const float* vn = a.m.vn + sd.x0;
   const float* ztz = a.ztz + sd.off;
   const float* ztx = a.ztx + sd.off;
+  const float* ptx = a.ptx + sd.off;
+  const float* ptz = a.ptz + sd.off;
+  const float* qa = img ? a.qa + sd.off : L;
+  const float* qb = img ? a.qb + sd.off : L;
   float* buf = xbuf[wave];
-  const int z0 = itm.z, z1 = itm.w;
   const int zb = nz - cpml;      // first row of the bottom C-PML strip
   const int zc = nz - cpml - 4;  // rows at or below this need the C-PML terms in z
-  float gq[9];         // gbar_z along z (rows z-4..z+4)
-  float lq[5], mq[5];  // mP, mF at rows z..z+4
-  if (!ghost) {
+  float* gh_ = a.gh + sd.gslot;  // per-shot gradient (ungrouped)
+  float* gn_ = a.gn + sd.gslot;
+  float* gp_ = a.gp + sd.gslot;
+
+  float gq[9];  // gbar_z along z (rows z-4..z+4)
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int r = z0 - 4 + k;
-      const unsigned ir = (unsigned)(cx + ps * r);
-      const float l = ldu(L, ir);
-      const float m = ldu(M, ir);
-      float v = dt2 * (ldu(vp, (unsigned)(cx + nxm * r)) * (l + m));
-      if (r >= zb) v = v + coef_z(a.g.ta, r, nz, cpml) * ldu(ztz, ir);
-      gq[k] = (act && r >= kRim) ? v : 0.f;
-      if (k >= 4) {
-        lq[k - 4] = l;
-        mq[k - 4] = m;
-      }
-    }
+  for (int k = 0; k < 8; ++k) {
+    const int r = z0 - 4 + k;
+    const unsigned ir = (unsigned)(cx + ps * r);
+    float v = dt2 * (ldu(vp, (unsigned)(cx + nxm * r)) * (ldu(L, ir) + ldu(M, ir)));
+    if (r >= zb) v = v + coef_z(a.g.ta, r, nz, cpml) * ldu(ztz, ir);
+    gq[k] = (act && r >= kRim) ? v : 0.f;
   }
-  for (int z = z0; z < z1; ++z) {
+  // The loads of a row are issued one row ahead, before the previous row's stores (the vector memory counter
+  // retires in issue order, so a load issued after a store can only be waited for together with it), and the
+  // loop is unrolled by two over two row buffers, so no register copy waits for them.
+  struct Row {
+    float l4, m4, vp4, zt4;                // row z+4: mP, mF, v_po, zt_z (bottom C-PML rows)
+    float lc, mc, vhc, vnc;                // row z at the lane's column: mP, mF (an L2 hit), v_hor, v_nmo
+    float vhh, vnh, lh, mh;                // row z at the halo column: v_hor, v_nmo, mP, mF
+    float xc, xh, pc, ph;                  // zt_x and a_x psi-bar_tot at the lane's and the halo column (XN)
+    float lold, mold, qa, qb, g0, g1, g2;  // centre-only values: m^{k+2}, imaging terms, gradient
+  };
+  // XN: band near a side strip; ZB: rows z >= zc (bottom C-PML terms, zt_z of row z+4)
+  auto load_row = [&](auto xn_t, auto zb_t, int z, Row& r) __attribute__((always_inline)) {
+    constexpr bool XN = decltype(xn_t)::value, ZB = decltype(zb_t)::value;
     const unsigned o = (unsigned)(cx + ps * z);
-    float ch = 0.f, cn = 0.f, cp = 0.f;  // this shot's imaging contributions (GROUP)
-    if (!ghost) {
     const unsigned mo = (unsigned)(cx + nxm * z);
     const unsigned ho = (unsigned)(hx + ps * z), hmo = (unsigned)(hx + nxm * z);
+    const unsigned ir = o + 4u * ps;
+    r.l4 = ldu(L, ir);
+    r.m4 = ldu(M, ir);
+    r.vp4 = ldu(vp, mo + 4u * nxm);
+    if (ZB) r.zt4 = ldu(ztz, ir);
+    r.lc = ldu(L, o);
+    r.mc = ldu(M, o);
+    r.vhc = ldu(vh, mo);
+    r.vnc = ldu(vn, mo);
+    r.vhh = ldu(vh, hmo);
+    r.lh = ldu(L, ho);
+    r.vnh = ldu(vn, hmo);
+    r.mh = ldu(M, ho);
+    if (XN) {
+      r.xc = ldu(ztx, o);
+      r.xh = ldu(ztx, ho);
+      r.pc = ldu(ptx, o);
+      r.ph = ldu(ptx, ho);
+    }
+    r.lold = ldn(Lo, o);
+    r.mold = ldn(Mo, o);
+    if (img) {
+      r.qa = ldn(qa, o);
+      r.qb = ldn(qb, o);
+      if (GROUP) {
+        r.g0 = bld<kBufNT>(grs, gcol + grow * z);
+      } else {
+        r.g0 = ldn(gh_, o);
+        r.g1 = ldn(gn_, o);
+        r.g2 = ldn(gp_, o);
+      }
+    }
+  };
+  // Row z from cur.  PIPE (interior items): row min(z+1, ze-1) is loaded into nxt once cur's stencil inputs
+  // are consumed and before the row's stores (an unused repeat on the last row keeps one path through the
+  // body).  The edge kernel loads each row at its start: its C-PML inputs leave no registers for a second row.
+  auto step = [&](auto xn_t, auto zb_t, auto pipe_t, int z, int ze, const Row& cur, Row& nxt)
+                  __attribute__((always_inline)) {
+    constexpr bool XN = decltype(xn_t)::value, ZB = decltype(zb_t)::value, PIPE = decltype(pipe_t)::value;
+    const unsigned o = (unsigned)(cx + ps * z);
     {
       const int r = z + 4;
-      const unsigned ir = o + 4u * ps;
-      const float l = ldu(L, ir);
-      const float m = ldu(M, ir);
-      lq[4] = l;
-      mq[4] = m;
-      float v = dt2 * (ldu(vp, mo + 4u * nxm) * (l + m));
-      if (r >= zb) v = v + coef_z(a.g.ta, r, nz, cpml) * ldu(ztz, ir);
+      float v = dt2 * (cur.vp4 * (cur.l4 + cur.m4));
+      if (ZB) v = v + coef_z_uniform(a.g.ta, r, nz, cpml) * cur.zt4;
       gq[8] = (act && r < nz - kRim) ? v : 0.f;
     }
     // gbar_x of the band row (zero outside Omega) and of the halo columns
-    float go = dt2 * (ldu(vh, mo) * lq[0] + ldu(vn, mo) * mq[0]);
-    float gh = dt2 * (ldu(vh, hmo) * ldu(L, ho) + ldu(vn, hmo) * ldu(M, ho));
-    if (xnear) {
-      go = go + axo * ldu(ztx, o);
-      gh = gh + axh * ldu(ztx, ho);
+    float go = dt2 * (cur.vhc * cur.lc + cur.vnc * cur.mc);
+    float gh = dt2 * (cur.vhh * cur.lh + cur.vnh * cur.mh);
+    if (XN) {
+      go = go + axo * cur.xc;
+      gh = gh + axh * cur.xh;
     }
     go = act ? go : 0.f;
     gh = hact ? gh : 0.f;
-    const float lold = ldn(Lo, o), mold = ldn(Mo, o);
+    float pz[8];
+    if (ZB) {
+      const unsigned snx = (unsigned)ps;
+      pz[0] = ldu(ptz, o - 4 * snx);
+      pz[1] = ldu(ptz, o - 3 * snx);
+      pz[2] = ldu(ptz, o - 2 * snx);
+      pz[3] = ldu(ptz, o - snx);
+      pz[4] = ldu(ptz, o + snx);
+      pz[5] = ldu(ptz, o + 2 * snx);
+      pz[6] = ldu(ptz, o + 3 * snx);
+      pz[7] = ldu(ptz, o + 4 * snx);
+    }
+    if (PIPE) load_row(xn_t, zb_t, min(z + 1, ze - 1), nxt);
     float gx[9];
     xrow9(buf, lane, go, gh, gx);
-    if (act) {
-      float ap = d2s<S>(gx[0], gx[1], gx[2], gx[3], gx[4], gx[5], gx[6], gx[7], gx[8], dh2);
-      float af = d2s<S>(gq[0], gq[1], gq[2], gq[3], gq[4], gq[5], gq[6], gq[7], gq[8], dh2);
-      if (xnear) {
-        const float* pt = a.ptx + sd.off;
-        ap = ap - d1s<S>(ldu(pt, o - 4), ldu(pt, o - 3), ldu(pt, o - 2), ldu(pt, o - 1), ldu(pt, o + 1),
-                         ldu(pt, o + 2), ldu(pt, o + 3), ldu(pt, o + 4), dh);
-      }
-      if (z >= zc) {
-        const float* pt = a.ptz + sd.off;
-        const unsigned snx = (unsigned)ps;
-        af = af - d1s<S>(ldu(pt, o - 4 * snx), ldu(pt, o - 3 * snx), ldu(pt, o - 2 * snx), ldu(pt, o - snx),
-                         ldu(pt, o + snx), ldu(pt, o + 2 * snx), ldu(pt, o + 3 * snx), ldu(pt, o + 4 * snx), dh);
-      }
-      float ln = 2.f * lq[0] + ap - lold;
-      float mn = 2.f * mq[0] + af - mold;
-      if (z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec) {
-        const float r = a.res[sd.goff + (long)a.rit * sd.nrec + (ix - sd.ir0)];
+    float ap = d2s<S>(gx[0], gx[1], gx[2], gx[3], gx[4], gx[5], gx[6], gx[7], gx[8], dh2);
+    float af = d2s<S>(gq[0], gq[1], gq[2], gq[3], gq[4], gq[5], gq[6], gq[7], gq[8], dh2);
+    if (XN) {
+      float px[9];
+      xrow9(buf, lane, cur.pc, cur.ph, px);
+      ap = ap - d1s<S>(px[0], px[1], px[2], px[3], px[5], px[6], px[7], px[8], dh);
+    }
+    if (ZB) af = af - d1s<S>(pz[0], pz[1], pz[2], pz[3], pz[4], pz[5], pz[6], pz[7], dh);
+    float ln = 2.f * cur.lc + ap - cur.lold;
+    float mn = 2.f * cur.mc + af - cur.mold;
+    if (z == sd.irz && sd.nrec > 0) {
+      const bool rec = ix >= sd.ir0 && ix < sd.ir0 + sd.nrec;
+      const float r = a.res[sd.goff + (long)a.rit * sd.nrec + (rec ? ix - sd.ir0 : 0)];
+      if (rec) {
         ln += r;
         mn += r;
       }
-      stn(Lo, o, ln);
-      stn(Mo, o, mn);
-      if (a.qa) {
-        const float qa = ldn(a.qa + sd.off, o), qb = ldn(a.qb + sd.off, o);
-        if (GROUP) {
-          ch = dt2 * ln * qa;
-          cn = dt2 * mn * qa;
-          cp = dt2 * (ln + mn) * qb;
-        } else {
-          float* gh_ = a.gh + sd.gslot;
-          float* gn_ = a.gn + sd.gslot;
-          float* gp_ = a.gp + sd.gslot;
-          stn(gh_, o, ldn(gh_, o) + dt2 * ln * qa);
-          stn(gn_, o, ldn(gn_, o) + dt2 * mn * qa);
-          stn(gp_, o, ldn(gp_, o) + dt2 * (ln + mn) * qb);
-        }
+    }
+    // Every lane stores: a lane off the updated cells (the rim, or a column past the shot clamped onto the
+    // rim) writes back the value it read, which no other wave changes.
+    stn(Lo, o, act ? ln : cur.lold);
+    stn(Mo, o, act ? mn : cur.mold);
+    float ch = 0.f, cn = 0.f, cp = 0.f;  // this shot's imaging contributions (GROUP)
+    if (img) {
+      if (GROUP) {
+        ch = act ? dt2 * ln * cur.qa : 0.f;
+        cn = act ? dt2 * mn * cur.qa : 0.f;
+        cp = act ? dt2 * (ln + mn) * cur.qb : 0.f;
+      } else {
+        stn(gh_, o, act ? cur.g0 + dt2 * ln * cur.qa : cur.g0);
+        stn(gn_, o, act ? cur.g1 + dt2 * mn * cur.qa : cur.g1);
+        stn(gp_, o, act ? cur.g2 + dt2 * (ln + mn) * cur.qb : cur.g2);
       }
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) gq[k] = gq[k + 1];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      lq[k] = lq[k + 1];
-      mq[k] = mq[k + 1];
-    }
-    }  // !ghost
-    if (GROUP && a.qa) {
+    if (GROUP && img) {
       // double-buffered by row parity: one barrier per row suffices, since a wave can only overwrite this
       // buffer two rows later, after every wave has passed the next row's barrier
       float(*gs)[3][64] = gsum[GROUP ? (z & 1) : 0];
@@ -728,14 +802,40 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       gs[wave][1][lane] = cn;
       gs[wave][2][lane] = cp;
       __syncthreads();
-      const int mc = sd.x0 + ix;  // model column of the lane, the same in every wave of the workgroup
-      if (wave < 3 && mc >= sd.gx0 && mc < sd.gx0 + sd.gnx) {
-        const float sum = ((gs[0][wave][lane] + gs[1][wave][lane]) + gs[2][wave][lane]) + gs[3][wave][lane];
-        float* g = (wave == 0 ? a.gh : wave == 1 ? a.gn : a.gp) + sd.gslot;
-        const unsigned go = (unsigned)(mc - sd.gx0 + sd.gps * z);
-        stn(g, go, ldn(g, go) + sum);
+      const float sum = ((gs[0][wave][lane] + gs[1][wave][lane]) + gs[2][wave][lane]) + gs[3][wave][lane];
+      bst<kBufNT>(grs, gcol + grow * z, cur.g0 + sum);
+    }
+  };
+  auto rows = [&](auto xn_t, auto zb_t, auto pipe_t, int za, int ze) __attribute__((always_inline)) {
+    if (za >= ze) return;
+    Row ra, rb;
+    if (decltype(pipe_t)::value) {
+      load_row(xn_t, zb_t, za, ra);
+      for (int z = za; z < ze; ++z) {
+        step(xn_t, zb_t, pipe_t, z, ze, ra, rb);
+        ra = rb;
+      }
+    } else {
+      for (int z = za; z < ze; ++z) {
+        load_row(xn_t, zb_t, z, ra);
+        step(xn_t, zb_t, pipe_t, z, ze, ra, rb);
       }
     }
+  };
+  // an item lies entirely above row zc or entirely at or below it (the items are split there); the interior
+  // kernel only receives items above zc on bands out of reach of the side strips
+  using T = std::true_type;
+  using F = std::false_type;
+  if (EDGE) {
+    if (xnear) {
+      if (z0 >= zc) rows(T{}, T{}, F{}, z0, z1);
+      else rows(T{}, F{}, F{}, z0, z1);
+    } else {
+      if (z0 >= zc) rows(F{}, T{}, F{}, z0, z1);
+      else rows(F{}, F{}, F{}, z0, z1);
+    }
+  } else {
+    rows(F{}, F{}, T{}, z0, z1);
   }
 }
 

@@ -81,6 +81,12 @@ __device__ __forceinline__ float coef_x(const float* __restrict__ tab, int i, in
 __device__ __forceinline__ float coef_z(const float* __restrict__ tab, int i, int n, int cpml) {
   return (i > n - cpml - 1) ? tab[n - 1 - i] : 0.f;
 }
+// The same for a wave-uniform row i, read through the scalar cache (constant address space), so that the
+// value lands in a scalar register and its wait does not involve the vector memory counter.
+__device__ __forceinline__ float coef_z_uniform(const float* __restrict__ tab, int i, int n, int cpml) {
+  const __attribute__((address_space(4))) float* t = (const __attribute__((address_space(4))) float*)tab;
+  return (i > n - cpml - 1) ? t[n - 1 - i] : 0.f;
+}
 
 // Wave-private row exchange through LDS.  Lane l holds the value v of band column l; lanes 0..3 also hold
 // the halo value h of column l-4, lanes 4..7 that of column 64+(l-4).  On return u[k] is the value of
@@ -123,6 +129,25 @@ __device__ __forceinline__ void stn(float* base, unsigned idx, float v) {
   __builtin_nontemporal_store(v, reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (idx << 2)));
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Raw buffer access (bounds-checked by the hardware against the resource's byte range): a load at an offset
+// past the range returns 0 and a store there is dropped.  Lane conditions become offsets instead of branches,
+// so a loop body has one path and the compiler's memory-counter waits need not assume a shorter one.
+// The resource is wave-uniform (scalar registers); word 3 is the gfx9 raw-buffer format.
+using brsrc = __amdgpu_buffer_rsrc_t;
+constexpr unsigned kOutOfRange = 0x80000000u;  // byte offset past every resource range used here (< 2 GiB)
+constexpr int kBufNT = 2;                      // cache policy bit: non-temporal
+__device__ __forceinline__ brsrc make_rsrc(const float* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <int POL = 0>
+__device__ __forceinline__ float bld(brsrc r, unsigned byte_off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, POL));
+}
+template <int POL = 0>
+__device__ __forceinline__ void bst(brsrc r, unsigned byte_off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)byte_off, 0, POL);
+}
 // Wave index within the workgroup as a wave-uniform (scalar) value.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 

@@ -45,6 +45,9 @@ struct Batch {
   int nitems = 0;
   int4* d_items_adj = nullptr;  // adjoint work items (grouped: whole workgroups per group and tile)
   int nitems_adj = 0;
+  int nitems_adj_int = 0;  // exact gradient: the first items, in whole workgroups, need no C-PML term
+  long long adj_cells_int = 0, adj_cells_edge = 0;  // shot cells of the interior / edge items
+  long long adj_model_int = 0;                      // model cells under the interior items
   int4* d_items_core = nullptr;  // step pairs: fused core items (56 output columns each)
   int nitems_core = 0;
   int4* d_items_shell = nullptr; // step pairs: single-step items of the shell, with their column ranges
@@ -105,14 +108,32 @@ struct fwi_plan {
   int max_count = 0;                // shots in the largest batch
   unsigned* d_ck_amax = nullptr;    // 16-bit checkpoints: max-abs bits per (checkpoint, shot, field)
   // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute),
-  // 2 = adjoint step, 3 / 4 = fused core step pair without / with store; every `timing_stride`-th launch of
-  // a kind is bracketed by events on its stream
+  // 2 = adjoint step (exact gradient: its interior kernel), 3 / 4 = fused core step pair without / with store,
+  // 5 = exact adjoint edge kernel; every `timing_stride`-th launch of a kind is bracketed by events on its stream
+  static constexpr int kKinds = 6;
   int timing_stride = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_open;  // (kind, event index of the start; end = start + 1)
-  double kt_ms[5] = {0, 0, 0, 0, 0};
-  long long kt_count[5] = {0, 0, 0, 0, 0};
-  long long launch_no[5] = {0, 0, 0, 0, 0};
+  double kt_ms[kKinds] = {};
+  long long kt_count[kKinds] = {};
+  long long launch_no[kKinds] = {};
+  // The exact adjoint runs its C-PML work (strip kernels, edge items) on a side stream, forked from and joined
+  // back to the step's stream around each step, concurrently with the interior kernel.
+  hipStream_t side_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int side_fork(hipStream_t st) {
+    if (!side_stream) CK(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
+    if (!ev_fork) CK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    if (!ev_join) CK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    CK(hipEventRecord(ev_fork, st));
+    CK(hipStreamWaitEvent(side_stream, ev_fork, 0));
+    return 0;
+  }
+  int side_join(hipStream_t st) {
+    CK(hipEventRecord(ev_join, side_stream));
+    CK(hipStreamWaitEvent(st, ev_join, 0));
+    return 0;
+  }
 
   int timed_begin(int kind, hipStream_t st) {
     if (timing_stride <= 0 || (launch_no[kind]++ % timing_stride) != 0) return -1;
@@ -266,6 +287,9 @@ struct fwi_plan {
   ~fwi_plan() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    if (side_stream) (void)hipStreamDestroy(side_stream);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     for (auto& b : batches) {
       (void)hipFree(b.d_shots);
       (void)hipFree(b.d_items);
@@ -288,12 +312,26 @@ struct fwi_plan {
 
   // ---------------------------------------------------------------------------------------------
   // Work items: (shot, 64-column band, first row, end row) per wave, four per workgroup.
-  static std::vector<int4> ungrouped_items(const Batch& b, int nz, int rows, int order) {
+  // Row runs of `rows` rows over [kRim, nz-kRim); a run that contains row zsplit (> its first row) is cut there.
+  static std::vector<std::pair<int, int>> row_runs(int nz, int rows, int zsplit) {
+    std::vector<std::pair<int, int>> runs;
+    for (int z0 = kRim; z0 < nz - kRim; z0 += rows) {
+      const int z1 = std::min(z0 + rows, nz - kRim);
+      if (zsplit > z0 && zsplit < z1) {
+        runs.push_back({z0, zsplit});
+        runs.push_back({zsplit, z1});
+      } else {
+        runs.push_back({z0, z1});
+      }
+    }
+    return runs;
+  }
+
+  static std::vector<int4> ungrouped_items(const Batch& b, int nz, int rows, int order, int zsplit = 0) {
     std::vector<int4> items;
     int maxnx = 0;
     for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
-    std::vector<std::pair<int, int>> runs;
-    for (int z0 = kRim; z0 < nz - kRim; z0 += rows) runs.push_back({z0, std::min(z0 + rows, nz - kRim)});
+    const std::vector<std::pair<int, int>> runs = row_runs(nz, rows, zsplit);
     if (order == 1) {
       for (int s = 0; s < b.count; ++s)
         for (auto& r : runs)
@@ -319,10 +357,9 @@ struct fwi_plan {
   // Grouped adjoint items: every workgroup is one gradient group on one tile, member k on wave k, and
   // -(leader+1) (a placeholder wave) where the group has fewer than 4 shots.  Orders: 0 row run -> band ->
   // group; 2 band -> group -> row run.
-  static std::vector<int4> grouped_items(const Batch& b, int nz, int rows, int order) {
+  static std::vector<int4> grouped_items(const Batch& b, int nz, int rows, int order, int zsplit) {
     std::vector<int4> items;
-    std::vector<std::pair<int, int>> runs;
-    for (int z0 = kRim; z0 < nz - kRim; z0 += rows) runs.push_back({z0, std::min(z0 + rows, nz - kRim)});
+    const std::vector<std::pair<int, int>> runs = row_runs(nz, rows, zsplit);
     int maxnx = 0;
     for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
     // one workgroup: the group's members on the 64 model columns from bm, each at its window's offset
@@ -424,10 +461,52 @@ struct fwi_plan {
     return 0;
   }
 
-  // Work items of the adjoint kernels: grouped (orders 0 and 2) when the batch has gradient groups.
+  // Work items of the adjoint kernels: grouped (orders 0 and 2) when the batch has gradient groups.  No item
+  // crosses row nz-cpml-4, the first row whose z stencil reaches the bottom C-PML strip: the exact adjoint
+  // kernel runs the rows of an item with or without the bottom C-PML terms, by its first row.
+  // For the exact gradient the items are partitioned, in whole workgroups and keeping their order, into
+  // interior workgroups (no member needs a C-PML term) followed by edge workgroups, one launch each.
   int build_adj_items(Batch& b, int rows, int order) {
-    return upload_items(b.d_items_adj, b.nitems_adj,
-                        b.grouped ? grouped_items(b, nz, rows, order) : ungrouped_items(b, nz, rows, order));
+    const int zsplit = nz - cpml - 4;
+    std::vector<int4> items = b.grouped ? grouped_items(b, nz, rows, order, zsplit)
+                                        : ungrouped_items(b, nz, rows, order, zsplit);
+    b.nitems_adj_int = 0;
+    if (opt.grad_mode == FWI_GRAD_EXACT) {
+      auto interior = [&](const int4& it) {
+        if (it.x < 0) return true;  // a placeholder wave computes nothing
+        const int nxs = b.h_shots[it.x].nx;
+        return it.y >= cpml + 4 && it.y + 64 <= nxs - cpml - 4 && it.w <= zsplit;
+      };
+      std::vector<int4> in, edge;
+      for (size_t w = 0; w < items.size(); w += 4) {
+        const size_t e = std::min(items.size(), w + 4);
+        bool all = true;
+        for (size_t k = w; k < e; ++k) all = all && interior(items[k]);
+        (all ? in : edge).insert((all ? in : edge).end(), items.begin() + w, items.begin() + e);
+      }
+      b.nitems_adj_int = (int)in.size();
+      auto cells = [&](const std::vector<int4>& v, std::vector<char>* mask) {
+        long long c = 0;
+        for (const int4& it : v) {
+          if (it.x < 0) continue;
+          const ShotDev& d = b.h_shots[it.x];
+          const int xa = std::max(it.y, 0), xb = std::min(it.y + 64, d.nx);
+          if (xb <= xa) continue;
+          c += (long long)(xb - xa) * (it.w - it.z);
+          if (mask)
+            for (int z = it.z; z < it.w; ++z)
+              for (int x = xa; x < xb; ++x) (*mask)[(size_t)z * nx + d.x0 + x] = 1;
+        }
+        return c;
+      };
+      std::vector<char> mask((size_t)nx * nz, 0);
+      b.adj_cells_int = cells(in, &mask);
+      b.adj_cells_edge = cells(edge, nullptr);
+      b.adj_model_int = std::count(mask.begin(), mask.end(), (char)1);
+      items = in;
+      items.insert(items.end(), edge.begin(), edge.end());
+    }
+    return upload_items(b.d_items_adj, b.nitems_adj, items);
   }
 
   GridArgs grid_args(const Batch& b) const {
@@ -569,6 +648,26 @@ struct fwi_plan {
   }
 
 
+  template <bool EDGE>
+  static int adj_ex_launch(bool grouped, int am, dim3 grid, const AdjExArgs& a, hipStream_t st) {
+    if (grouped) {
+      if (am == FWI_ARITH_MIXED)
+        hipLaunchKernelGGL((k_adj_ex_main<double, true, EDGE>), grid, dim3(256), 0, st, a);
+      else if (am == FWI_ARITH_FP32D)
+        hipLaunchKernelGGL((k_adj_ex_main<F32D, true, EDGE>), grid, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_adj_ex_main<float, true, EDGE>), grid, dim3(256), 0, st, a);
+    } else {
+      if (am == FWI_ARITH_MIXED)
+        hipLaunchKernelGGL((k_adj_ex_main<double, false, EDGE>), grid, dim3(256), 0, st, a);
+      else if (am == FWI_ARITH_FP32D)
+        hipLaunchKernelGGL((k_adj_ex_main<F32D, false, EDGE>), grid, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_adj_ex_main<float, false, EDGE>), grid, dim3(256), 0, st, a);
+    }
+    return 0;
+  }
+
   // one adjoint step k (produces m^k from m^{k+1}, m^{k+2}); q slot pointers may be null
   int adj_step(const Batch& b, const ModelArgs& m, int k, const float* qa, const float* qb, hipStream_t st) {
     const int cur = k & 1;
@@ -645,23 +744,42 @@ struct fwi_plan {
       a.gh = sgrad(0);
       a.gp = sgrad(1);
       a.gn = sgrad(2);
+      // The interior items touch no C-PML array, so the C-PML chain (strip kernels, then the edge items)
+      // runs on the side stream while the interior kernel runs on st; both write disjoint cells.
+      const int nin = b.nitems_adj_int, ned = b.nitems_adj - b.nitems_adj_int;
+      const bool side = nin > 0 && (ned > 0 || b.nsblocks > 0);
+      hipStream_t s2 = st;
+      if (side) {
+        if (int r = side_fork(st)) return r;
+        s2 = side_stream;
+      }
       if (b.nsblocks > 0) {
-        hipLaunchKernelGGL(k_adj_ex_strip1, dim3(b.nsblocks), dim3(256), 0, st, sa, a.dt2, Lc, Mc, a.zx, a.zz, a.ztx,
+        hipLaunchKernelGGL(k_adj_ex_strip1, dim3(b.nsblocks), dim3(256), 0, s2, sa, a.dt2, Lc, Mc, a.zx, a.zz, a.ztx,
                            a.ztz);
-        ARITH_LAUNCH(k_adj_ex_strip2, dim3(b.nsblocks), sa, a);
-      }
-      const int tev = timed_begin(2, st);
-      if (b.grouped) {
         if (am == FWI_ARITH_MIXED)
-          hipLaunchKernelGGL((k_adj_ex_main<double, true>), grid, dim3(256), 0, st, a);
+          hipLaunchKernelGGL(k_adj_ex_strip2<double>, dim3(b.nsblocks), dim3(256), 0, s2, sa, a);
         else if (am == FWI_ARITH_FP32D)
-          hipLaunchKernelGGL((k_adj_ex_main<F32D, true>), grid, dim3(256), 0, st, a);
+          hipLaunchKernelGGL(k_adj_ex_strip2<F32D>, dim3(b.nsblocks), dim3(256), 0, s2, sa, a);
         else
-          hipLaunchKernelGGL((k_adj_ex_main<float, true>), grid, dim3(256), 0, st, a);
-      } else {
-        ARITH_LAUNCH(k_adj_ex_main, grid, a);
+          hipLaunchKernelGGL(k_adj_ex_strip2<float>, dim3(b.nsblocks), dim3(256), 0, s2, sa, a);
       }
-      timed_end(tev, st);
+      if (ned > 0) {
+        AdjExArgs ae = a;
+        ae.items = b.d_items_adj + nin;
+        ae.nitems = ned;
+        const int tev = timed_begin(5, s2);
+        if (int r = adj_ex_launch<true>(b.grouped, am, dim3((ned + 3) / 4), ae, s2)) return r;
+        timed_end(tev, s2);
+      }
+      if (nin > 0) {
+        a.items = b.d_items_adj;
+        a.nitems = nin;
+        const int tev = timed_begin(2, st);
+        if (int r = adj_ex_launch<false>(b.grouped, am, dim3((nin + 3) / 4), a, st)) return r;
+        timed_end(tev, st);
+      }
+      if (side)
+        if (int r = side_join(st)) return r;
     }
 #undef ARITH_LAUNCH
     CK(hipGetLastError());
@@ -1326,7 +1444,7 @@ int fwi_plan_shot_misfits(fwi_plan* p, double* out, void* stream) {
 void fwi_plan_set_timing(fwi_plan* p, int stride) {
   p->timing_collect();
   p->timing_stride = stride;
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < fwi_plan::kKinds; ++k) {
     p->kt_ms[k] = 0;
     p->kt_count[k] = 0;
     p->launch_no[k] = 0;
@@ -1334,7 +1452,7 @@ void fwi_plan_set_timing(fwi_plan* p, int stride) {
 }
 
 int fwi_plan_kernel_time(fwi_plan* p, int kind, double* total_ms, long long* count) {
-  if (kind < 0 || kind > 4) return 1;
+  if (kind < 0 || kind >= fwi_plan::kKinds) return 1;
   p->timing_collect();
   *total_ms = p->kt_ms[kind];
   *count = p->kt_count[kind];
@@ -1349,6 +1467,20 @@ long long fwi_plan_cells(const fwi_plan* p, long long* strip_cells) {
   }
   if (strip_cells) *strip_cells = s;
   return c;
+}
+
+int fwi_plan_adj_split(const fwi_plan* p, long long* cells_interior, long long* cells_edge,
+                       long long* model_interior) {
+  long long a = 0, e = 0, m = 0;
+  for (auto& b : p->batches) {
+    a += b.adj_cells_int;
+    e += b.adj_cells_edge;
+    m += b.adj_model_int;
+  }
+  if (cells_interior) *cells_interior = a;
+  if (cells_edge) *cells_edge = e;
+  if (model_interior) *model_interior = m;
+  return (int)p->batches.size();
 }
 
 int fwi_plan_ckpt_interval(const fwi_plan* p) { return p->K; }
