@@ -226,29 +226,18 @@ def main():
         assert phi_i == phi
         log(f"instrumented eager gradient: {sec_instr:.3f} s")
 
-    # roofline of the adjoint kernel (dominant): algorithmic bytes per launch / mean event time.  The exact
-    # gradient's adjoint step is two kernels: the interior items (kind 2, the dominant one, measured here) and
-    # the edge items with C-PML terms (kind 5), which run concurrently on a side stream.
+    # roofline of the adjoint step kernel (dominant): algorithmic bytes per launch / mean event time
     adj_ms, adj_n = plan.kernel_time(2)
-    edge_ms, edge_n = plan.kernel_time(5)
     fwd_ms, fwd_n = plan.kernel_time(0)
     rec_ms, rec_n = plan.kernel_time(1)
     nmodel = w.nx * w.nz
-    adj_cells, adj_model, adj_kernel_cells = ncell, nmodel, "all cells"
-    if grad_mode == 1:
-        ci, ce, mi, nb = plan.adj_split()
-        adj_cells, adj_model = ci / nb, mi / nb
-        adj_kernel_cells = f"interior items: {ci / nb:.0f} of {(ci + ce) / nb:.0f} item cells per launch"
-    adj_bytes = 56.0 * adj_cells + 12.0 * adj_model
+    adj_bytes = 56.0 * ncell + 12.0 * nmodel
     adj_avg = adj_ms / max(adj_n, 1) / 1e3
     achieved = adj_bytes / adj_avg / 1e9 if adj_n else None
     fwd_bytes = 24.0 * ncell + 12.0 * nmodel
     fwd_avg = fwd_ms / max(fwd_n, 1) / 1e3
     if adj_n:
-        log(f"adjoint step ({adj_kernel_cells}): {adj_n} timed launches, mean {adj_avg * 1e6:.1f} us -> "
-            f"{achieved:.0f} GB/s algorithmic")
-    if edge_n:
-        log(f"adjoint edge kernel (side stream): {edge_n} timed launches, mean {edge_ms / edge_n * 1e3:.1f} us")
+        log(f"adjoint step: {adj_n} timed launches, mean {adj_avg * 1e6:.1f} us -> {achieved:.0f} GB/s algorithmic")
     pair_ms, pair_n = plan.kernel_time(3)
     pairs_ms, pairs_n = plan.kernel_time(4)
     if pair_n or pairs_n:
@@ -261,7 +250,7 @@ def main():
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "fused_steps_survey1600_hbm_traffic.json")
-    kname = "k_adj_ex_main interior" if grad_mode == 1 else "k_adj_ref_main"
+    kname = "k_adj_ex_main" if grad_mode == 1 else "k_adj_ref_main"
     if os.path.exists(tfile) and w.name == "survey1600" and len(w.shots) == 64:
         with open(tfile) as fh:
             traffic = json.load(fh)["bytes_per_launch"].get(kname)
@@ -329,7 +318,6 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": traffic,
             "bytes_per_launch": adj_bytes,
-            "cells_per_launch": adj_kernel_cells,
             "mean_launch_us": round(adj_avg * 1e6, 2) if adj_n else None,
         },
         "cpu_baseline": cpu,

@@ -90,7 +90,6 @@ BATCHED = {
     "fwi_plan_set_timing": ([vp_, c_int], None),
     "fwi_plan_kernel_time": ([vp_, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll)], c_int),
     "fwi_plan_cells": ([vp_, ctypes.POINTER(c_ll)], c_ll),
-    "fwi_plan_adj_split": ([vp_, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)], c_int),
     "fwi_plan_shot_misfits": ([vp_, ctypes.POINTER(c_double), vp_], c_int),
     "fwi_plan_ckpt_interval": ([vp_], c_int),
     "fwi_plan_rows_per_item": ([vp_], c_int),

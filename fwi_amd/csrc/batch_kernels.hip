@@ -49,6 +49,109 @@ __global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __r
   }
 }
 
+// The same step on an interior work item: a band within [cpml, nx-cpml) and rows above nz-cpml, where no cell
+// carries a C-PML term and every lane updates its cell.  The body has one path (no lane-masked memory access)
+// and loads each row one row ahead, before the previous row's stores: the vector memory counter retires in
+// issue order, so a load issued after a store could only be waited for together with that store.  The
+// stored values are those of k_fwd_main, expression by expression (its C-PML terms are +0 here).
+template <int QM, class S, bool INC>
+__device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, const ShotDev& sd, int lane,
+                                             float* buf) {
+  const int nx = sd.nx, ps = sd.ps, nxm = a.m.nxm;
+  const int ix = itm.y + lane;
+  const int hx = lane < 8 ? clampi(itm.y + halo_col(lane), 0, nx - 1) : ix;
+  const float dh2 = a.g.dh2, C = a.C;
+  const float* P = a.Pc + sd.off;
+  const float* F = a.Fc + sd.off;
+  float* Po = a.Po + sd.off;
+  float* Fo = a.Fo + sd.off;
+  const float* Qc = INC ? a.Qc + sd.off : nullptr;
+  const float* Rc = INC ? a.Rc + sd.off : nullptr;
+  float* Qo = INC ? a.Qo + sd.off : nullptr;
+  float* Ro = INC ? a.Ro + sd.off : nullptr;
+  const float* vh = a.m.vh + sd.x0;
+  const float* vp = a.m.vp + sd.x0;
+  const float* vn = a.m.vn + sd.x0;
+  const float srcv = a.src[a.it];
+  const int z0 = itm.z, z1 = itm.w;
+  float fq[9];  // F along z (rows z-4..z+4)
+  struct Row {
+    float f4;                // F at row z+4
+    float c, h;              // P at row z: the lane's column, the halo column
+    float pold, fold;        // two-level form: P^{n-1}, F^{n-1}; increment form: Q^n, R^n
+    float vhv, vpv, vnv;     // model at row z
+  };
+  auto load_row = [&](int z, Row& r) __attribute__((always_inline)) {
+    const unsigned o = (unsigned)(ix + ps * z);
+    const unsigned mo = (unsigned)(ix + nxm * z);
+    r.f4 = ldu(F, o + 4u * ps);
+    r.c = ldu(P, o);
+    r.h = ldu(P, (unsigned)(hx + ps * z));
+    r.pold = INC ? ldn(Qc, o) : ldn(Po, o);
+    r.fold = INC ? ldn(Rc, o) : ldn(Fo, o);
+    r.vhv = ldu(vh, mo);
+    r.vpv = ldu(vp, mo);
+    r.vnv = ldu(vn, mo);
+  };
+  auto step = [&](int z, const Row& cur, Row& nxt) __attribute__((always_inline)) {
+    const unsigned o = (unsigned)(ix + ps * z);
+    fq[8] = cur.f4;
+    float u[9];
+    xrow9(buf, lane, cur.c, cur.h, u);
+    load_row(min(z + 1, z1 - 1), nxt);  // an unused repeat on the last row keeps one path
+    const float p2 = d2s<S>(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8], dh2);
+    const float r2 = d2s<S>(fq[0], fq[1], fq[2], fq[3], fq[4], fq[5], fq[6], fq[7], fq[8], dh2);
+    const float atp = 0.f, atf = 0.f;
+    const float tp = dh2 * (p2 + atp);
+    const float tf = dh2 * (r2 + atf);
+    float pn, fn;
+    if (INC) {
+      float dq = cur.vhv * C * tp + cur.vpv * C * tf;
+      float dr = cur.vnv * C * tp + cur.vpv * C * tf;
+      if (ix == sd.isx && z == sd.isz) {
+        dq += srcv;
+        dr += srcv;
+      }
+      const float qn = cur.pold + dq, rn = cur.fold + dr;
+      pn = cur.c + qn;
+      fn = fq[4] + rn;
+      stn(Qo, o, qn);
+      stn(Ro, o, rn);
+    } else {
+      pn = 2.f * cur.c - cur.pold + cur.vhv * C * tp + cur.vpv * C * tf;
+      fn = 2.f * fq[4] - cur.fold + cur.vnv * C * tp + cur.vpv * C * tf;
+      if (ix == sd.isx && z == sd.isz) {
+        pn += srcv;
+        fn += srcv;
+      }
+    }
+    stn(Po, o, pn);
+    stn(Fo, o, fn);
+    if (QM == 1) {
+      stn(a.qa + sd.off, o, p2 + atp);
+      stn(a.qb + sd.off, o, r2 + atf);
+    } else if (QM == 2) {
+      stn(a.qa + sd.off, o, p2);
+      stn(a.qb + sd.off, o, r2);
+    }
+    if (a.gather && z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec)
+      a.gather[sd.goff + (long)a.it * sd.nrec + (ix - sd.ir0)] = pn + fn;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) fq[k] = fq[k + 1];
+  };
+#pragma unroll
+  for (int k = 0; k < 8; ++k) fq[k] = ldu(F, (unsigned)(ix + ps * (z0 - 4 + k)));
+  // The first row is peeled, so the loop is entered, as it is repeated, after a row's stores that follow the
+  // next row's loads: the memory-counter waits at its head are the same on both paths.
+  Row ra, rb;
+  load_row(z0, ra);
+  step(z0, ra, rb);
+  for (int z = z0 + 1; z < z1; ++z) {
+    ra = rb;
+    step(z, ra, rb);
+  }
+}
+
 // One forward step of every shot (parallelcalls.cu:120-130 SecondDerivate_P, :65-86 ZETA_F, :88-117
 // propagator_F).  QM selects what is kept for the adjoint pass: 0 nothing, 1 the model-multiplied terms
 // a = p2 + aten_p, b = r2 + aten_f (exact gradient), 2 p2, r2 (reference-compatible gradient).
@@ -66,6 +169,13 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
   const int4 itm = a.items[item];
   const ShotDev sd = a.g.shots[itm.x];
   const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
+  // Interior items (no C-PML cell; the items are cut at row nz-e, e = max(cpml,4)) take the pipelined body.
+  // Items with C-PML cells take the general body below, interleaved with them in the launch.
+  const int e = cpml > kRim ? cpml : kRim;
+  if (!a.xr && itm.y >= e && itm.y + 64 <= nx - e && itm.w <= nz - e) {
+    fwd_interior<QM, S, INC>(a, itm, sd, lane, xbuf[wave]);
+    return;
+  }
   const int ix = itm.y + lane;
   bool act = ix >= kRim && ix < nx - kRim;
   if (a.xr) {  // shell items of the step-pair scheme update only their column range
@@ -591,10 +701,11 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
 // cell and shot instead of 24 when the windows coincide.  Items are padded to whole workgroups, so every
 // wave reaches every barrier.
 //
-// EDGE: the launch over the work items that need C-PML terms (bands within reach of a side strip, rows from
-// nz-cpml-4 down); the other launch runs the interior items, whose rows are software-pipelined.  Registers are
-// allocated per kernel, so the interior kernel does not pay for the C-PML paths.
-template <class S, bool GROUP, bool EDGE>
+// Interior items (bands out of reach of the side strips, rows above nz-cpml-4) are software-pipelined; items
+// that need C-PML terms take a general body per row, interleaved with them in the launch.  Every path of a
+// wave passes one barrier per row, so a workgroup may mix them.
+// IMG: the step adds imaging terms (a.qa, a.qb are set), a launch constant so that the body has one path.
+template <class S, bool GROUP, bool IMG>
 __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   __shared__ float xbuf[4][72];
   __shared__ float gsum[GROUP ? 2 : 1][4][3][64];
@@ -608,7 +719,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   const int ix = itm.y + lane;
   const int z0 = itm.z, z1 = itm.w;
-  const bool img = a.qa != nullptr;
+  constexpr bool img = IMG;
   // GROUP: field `wave` of the group's slot, read-modify-written by waves 0..2 on the model columns of the
   // union window; other lanes (and wave 3) address past the resource, so their accesses drop out
   const int mc = sd.x0 + ix;  // model column of the lane, the same in every wave of the workgroup
@@ -810,10 +921,13 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     if (za >= ze) return;
     Row ra, rb;
     if (decltype(pipe_t)::value) {
+      // the first row is peeled, so the loop is entered, as it is repeated, after a row's stores that follow
+      // the next row's loads: the memory-counter waits at its head are the same on both paths
       load_row(xn_t, zb_t, za, ra);
-      for (int z = za; z < ze; ++z) {
-        step(xn_t, zb_t, pipe_t, z, ze, ra, rb);
+      step(xn_t, zb_t, pipe_t, za, ze, ra, rb);
+      for (int z = za + 1; z < ze; ++z) {
         ra = rb;
+        step(xn_t, zb_t, pipe_t, z, ze, ra, rb);
       }
     } else {
       for (int z = za; z < ze; ++z) {
@@ -822,20 +936,15 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       }
     }
   };
-  // an item lies entirely above row zc or entirely at or below it (the items are split there); the interior
-  // kernel only receives items above zc on bands out of reach of the side strips
+  // an item lies entirely above row zc or entirely at or below it (the items are split there)
   using T = std::true_type;
   using F = std::false_type;
-  if (EDGE) {
-    if (xnear) {
-      if (z0 >= zc) rows(T{}, T{}, F{}, z0, z1);
-      else rows(T{}, F{}, F{}, z0, z1);
-    } else {
-      if (z0 >= zc) rows(F{}, T{}, F{}, z0, z1);
-      else rows(F{}, F{}, F{}, z0, z1);
-    }
+  if (xnear) {
+    if (z0 >= zc) rows(T{}, T{}, F{}, z0, z1);
+    else rows(T{}, F{}, F{}, z0, z1);
   } else {
-    rows(F{}, F{}, T{}, z0, z1);
+    if (z0 >= zc) rows(F{}, T{}, F{}, z0, z1);
+    else rows(F{}, F{}, T{}, z0, z1);
   }
 }
 

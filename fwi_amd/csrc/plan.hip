@@ -45,9 +45,6 @@ struct Batch {
   int nitems = 0;
   int4* d_items_adj = nullptr;  // adjoint work items (grouped: whole workgroups per group and tile)
   int nitems_adj = 0;
-  int nitems_adj_int = 0;  // exact gradient: the first items, in whole workgroups, need no C-PML term
-  long long adj_cells_int = 0, adj_cells_edge = 0;  // shot cells of the interior / edge items
-  long long adj_model_int = 0;                      // model cells under the interior items
   int4* d_items_core = nullptr;  // step pairs: fused core items (56 output columns each)
   int nitems_core = 0;
   int4* d_items_shell = nullptr; // step pairs: single-step items of the shell, with their column ranges
@@ -108,32 +105,15 @@ struct fwi_plan {
   int max_count = 0;                // shots in the largest batch
   unsigned* d_ck_amax = nullptr;    // 16-bit checkpoints: max-abs bits per (checkpoint, shot, field)
   // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute),
-  // 2 = adjoint step (exact gradient: its interior kernel), 3 / 4 = fused core step pair without / with store,
-  // 5 = exact adjoint edge kernel; every `timing_stride`-th launch of a kind is bracketed by events on its stream
-  static constexpr int kKinds = 6;
+  // 2 = adjoint step, 3 / 4 = fused core step pair without / with store; every `timing_stride`-th launch of a
+  // kind is bracketed by events on its stream
+  static constexpr int kKinds = 5;
   int timing_stride = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_open;  // (kind, event index of the start; end = start + 1)
   double kt_ms[kKinds] = {};
   long long kt_count[kKinds] = {};
   long long launch_no[kKinds] = {};
-  // The exact adjoint runs its C-PML work (strip kernels, edge items) on a side stream, forked from and joined
-  // back to the step's stream around each step, concurrently with the interior kernel.
-  hipStream_t side_stream = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int side_fork(hipStream_t st) {
-    if (!side_stream) CK(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
-    if (!ev_fork) CK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    if (!ev_join) CK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-    CK(hipEventRecord(ev_fork, st));
-    CK(hipStreamWaitEvent(side_stream, ev_fork, 0));
-    return 0;
-  }
-  int side_join(hipStream_t st) {
-    CK(hipEventRecord(ev_join, side_stream));
-    CK(hipStreamWaitEvent(st, ev_join, 0));
-    return 0;
-  }
 
   int timed_begin(int kind, hipStream_t st) {
     if (timing_stride <= 0 || (launch_no[kind]++ % timing_stride) != 0) return -1;
@@ -287,9 +267,6 @@ struct fwi_plan {
   ~fwi_plan() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
-    if (side_stream) (void)hipStreamDestroy(side_stream);
-    if (ev_fork) (void)hipEventDestroy(ev_fork);
-    if (ev_join) (void)hipEventDestroy(ev_join);
     for (auto& b : batches) {
       (void)hipFree(b.d_shots);
       (void)hipFree(b.d_items);
@@ -312,22 +289,25 @@ struct fwi_plan {
 
   // ---------------------------------------------------------------------------------------------
   // Work items: (shot, 64-column band, first row, end row) per wave, four per workgroup.
-  // Row runs of `rows` rows over [kRim, nz-kRim); a run that contains row zsplit (> its first row) is cut there.
-  static std::vector<std::pair<int, int>> row_runs(int nz, int rows, int zsplit) {
+  // Row runs of `rows` rows over [kRim, nz-kRim); a run is cut at every split row strictly inside it.
+  static std::vector<std::pair<int, int>> row_runs(int nz, int rows, std::vector<int> splits) {
+    std::sort(splits.begin(), splits.end());
     std::vector<std::pair<int, int>> runs;
     for (int z0 = kRim; z0 < nz - kRim; z0 += rows) {
       const int z1 = std::min(z0 + rows, nz - kRim);
-      if (zsplit > z0 && zsplit < z1) {
-        runs.push_back({z0, zsplit});
-        runs.push_back({zsplit, z1});
-      } else {
-        runs.push_back({z0, z1});
-      }
+      int a = z0;
+      for (int sp : splits)
+        if (sp > a && sp < z1) {
+          runs.push_back({a, sp});
+          a = sp;
+        }
+      runs.push_back({a, z1});
     }
     return runs;
   }
 
-  static std::vector<int4> ungrouped_items(const Batch& b, int nz, int rows, int order, int zsplit = 0) {
+  static std::vector<int4> ungrouped_items(const Batch& b, int nz, int rows, int order,
+                                           const std::vector<int>& zsplit = {}) {
     std::vector<int4> items;
     int maxnx = 0;
     for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
@@ -357,7 +337,8 @@ struct fwi_plan {
   // Grouped adjoint items: every workgroup is one gradient group on one tile, member k on wave k, and
   // -(leader+1) (a placeholder wave) where the group has fewer than 4 shots.  Orders: 0 row run -> band ->
   // group; 2 band -> group -> row run.
-  static std::vector<int4> grouped_items(const Batch& b, int nz, int rows, int order, int zsplit) {
+  static std::vector<int4> grouped_items(const Batch& b, int nz, int rows, int order,
+                                         const std::vector<int>& zsplit) {
     std::vector<int4> items;
     const std::vector<std::pair<int, int>> runs = row_runs(nz, rows, zsplit);
     int maxnx = 0;
@@ -391,6 +372,32 @@ struct fwi_plan {
     return items;
   }
 
+  // Work items with C-PML cells take longer per row than interior items.  Workgroups are dealt to the XCDs in
+  // contiguous ranges of the item order (xcd_logical_block) and each XCD dispatches its range in order, so
+  // within every range the units (of `unit` items: 4 for a workgroup that must stay together) containing such
+  // an item are moved to the front, keeping the order otherwise; their longer rows then overlap the rest.
+  template <class IsEdge>
+  static void edge_first(std::vector<int4>& items, int unit, IsEdge&& is_edge) {
+    const long long nwg = ((long long)items.size() + 3) / 4;
+    const long long q = nwg / 8, r = nwg % 8;
+    std::vector<int4> out;
+    out.reserve(items.size());
+    for (long long x = 0; x < 8; ++x) {
+      const long long w0 = x * q + std::min(x, r), w1 = w0 + q + (x < r ? 1 : 0);
+      const size_t i0 = (size_t)std::min<long long>(4 * w0, (long long)items.size());
+      const size_t i1 = (size_t)std::min<long long>(4 * w1, (long long)items.size());
+      std::vector<int4> rest;
+      for (size_t u = i0; u < i1; u += unit) {
+        const size_t e = std::min(i1, u + unit);
+        bool edge = false;
+        for (size_t k = u; k < e; ++k) edge = edge || is_edge(items[k]);
+        (edge ? out : rest).insert((edge ? out : rest).end(), items.begin() + u, items.begin() + e);
+      }
+      out.insert(out.end(), rest.begin(), rest.end());
+    }
+    items.swap(out);
+  }
+
   int upload_items(int4*& dst, int& n, const std::vector<int4>& items) {
     (void)hipFree(dst);
     dst = nullptr;
@@ -409,7 +416,16 @@ struct fwi_plan {
   //   2  band -> shot group of 4 -> row run -> shot: consecutive workgroups walk down a column of tiles,
   //      so z halos are reused from the L2 while the neighbour is still resident.
   // Workgroup ids are remapped so each XCD receives a contiguous range of this order.
-  int build_items(Batch& b, int rows, int order) { return upload_items(b.d_items, b.nitems, ungrouped_items(b, nz, rows, order)); }
+  // Forward items, cut at row nz-e (e = max(cpml,4), the first bottom C-PML row): the kernel runs items without
+  // C-PML cells with its pipelined interior body; the others go first in each XCD's range.
+  int build_items(Batch& b, int rows, int order) {
+    const int e = std::max(cpml, kRim);
+    std::vector<int4> items = ungrouped_items(b, nz, rows, order, {nz - e});
+    edge_first(items, 1, [&](const int4& it) {
+      return !(it.y >= e && it.y + 64 <= b.h_shots[it.x].nx - e && it.w <= nz - e);
+    });
+    return upload_items(b.d_items, b.nitems, items);
+  }
   // Step-pair items: the fused core items (56 output columns, runs of `rows`, row run -> band -> shot) and the
   // shell's single-step items with the column range each one updates.  The core of a window is columns
   // [cpml+4, nx-cpml-4) and rows [4, nz-cpml-4); a window without a core is all shell.
@@ -464,48 +480,16 @@ struct fwi_plan {
   // Work items of the adjoint kernels: grouped (orders 0 and 2) when the batch has gradient groups.  No item
   // crosses row nz-cpml-4, the first row whose z stencil reaches the bottom C-PML strip: the exact adjoint
   // kernel runs the rows of an item with or without the bottom C-PML terms, by its first row.
-  // For the exact gradient the items are partitioned, in whole workgroups and keeping their order, into
-  // interior workgroups (no member needs a C-PML term) followed by edge workgroups, one launch each.
   int build_adj_items(Batch& b, int rows, int order) {
     const int zsplit = nz - cpml - 4;
-    std::vector<int4> items = b.grouped ? grouped_items(b, nz, rows, order, zsplit)
-                                        : ungrouped_items(b, nz, rows, order, zsplit);
-    b.nitems_adj_int = 0;
-    if (opt.grad_mode == FWI_GRAD_EXACT) {
-      auto interior = [&](const int4& it) {
-        if (it.x < 0) return true;  // a placeholder wave computes nothing
+    std::vector<int4> items = b.grouped ? grouped_items(b, nz, rows, order, {zsplit})
+                                        : ungrouped_items(b, nz, rows, order, {zsplit});
+    if (opt.grad_mode == FWI_GRAD_EXACT)
+      edge_first(items, b.grouped ? 4 : 1, [&](const int4& it) {
+        if (it.x < 0) return false;  // a placeholder wave computes nothing
         const int nxs = b.h_shots[it.x].nx;
-        return it.y >= cpml + 4 && it.y + 64 <= nxs - cpml - 4 && it.w <= zsplit;
-      };
-      std::vector<int4> in, edge;
-      for (size_t w = 0; w < items.size(); w += 4) {
-        const size_t e = std::min(items.size(), w + 4);
-        bool all = true;
-        for (size_t k = w; k < e; ++k) all = all && interior(items[k]);
-        (all ? in : edge).insert((all ? in : edge).end(), items.begin() + w, items.begin() + e);
-      }
-      b.nitems_adj_int = (int)in.size();
-      auto cells = [&](const std::vector<int4>& v, std::vector<char>* mask) {
-        long long c = 0;
-        for (const int4& it : v) {
-          if (it.x < 0) continue;
-          const ShotDev& d = b.h_shots[it.x];
-          const int xa = std::max(it.y, 0), xb = std::min(it.y + 64, d.nx);
-          if (xb <= xa) continue;
-          c += (long long)(xb - xa) * (it.w - it.z);
-          if (mask)
-            for (int z = it.z; z < it.w; ++z)
-              for (int x = xa; x < xb; ++x) (*mask)[(size_t)z * nx + d.x0 + x] = 1;
-        }
-        return c;
-      };
-      std::vector<char> mask((size_t)nx * nz, 0);
-      b.adj_cells_int = cells(in, &mask);
-      b.adj_cells_edge = cells(edge, nullptr);
-      b.adj_model_int = std::count(mask.begin(), mask.end(), (char)1);
-      items = in;
-      items.insert(items.end(), edge.begin(), edge.end());
-    }
+        return it.y < cpml + 4 || it.y + 64 > nxs - cpml - 4 || it.z >= zsplit;
+      });
     return upload_items(b.d_items_adj, b.nitems_adj, items);
   }
 
@@ -648,23 +632,29 @@ struct fwi_plan {
   }
 
 
-  template <bool EDGE>
-  static int adj_ex_launch(bool grouped, int am, dim3 grid, const AdjExArgs& a, hipStream_t st) {
+  template <bool IMG>
+  static void adj_ex_launch_img(bool grouped, int am, dim3 grid, const AdjExArgs& a, hipStream_t st) {
     if (grouped) {
       if (am == FWI_ARITH_MIXED)
-        hipLaunchKernelGGL((k_adj_ex_main<double, true, EDGE>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_adj_ex_main<double, true, IMG>), grid, dim3(256), 0, st, a);
       else if (am == FWI_ARITH_FP32D)
-        hipLaunchKernelGGL((k_adj_ex_main<F32D, true, EDGE>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_adj_ex_main<F32D, true, IMG>), grid, dim3(256), 0, st, a);
       else
-        hipLaunchKernelGGL((k_adj_ex_main<float, true, EDGE>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_adj_ex_main<float, true, IMG>), grid, dim3(256), 0, st, a);
     } else {
       if (am == FWI_ARITH_MIXED)
-        hipLaunchKernelGGL((k_adj_ex_main<double, false, EDGE>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_adj_ex_main<double, false, IMG>), grid, dim3(256), 0, st, a);
       else if (am == FWI_ARITH_FP32D)
-        hipLaunchKernelGGL((k_adj_ex_main<F32D, false, EDGE>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_adj_ex_main<F32D, false, IMG>), grid, dim3(256), 0, st, a);
       else
-        hipLaunchKernelGGL((k_adj_ex_main<float, false, EDGE>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_adj_ex_main<float, false, IMG>), grid, dim3(256), 0, st, a);
     }
+  }
+  static int adj_ex_launch(bool grouped, int am, dim3 grid, const AdjExArgs& a, hipStream_t st) {
+    if (a.qa)
+      adj_ex_launch_img<true>(grouped, am, grid, a, st);
+    else
+      adj_ex_launch_img<false>(grouped, am, grid, a, st);
     return 0;
   }
 
@@ -744,42 +734,14 @@ struct fwi_plan {
       a.gh = sgrad(0);
       a.gp = sgrad(1);
       a.gn = sgrad(2);
-      // The interior items touch no C-PML array, so the C-PML chain (strip kernels, then the edge items)
-      // runs on the side stream while the interior kernel runs on st; both write disjoint cells.
-      const int nin = b.nitems_adj_int, ned = b.nitems_adj - b.nitems_adj_int;
-      const bool side = nin > 0 && (ned > 0 || b.nsblocks > 0);
-      hipStream_t s2 = st;
-      if (side) {
-        if (int r = side_fork(st)) return r;
-        s2 = side_stream;
-      }
       if (b.nsblocks > 0) {
-        hipLaunchKernelGGL(k_adj_ex_strip1, dim3(b.nsblocks), dim3(256), 0, s2, sa, a.dt2, Lc, Mc, a.zx, a.zz, a.ztx,
+        hipLaunchKernelGGL(k_adj_ex_strip1, dim3(b.nsblocks), dim3(256), 0, st, sa, a.dt2, Lc, Mc, a.zx, a.zz, a.ztx,
                            a.ztz);
-        if (am == FWI_ARITH_MIXED)
-          hipLaunchKernelGGL(k_adj_ex_strip2<double>, dim3(b.nsblocks), dim3(256), 0, s2, sa, a);
-        else if (am == FWI_ARITH_FP32D)
-          hipLaunchKernelGGL(k_adj_ex_strip2<F32D>, dim3(b.nsblocks), dim3(256), 0, s2, sa, a);
-        else
-          hipLaunchKernelGGL(k_adj_ex_strip2<float>, dim3(b.nsblocks), dim3(256), 0, s2, sa, a);
+        ARITH_LAUNCH(k_adj_ex_strip2, dim3(b.nsblocks), sa, a);
       }
-      if (ned > 0) {
-        AdjExArgs ae = a;
-        ae.items = b.d_items_adj + nin;
-        ae.nitems = ned;
-        const int tev = timed_begin(5, s2);
-        if (int r = adj_ex_launch<true>(b.grouped, am, dim3((ned + 3) / 4), ae, s2)) return r;
-        timed_end(tev, s2);
-      }
-      if (nin > 0) {
-        a.items = b.d_items_adj;
-        a.nitems = nin;
-        const int tev = timed_begin(2, st);
-        if (int r = adj_ex_launch<false>(b.grouped, am, dim3((nin + 3) / 4), a, st)) return r;
-        timed_end(tev, st);
-      }
-      if (side)
-        if (int r = side_join(st)) return r;
+      const int tev = timed_begin(2, st);
+      if (int r = adj_ex_launch(b.grouped, am, grid, a, st)) return r;
+      timed_end(tev, st);
     }
 #undef ARITH_LAUNCH
     CK(hipGetLastError());
@@ -1467,20 +1429,6 @@ long long fwi_plan_cells(const fwi_plan* p, long long* strip_cells) {
   }
   if (strip_cells) *strip_cells = s;
   return c;
-}
-
-int fwi_plan_adj_split(const fwi_plan* p, long long* cells_interior, long long* cells_edge,
-                       long long* model_interior) {
-  long long a = 0, e = 0, m = 0;
-  for (auto& b : p->batches) {
-    a += b.adj_cells_int;
-    e += b.adj_cells_edge;
-    m += b.adj_model_int;
-  }
-  if (cells_interior) *cells_interior = a;
-  if (cells_edge) *cells_edge = e;
-  if (model_interior) *model_interior = m;
-  return (int)p->batches.size();
 }
 
 int fwi_plan_ckpt_interval(const fwi_plan* p) { return p->K; }

@@ -162,13 +162,6 @@ class Plan:
         c = _lib.load().fwi_plan_cells(self._h, ctypes.byref(s))
         return int(c), int(s.value)
 
-    def adj_split(self):
-        """Exact gradient: (interior item cells, edge item cells, model cells under the interior items, batches)
-        of the adjoint kernels (timed as kinds 2 and 5)."""
-        a, e, m = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
-        nb = _lib.load().fwi_plan_adj_split(self._h, ctypes.byref(a), ctypes.byref(e), ctypes.byref(m))
-        return int(a.value), int(e.value), int(m.value), int(nb)
-
     def shot_misfits(self) -> np.ndarray:
         """Per-shot FP64 misfits of the last misfit or gradient call (they sum, in shot order, to the total)."""
         out = np.zeros(len(self.shots), np.float64)
@@ -180,9 +173,8 @@ class Plan:
         _lib.load().fwi_plan_set_timing(self._h, stride)
 
     def kernel_time(self, kind: int):
-        """(total ms, launches timed) of kind 0 forward step, 1 forward step storing, 2 adjoint step (exact
-        gradient: its interior kernel), 3 / 4 fused core step pair without / with store, 5 exact adjoint edge
-        kernel (concurrent with kind 2 on a side stream)."""
+        """(total ms, launches timed) of kind 0 forward step, 1 forward step storing, 2 adjoint step, 3 / 4 fused
+        core step pair without / with store."""
         ms = ctypes.c_double()
         n = ctypes.c_longlong()
         _lib.check(_lib.load().fwi_plan_kernel_time(self._h, kind, ctypes.byref(ms), ctypes.byref(n)),

@@ -175,18 +175,12 @@ int fwi_dot3(const double* a, const double* b, long long n, double* out, void* s
  * main.c:323,368-372).  With stride > 0, every stride-th launch of each fused step kernel is bracketed by
  * HIP events on the stream it runs on; fwi_plan_kernel_time returns the summed event time and the
  * number of timed launches of a kind (0 forward step, 1 forward step storing imaging terms, 2 adjoint
- * step (exact gradient: its interior kernel), 3 / 4 fused core step pair without / with stored imaging
- * terms, 5 exact adjoint edge kernel, which runs on a side stream concurrently with kind 2) since the last
- * fwi_plan_set_timing call. */
+ * step, 3 / 4 fused core step pair without / with stored imaging terms) since the last fwi_plan_set_timing
+ * call. */
 void fwi_plan_set_timing(fwi_plan* plan, int stride);
 int fwi_plan_kernel_time(fwi_plan* plan, int kind, double* total_ms, long long* count);
 /* Cells of all shot windows (nx_s*nz summed over shots) and, in *strip_cells, their C-PML cells. */
 long long fwi_plan_cells(const fwi_plan* plan, long long* strip_cells);
-/* Exact gradient: shot cells of the adjoint's interior work items (the kernel timed as kind 2) and of its
- * edge items (kind 5), and the model cells under the interior items, summed over the plan's batches (as
- * last built); returns the number of batches. */
-int fwi_plan_adj_split(const fwi_plan* plan, long long* cells_interior, long long* cells_edge,
-                       long long* model_interior);
 /* Checkpoint interval K, z rows per work item and work-item order in use (after the first call): the
  * forward kernels' and, through fwi_plan_adj_tiling, the adjoint kernels' (grouped = 1 when the first
  * batch accumulates gradients per group of shots). */
