@@ -173,6 +173,8 @@ struct AdjExArgs {
   float* ztz;
   float* ptx;  // a_x * psi-bar_tot of this step
   float* ptz;
+  float* gbx;  // gbar_x, gbar_z of this step on the updated C-PML cells (zero on every other cell)
+  float* gbz;
   const float* res;
   int rit;
   const float* qa;  // forward a^n, b^n of the paired step, or null

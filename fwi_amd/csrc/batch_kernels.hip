@@ -606,11 +606,14 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
 //   m^k = 2 m^{k+1} + D2x gbar_x - D1x(a_x pt_x) - m^{k+2} (+ residual at the receivers), likewise z.
 // ---------------------------------------------------------------------------------------------
 
-// Phase 1 on the C-PML cells: zt (kept for this step) and the new zeta-bar.
+// Phase 1 on the C-PML cells: zt (kept for this step), the new zeta-bar, and the cell's gbar_x, gbar_z, which
+// phase 2 reads at the neighbours of each C-PML cell (the C-PML cells are Gamma without the rim, so the arrays,
+// zero elsewhere, hold the Gamma-masked gbar).
 __global__ void __launch_bounds__(256) k_adj_ex_strip1(StripArgs a, float dt2, const float* __restrict__ Lc,
                                                        const float* __restrict__ Mc, float* __restrict__ zx,
                                                        float* __restrict__ zz, float* __restrict__ ztx,
-                                                       float* __restrict__ ztz) {
+                                                       float* __restrict__ ztz, float* __restrict__ gbx,
+                                                       float* __restrict__ gbz) {
   const int2 blk = a.blocks[blockIdx.x];
   const ShotDev sd = a.g.shots[blk.x];
   const int k = blk.y + threadIdx.x;
@@ -621,45 +624,25 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip1(StripArgs a, float dt2, c
   const long i = sd.off + ix + (long)sd.ps * iz;
   const long mi = sd.x0 + ix + (long)a.m.nxm * iz;
   const float mP = Lc[i], mF = Mc[i];
+  const float ab = dt2 * (a.m.vh[mi] * mP + a.m.vn[mi] * mF);
+  const float bb = dt2 * (a.m.vp[mi] * (mP + mF));
+  float tx = ztx[i], tz = ztz[i];  // zero off chi_x / chi_z
   if (ix < cpml || ix > nx - cpml - 1) {
-    const float ab = dt2 * (a.m.vh[mi] * mP + a.m.vn[mi] * mF);
-    const float t = zx[i] + ab;
-    ztx[i] = t;
-    zx[i] = coef_x(a.g.tb, ix, nx, cpml) * t;
+    tx = zx[i] + ab;
+    ztx[i] = tx;
+    zx[i] = coef_x(a.g.tb, ix, nx, cpml) * tx;
   }
   if (iz > nz - cpml - 1) {
-    const float bb = dt2 * (a.m.vp[mi] * (mP + mF));
-    const float t = zz[i] + bb;
-    ztz[i] = t;
-    zz[i] = coef_z(a.g.tb, iz, nz, cpml) * t;
+    tz = zz[i] + bb;
+    ztz[i] = tz;
+    zz[i] = coef_z(a.g.tb, iz, nz, cpml) * tz;
   }
+  gbx[i] = ab + coef_x(a.g.ta, ix, nx, cpml) * tx;
+  gbz[i] = bb + coef_z(a.g.ta, iz, nz, cpml) * tz;
 }
 
-// gbar at cell (jx, jz) of one shot: Omega & Gamma masked on request.
-__device__ __forceinline__ float gbar_x(const AdjExArgs& a, const ShotDev& sd, const float* __restrict__ L,
-                                        const float* __restrict__ M, int jx, int jz, bool gamma_only) {
-  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
-  if (jx < kRim || jx >= nx - kRim || jz < kRim || jz >= nz - kRim) return 0.f;
-  const bool gam = jx < cpml || jx > nx - cpml - 1 || jz > nz - cpml - 1;
-  if (gamma_only && !gam) return 0.f;
-  const long i = jx + (long)sd.ps * jz;
-  const long mi = sd.x0 + jx + (long)a.m.nxm * jz;
-  const float ab = a.dt2 * (a.m.vh[mi] * L[i] + a.m.vn[mi] * M[i]);
-  return ab + coef_x(a.g.ta, jx, nx, cpml) * a.ztx[sd.off + i];
-}
-__device__ __forceinline__ float gbar_z(const AdjExArgs& a, const ShotDev& sd, const float* __restrict__ L,
-                                        const float* __restrict__ M, int jx, int jz, bool gamma_only) {
-  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
-  if (jx < kRim || jx >= nx - kRim || jz < kRim || jz >= nz - kRim) return 0.f;
-  const bool gam = jx < cpml || jx > nx - cpml - 1 || jz > nz - cpml - 1;
-  if (gamma_only && !gam) return 0.f;
-  const long i = jx + (long)sd.ps * jz;
-  const long mi = sd.x0 + jx + (long)a.m.nxm * jz;
-  const float bb = a.dt2 * (a.m.vp[mi] * (L[i] + M[i]));
-  return bb + coef_z(a.g.ta, jz, nz, cpml) * a.ztz[sd.off + i];
-}
-
-// Phase 2 on the C-PML cells: psi-bar_tot -> a*psi-bar_tot (kept for this step) and the new psi-bar.
+// Phase 2 on the C-PML cells: psi-bar_tot -> a*psi-bar_tot (kept for this step) and the new psi-bar, from the
+// Gamma-masked gbar of phase 1 at the 8 neighbours (zero outside the window).
 template <class S>
 __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a) {
   const int2 blk = s.blocks[blockIdx.x];
@@ -670,13 +653,14 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
   int ix, iz;
   strip_cell(k, nx, nz, cpml, ix, iz);
   const long i = sd.off + ix + (long)sd.ps * iz;
-  const float* L = a.Lc + sd.off;
-  const float* M = a.Mc + sd.off;
   const float dh = a.g.dh;
   if (ix < cpml || ix > nx - cpml - 1) {
     float w[9];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) w[j] = (j == 4) ? 0.f : gbar_x(a, sd, L, M, ix + j - 4, iz, true);
+    for (int j = 0; j < 9; ++j) {
+      const int jx = ix + j - 4;
+      w[j] = (j == 4 || jx < 0 || jx >= nx) ? 0.f : a.gbx[i + j - 4];
+    }
     const float t = a.sx[i] - d1s<S>(w[0], w[1], w[2], w[3], w[5], w[6], w[7], w[8], dh);
     a.sx[i] = coef_x(a.g.tb, ix, nx, cpml) * t;
     a.ptx[i] = coef_x(a.g.ta, ix, nx, cpml) * t;
@@ -684,7 +668,10 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
   if (iz > nz - cpml - 1) {
     float w[9];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) w[j] = (j == 4) ? 0.f : gbar_z(a, sd, L, M, ix, iz + j - 4, true);
+    for (int j = 0; j < 9; ++j) {
+      const int jz = iz + j - 4;
+      w[j] = (j == 4 || jz < 0 || jz >= nz) ? 0.f : a.gbz[i + (long)(j - 4) * sd.ps];
+    }
     const float t = a.sz[i] - d1s<S>(w[0], w[1], w[2], w[3], w[5], w[6], w[7], w[8], dh);
     a.sz[i] = coef_z(a.g.tb, iz, nz, cpml) * t;
     a.ptz[i] = coef_z(a.g.ta, iz, nz, cpml) * t;

@@ -147,7 +147,7 @@ struct fwi_plan {
   // forward fields: the state of step n in set n mod 3 in the increment form (P 0..2, F 3..5, Q = P^n - P^{n-1}
   // 6..8, R = F^n - F^{n-1} 9..11; three sets so that fused step pairs can keep step n while writing n+1 and
   // n+2), in set n mod 2 of P 0..1 and F 3..4 in the two-level form; C-PML memory psx 12, psz 13, zx 14, zz 15
-  static constexpr int kFwdF = 16, kAdjF = 12, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
+  static constexpr int kFwdF = 16, kAdjF = 14, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
   int fset(int n) const { return inc() ? n % 3 : (n & 1); }
   float* fP(int n) const { return fwd(fset(n)); }
   float* fF(int n) const { return fwd(3 + fset(n)); }
@@ -727,6 +727,8 @@ struct fwi_plan {
       a.ztz = adj(9);
       a.ptx = adj(10);
       a.ptz = adj(11);
+      a.gbx = adj(12);
+      a.gbz = adj(13);
       a.res = resid();
       a.rit = k - 1;
       a.qa = qa;
@@ -736,7 +738,7 @@ struct fwi_plan {
       a.gn = sgrad(2);
       if (b.nsblocks > 0) {
         hipLaunchKernelGGL(k_adj_ex_strip1, dim3(b.nsblocks), dim3(256), 0, st, sa, a.dt2, Lc, Mc, a.zx, a.zz, a.ztx,
-                           a.ztz);
+                           a.ztz, a.gbx, a.gbz);
         ARITH_LAUNCH(k_adj_ex_strip2, dim3(b.nsblocks), sa, a);
       }
       const int tev = timed_begin(2, st);
