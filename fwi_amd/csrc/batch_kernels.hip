@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip1(StripArgs a, float dt2, c
   const float mP = Lc[i], mF = Mc[i];
   const float ab = dt2 * (a.m.vh[mi] * mP + a.m.vn[mi] * mF);
   const float bb = dt2 * (a.m.vp[mi] * (mP + mF));
-  float tx = ztx[i], tz = ztz[i];  // zero off chi_x / chi_z
+  float tx = 0.f, tz = 0.f;  // zt off chi_x / chi_z: never written, zero since the pass began
   if (ix < cpml || ix > nx - cpml - 1) {
     tx = zx[i] + ab;
     ztx[i] = tx;

@@ -77,6 +77,39 @@ def test_gradient_matches_oracle(torch, exact, arith, mode):
         assert e <= TOL
 
 
+@pytest.mark.parametrize("cpml", [5, 12])
+def test_odd_geometries_match_oracle(torch, cpml):
+    """Windows that exercise the kernels' choice between the pipelined interior bodies and the general body and
+    the cuts of the row runs: narrower than one 64-column band, exactly one interior band wide, one column more,
+    a source next to a C-PML strip, a short last row run (nz - 8 not a multiple of the rows per item), with the
+    thinnest C-PML the plan accepts (one cell past the rim) or a thicker one; gathers, misfit and both
+    gradients against the oracle."""
+    nx, nz, nt = 230, 45, 90
+    w = small_workload(nx=nx, nz=nz, nt=nt, cpml=cpml, nshots=1)
+    e = max(cpml, 4)
+    w.shots = [M.Shot(x0=0, nx=40, isx=20, isz=6, irz=6, ir0=cpml, nrec=40 - 2 * cpml),
+               M.Shot(x0=10, nx=64 + 2 * e, isx=e + 1, isz=7, irz=6, ir0=cpml, nrec=64 + 2 * e - 2 * cpml),
+               M.Shot(x0=100, nx=65 + 2 * e, isx=40, isz=9, irz=5, ir0=cpml + 3, nrec=30),
+               M.Shot(x0=0, nx=nx, isx=nx - cpml - 1, isz=6, irz=6, ir0=cpml, nrec=nx - 2 * cpml)]
+    say(f"odd geometries: cpml={cpml}, nz={nz}, windows 40 / {64 + 2 * e} / {65 + 2 * e} / {nx}")
+    for exact in (True, False):
+        tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+        obs = oracle_gathers(w, "f32i", sq_true, tab, src)
+        phi_ref, G_ref, syn_ref = oracle_gradient(w, "f32i", exact, sq_init, tab, src, obs)
+        plan = _plan(w, tab, src, arith=3, grad_mode=1 if exact else 0, ckpt_interval=13, store_segments=1)
+        g = plan.forward(_dev_model(torch, sq_true))
+        phi, G, syn = plan.gradient(_dev_model(torch, sq_init), torch.from_numpy(_cat(obs)).cuda(), want_syn=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(g.cpu().numpy(), _cat(obs))
+        assert np.array_equal(syn.cpu().numpy(), _cat(syn_ref))
+        assert abs(phi - phi_ref) <= 1e-9 * phi_ref
+        G = G.cpu().numpy()
+        for k in range(3):
+            e_k = rel_l2(G[k], G_ref[k])
+            print(f"exact={exact} grad[{k}] rel-L2 {e_k:.3e}")
+            assert e_k <= TOL
+
+
 def test_checkpoint_interval_and_batching_invariance(torch):
     """Recomputation from checkpoints replays the same kernels: every interval K (including storing all
     steps) gives bit-identical gradients, in the two-level and the increment form; batch splits and group
