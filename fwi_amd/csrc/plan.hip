@@ -943,7 +943,7 @@ struct fwi_plan {
   // candidate on the plan's own buffers and its largest batch.
   int tune(const ModelArgs& m, hipStream_t st) {
     if (tuned) return 0;
-    std::vector<int> rows_c = {16, 24, 32, 48, 64, 96};
+    std::vector<int> rows_c = {16, 8, 4, 24, 32, 48, 64, 96};  // 8, 4: small grids, whose few items are latency-bound
     if (opt.rows_per_item > 0) rows_c = {opt.rows_per_item};
     Batch& b = batches[0];
     std::vector<int> order_f = {0, 1, 2};
