@@ -1,7 +1,9 @@
 """GPU parity of the batched FWI path (include/fwi.h) against the oracle.
 
-Each arithmetic (FP32, MIXED, FP32D, FP32I) is compared with the oracle's instantiation of the same name: same expressions, same order, so the bar is bit-level (tolerance rel-L2 <= 1e-6, measured
-value printed).  Different checkpoint intervals and batch splits must give identical results, because
+Each arithmetic (FP32, MIXED, FP32D, FP32I) is compared with the oracle's instantiation of the same name:
+same expressions, same order, so gathers and gradients must be equal bit for bit (the rel-L2 is printed
+beside); the FP64 misfit is held to 1e-9 relative, since the device adds its per-block partial sums in
+another order than the oracle's sequential sum.  Different checkpoint intervals and batch splits must give identical results, because
 recomputation replays the same deterministic kernels.  The error of the FP32 path against the FP64
 oracle is measured and bounded separately (the FP32 rounding floor of the leapfrog: a few 1e-4 in the
 two-level form, ~2e-5 in the default increment form).
@@ -14,8 +16,6 @@ from helpers import oracle_gathers, oracle_gradient, oracle_setup, rel_l2, say, 
 from fwi_amd import models as M
 
 pytestmark = pytest.mark.gpu
-
-TOL = 1e-6
 
 
 @pytest.fixture(scope="module")
@@ -51,7 +51,7 @@ def test_forward_plain200_matches_oracle(torch, arith, mode):
     got = g.cpu().numpy().reshape(w.nt, -1)
     err = rel_l2(got, ref)
     print(f"forward plain200 {mode}: rel-L2 {err:.3e}, max|diff| {np.abs(got - ref).max():.3e}")
-    assert err <= TOL
+    assert np.array_equal(got, ref)
 
 
 @pytest.mark.parametrize("exact", [True, False])
@@ -69,12 +69,12 @@ def test_gradient_matches_oracle(torch, exact, arith, mode):
     G = G.cpu().numpy()
     es = rel_l2(syn.cpu().numpy(), _cat(syn_ref))
     print(f"syn rel-L2 {es:.3e}; misfit {phi:.9e} vs {phi_ref:.9e}")
-    assert es <= TOL
+    assert np.array_equal(syn.cpu().numpy(), _cat(syn_ref))
     assert abs(phi - phi_ref) <= 1e-9 * phi_ref
     for k, name in enumerate(("v_hor", "v_po", "v_nmo")):
         e = rel_l2(G[k], G_ref[k])
         print(f"grad {name}: rel-L2 {e:.3e}")
-        assert e <= TOL
+        assert np.array_equal(G[k], G_ref[k])
 
 
 @pytest.mark.parametrize("cpml", [5, 12])
@@ -107,7 +107,7 @@ def test_odd_geometries_match_oracle(torch, cpml):
         for k in range(3):
             e_k = rel_l2(G[k], G_ref[k])
             print(f"exact={exact} grad[{k}] rel-L2 {e_k:.3e}")
-            assert e_k <= TOL
+            assert np.array_equal(G[k], G_ref[k])
 
 
 def test_tiling_invariance(torch):
@@ -209,7 +209,7 @@ def test_gradient_groups_match_oracle(torch, layout):
     for k in range(3):
         e = rel_l2(G[k], G_ref[k])
         print(f"groups {layout} grad[{k}] rel-L2 {e:.3e}")
-        assert e <= TOL
+        assert np.array_equal(G[k], G_ref[k])
 
 
 @pytest.mark.parametrize("fmt", [1, 2])
@@ -239,7 +239,7 @@ def test_16bit_checkpoints_match_oracle(torch, fmt):
     for k in range(3):
         e = rel_l2(G[k], G_ref[k])
         print(f"ckpt fmt {fmt} grad[{k}] vs oracle rel-L2 {e:.3e}, vs FP32 checkpoints {rel_l2(G[k], G32[k]):.3e}")
-        assert e <= TOL
+        assert np.array_equal(G[k], G_ref[k])
 
 
 def test_ragged_windows_match_oracle(torch):
@@ -262,7 +262,7 @@ def test_ragged_windows_match_oracle(torch):
         for k in range(3):
             e = rel_l2(G[k], G_ref[k])
             print(f"ragged exact={exact} grad[{k}] rel-L2 {e:.3e}")
-            assert e <= TOL
+            assert np.array_equal(G[k], G_ref[k])
 
 
 def test_misfit_matches_gradient_misfit(torch):
@@ -318,7 +318,7 @@ def test_adjoint128_checkpoint28_fp32(torch):
     torch.cuda.synchronize()
     Gn = G.cpu().numpy()
     for k in range(3):
-        assert rel_l2(Gn[k], G_ref[k]) <= TOL
+        assert np.array_equal(Gn[k], G_ref[k])
     # FP64 oracle on the same observed data
     tab64, src64, _, si64 = oracle_setup(w, "f64")
     _, G64, _ = oracle_gradient(w, "f64", True, si64, tab64, src64.astype(np.float64), [o.astype(np.float64) for o in obs])

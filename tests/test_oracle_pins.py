@@ -200,6 +200,60 @@ def test_forward_linear_in_source():
     np.testing.assert_allclose(g12, 2 * g1 - 3 * g2, rtol=0, atol=1e-10 * np.abs(g12).max())
 
 
+@pytest.mark.parametrize("mode", ["mix", "f32", "f64", "f32d", "f32i"])
+def test_forward_time_shift_invariance(mode):
+    """The step has no explicit time dependence: a source delayed by m steps leaves every field exactly zero
+    for m steps, then repeats the undelayed run, so the gather is the undelayed gather delayed by m steps,
+    bit for bit, in every arithmetic (a wrong source index, a lagged receiver read or state that is not reset
+    would break it)."""
+    nx, nz, nt, cpml, m = 60, 52, 180, 10, 37
+    p = M.CpmlParams(cpml=cpml, dh=10.0, dt=1e-3, f=12.0)
+    T = np.float64 if mode == "f64" else np.float32
+    vp, eps, dl = M.gradient_model(nx, nz, seed=4, cpml=cpml, sigma=5.0)
+    sq = O.pre_variables(vp.astype(T), eps.astype(T), dl.astype(T), dtype=T)
+    ta, tb = O.cpml_table(p)
+    src = O.ricker(nt, p.f, p.dt).astype(T)
+    late = np.zeros_like(src)
+    late[m:] = src[:nt - m]
+    shot = M.reference_shot(nx, cpml, 27, M.depth_row(3))
+    g = O.forward(mode, sq, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    gl = O.forward(mode, sq, (ta, tb), late, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    assert not gl[:m].any()
+    assert np.array_equal(gl[m:], g[:nt - m])
+    assert np.abs(g[:nt - m]).max() > 0.01
+
+
+def test_forward_source_receiver_reciprocity():
+    """Isotropic (epsilon = delta = 0, so P = F) heterogeneous medium, FP64.  The step is
+    P^{n+1} - 2 P^n + P^{n-1} = C v L P^n + s_n e_A with L the symmetric undivided 8th-order Laplacian on the
+    live cells (zero outside) and v the squared velocity, i.e. (1/v) D_tt P - C L P = s_n e_A / v_A with a
+    symmetric spatial operator; its Green's function is symmetric, so v_A P_B(source at A) = v_B P_A(source at B).
+    The run ends before either wave reaches the C-PML strips, whose discrete update is not symmetric.  A swapped
+    grid index, a transposed stencil or a source scaled by the local velocity would break the identity."""
+    nx = nz = 220
+    cpml, nt = 10, 240
+    p = M.CpmlParams(cpml=cpml, dh=10.0, dt=1e-3, f=20.0)
+    vp, _, _ = M.gradient_model(nx, nz, seed=5, cpml=cpml, sigma=12.0)
+    zero = np.zeros_like(vp, np.float64)
+    sq = O.pre_variables(vp.astype(np.float64), zero, zero, dtype=np.float64)
+    v = sq[1]
+    ta, tb = O.cpml_table(p)
+    src = O.ricker(nt, p.f, p.dt).astype(np.float64)
+    (xa, za), (xb, zb) = (100, 100), (125, 115)
+    # every wave stays inside: 85 cells to the nearest strip, at most 0.35 cells per step for 240 steps
+    assert np.sqrt(v.max()) * p.dt / p.dh * nt < min(xa - cpml, nx - cpml - xb, nz - cpml - zb)
+    ga = O.forward("f64", sq, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt,
+                   M.Shot(x0=0, nx=nx, isx=xa, isz=za, irz=zb, ir0=xb, nrec=1))[:, 0]
+    gb = O.forward("f64", sq, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt,
+                   M.Shot(x0=0, nx=nx, isx=xb, isz=zb, irz=za, ir0=xa, nrec=1))[:, 0]
+    lhs, rhs = v[za, xa] * ga, v[zb, xb] * gb
+    assert np.abs(lhs).max() > 1e-3 * np.abs(v[za, xa] * src).max()  # the wave has arrived
+    np.testing.assert_allclose(lhs, rhs, rtol=0, atol=1e-9 * np.abs(lhs).max())
+    # without the velocity weights the two traces differ (v_A != v_B here)
+    assert abs(v[za, xa] / v[zb, xb] - 1) > 0.05
+    assert np.abs(ga - gb).max() > 1e-3 * np.abs(ga).max()
+
+
 # ------------------------------------------------------------------------------------------------
 # Exact adjoint and gradients
 # ------------------------------------------------------------------------------------------------
