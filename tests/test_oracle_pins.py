@@ -2,9 +2,11 @@
 
 The reference ships no tests, fixtures or golden vectors, so each oracle function is checked here against
 an independent fact: polynomial exactness and the Nyquist symbol of the stencils, the C-PML profile shapes
-recovered from the tables, closed-form properties of the Ricker wavelet, the isotropic reduction and the
-analytic 2-D Green's function of the forward, a dot-product test and finite-difference directional
-derivatives of the exact adjoint, and the relation between the reference-compatible and exact gradients.
+recovered from the tables, closed-form properties of the Ricker wavelet, the isotropic reduction, the
+analytic 2-D Green's function, linearity, time-shift invariance and source-receiver reciprocity of the
+forward, a dot-product test, finite-difference directional derivatives and a per-cell brute-force
+finite-difference gradient of the exact adjoint, and the relation between the reference-compatible and exact
+gradients.
 """
 import math
 
@@ -336,6 +338,51 @@ def test_exact_gradient_fd_directional_derivatives():
             errs.append(abs(fd - pred) / abs(pred))
         assert errs[2] < 1e-7, (comp, errs)
         assert errs[0] / errs[1] > 50, (comp, errs)  # second-order convergence
+
+
+def test_exact_gradient_brute_force_per_cell():
+    """Central differences of the FP64 misfit with respect to every cell of v_hor, v_po and v_nmo on a 24 x 24
+    grid (rim, C-PML strips and interior; 3456 forward runs) equal the exact gradient cell by cell: a smooth
+    direction cannot see a gradient shifted by one cell or a term assigned to the wrong field, this does."""
+    nx = nz = 24
+    cpml, nt = 6, 80
+    p = M.CpmlParams(cpml=cpml, dh=10.0, dt=1e-3, f=25.0)
+    ta, tb = O.cpml_table(p)
+    src = O.ricker(nt, p.f, p.dt).astype(np.float64)
+    shot = M.reference_shot(nx, cpml, 11, M.depth_row(2))
+    Z = np.arange(nz, dtype=np.float64)[:, None] * np.ones((1, nx))
+
+    def model(seed):  # VTI with depth trends and seeded cell-scale noise, so no two cells are alike
+        r = np.random.default_rng(seed)
+        vp = 2000 + 40 * Z + 60 * r.standard_normal((nz, nx))
+        return O.pre_variables(vp, 0.1 + 0.004 * Z + 0.01 * r.standard_normal((nz, nx)),
+                               0.05 + 0.002 * Z + 0.005 * r.standard_normal((nz, nx)), dtype=np.float64)
+
+    sqt, sq0 = model(8), model(9)
+    obs = O.forward("f64", sqt, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt, shot)
+
+    def J(model):
+        g = O.forward("f64", model, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt, shot)
+        return 0.5 * np.sum((g - obs) ** 2)
+
+    _, _, _, gh, gp, gn = O.gradient("f64", True, sq0, (ta, tb), src, obs, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    h = 1e-5
+    for f, grad in enumerate((gh, gp, gn)):
+        fd = np.zeros((nz, nx))
+        for iz in range(nz):
+            for ix in range(nx):
+                d = h * sq0[f][iz, ix]
+                up = [m.copy() for m in sq0]
+                dn = [m.copy() for m in sq0]
+                up[f][iz, ix] += d
+                dn[f][iz, ix] -= d
+                fd[iz, ix] = (J(up) - J(dn)) / (2 * d)
+        # compare the sensitivities to a relative change of each cell
+        scale = np.abs(grad * sq0[f]).max()
+        assert scale > 0
+        np.testing.assert_allclose(fd * sq0[f], grad * sq0[f], rtol=0, atol=1e-6 * scale)
+        # the rim is never updated, so neither side sees it
+        assert not grad[:4].any() and not grad[-4:].any() and not grad[:, :4].any() and not grad[:, -4:].any()
 
 
 def test_refcompat_gradient_relates_to_exact():
