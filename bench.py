@@ -302,6 +302,7 @@ def main():
             "batch_shots": plan.batch_size,
             "rows_per_item": plan.rows_per_item,
             "item_order": plan.item_order,
+            "adjoint_tiling": dict(zip(("rows_per_item", "item_order", "grouped"), plan.adj_tiling)),
             "s_per_gradient_batched": round(sec, 4),
             "graphs": {"captures": graphs[0], "replays": graphs[1]},
             "s_per_gradient_eager_instrumented": round(sec_instr, 4) if sec_instr else None,
