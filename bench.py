@@ -155,8 +155,9 @@ def cpu_baseline(w, nsteps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (shots would shard across GPUs "
-              "with one FP64 all-reduce of the gradient; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported.  Shots are independent, so "
+              "several GPUs would each take a share of the shots and add their FP64 gradients and misfits "
+              "with one all-reduce per gradient (3*nx*nz + 1 doubles).", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

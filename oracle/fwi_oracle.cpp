@@ -30,9 +30,11 @@
 //   cpml_table          : closed form of Komatitsch & Martin C-PML, kappa=1 (test_oracle_pins.py)
 //   ricker              : closed form                                     (test_oracle_pins.py)
 //   forward (all modes) : isotropic reduction (eps=delta=0 => P==F), analytic 2-D Green's function
-//                         before the first reflection, linearity, reciprocity (test_oracle_pins.py);
+//                         before the first reflection, linearity, time-shift invariance (bit for bit),
+//                         source-receiver reciprocity (test_oracle_pins.py);
 //                         f32i against f64 at 1e-5 and ahead of f32 on a long, finely sampled run
-//   exact adjoint       : dot-product test of the one-step transpose, FD directional derivatives
+//   exact adjoint       : dot-product test of the one-step transpose, FD directional derivatives,
+//                         brute-force central differences for every cell of a 24x24 grid
 //   ref-compat gradient : restated formula; relation to the exact gradient (lag + 1/dt) on tiny grids
 //   L-BFGS two-loop     : dense BFGS inverse update with H0 = gamma*I (Nocedal & Wright Alg. 7.4)
 // The reference's own rounding (nvcc FMA contraction, device exp/pow in CPML_x/CPML_z) cannot be
