@@ -1,0 +1,114 @@
+"""GPU checks at the benchmark's own size: survey1600, 1600 x 400 cells, 64 full-width shots, 4000 steps.
+
+The oracle cannot run 64 shots x 4000 steps in seconds, so the full size is covered three ways:
+  * three shots of the full 64-shot batch (the first, a middle and the last, whose fields lie gigabytes into
+    the workspace) against the oracle's per-shot forward, bit for bit, on a 300-step time axis;
+  * one full-width shot's exact gradient against the oracle, bit for bit, on a 400-step time axis with
+    recomputation from checkpoints;
+  * size-independent properties of the full 4000-step, 64-shot gradient in the benchmark's configuration
+    (default options, captured graphs, the tuner's tiling, trailing segments kept to the memory budget):
+    a source delayed by m steps delays every gather by m steps bit for bit, and a second plan with another
+    checkpoint interval, one kept segment, eager launches and another tiling returns the same misfit and the
+    same gradient bits (recomputation and the schedule change nothing).
+"""
+import numpy as np
+import pytest
+
+from helpers import oracle_gathers, oracle_gradient, oracle_setup, rel_l2, say
+from fwi_amd import models as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+
+    return t
+
+
+def _plan(w, tab, src, **kw):
+    from fwi_amd.survey import Plan
+
+    return Plan(w.nx, w.nz, w.nt, w.dh, w.dt, w.cpml, w.shots, src, tab, **kw)
+
+
+def _dev_model(torch, sq):
+    return tuple(torch.from_numpy(np.ascontiguousarray(x, np.float32)).cuda() for x in sq)
+
+
+def test_survey1600_batch_gathers_match_oracle(torch):
+    w = M.workload("survey1600", nt=300)
+    say("survey1600 forward, 64 shots x 300 steps, shots 0/31/63 against the oracle")
+    tab, src, sq_true, _ = oracle_setup(w, "f32i")
+    plan = _plan(w, tab, src, mem_budget=40e9)
+    try:
+        g = plan.split_gathers(plan.forward(_dev_model(torch, sq_true)))
+        torch.cuda.synchronize()
+        for i in (0, 31, 63):
+            sub = M.Workload(w.name, w.nx, w.nz, w.dh, w.dt, w.nt, w.f, w.cpml, w.true_model, w.init_model,
+                             [w.shots[i]])
+            ref = oracle_gathers(sub, "f32i", sq_true, tab, src)[0]
+            got = g[i].cpu().numpy()
+            print(f"shot {i} (x={w.shots[i].isx}): rel-L2 {rel_l2(got, ref):.3e}")
+            assert np.array_equal(got, ref), i
+    finally:
+        plan.close()
+
+
+def test_full_width_shot_gradient_matches_oracle(torch):
+    w = M.workload("survey1600", nshots=1, nt=400)
+    say("survey1600 grid, one full-width shot, 400 steps: exact gradient against the oracle")
+    tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+    obs = oracle_gathers(w, "f32i", sq_true, tab, src)
+    phi_ref, G_ref, _ = oracle_gradient(w, "f32i", True, sq_init, tab, src, obs)
+    plan = _plan(w, tab, src, store_segments=1)
+    try:
+        assert plan.kept_steps < w.nt
+        phi, G, _ = plan.gradient(_dev_model(torch, sq_init), torch.from_numpy(obs[0].reshape(-1)).cuda())
+        torch.cuda.synchronize()
+        G = G.cpu().numpy()
+        assert abs(phi - phi_ref) <= 1e-9 * phi_ref
+        for k in range(3):
+            print(f"grad[{k}] rel-L2 {rel_l2(G[k], G_ref[k]):.3e}")
+            assert np.array_equal(G[k], G_ref[k]), k
+    finally:
+        plan.close()
+
+
+def test_survey1600_full_gradient_properties(torch):
+    w = M.workload("survey1600")
+    say("survey1600 at full size: 64 shots x 4000 steps; time shift and schedule invariance")
+    tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+    true_d, init_d = _dev_model(torch, sq_true), _dev_model(torch, sq_init)
+    plan = _plan(w, tab, src)  # the benchmark's configuration
+    try:
+        obs = plan.forward(true_d)
+        phi_a, G_a, _ = plan.gradient(init_d, obs)
+        torch.cuda.synchronize()
+        assert np.isfinite(phi_a) and phi_a > 0 and torch.isfinite(G_a).all().item()
+        kept, tiling = plan.kept_steps, (plan.rows_per_item, plan.item_order, plan.adj_tiling)
+        print(f"misfit {phi_a:.9e}; kept steps {kept}, K {plan.ckpt_interval}, tiling {tiling}")
+        # time shift: the whole survey's gathers move by m steps, nothing else changes
+        m = 173
+        late = np.zeros_like(src)
+        late[m:] = src[:w.nt - m]
+        plan.set_wavelet(late)
+        shifted = plan.forward(true_d)
+        torch.cuda.synchronize()
+        for i, (a, b) in enumerate(zip(plan.split_gathers(obs), plan.split_gathers(shifted))):
+            assert not b[:m].any().item(), i
+            assert torch.equal(b[m:], a[:w.nt - m]), i
+        del shifted
+    finally:
+        plan.close()
+    other = _plan(w, tab, src, ckpt_interval=50, store_segments=1, use_graph=0, rows_per_item=32, item_order=0)
+    try:
+        assert other.kept_steps != kept
+        phi_b, G_b, _ = other.gradient(init_d, obs)
+        torch.cuda.synchronize()
+        print(f"second plan: K {other.ckpt_interval}, kept {other.kept_steps}, misfit {phi_b:.9e}")
+        assert phi_b == phi_a
+        assert torch.equal(G_b, G_a)
+    finally:
+        other.close()
