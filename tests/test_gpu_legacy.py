@@ -101,6 +101,25 @@ def test_legacy_forward_plain200_matches_oracle(env):
     assert np.all(g[:, :w.cpml] == 0) and np.all(g[:, w.nx - w.cpml:] == 0)
 
 
+
+def test_legacy_forward_survey1600_grid_matches_oracle(env):
+    """The per-step launchers on the 1600 x 400 grid (one shot, 300 steps): the reference's observed-data loop
+    at the benchmark's grid size, bit-equal to the oracle's mixed arithmetic."""
+    torch, L, _lib = env
+    w = M.workload("survey1600", nshots=1, nt=300)
+    say("legacy forward survey1600 grid 1600x400 nt=300")
+    d = _setup_legacy(torch, L, w, w.true_model)
+    g = _legacy_forward(torch, L, w, d, w.shots[0])
+    _lib.check_last("legacy forward")
+    tab = O.cpml_table(w.cpml_params)
+    ref = O.forward("mix", _oracle_sq(w, w.true_model), tab, O.ricker(w.nt, w.f, w.dt), w.nx, w.nz, w.nt, w.cpml,
+                    w.dh, w.dt, w.shots[0])
+    s = w.shots[0]
+    got = g[:, s.ir0:s.ir0 + s.nrec]
+    print(f"legacy forward 1600x400 vs oracle(mix): rel-L2 {rel_l2(got, ref):.3e}")
+    assert np.abs(ref).max() > 0
+    assert np.array_equal(got, ref)
+
 def test_legacy_gradient_matches_oracle(env):
     """main.c:410-496 with launchPropagation / res_h / launchBackPropagation and pinned PF_h snapshots."""
     torch, L, _lib = env
