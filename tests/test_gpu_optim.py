@@ -23,13 +23,14 @@ def torch():
     return t
 
 
-def test_lbfgs_iterations_match_numpy_oracle(torch):
-    """SD step then 7 L-BFGS iterations with memory 3 (the history ring wraps twice): direction, trial
-    and restore against the FP64 oracle."""
+@pytest.mark.parametrize("nx,nz,m,its", [(40, 30, 3, 7), (1600, 400, 5, 7), (4000, 1000, 5, 3)])
+def test_lbfgs_iterations_match_numpy_oracle(torch, nx, nz, m, its):
+    """SD step then L-BFGS iterations (with memory 3 over 7 iterations the history ring wraps twice):
+    direction, trial and restore against the FP64 oracle, on a small grid and on the survey1600 and large4000
+    grids, whose FP64 dot products and norms reduce over 0.6 and 4 million cells per block."""
     from fwi_amd.optim import LBFGS
 
-    say("device L-BFGS vs numpy oracle, 40x30, m=3, 7 iterations")
-    nx, nz, m = 40, 30, 3
+    say(f"device L-BFGS vs numpy oracle, {nx}x{nz}, m={m}, {its} iterations")
     rng = np.random.default_rng(5)
     model_h = [(2.5e6 + 1e5 * rng.standard_normal((nz, nx))).astype(np.float32) for _ in range(3)]
     model_h[0] = np.maximum(model_h[0], model_h[2])
@@ -43,7 +44,7 @@ def test_lbfgs_iterations_match_numpy_oracle(torch):
     for k in range(3):  # norms are FP64 tree sums on the device, pairwise sums in numpy: <= 1 ulp apart
         np.testing.assert_allclose(dev[k].cpu().numpy(), orc_model[k], rtol=1e-6)
     cur = [d.cpu().numpy() for d in dev]
-    for it in range(7):
+    for it in range(its):
         # a quadratic-like gradient so that curvature pairs are positive
         g = np.stack([1e-6 * (c.astype(np.float64) - 2.4e6) + 0.01 * rng.standard_normal((nz, nx)) for c in cur])
         r_ref = orc.direction(cur, list(g))
