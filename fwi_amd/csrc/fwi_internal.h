@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -12,6 +13,19 @@ namespace fwi {
 void record_error(hipError_t e);
 // Sets the thread's message for the status-returning API (fwi_error_message()).
 void set_message(const std::string& msg);
+
+// An array argument of a status-returning entry point and its name in messages.
+struct ArrayArg {
+  const void* ptr;
+  const char* name;
+};
+// Every array must be non-null and memory the GPU can address (device, managed, or pinned host memory): a
+// pointer the runtime classifies as unregistered host memory would fault the GPU inside a kernel, so it is
+// refused.  Pointers the runtime cannot classify are let through.  On the first bad argument sets the message
+// "<fn>: <name> ..." and returns 1.
+int check_arrays(const char* fn, std::initializer_list<ArrayArg> args);
+// Host-side arguments (read or written by the host): non-null only.
+int check_host(const char* fn, std::initializer_list<ArrayArg> args);
 
 // C-PML coefficient table by distance e from the edge (0 = outermost), e < cpml, for the profile of the
 // reference's CPML_x/CPML_z kernels (src/parallelcalls.cu:18-46): depth l = (cpml-e)*dh,

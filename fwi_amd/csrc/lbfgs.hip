@@ -187,6 +187,7 @@ struct fwi_lbfgs {
 extern "C" {
 
 int fwi_lbfgs_create(int nx, int nz, int m, fwi_lbfgs** out) {
+  if (check_host("fwi_lbfgs_create", {{out, "out"}})) return 1;
   *out = nullptr;
   if (nx <= 0 || nz <= 0 || m <= 0) {
     set_message("fwi_lbfgs_create: bad sizes");
@@ -231,6 +232,9 @@ static int backup(fwi_lbfgs* h, const float* vh, const float* vp, const float* v
 
 int fwi_lbfgs_sd_step(fwi_lbfgs* h, float* vh, float* vp, float* vn, const double* grad, const double* alpha,
                       void* stream) {
+  if (check_host("fwi_lbfgs_sd_step", {{h, "optimizer"}, {alpha, "alpha"}}) ||
+      check_arrays("fwi_lbfgs_sd_step", {{vh, "v_hor"}, {vp, "v_po"}, {vn, "v_nmo"}, {grad, "grad"}}))
+    return 1;
   hipStream_t st = (hipStream_t)stream;
   if (int r = backup(h, vh, vp, vn, grad, st)) return r;
   double nn[3];
@@ -247,6 +251,9 @@ int fwi_lbfgs_sd_step(fwi_lbfgs* h, float* vh, float* vp, float* vn, const doubl
 
 int fwi_lbfgs_direction(fwi_lbfgs* h, const float* vh, const float* vp, const float* vn, const double* grad,
                         void* stream) {
+  if (check_host("fwi_lbfgs_direction", {{h, "optimizer"}}) ||
+      check_arrays("fwi_lbfgs_direction", {{vh, "v_hor"}, {vp, "v_po"}, {vn, "v_nmo"}, {grad, "grad"}}))
+    return 1;
   hipStream_t st = (hipStream_t)stream;
   if (!h->have_back) {
     set_message("fwi_lbfgs_direction: no previous model/gradient (run fwi_lbfgs_sd_step first)");
@@ -300,6 +307,9 @@ int fwi_lbfgs_direction(fwi_lbfgs* h, const float* vh, const float* vp, const fl
 }
 
 int fwi_lbfgs_trial(fwi_lbfgs* h, float* vh, float* vp, float* vn, const double* alpha, void* stream) {
+  if (check_host("fwi_lbfgs_trial", {{h, "optimizer"}, {alpha, "alpha"}}) ||
+      check_arrays("fwi_lbfgs_trial", {{vh, "v_hor"}, {vp, "v_po"}, {vn, "v_nmo"}}))
+    return 1;
   hipStream_t st = (hipStream_t)stream;
   Coef3 a{{alpha[0], alpha[1], alpha[2]}}, one{{1, 1, 1}};
   hipLaunchKernelGGL(k_update3, dim3(h->blocks(h->n)), dim3(256), 0, st, vh, vp, vn, h->mback, h->q, a, one, h->nx,
@@ -309,6 +319,9 @@ int fwi_lbfgs_trial(fwi_lbfgs* h, float* vh, float* vp, float* vn, const double*
 }
 
 int fwi_lbfgs_restore(fwi_lbfgs* h, float* vh, float* vp, float* vn, void* stream) {
+  if (check_host("fwi_lbfgs_restore", {{h, "optimizer"}}) ||
+      check_arrays("fwi_lbfgs_restore", {{vh, "v_hor"}, {vp, "v_po"}, {vn, "v_nmo"}}))
+    return 1;
   hipStream_t st = (hipStream_t)stream;
   const size_t b = (size_t)h->n * sizeof(float);
   hipError_t e = hipMemcpyAsync(vh, h->mback, b, hipMemcpyDeviceToDevice, st);
@@ -318,6 +331,9 @@ int fwi_lbfgs_restore(fwi_lbfgs* h, float* vh, float* vp, float* vn, void* strea
 }
 
 int fwi_lbfgs_get_direction(fwi_lbfgs* h, double* out, void* stream) {
+  if (check_host("fwi_lbfgs_get_direction", {{h, "optimizer"}}) ||
+      check_arrays("fwi_lbfgs_get_direction", {{out, "out"}}))
+    return 1;
   hipError_t e = hipMemcpyAsync(out, h->q, 3 * (size_t)h->n * sizeof(double), hipMemcpyDeviceToDevice,
                                 (hipStream_t)stream);
   return e == hipSuccess ? 0 : h->fail(e, "get_direction");
@@ -325,6 +341,7 @@ int fwi_lbfgs_get_direction(fwi_lbfgs* h, double* out, void* stream) {
 
 // FP64 dot products of three blocks each: out[k] = sum a_k b_k (deterministic).
 int fwi_dot3(const double* a, const double* b, long long n, double* out, void* stream) {
+  if (check_host("fwi_dot3", {{out, "out"}}) || check_arrays("fwi_dot3", {{a, "a"}, {b, "b"}})) return 1;
   hipStream_t st = (hipStream_t)stream;
   double* partial = nullptr;
   double* d = nullptr;
@@ -345,14 +362,13 @@ int fwi_dot3(const double* a, const double* b, long long n, double* out, void* s
   return 0;
 }
 
-}  // extern "C"
-
 int fwi_model_clamp(float* vh, float* vp, float* vn, long long n, float lo, float hi, void* stream) {
   if (!(lo >= 0.f) || !(hi >= lo)) {
     set_message("fwi_model_clamp: need 0 <= lo <= hi");
     return 1;
   }
   if (n <= 0) return 0;
+  if (check_arrays("fwi_model_clamp", {{vh, "v_hor"}, {vp, "v_po"}, {vn, "v_nmo"}})) return 1;
   hipLaunchKernelGGL(k_clamp3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, vh, vp, vn, n, lo,
                      hi);
   hipError_t e = hipGetLastError();
@@ -362,3 +378,5 @@ int fwi_model_clamp(float* vh, float* vp, float* vn, long long n, float lo, floa
   }
   return 0;
 }
+
+}  // extern "C"

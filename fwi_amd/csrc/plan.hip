@@ -1098,6 +1098,7 @@ void fwi_options_default(fwi_options* o) {
 }
 
 int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out) {
+  if (check_host("fwi_plan_create", {{out, "plan"}})) return 1;
   *out = nullptr;
   std::string why;
   if (validate(s, why)) {
@@ -1322,12 +1323,14 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
 void fwi_plan_destroy(fwi_plan* p) { delete p; }
 
 int fwi_plan_set_wavelet(fwi_plan* p, const float* w) {
+  if (check_host("fwi_plan_set_wavelet", {{p, "plan"}, {w, "wavelet"}})) return 1;
   p->wavelet.assign(w, w + p->nt);
   hipError_t e = hipMemcpy(p->d_src, w, p->nt * sizeof(float), hipMemcpyHostToDevice);
   return e == hipSuccess ? 0 : p->fail(e, "set_wavelet");
 }
 
 int fwi_plan_set_cpml(fwi_plan* p, const float* a, const float* b) {
+  if (check_host("fwi_plan_set_cpml", {{p, "plan"}, {a, "cpml_a"}, {b, "cpml_b"}})) return 1;
   p->ta.assign(a, a + p->cpml);
   p->tb.assign(b, b + p->cpml);
   hipError_t e = hipMemcpy(p->d_ta, a, p->cpml * sizeof(float), hipMemcpyHostToDevice);
@@ -1340,6 +1343,9 @@ int fwi_plan_batch_size(const fwi_plan* p) { return p->batches.empty() ? 0 : p->
 double fwi_plan_workspace_bytes(const fwi_plan* p) { return (double)p->work_bytes; }
 
 int fwi_forward(fwi_plan* p, const float* vh, const float* vp, const float* vn, float* gathers, void* stream) {
+  if (check_host("fwi_forward", {{p, "plan"}}) ||
+      check_arrays("fwi_forward", {{vh, "v_hor"}, {vp, "v_po"}, {vn, "v_nmo"}, {gathers, "gathers"}}))
+    return 1;
   hipStream_t st = (hipStream_t)stream;
   ModelArgs m{vh, vp, vn, p->nx};
   if (int r = p->tune(m, st)) return r;
@@ -1355,6 +1361,9 @@ int fwi_forward(fwi_plan* p, const float* vh, const float* vp, const float* vn, 
 
 int fwi_misfit(fwi_plan* p, const float* vh, const float* vp, const float* vn, const float* obs, double* misfit,
                void* stream) {
+  if (check_host("fwi_misfit", {{p, "plan"}, {misfit, "misfit"}}) ||
+      check_arrays("fwi_misfit", {{vh, "v_hor"}, {vp, "v_po"}, {vn, "v_nmo"}, {obs, "obs"}}))
+    return 1;
   hipStream_t st = (hipStream_t)stream;
   ModelArgs m{vh, vp, vn, p->nx};
   if (int r = p->tune(m, st)) return r;
@@ -1375,6 +1384,10 @@ int fwi_misfit(fwi_plan* p, const float* vh, const float* vp, const float* vn, c
 
 int fwi_gradient(fwi_plan* p, const float* vh, const float* vp, const float* vn, const float* obs, double* grad,
                  double* misfit, float* syn, void* stream) {
+  if (check_host("fwi_gradient", {{p, "plan"}}) ||
+      check_arrays("fwi_gradient", {{vh, "v_hor"}, {vp, "v_po"}, {vn, "v_nmo"}, {obs, "obs"}, {grad, "grad"}}) ||
+      (syn && check_arrays("fwi_gradient", {{syn, "syn"}})))
+    return 1;
   hipStream_t st = (hipStream_t)stream;
   ModelArgs m{vh, vp, vn, p->nx};
   if (int r = p->tune(m, st)) return r;
@@ -1399,6 +1412,7 @@ int fwi_gradient(fwi_plan* p, const float* vh, const float* vp, const float* vn,
 }
 
 int fwi_plan_shot_misfits(fwi_plan* p, double* out, void* stream) {
+  if (check_host("fwi_plan_shot_misfits", {{p, "plan"}, {out, "out"}})) return 1;
   hipStream_t st = (hipStream_t)stream;
   hipError_t e = hipMemcpyAsync(out, p->d_shot_misfit, p->shots.size() * sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1454,6 +1468,11 @@ int fwi_plan_adj_tiling(const fwi_plan* p, int* rows, int* order, int* grouped) 
 
 int fwi_gradient_src(fwi_plan* p, const float* vh, const float* vp, const float* vn, const float* adj_src,
                      double* grad, float* syn, void* stream) {
+  if (check_host("fwi_gradient_src", {{p, "plan"}}) ||
+      check_arrays("fwi_gradient_src",
+                   {{vh, "v_hor"}, {vp, "v_po"}, {vn, "v_nmo"}, {adj_src, "adj_src"}, {grad, "grad"}}) ||
+      (syn && check_arrays("fwi_gradient_src", {{syn, "syn"}})))
+    return 1;
   hipStream_t st = (hipStream_t)stream;
   ModelArgs m{vh, vp, vn, p->nx};
   if (int r = p->tune(m, st)) return r;
@@ -1473,6 +1492,8 @@ int fwi_gradient_src(fwi_plan* p, const float* vh, const float* vp, const float*
 
 int fwi_chain_rule(const float* vp, const float* eps, const float* dl, const double* g, double* out, long long n,
                    void* stream) {
+  if (n <= 0) return 0;
+  if (check_arrays("fwi_chain_rule", {{vp, "vp"}, {eps, "eps"}, {dl, "delta"}, {g, "grad"}, {out, "out"}})) return 1;
   hipLaunchKernelGGL(k_chain_rule, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, vp, eps, dl,
                      g, out, n);
   hipError_t e = hipGetLastError();

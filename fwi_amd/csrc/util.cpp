@@ -19,6 +19,32 @@ void record_error(hipError_t e) {
 
 void set_message(const std::string& msg) { g_message = msg; }
 
+int check_host(const char* fn, std::initializer_list<ArrayArg> args) {
+  for (const ArrayArg& a : args)
+    if (!a.ptr) {
+      set_message(std::string(fn) + ": " + a.name + " is null");
+      return 1;
+    }
+  return 0;
+}
+
+int check_arrays(const char* fn, std::initializer_list<ArrayArg> args) {
+  if (int r = check_host(fn, args)) return r;
+  for (const ArrayArg& a : args) {
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, a.ptr) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    if (at.type == hipMemoryTypeUnregistered || (at.type == hipMemoryTypeHost && !at.devicePointer)) {
+      set_message(std::string(fn) + ": " + a.name +
+                  " is host memory the GPU cannot address (pass device, managed or pinned memory)");
+      return 1;
+    }
+  }
+  return 0;
+}
+
 void cpml_table_host(int cpml, float vmax, float dt, float dh, float f, float d0, float L, std::vector<float>& a,
                      std::vector<float>& b) {
   a.assign((size_t)(cpml > 0 ? cpml : 0), 0.f);

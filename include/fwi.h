@@ -11,7 +11,9 @@
  *
  * All array arguments are device pointers unless marked host.  Fields are x fastest (ix + nx*iz).
  * Status-returning functions return 0 on success and a nonzero code with fwi_error_message() set on
- * failure; they never fall back to another implementation.
+ * failure; they never fall back to another implementation.  Before any device work they refuse a null
+ * plan, optimizer or required array, and a device-array argument that the runtime reports as pageable
+ * host memory (a kernel would fault on it); device, managed and pinned host memory are accepted.
  */
 #ifndef FWI_H
 #define FWI_H

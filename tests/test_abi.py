@@ -79,3 +79,27 @@ def test_plan_rejects_bad_options_before_touching_the_gpu(kw, msg):
     tab = (np.zeros(cpml, np.float32), np.ones(cpml, np.float32))
     with pytest.raises(RuntimeError, match=msg):
         Plan(nx, nz, 50, 10.0, 1e-3, cpml, shots, np.zeros(50, np.float32), tab, **kw)
+
+
+def test_entry_points_reject_null_arguments_before_touching_the_gpu():
+    """The status-returning entry points refuse a null plan, optimizer or array with a message that names the
+    argument, before any device call (the legacy launchers keep the reference's unchecked semantics)."""
+    import ctypes
+
+    from fwi_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libfwi.so not built")
+    L = _lib.load()
+    msg = lambda: L.fwi_error_message().decode()
+    assert L.fwi_forward(None, None, None, None, None, None) == 1
+    assert msg() == "fwi_forward: plan is null"
+    assert L.fwi_gradient(None, None, None, None, None, None, None, None, None) == 1
+    assert msg() == "fwi_gradient: plan is null"
+    assert L.fwi_plan_set_wavelet(None, None) == 1 and msg() == "fwi_plan_set_wavelet: plan is null"
+    assert L.fwi_chain_rule(None, None, None, None, None, 10, None) == 1 and msg() == "fwi_chain_rule: vp is null"
+    assert L.fwi_chain_rule(None, None, None, None, None, 0, None) == 0  # nothing to do
+    alpha = (ctypes.c_double * 3)(1, 1, 1)
+    assert L.fwi_lbfgs_trial(None, None, None, None, alpha, None) == 1
+    assert msg() == "fwi_lbfgs_trial: optimizer is null"
+    assert L.fwi_plan_create(None, None, None) == 1 and msg() == "fwi_plan_create: plan is null"

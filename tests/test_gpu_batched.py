@@ -3,8 +3,8 @@
 Each arithmetic (FP32, MIXED, FP32D, FP32I) is compared with the oracle's instantiation of the same name:
 same expressions, same order, so gathers and gradients must be equal bit for bit (the rel-L2 is printed
 beside); the FP64 misfit is held to 1e-9 relative, since the device adds its per-block partial sums in
-another order than the oracle's sequential sum.  Different checkpoint intervals and batch splits must give identical results, because
-recomputation replays the same deterministic kernels.  The error of the FP32 path against the FP64
+another order than the oracle's sequential sum.  Different checkpoint intervals and batch splits must give
+identical results, because recomputation replays the same deterministic kernels.  The error of the FP32 path against the FP64
 oracle is measured and bounded separately (the FP32 rounding floor of the leapfrog: a few 1e-4 in the
 two-level form, ~2e-5 in the default increment form).
 """
@@ -366,3 +366,38 @@ def test_step_pairs_identical_to_single_steps(torch):
         assert np.array_equal(g1, g0) and np.array_equal(s1, s0)
         assert p1 == p0
         assert np.array_equal(G1, G0), exact
+
+
+def test_entry_points_refuse_null_and_pageable_arrays(torch):
+    """With a live plan, a null array and a pageable host array (which a kernel would fault on) are refused with
+    a message naming the argument, before any kernel runs; the plan works normally afterwards (at the true
+    model the misfit and gradient are exactly zero)."""
+    import ctypes
+
+    from fwi_amd import _lib
+
+    w = small_workload(nshots=1, nt=40)
+    say("argument checks: null and pageable host arrays refused before any kernel")
+    tab, src, sq_true, _ = oracle_setup(w, "f32i")
+    plan = _plan(w, tab, src)
+    try:
+        L = _lib.load()
+        msg = lambda: L.fwi_error_message().decode()
+        model = _dev_model(torch, sq_true)
+        ptrs = [t.data_ptr() for t in model]
+        g = torch.empty(plan.gather_size, dtype=torch.float32, device="cuda")
+        host_f = np.zeros(1 << 20, np.float32)  # 4 MiB of pageable memory, classified as unregistered
+        host_d = np.zeros(1 << 20, np.float64)
+        assert L.fwi_forward(plan._h, ptrs[0], ptrs[1], ptrs[2], None, None) == 1
+        assert msg() == "fwi_forward: gathers is null"
+        assert L.fwi_forward(plan._h, host_f.ctypes.data, ptrs[1], ptrs[2], g.data_ptr(), None) == 1
+        assert msg().startswith("fwi_forward: v_hor is host memory the GPU cannot address"), msg()
+        obs = plan.forward(model)
+        m = ctypes.c_double()
+        assert L.fwi_gradient(plan._h, *ptrs, obs.data_ptr(), host_d.ctypes.data, ctypes.byref(m), None, None) == 1
+        assert msg().startswith("fwi_gradient: grad is host memory"), msg()
+        phi, G, _ = plan.gradient(model, obs)
+        torch.cuda.synchronize()
+        assert phi == 0.0 and not G.any().item()
+    finally:
+        plan.close()
