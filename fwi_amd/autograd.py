@@ -17,6 +17,8 @@ from .survey import Plan, chain_rule, pre_variables
 class SurveyModeling(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vp: torch.Tensor, eps: torch.Tensor, delta: torch.Tensor, plan: Plan):
+        if plan.grad_mode != 1:
+            raise RuntimeError("autograd needs the exact adjoint: build the Plan with grad_mode=GRAD_EXACT")
         model = pre_variables(vp, eps, delta)
         syn = plan.forward(model)
         ctx.save_for_backward(vp, eps, delta)
@@ -28,8 +30,6 @@ class SurveyModeling(torch.autograd.Function):
         vp, eps, delta = ctx.saved_tensors
         plan: Plan = ctx.plan
         model = pre_variables(vp, eps, delta)
-        if plan.grad_mode != 1:
-            raise RuntimeError("autograd needs the exact adjoint: build the Plan with grad_mode=GRAD_EXACT")
         g_sq, _ = plan.gradient_src(model, gsyn.contiguous().float())
         g = chain_rule(vp, eps, delta, g_sq)
         return g[0].to(vp.dtype), g[1].to(eps.dtype), g[2].to(delta.dtype), None
