@@ -36,7 +36,7 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="survey1600")
     ap.add_argument("--shots", type=int, default=0, help="shots (0 = the workload's own count)")
@@ -305,6 +305,7 @@ def main():
             "item_order": plan.item_order,
             "adjoint_tiling": dict(zip(("rows_per_item", "item_order", "grouped"), plan.adj_tiling)),
             "s_per_gradient_batched": round(sec, 4),
+            "s_per_gradient_median": round(float(np.median(times)), 4),
             "graphs": {"captures": graphs[0], "replays": graphs[1]},
             "s_per_gradient_eager_instrumented": round(sec_instr, 4) if sec_instr else None,
             "s_per_gradient_plain": round(plain, 3) if plain else None,
