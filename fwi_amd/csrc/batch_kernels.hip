@@ -98,7 +98,7 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
     fq[8] = cur.f4;
     float u[9];
     xrow9(buf, lane, cur.c, cur.h, u);
-    load_row(min(z + 1, z1 - 1), nxt);  // an unused repeat on the last row keeps one path
+    load_row(min(z + 2, z1 - 1), nxt);  // two rows ahead; unused repeats on the last rows keep one path
     const float p2 = d2s<S>(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8], dh2);
     const float r2 = d2s<S>(fq[0], fq[1], fq[2], fq[3], fq[4], fq[5], fq[6], fq[7], fq[8], dh2);
     const float atp = 0.f, atf = 0.f;
@@ -141,15 +141,27 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
   };
 #pragma unroll
   for (int k = 0; k < 8; ++k) fq[k] = ldu(F, (unsigned)(ix + ps * (z0 - 4 + k)));
-  // The first row is peeled, so the loop is entered, as it is repeated, after a row's stores that follow the
-  // next row's loads: the memory-counter waits at its head are the same on both paths.
-  Row ra, rb;
+  // Two rows are in flight: each step loads the row two ahead into the buffer its predecessor consumed.  Three
+  // buffers in rotation, unrolled by three, need no register copy (a copy of a value still being loaded would
+  // wait for it); the first three rows are peeled, so the loop is entered, as it is repeated, after a row's
+  // stores that follow the loads of the rows ahead, and the memory-counter waits at its head are the same on
+  // both paths.  The last one or two rows follow the loop.
+  Row ra, rb, rc;
   load_row(z0, ra);
-  step(z0, ra, rb);
-  for (int z = z0 + 1; z < z1; ++z) {
-    ra = rb;
-    step(z, ra, rb);
+  load_row(min(z0 + 1, z1 - 1), rb);
+  int z = z0;
+  if (z + 2 < z1) {
+    step(z, ra, rc);
+    step(z + 1, rb, ra);
+    step(z + 2, rc, rb);
+    for (z += 3; z + 2 < z1; z += 3) {
+      step(z, ra, rc);
+      step(z + 1, rb, ra);
+      step(z + 2, rc, rb);
+    }
   }
+  if (z < z1) step(z, ra, rc);
+  if (z + 1 < z1) step(z + 1, rb, ra);
 }
 
 // One forward step of every shot (parallelcalls.cu:120-130 SecondDerivate_P, :65-86 ZETA_F, :88-117
@@ -770,9 +782,10 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     if (r >= zb) v = v + coef_z(a.g.ta, r, nz, cpml) * ldu(ztz, ir);
     gq[k] = (act && r >= kRim) ? v : 0.f;
   }
-  // The loads of a row are issued one row ahead, before the previous row's stores (the vector memory counter
-  // retires in issue order, so a load issued after a store can only be waited for together with it), and the
-  // loop is unrolled by two over two row buffers, so no register copy waits for them.
+  // The loads of a row are issued two rows ahead, before the current row's stores (the vector memory counter
+  // retires in issue order, so a load issued after a store can only be waited for together with it), into
+  // three row buffers used in rotation, so no register copy waits for them (interior items; the general body
+  // loads each row at its start).
   struct Row {
     float l4, m4, vp4, zt4;                // row z+4: mP, mF, v_po, zt_z (bottom C-PML rows)
     float lc, mc, vhc, vnc;                // row z at the lane's column: mP, mF (an L2 hit), v_hor, v_nmo
@@ -819,8 +832,8 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       }
     }
   };
-  // Row z from cur.  PIPE (interior items): row min(z+1, ze-1) is loaded into nxt once cur's stencil inputs
-  // are consumed and before the row's stores (an unused repeat on the last row keeps one path through the
+  // Row z from cur.  PIPE (interior items): row min(z+2, ze-1) is loaded into nxt once cur's stencil inputs
+  // are consumed and before the row's stores (unused repeats on the last rows keep one path through the
   // body).  The edge kernel loads each row at its start: its C-PML inputs leave no registers for a second row.
   auto step = [&](auto xn_t, auto zb_t, auto pipe_t, int z, int ze, const Row& cur, Row& nxt)
                   __attribute__((always_inline)) {
@@ -853,7 +866,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       pz[6] = ldu(ptz, o + 3 * snx);
       pz[7] = ldu(ptz, o + 4 * snx);
     }
-    if (PIPE) load_row(xn_t, zb_t, min(z + 1, ze - 1), nxt);
+    if (PIPE) load_row(xn_t, zb_t, min(z + 2, ze - 1), nxt);
     float gx[9];
     xrow9(buf, lane, go, gh, gx);
     float ap = d2s<S>(gx[0], gx[1], gx[2], gx[3], gx[4], gx[5], gx[6], gx[7], gx[8], dh2);
@@ -908,14 +921,24 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     if (za >= ze) return;
     Row ra, rb;
     if (decltype(pipe_t)::value) {
-      // the first row is peeled, so the loop is entered, as it is repeated, after a row's stores that follow
-      // the next row's loads: the memory-counter waits at its head are the same on both paths
+      // two rows in flight, three buffers in rotation unrolled by three (no register copy waits for a load),
+      // the first three rows peeled so the loop head sees the same memory-counter state from both paths
+      Row rc;
       load_row(xn_t, zb_t, za, ra);
-      step(xn_t, zb_t, pipe_t, za, ze, ra, rb);
-      for (int z = za + 1; z < ze; ++z) {
-        ra = rb;
-        step(xn_t, zb_t, pipe_t, z, ze, ra, rb);
+      load_row(xn_t, zb_t, min(za + 1, ze - 1), rb);
+      int z = za;
+      if (z + 2 < ze) {
+        step(xn_t, zb_t, pipe_t, z, ze, ra, rc);
+        step(xn_t, zb_t, pipe_t, z + 1, ze, rb, ra);
+        step(xn_t, zb_t, pipe_t, z + 2, ze, rc, rb);
+        for (z += 3; z + 2 < ze; z += 3) {
+          step(xn_t, zb_t, pipe_t, z, ze, ra, rc);
+          step(xn_t, zb_t, pipe_t, z + 1, ze, rb, ra);
+          step(xn_t, zb_t, pipe_t, z + 2, ze, rc, rb);
+        }
       }
+      if (z < ze) step(xn_t, zb_t, pipe_t, z, ze, ra, rc);
+      if (z + 1 < ze) step(xn_t, zb_t, pipe_t, z + 1, ze, rb, ra);
     } else {
       for (int z = za; z < ze; ++z) {
         load_row(xn_t, zb_t, z, ra);
