@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __r
 
 // The same step on an interior work item: a band within [cpml, nx-cpml) and rows above nz-cpml, where no cell
 // carries a C-PML term and every lane updates its cell.  The body has one path (no lane-masked memory access)
-// and loads each row one row ahead, before the previous row's stores: the vector memory counter retires in
+// and loads each row two rows ahead, before the current row's stores: the vector memory counter retires in
 // issue order, so a load issued after a store could only be waited for together with that store.  The
 // stored values are those of k_fwd_main, expression by expression (its C-PML terms are +0 here).
 template <int QM, class S, bool INC>
