@@ -12,6 +12,7 @@ Beside it, on the same workload:
   * roofline of the dominant kernel (the adjoint step): algorithmic bytes per launch over its average
     duration, measured with HIP events on the stream the kernel runs on during the timed region;
   * cpu_baseline: the CPU oracle (f32i, exact gradient, OpenMP) on one shot for a bounded number of steps.
+  * the device optimizer of one iteration (steepest-descent step, L-BFGS direction, trial), timed apart.
 """
 from __future__ import annotations
 
@@ -152,6 +153,36 @@ def cpu_baseline(w, nsteps):
     return 2.0 * shot.nx * w.nz * nsteps / sec / 1e9, threads, sec
 
 
+def optimizer_ms(w, model, grad):
+    """The device optimizer of one inversion iteration (src/main.c:539-800) on this model and gradient, timed
+    apart from the gradient as the reference's flow separates them: a steepest-descent step, an L-BFGS
+    direction from one curvature pair (FP64 two-loop recursion with device dot products) and one trial update,
+    in milliseconds each.  The model is a copy; the second gradient of the pair is a scaled copy of the first
+    (the timing does not depend on the values)."""
+    import torch
+
+    from fwi_amd.optim import LBFGS, SD_STEP
+
+    m = [t.clone() for t in model]
+    opt = LBFGS(w.nx, w.nz, 5)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return round((time.perf_counter() - t0) * 1e3, 3)
+
+    try:
+        out = {"sd_step_ms": timed(lambda: opt.sd_step(m, grad, SD_STEP))}
+        g2 = grad * 1.5
+        out["lbfgs_direction_ms"] = timed(lambda: opt.direction(m, g2))
+        out["trial_ms"] = timed(lambda: opt.trial(m, (1.0, 1.0, 1.0)))
+    finally:
+        opt.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -256,6 +287,10 @@ def main():
         with open(tfile) as fh:
             traffic = json.load(fh)["bytes_per_launch"].get(kname)
 
+    opt_ms = optimizer_ms(w, init_sq, grad)
+    log(f"optimizer: SD step {opt_ms['sd_step_ms']} ms, L-BFGS direction {opt_ms['lbfgs_direction_ms']} ms, "
+        f"trial {opt_ms['trial_ms']} ms")
+
     plain = None
     if not args.no_plain:
         ns = min(args.plain_steps, nt)
@@ -306,6 +341,7 @@ def main():
             "adjoint_tiling": dict(zip(("rows_per_item", "item_order", "grouped"), plan.adj_tiling)),
             "s_per_gradient_batched": round(sec, 4),
             "s_per_gradient_median": round(float(np.median(times)), 4),
+            "optimizer_ms": opt_ms,
             "graphs": {"captures": graphs[0], "replays": graphs[1]},
             "s_per_gradient_eager_instrumented": round(sec_instr, 4) if sec_instr else None,
             "s_per_gradient_plain": round(plain, 3) if plain else None,
