@@ -80,9 +80,13 @@ struct fwi_plan {
   fwi_options opt{};
   int K = 1, J = 1;  // checkpoint interval and number of segments
   int S = 1;         // trailing segments whose imaging terms the forward keeps (no checkpoint, no recomputation)
-  int sfirst() const { return (J - S) * K; }                       // first step kept from the forward
-  int qslots() const { return std::max(nt - sfirst(), K); }         // imaging-term slots (2 fields each)
-  int nckpt() const { return std::max(J - S - 1, 0); }              // checkpoints 1..J-S-1
+  // with s trailing segments kept (default: the plan's S)
+  int sfirst(int s) const { return (J - s) * K; }                  // first step kept from the forward
+  int qslots(int s) const { return std::max(nt - sfirst(s), K); }  // imaging-term slots (2 fields each)
+  int nckpt(int s) const { return std::max(J - s - 1, 0); }        // checkpoints 1..J-s-1
+  int sfirst() const { return sfirst(S); }
+  int qslots() const { return qslots(S); }
+  int nckpt() const { return nckpt(S); }
   int tz = 32;       // rows per work item (forward kernels)
   int order = 0;     // work-item order (build_items)
   int tz_adj = 32;   // rows per work item and order of the adjoint kernels
@@ -169,13 +173,9 @@ struct fwi_plan {
   float* gathers() const {
     return (float*)((char*)(d_work + (size_t)(kFixedF + 2 * (size_t)qslots()) * cap_cell) + (size_t)nckpt() * ck_bytes());
   }
-  size_t workspace_floats(int S_) const {
-    const int keep = S;
-    const_cast<fwi_plan*>(this)->S = S_;
-    const size_t n = (size_t)(kFixedF + 2 * (size_t)qslots()) * cap_cell + (ck_bytes() * (size_t)nckpt() + 3) / 4 +
-                     2 * (size_t)cap_gather;
-    const_cast<fwi_plan*>(this)->S = keep;
-    return n;
+  size_t workspace_floats(int s) const {
+    return (size_t)(kFixedF + 2 * (size_t)qslots(s)) * cap_cell + (ck_bytes() * (size_t)nckpt(s) + 3) / 4 +
+           2 * (size_t)cap_gather;
   }
   float* resid() const { return gathers() + cap_gather; }
   bool inc() const { return opt.arith == FWI_ARITH_FP32I; }
