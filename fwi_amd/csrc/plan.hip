@@ -255,6 +255,8 @@ struct fwi_plan {
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess) return fail(ei, "hipGraphInstantiate");
     if (graphs.size() >= kMaxGraphs) {  // least recently used first
+      // its last launch may still run on the stream it was given: wait for the device before destroying it
+      (void)hipDeviceSynchronize();
       (void)hipGraphExecDestroy(graphs.front().exec);
       graphs.erase(graphs.begin());
     }
@@ -265,6 +267,7 @@ struct fwi_plan {
   }
 
   ~fwi_plan() {
+    (void)hipDeviceSynchronize();  // launches of this plan may still be running on the caller's streams
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (auto& b : batches) {

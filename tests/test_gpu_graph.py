@@ -52,3 +52,31 @@ def test_graph_replays_match_eager():
     # 2 batches x 4 entry points captured on the first pass, replayed on the three others
     assert captures == 8 and replays == 3 * 8
     assert pe.graph_stats() == (0, 0)
+
+
+def test_graph_cache_eviction():
+    """More buffer sets than the plan keeps graphs for (8): forwards into 11 output buffers, twice over, evict
+    and recapture least recently used graphs, and every result still equals the eager plan's."""
+    import torch
+
+    from fwi_amd.survey import Plan
+
+    w = small_workload(nshots=2, nt=60)
+    say("graph cache eviction: 11 output buffers, 2 rounds")
+    tab, src, sq_true, _ = oracle_setup(w, "f32i")
+    mk = lambda g: Plan(w.nx, w.nz, w.nt, w.dh, w.dt, w.cpml, w.shots, src, tab, use_graph=g)
+    pg, pe = mk(1), mk(0)
+    model = [torch.from_numpy(np.ascontiguousarray(x, np.float32)).cuda() for x in sq_true]
+    ref = pe.forward(model)
+    outs = [torch.empty(pg.gather_size, dtype=torch.float32, device="cuda") for _ in range(11)]
+    for rnd in range(2):
+        for k, o in enumerate(outs):
+            o.fill_(float("nan"))
+            pg.forward(model, out=o)
+        torch.cuda.synchronize()
+        for k, o in enumerate(outs):
+            assert torch.equal(o, ref), (rnd, k)
+    captures, replays = pg.graph_stats()
+    print("graph captures, replays:", captures, replays)
+    # one batch: 11 distinct keys per round and a cache of 8, visited in order, so every call recaptures
+    assert captures == 22 and replays == 0
