@@ -352,9 +352,10 @@ int fwi_dot3(const double* a, const double* b, long long n, double* out, void* s
     hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, st, partial, kDotBlocks, d);
     e = hipMemcpyAsync(out, d, 3 * sizeof(double), hipMemcpyDeviceToHost, st);
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  (void)hipFree(partial);
-  (void)hipFree(d);
+  if (partial) (void)hipFreeAsync(partial, st);
+  if (d) (void)hipFreeAsync(d, st);
+  const hipError_t es = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = es;
   if (e != hipSuccess) {
     set_message(std::string("fwi_dot3: ") + hipGetErrorString(e));
     return (int)e;

@@ -158,3 +158,24 @@ def test_model_clamp_matches_numpy(torch):
     c[0] = np.where(c[0] < c[2], c[2], c[0])
     for k in range(3):
         np.testing.assert_array_equal(dev[k].cpu().numpy(), c[k])
+
+
+@pytest.mark.parametrize("n", [1, 1000, 640000 + 17])
+def test_dot3_matches_numpy(torch, n):
+    """fwi_dot3: the three FP64 block dot products the optimizer uses (main.c:576-676 in FP64), against numpy,
+    for one element per block, a small block and a block of the survey1600 size plus a ragged tail."""
+    import ctypes
+
+    from fwi_amd import _lib
+
+    say(f"fwi_dot3 vs numpy, n={n}")
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((3, n))
+    b = rng.standard_normal((3, n))
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    out = (ctypes.c_double * 3)()
+    _lib.check(_lib.load().fwi_dot3(da.data_ptr(), db.data_ptr(), n, out, None), "fwi_dot3")
+    ref = [float(np.dot(a[k], b[k])) for k in range(3)]
+    scale = [float(np.dot(np.abs(a[k]), np.abs(b[k]))) for k in range(3)]
+    for k in range(3):
+        assert abs(out[k] - ref[k]) <= 1e-13 * scale[k], (k, out[k], ref[k])
