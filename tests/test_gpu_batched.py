@@ -401,3 +401,31 @@ def test_entry_points_refuse_null_and_pageable_arrays(torch):
         assert phi == 0.0 and not G.any().item()
     finally:
         plan.close()
+
+
+def test_plan_retargets_to_another_stage_frequency(torch):
+    """fwi_plan_set_wavelet + fwi_plan_set_cpml move a plan to another stage's frequency (a multiscale
+    inversion's next stage): its gathers then equal those of a plan created at that frequency, bit for bit."""
+    import dataclasses
+
+    w = small_workload(nshots=2, nt=120, f=10.0)
+    say("plan retargeted from 10 Hz to 15 Hz: wavelet and C-PML table replaced")
+    tab10, src10, sq_true, _ = oracle_setup(w, "f32i")
+    p15 = dataclasses.replace(w.cpml_params, f=15.0)
+    tab15, src15 = O.cpml_table(p15), O.ricker(w.nt, 15.0, w.dt)
+    assert not np.array_equal(tab15[0], tab10[0])
+    model = _dev_model(torch, sq_true)
+    moved = _plan(w, tab10, src10)
+    fresh = _plan(w, tab15, src15)
+    try:
+        g10 = moved.forward(model).clone()
+        moved.set_wavelet(src15)
+        moved.set_cpml(tab15)
+        g_moved = moved.forward(model)
+        g_fresh = fresh.forward(model)
+        torch.cuda.synchronize()
+        assert not torch.equal(g10, g_fresh)
+        assert torch.equal(g_moved, g_fresh)
+    finally:
+        moved.close()
+        fresh.close()
