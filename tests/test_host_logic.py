@@ -162,3 +162,16 @@ def test_workload_geometry_rules():
         assert 300 + 40 <= s.nx <= 1300 + 40 and 200 <= s.nrec <= 1200 or s.nrec == s.nx - 40
         assert s.x0 + s.nx <= 1600 and s.ir0 >= 20 and s.ir0 + s.nrec <= s.nx - 20
         assert 20 <= s.isx < s.nx - 20
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says how shots would shard, before any
+    device work (so it runs here without a GPU)."""
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "measures one GPU" in r.stderr and "all-reduce" in r.stderr
+    assert r.stdout == ""
