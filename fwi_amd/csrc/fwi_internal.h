@@ -21,8 +21,8 @@ struct ArrayArg {
 };
 // Every array must be non-null and memory the GPU can address (device, managed, or pinned host memory): a
 // pointer the runtime classifies as unregistered host memory would fault the GPU inside a kernel, so it is
-// refused.  Pointers the runtime cannot classify are let through.  On the first bad argument sets the message
-// "<fn>: <name> ..." and returns 1.
+// refused, and so is a pointer the runtime cannot classify at all (the query fails, e.g. without a device).  On
+// the first bad argument sets the message "<fn>: <name> ..." and returns 1.
 int check_arrays(const char* fn, std::initializer_list<ArrayArg> args);
 // Host-side arguments (read or written by the host): non-null only.
 int check_host(const char* fn, std::initializer_list<ArrayArg> args);

@@ -319,6 +319,16 @@ struct fwi_plan {
       for (int s = 0; s < b.count; ++s)
         for (auto& r : runs)
           for (int x0 = 0; x0 < b.h_shots[s].nx; x0 += 64) items.push_back(make_int4(s, x0, r.first, r.second));
+    } else if (order == 3) {
+      for (int g = 0; g < b.count; g += 4) {
+        const int ge = std::min(b.count, g + 4);
+        int gnx = 0;
+        for (int s = g; s < ge; ++s) gnx = std::max(gnx, b.h_shots[s].nx);
+        for (auto& r : runs)
+          for (int x0 = 0; x0 < gnx; x0 += 64)
+            for (int s = g; s < ge; ++s)
+              if (x0 < b.h_shots[s].nx) items.push_back(make_int4(s, x0, r.first, r.second));
+      }
     } else if (order == 2) {
       for (int x0 = 0; x0 < maxnx; x0 += 64) {
         std::vector<int> sh;
@@ -339,7 +349,7 @@ struct fwi_plan {
 
   // Grouped adjoint items: every workgroup is one gradient group on one tile, member k on wave k, and
   // -(leader+1) (a placeholder wave) where the group has fewer than 4 shots.  Orders: 0 row run -> band ->
-  // group; 2 band -> group -> row run.
+  // group; 2 band -> group -> row run; 3 group -> row run -> band.
   static std::vector<int4> grouped_items(const Batch& b, int nz, int rows, int order,
                                          const std::vector<int>& zsplit) {
     std::vector<int4> items;
@@ -365,6 +375,10 @@ struct fwi_plan {
         for (auto& g : b.groups)
           if (x < g.unx)
             for (auto& r : runs) push(g, g.ux0 + x, r);
+    } else if (order == 3) {
+      for (auto& g : b.groups)
+        for (auto& r : runs)
+          for (int x = 0; x < g.unx; x += 64) push(g, g.ux0 + x, r);
     } else {
       for (auto& r : runs)
         for (int x = 0; x < maxunx; x += 64)
@@ -411,13 +425,16 @@ struct fwi_plan {
     return 0;
   }
 
-  // Work items of the forward kernels in one of three orders (the tuner picks one per plan):
+  // Work items of the forward kernels in one of four orders (the tuner picks one per plan):
   //   0  row run -> band -> shot: the four waves of a workgroup are four shots on one tile (model rows
   //      shared through the L1), x-neighbour tiles are a few workgroups apart;
   //   1  shot -> row run -> band: a workgroup is four adjacent bands of one shot (x halos shared through
   //      the L1), z-neighbour row runs one band row apart;
   //   2  band -> shot group of 4 -> row run -> shot: consecutive workgroups walk down a column of tiles,
-  //      so z halos are reused from the L2 while the neighbour is still resident.
+  //      so z halos are reused from the L2 while the neighbour is still resident;
+  //   3  shot group of 4 -> row run -> band -> shot: a workgroup is four shots on one tile (model rows shared
+  //      through the L1) and consecutive workgroups are its x neighbours, with the z neighbours a row of bands
+  //      later, so both halos are read while the neighbour tile is resident in the L2.
   // Workgroup ids are remapped so each XCD receives a contiguous range of this order.
   // Forward items, cut at row nz-e (e = max(cpml,4), the first bottom C-PML row): the kernel runs items without
   // C-PML cells with its pipelined interior body; the others go first in each XCD's range.
@@ -480,7 +497,7 @@ struct fwi_plan {
     return 0;
   }
 
-  // Work items of the adjoint kernels: grouped (orders 0 and 2) when the batch has gradient groups.  No item
+  // Work items of the adjoint kernels: grouped (orders 0, 2 and 3) when the batch has gradient groups.  No item
   // crosses row nz-cpml-4, the first row whose z stencil reaches the bottom C-PML strip: the exact adjoint
   // kernel runs the rows of an item with or without the bottom C-PML terms, by its first row.
   int build_adj_items(Batch& b, int rows, int order) {
@@ -949,8 +966,8 @@ struct fwi_plan {
     std::vector<int> rows_c = {16, 8, 4, 24, 32, 48, 64, 96};  // 8, 4: small grids, whose few items are latency-bound
     if (opt.rows_per_item > 0) rows_c = {opt.rows_per_item};
     Batch& b = batches[0];
-    std::vector<int> order_f = {0, 1, 2};
-    std::vector<int> order_a = b.grouped ? std::vector<int>{0, 2} : std::vector<int>{0, 1, 2};
+    std::vector<int> order_f = {0, 1, 2, 3};
+    std::vector<int> order_a = b.grouped ? std::vector<int>{0, 2, 3} : std::vector<int>{0, 1, 2, 3};
     if (opt.item_order >= 0) {
       order_f = {opt.item_order};
       order_a = {b.grouped && opt.item_order == 1 ? 0 : opt.item_order};
@@ -1312,7 +1329,7 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
            "copy sblocks");
     }
     if (rc == 0) rc = p->build_items(b, p->tz, p->opt.item_order > 0 ? p->opt.item_order : 0);
-    if (rc == 0) rc = p->build_adj_items(b, p->tz, p->opt.item_order == 2 ? 2 : 0);
+    if (rc == 0) rc = p->build_adj_items(b, p->tz, p->opt.item_order == 2 || p->opt.item_order == 3 ? p->opt.item_order : 0);
     if (rc == 0 && p->inc() && p->opt.step_pairs != 0) rc = p->build_pair_items(b, p->tz_pair);
   }
   if (rc) {

@@ -32,9 +32,13 @@ int check_arrays(const char* fn, std::initializer_list<ArrayArg> args) {
   if (int r = check_host(fn, args)) return r;
   for (const ArrayArg& a : args) {
     hipPointerAttribute_t at{};
-    if (hipPointerGetAttributes(&at, a.ptr) != hipSuccess) {
+    const hipError_t e = hipPointerGetAttributes(&at, a.ptr);
+    if (e != hipSuccess) {
+      // fail closed: memory the runtime cannot classify never reaches a kernel
       (void)hipGetLastError();
-      continue;
+      set_message(std::string(fn) + ": " + a.name + " is memory the runtime cannot classify (hipPointerGetAttributes: " +
+                  hipGetErrorString(e) + "); pass device, managed or pinned memory");
+      return 1;
     }
     if (at.type == hipMemoryTypeUnregistered || (at.type == hipMemoryTypeHost && !at.devicePointer)) {
       set_message(std::string(fn) + ": " + a.name +

@@ -69,7 +69,7 @@ typedef struct fwi_options {
   int max_batch;       /* shots per batch, 0 = as many as mem_budget allows */
   double mem_budget;   /* bytes of device memory the plan may use, 0 = 85 % of free memory */
   int rows_per_item;   /* z rows per wave in the stencil kernels, 0 = chosen by the tuner */
-  int item_order;      /* work-item order 0..2 (see plan.hip build_items), -1 = chosen by the tuner */
+  int item_order;      /* work-item order 0..3 (see plan.hip build_items), -1 = chosen by the tuner */
   int grad_group;      /* exact mode: up to this many consecutive shots with identical windows share one
                           FP32 gradient slot, their per-step imaging terms added in shot order before one
                           read-modify-write (0 = 4, the default; 1 = one slot per shot) */

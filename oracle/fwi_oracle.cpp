@@ -35,7 +35,10 @@
 //                         f32i against f64 at 1e-5 and ahead of f32 on a long, finely sampled run
 //   exact adjoint       : dot-product test of the one-step transpose, FD directional derivatives,
 //                         brute-force central differences for every cell of a 24x24 grid
-//   ref-compat gradient : restated formula; relation to the exact gradient (lag + 1/dt) on tiny grids
+//   ref-compat gradient : with the C-PML off its adjoint fields equal the exact adjoint's (2e-14, FP64); its
+//                         gradient equals dt sum_k lam^k D2(u^(k-2)) assembled in numpy from the recorded fields
+//                         (a lag of 0 or 2 or a dt^2 scale miss by > 5e-2); with the C-PML on, a step-by-step
+//                         composition of the pinned forward routines on aux1..4 reproduces it bit for bit
 //   L-BFGS two-loop     : dense BFGS inverse update with H0 = gamma*I (Nocedal & Wright Alg. 7.4)
 // The reference's own rounding (nvcc FMA contraction, device exp/pow in CPML_x/CPML_z) cannot be
 // reproduced without running it and is "parity unpinned" at the last-ulp level.
@@ -678,18 +681,26 @@ struct Scal {
 //   qmode 2: PF_h snapshot of (P_past, F_past) taken before the update (serialcalls.cu:60), i.e. u^{n-1}
 template <class T, class S>
 void forward_shot(const T* vh, const T* vp, const T* vn, const Tables<T>& tb, const T* src, const Geo& g,
-                  const ShotGeo& sg, int nt, Scal sc, T* gather, int qmode, std::vector<T>* q) {
+                  const ShotGeo& sg, int nt, Scal sc, T* gather, int qmode, std::vector<T>* q, T* rec = nullptr) {
   FwdState<T, S> st(g);
   const T dh = (T)sc.dh, dt = (T)sc.dt, dh2 = dh * dh, C = (dt * dt) / dh2;
   const long n = st.n;
   if (q) q->assign((size_t)nt * 2 * n, (T)0);
+  // rec (optional): the state P^n, F^n that step n starts from, n = 0..nt (2 fields per n)
+  auto record = [&](int k) {
+    if (!rec) return;
+    std::memcpy(rec + (size_t)k * 2 * n, st.Pcur(), n * sizeof(T));
+    std::memcpy(rec + (size_t)k * 2 * n + n, st.Fcur(), n * sizeof(T));
+  };
   for (int it = 0; it < nt; ++it) {
+    record(it);
     if (qmode == 2) {
       std::memcpy(q->data() + (size_t)it * 2 * n, st.Pold(), n * sizeof(T));
       std::memcpy(q->data() + (size_t)it * 2 * n + n, st.Fold(), n * sizeof(T));
     }
     fwd_step<T, S>(st, vh, vp, vn, tb.ax.data(), tb.bx.data(), tb.az.data(), tb.bz.data(), src, C, dh, dh2, sg, it,
                    gather, nt);
+    if (it == nt - 1) record(nt);
     if (qmode == 1) {
       T* qa = q->data() + (size_t)it * 2 * n;
       T* qb = qa + n;
@@ -712,7 +723,8 @@ void forward_shot(const T* vh, const T* vp, const T* vn, const Tables<T>& tb, co
 // time-reversed at the receivers), GRADIENT with the snapshot PF_h[nt-it-1].
 template <class T, class S>
 void adjoint_ref_shot(const T* vh, const T* vp, const T* vn, const Tables<T>& tb, const Geo& g, const ShotGeo& sg,
-                      int nt, Scal sc, const T* res_g, const std::vector<T>& snaps, T* gcx, T* gcz, T* gdx) {
+                      int nt, Scal sc, const T* res_g, const std::vector<T>& snaps, T* gcx, T* gcz, T* gdx,
+                      T* rec = nullptr) {
   const long n = (long)g.nx * g.nz;
   const T dh = (T)sc.dh, dt = (T)sc.dt, dh2 = dh * dh, C = (dt * dt) / dh2;
   Fields<T> f(n, 4 + 4 + 4 + 4 + 4 + 4);
@@ -749,6 +761,10 @@ void adjoint_ref_shot(const T* vh, const T* vp, const T* vn, const Tables<T>& tb
       T v = resw[(size_t)(nt - it - 1) * g.nx + sg.ir0 + r];
       Pm[i] += v;
       Fm[i] += v;
+    }
+    if (rec) {  // (lambda_P, lambda_F) after step it, stored at k - 1 with k = nt - it (App. A.4: mu^k)
+      std::memcpy(rec + (size_t)(nt - it - 1) * 2 * n, Pm, n * sizeof(T));
+      std::memcpy(rec + (size_t)(nt - it - 1) * 2 * n + n, Fm, n * sizeof(T));
     }
     k_gradient<T, S>(Pf, Ff, Pm, Fm, gcx, gcz, gdx, dh2, dt, g);
     std::swap(Pp, Pm);
@@ -854,7 +870,8 @@ struct AdjOp {
 // where mu^k is the adjoint of u^k (residual res^{k-1} injected at the receivers into P and F).
 template <class T, class S>
 void adjoint_exact_shot(const T* vh, const T* vp, const T* vn, const Tables<T>& tb, const Geo& g, const ShotGeo& sg,
-                        int nt, Scal sc, const T* res_g, const std::vector<T>& q, T* gh, T* gp, T* gn) {
+                        int nt, Scal sc, const T* res_g, const std::vector<T>& q, T* gh, T* gp, T* gn,
+                        T* rec = nullptr) {
   const long n = (long)g.nx * g.nz;
   const T dh = (T)sc.dh, dt = (T)sc.dt, dt2 = dt * dt;
   AdjOp<T, S> op(g, tb, vh, vp, vn, dh, dt);
@@ -875,6 +892,10 @@ void adjoint_exact_shot(const T* vh, const T* vp, const T* vn, const Tables<T>& 
     std::swap(muF, nF);
     std::swap(shP, oP);
     std::swap(shF, oF);
+    if (rec) {  // (mu_P^k, mu_F^k) at k - 1
+      std::memcpy(rec + (size_t)(k - 1) * 2 * n, muP, n * sizeof(T));
+      std::memcpy(rec + (size_t)(k - 1) * 2 * n + n, muF, n * sizeof(T));
+    }
     const T* qa = q.data() + (size_t)(k - 1) * 2 * n;
     const T* qb = qa + n;
 #pragma omp parallel for
@@ -891,13 +912,14 @@ void adjoint_exact_shot(const T* vh, const T* vp, const T* vn, const Tables<T>& 
 template <class T, class S>
 double shot_gradient(int mode_exact, const T* vh, const T* vp, const T* vn, const T* tab_a, const T* tab_b, const T* src,
                      const T* obs, int nx, int nz, int nt, int cpml, double dh, double dt, int isx, int isz, int irz,
-                     int ir0, int nrec, T* syn, T* g_hor, T* g_po, T* g_nmo, double* misfit_ref32) {
+                     int ir0, int nrec, T* syn, T* g_hor, T* g_po, T* g_nmo, double* misfit_ref32,
+                     T* fwd_rec = nullptr, T* adj_rec = nullptr) {
   Geo g{nx, nz, cpml};
   ShotGeo sg{isx, isz, irz, ir0, nrec};
   Tables<T> tb(tab_a, tab_b, cpml, nx, nz);
   Scal sc{dh, dt};
   std::vector<T> q;
-  forward_shot<T, S>(vh, vp, vn, tb, src, g, sg, nt, sc, syn, mode_exact ? 1 : 2, &q);
+  forward_shot<T, S>(vh, vp, vn, tb, src, g, sg, nt, sc, syn, mode_exact ? 1 : 2, &q, fwd_rec);
   std::vector<T> res((size_t)nt * nrec);
   double phi = 0;
   float phi32 = 0;
@@ -912,9 +934,9 @@ double shot_gradient(int mode_exact, const T* vh, const T* vp, const T* vn, cons
   phi32 = phi32 / 2;
   if (misfit_ref32) *misfit_ref32 = phi32;
   if (mode_exact)
-    adjoint_exact_shot<T, S>(vh, vp, vn, tb, g, sg, nt, sc, res.data(), q, g_hor, g_po, g_nmo);
+    adjoint_exact_shot<T, S>(vh, vp, vn, tb, g, sg, nt, sc, res.data(), q, g_hor, g_po, g_nmo, adj_rec);
   else  // reference arrays: gcx <-> v_hor, gcz <-> v_po, gdx <-> v_nmo (main.c:573)
-    adjoint_ref_shot<T, S>(vh, vp, vn, tb, g, sg, nt, sc, res.data(), q, g_hor, g_po, g_nmo);
+    adjoint_ref_shot<T, S>(vh, vp, vn, tb, g, sg, nt, sc, res.data(), q, g_hor, g_po, g_nmo, adj_rec);
   return phi;
 }
 
@@ -1152,6 +1174,21 @@ ORC_GRAD(f32, float, float)
 ORC_GRAD(f64, double, double)
 ORC_GRAD(f32d, float, F32D)
 ORC_GRAD(f32i, float, F32I)
+
+// orc_gradient_* that also records the fields it went through: fwd_states[n] = (P^n, F^n) for n = 0..nt (the
+// state forward step n starts from; (nt+1) x 2 x nx*nz) and adj_states[k-1] = the adjoint (P, F) fields
+// paired with k = nt..1 ((nt) x 2 x nx*nz): mu^k of the exact adjoint, or the reference-compatible adjoint's
+// (P_pre, F_pre) after its step it = nt - k (main.c:478-492).
+#define ORC_GRAD_STATES(SUFFIX, T, S)                                                                              \
+  double orc_gradient_states_##SUFFIX(int mode_exact, const T* vh, const T* vp, const T* vn, const T* tab_a,        \
+                                      const T* tab_b, const T* src, const T* obs, int nx, int nz, int nt, int cpml, \
+                                      double dh, double dt, int isx, int isz, int irz, int ir0, int nrec, T* syn,   \
+                                      T* g_hor, T* g_po, T* g_nmo, T* fwd_states, T* adj_states) {                  \
+    return shot_gradient<T, S>(mode_exact, vh, vp, vn, tab_a, tab_b, src, obs, nx, nz, nt, cpml, dh, dt, isx, isz,  \
+                               irz, ir0, nrec, syn, g_hor, g_po, g_nmo, nullptr, fwd_states, adj_states);           \
+  }
+ORC_GRAD_STATES(mix, float, double)
+ORC_GRAD_STATES(f64, double, double)
 
 // Exact gradient of a gradient group (1..4 shots with their own windows of the nxm-wide model), accumulated
 // into g_* over the union of the windows (union width x nz); per-shot FP64 misfits into misfits[ns].

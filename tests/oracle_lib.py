@@ -71,6 +71,10 @@ def _declare(L):
         fn = getattr(L, f"orc_group_gradient_{suf}")
         fn.argtypes = [P] * 6 + [_i, _i, _i, _i, _d, _d, _i, ctypes.c_void_p, P, P, P, P, P, _D]
         fn.restype = None
+    for suf, P in (("mix", _F), ("f64", _D)):
+        fn = getattr(L, f"orc_gradient_states_{suf}")
+        fn.argtypes = [_i] + [P] * 7 + [_i, _i, _i, _i, _d, _d, _i, _i, _i, _i, _i, P, P, P, P, P, P]
+        fn.restype = _d
     L.orc_group_gradient_ck_f32i.argtypes = [_F] * 6 + [_i, _i, _i, _i, _d, _d, _i, ctypes.c_void_p] + [_F] * 5 + [
         _D, _i, _i]
     L.orc_group_gradient_ck_f32i.restype = None
@@ -154,6 +158,26 @@ def gradient(mode, exact, model_sq, tab, src, obs, nx, nz, nt, cpml, dh, dt, sho
                                                  shot.irz, shot.ir0, shot.nrec, ptr(syn), ptr(gh), ptr(gp), ptr(gn),
                                                  ptr(m32))
     return phi, float(m32[0]), syn, gh, gp, gn
+
+
+def gradient_states(mode, exact, model_sq, tab, src, obs, nx, nz, nt, cpml, dh, dt, shot):
+    """gradient() that also returns the fields it went through (mode "mix" or "f64"): fwd[n] = (P^n, F^n), the
+    state forward step n starts from, n = 0..nt, shape [nt+1, 2, nz, nx]; adj[k-1] = the adjoint (P, F) fields
+    paired with k = nt..1, shape [nt, 2, nz, nx]: mu^k of the exact adjoint, or the reference-compatible
+    adjoint's fields after its step nt - k.  Returns (misfit_f64, syn, g_hor, g_po, g_nmo, fwd, adj)."""
+    T = _dtype_of(mode)
+    vh, vp, vn = (np.ascontiguousarray(x, T) for x in model_sq)
+    ta, tb = (np.ascontiguousarray(x, T) for x in tab)
+    s = np.ascontiguousarray(src, T)
+    o = np.ascontiguousarray(obs, T)
+    syn = np.zeros((nt, shot.nrec), T)
+    gh, gp, gn = (np.zeros((nz, nx), T) for _ in range(3))
+    fwd = np.zeros((nt + 1, 2, nz, nx), T)
+    adj = np.zeros((nt, 2, nz, nx), T)
+    phi = getattr(lib(), f"orc_gradient_states_{mode}")(
+        1 if exact else 0, ptr(vh), ptr(vp), ptr(vn), ptr(ta), ptr(tb), ptr(s), ptr(o), nx, nz, nt, cpml, dh, dt,
+        shot.isx, shot.isz, shot.irz, shot.ir0, shot.nrec, ptr(syn), ptr(gh), ptr(gp), ptr(gn), ptr(fwd), ptr(adj))
+    return phi, syn, gh, gp, gn, fwd, adj
 
 
 def group_gradient(mode, model_sq, tab, src, obs_list, nxm, nz, nt, cpml, dh, dt, shots, ckpt=None):

@@ -103,3 +103,25 @@ def test_entry_points_reject_null_arguments_before_touching_the_gpu():
     assert L.fwi_lbfgs_trial(None, None, None, None, alpha, None) == 1
     assert msg() == "fwi_lbfgs_trial: optimizer is null"
     assert L.fwi_plan_create(None, None, None) == 1 and msg() == "fwi_plan_create: plan is null"
+
+
+def test_array_checks_fail_closed():
+    """An array the runtime cannot classify is refused, not passed on to a kernel: without a GPU every array
+    query fails, so a non-null host array is refused with a message naming it (on a GPU machine the same
+    pageable array is classified as unregistered host memory and refused as well).  Nothing is launched."""
+    import numpy as np
+
+    from fwi_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libfwi.so not built")
+    L = _lib.load()
+    a = np.zeros(16, np.float32)
+    g = np.zeros(48, np.float64)
+    o = np.zeros(48, np.float64)
+    p = a.ctypes.data
+    assert L.fwi_chain_rule(p, p, p, g.ctypes.data, o.ctypes.data, 16, None) == 1
+    m = L.fwi_error_message().decode()
+    assert m.startswith("fwi_chain_rule: vp is "), m
+    assert ("cannot classify" in m) or ("host memory the GPU cannot address" in m), m
+    assert not o.any()

@@ -118,7 +118,7 @@ def test_tiling_invariance(torch):
     w.shots = [M.Shot(x0=0, nx=150, isx=70, isz=6, irz=6, ir0=12, nrec=126),
                M.Shot(x0=20, nx=220, isx=100, isz=7, irz=6, ir0=12, nrec=196),
                M.Shot(x0=170, nx=70, isx=30, isz=6, irz=8, ir0=14, nrec=40)]
-    say("tiling invariance: rows {4, 8, 16, 24} x orders {0, 1, 2}, ragged windows")
+    say("tiling invariance: rows {4, 8, 16, 24} x orders {0, 1, 2, 3}, ragged windows")
     tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
     obs = torch.from_numpy(_cat(oracle_gathers(w, "f32i", sq_true, tab, src))).cuda()
     model = _dev_model(torch, sq_init)
@@ -126,7 +126,7 @@ def test_tiling_invariance(torch):
     for exact in (1, 0):
         ref = None
         for rows in (4, 8, 16, 24):
-            for order in (0, 1, 2):
+            for order in (0, 1, 2, 3):
                 plan = _plan(w, tab, src, grad_mode=exact, rows_per_item=rows, item_order=order,
                              ckpt_interval=9, store_segments=1)
                 g = plan.forward(true_model)

@@ -405,6 +405,186 @@ def test_refcompat_gradient_relates_to_exact():
         assert 0.7 < np.linalg.norm(b) / np.linalg.norm(a) < 1.4
 
 
+# ------------------------------------------------------------------------------------------------
+# Reference-compatible gradient (serialcalls.cu:55-90, main.c:422-492, parallelcalls.cu:144-259)
+# ------------------------------------------------------------------------------------------------
+
+_W2 = (-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0)  # Fornberg 8th-order D2 weights
+
+
+def _np_d2(u, axis, h2):
+    """8th-order second derivative along x (axis 1) or z (axis 0) on the interior, zero on the 4-cell rim,
+    written from the Fornberg weights in FP64 (independent of the oracle's stencil code)."""
+    nz, nx = u.shape
+    c = _W2[0] * u[4:nz - 4, 4:nx - 4]
+    for k in range(1, 5):
+        if axis == 1:
+            c = c + _W2[k] * (u[4:nz - 4, 4 + k:nx - 4 + k] + u[4:nz - 4, 4 - k:nx - 4 - k])
+        else:
+            c = c + _W2[k] * (u[4 + k:nz - 4 + k, 4:nx - 4] + u[4 - k:nz - 4 - k, 4:nx - 4])
+    out = np.zeros_like(u, dtype=np.float64)
+    out[4:nz - 4, 4:nx - 4] = c / h2
+    return out
+
+
+def _refcompat_case(mode, cpml_on, nt=100, nx=40, nz=36, cpml=8):
+    """A small VTI shot (true and initial models differ) in `mode` ("f64" or "mix"); C-PML tables zeroed when
+    not cpml_on.  Returns the parameters and the oracle's states of both gradient modes."""
+    p = M.CpmlParams(cpml=cpml, dh=10.0, dt=1e-3, f=20.0)
+    ta, tb = O.cpml_table(p)
+    if not cpml_on:
+        ta, tb = ta * 0, tb * 0
+    T = np.float64 if mode == "f64" else np.float32
+    src = O.ricker(nt, p.f, p.dt).astype(T)
+    shot = M.reference_shot(nx, cpml, 17, M.depth_row(2))
+    sqt = O.pre_variables(*M.gradient_model(nx, nz, seed=5, cpml=cpml, sigma=4.0), dtype=T)
+    sq = O.pre_variables(*M.gradient_model(nx, nz, seed=2, cpml=cpml, sigma=4.0), dtype=T)
+    obs = O.forward(mode, sqt, (ta, tb), src, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    out = {}
+    for exact in (True, False):
+        out[exact] = O.gradient_states(mode, exact, sq, (ta, tb), src, obs, nx, nz, nt, cpml, p.dh, p.dt, shot)
+    return p, (ta, tb), src, shot, sq, obs, out
+
+
+def test_refcompat_adjoint_equals_exact_adjoint_without_cpml():
+    """With the C-PML tables zeroed, the reference's adjoint recursion (entryPoint, secondDerivate, PSI_R, ZETA_R,
+    retroPropagation; parallelcalls.cu:144-243) is the transpose of the forward step: the fields after its step
+    it equal the exact discrete adjoint mu^(nt-it), which the dot-product, FD and per-cell brute-force tests
+    pin.  A field taken one step off (the pairing a lag error would produce) differs at the percent level."""
+    p, tab, src, shot, sq, obs, out = _refcompat_case("f64", cpml_on=False)
+    mu = out[True][6]
+    lam = out[False][6]
+    scale = np.abs(mu).max()
+    assert scale > 0
+    assert np.abs(lam - mu).max() <= 1e-11 * scale
+    assert np.abs(lam[1:] - mu[:-1]).max() > 1e-2 * scale
+
+
+def test_refcompat_gradient_is_the_lagged_dt_scaled_imaging():
+    """The reference's gradient (GRADIENT after each retroPropagation, with the snapshot PF_h[nt-it-1] that
+    launchPropagation copied from P_past before its update; serialcalls.cu:60,76,88) is
+        g_hor = dt sum_k lamP^k D2x P^(k-2),  g_nmo = dt sum_k lamF^k D2x P^(k-2),
+        g_po  = dt sum_k (lamP^k + lamF^k) D2z F^(k-2)
+    with lam^k the adjoint fields paired with k (the exact mu^k when the C-PML is off) and P^(-1) = 0.  It is
+    assembled here in numpy (its own D2) from the oracle's FP64 forward fields and adjoint fields, with and
+    without C-PML; the exact pairing (lag 0), a lag of 2 and a dt^2 scale each miss by orders of magnitude."""
+    for cpml_on in (False, True):
+        p, tab, src, shot, sq, obs, out = _refcompat_case("f64", cpml_on=cpml_on)
+        nt = out[False][5].shape[0] - 1
+        fwd = out[False][5]
+        lam = out[True][6] if not cpml_on else out[False][6]
+        dt, h2 = p.dt, p.dh * p.dh
+        ref = out[False][2:5]  # g_hor, g_po, g_nmo of the reference-compatible mode
+
+        def assemble(lag, scale):
+            gh, gp, gn = (np.zeros_like(ref[0]) for _ in range(3))
+            for k in range(1, nt + 1):
+                j = k - 1 - lag
+                if j < 0:
+                    continue
+                dxp = _np_d2(fwd[j, 0], 1, h2)
+                dzf = _np_d2(fwd[j, 1], 0, h2)
+                lp, lf = lam[k - 1, 0], lam[k - 1, 1]
+                gh += scale * dxp * lp
+                gn += scale * dxp * lf
+                gp += scale * dzf * (lp + lf)
+            return gh, gp, gn
+
+        def err(g):
+            return max(np.abs(a - b).max() / np.abs(b).max() for a, b in zip(g, ref))
+
+        assert err(assemble(1, dt)) < 1e-10, cpml_on
+        assert err(assemble(0, dt)) > 5e-2, cpml_on
+        assert err(assemble(2, dt)) > 5e-2, cpml_on
+        assert err(assemble(1, dt * dt)) > 0.5, cpml_on
+
+
+def test_refcompat_step_composition_with_cpml_bit_exact():
+    """With C-PML on, in the reference's arithmetic (mix: float storage, double-literal stencil sums): one
+    reference adjoint step is the forward routines applied to the auxiliary fields -- aux1 = v_hor lP, aux2 =
+    v_po lP, aux3 = v_nmo lF, aux4 = v_po lF (entryPoint); D2x aux1, D2z aux2, D2x aux3, D2z aux4 by
+    SecondDerivate_P; PSI_R = PSI_F on (aux1, aux2) and on (aux3, aux4); ZETA_R = ZETA_F on (p2cx, p2cz, psi1,
+    psi2) and on (r2dx, r2cz, psi3, psi4) -- then the retroPropagation leapfrog with the residual injected at
+    time nt-it-1, and GRADIENT with the forward snapshot u^(nt-it-2).  Composed here step by step from the
+    oracle's pinned per-kernel forward routines, it reproduces the oracle's reference-compatible adjoint fields
+    and gradient bit for bit; a snapshot one step off or the v_po/v_hor products swapped do not."""
+    nt = 80
+    p, (ta, tb), src, shot, sq, obs, out = _refcompat_case("mix", cpml_on=True, nt=nt)
+    nx, nz, cpml = 40, 36, 8
+    L = O.lib()
+    vh, vp, vn = (np.ascontiguousarray(x, np.float32) for x in sq)
+    fwd = out[False][5]
+    syn = out[False][1]
+    res = (syn - obs).astype(np.float32)
+    # C-PML coefficients per column / row (a, b = 0 off the strips; no C-PML at the top)
+    ax, bx, az, bz = (np.zeros(n, np.float32) for n in (nx, nx, nz, nz))
+    for ix in range(nx):
+        e = ix if ix < cpml else (nx - 1 - ix if ix > nx - cpml - 1 else -1)
+        if e >= 0:
+            ax[ix], bx[ix] = ta[e], tb[e]
+    for iz in range(nz - cpml, nz):
+        az[iz], bz[iz] = ta[nz - 1 - iz], tb[nz - 1 - iz]
+    dh = np.float32(p.dh)
+    dt = np.float32(p.dt)
+    dh2 = dh * dh
+    C = (dt * dt) / dh2
+    Z = lambda: np.zeros((nz, nx), np.float32)
+
+    def run(snap_lag=2, swap=False):
+        lP, lF, lPm, lFm = Z(), Z(), Z(), Z()
+        psi = [Z() for _ in range(4)]
+        zeta = [Z() for _ in range(4)]
+        gh, gp, gn = Z(), Z(), Z()
+        states = np.zeros((nt, 2, nz, nx), np.float32)
+        for it in range(nt):
+            a1 = (vp if swap else vh) * lP
+            a2 = vp * lP
+            a3 = vn * lF
+            a4 = vp * lF
+            p2cx, p2cz, r2dx, r2cz = Z(), Z(), Z(), Z()
+            L.orc_SecondDerivate_P(0, O.ptr(a1), O.ptr(a2), O.ptr(p2cx), O.ptr(p2cz), dh2, nx, nz)
+            L.orc_SecondDerivate_P(0, O.ptr(a3), O.ptr(a4), O.ptr(r2dx), O.ptr(r2cz), dh2, nx, nz)
+            L.orc_PSI_F(0, O.ptr(a1), O.ptr(a2), O.ptr(ax), O.ptr(bx), O.ptr(az), O.ptr(bz), O.ptr(psi[0]),
+                        O.ptr(psi[1]), nx, nz, dh, cpml)
+            L.orc_PSI_F(0, O.ptr(a3), O.ptr(a4), O.ptr(ax), O.ptr(bx), O.ptr(az), O.ptr(bz), O.ptr(psi[2]),
+                        O.ptr(psi[3]), nx, nz, dh, cpml)
+            at = [Z() for _ in range(4)]
+            L.orc_ZETA_F(0, O.ptr(p2cx), O.ptr(p2cz), O.ptr(ax), O.ptr(bx), O.ptr(az), O.ptr(bz), O.ptr(psi[0]),
+                         O.ptr(psi[1]), O.ptr(zeta[0]), O.ptr(zeta[1]), O.ptr(at[0]), O.ptr(at[1]), nx, nz, dh, cpml)
+            L.orc_ZETA_F(0, O.ptr(r2dx), O.ptr(r2cz), O.ptr(ax), O.ptr(bx), O.ptr(az), O.ptr(bz), O.ptr(psi[2]),
+                         O.ptr(psi[3]), O.ptr(zeta[2]), O.ptr(zeta[3]), O.ptr(at[2]), O.ptr(at[3]), nx, nz, dh, cpml)
+            i = (slice(4, nz - 4), slice(4, nx - 4))
+            nP, nF = lPm.copy(), lFm.copy()
+            nP[i] = np.float32(2) * lP[i] - lPm[i] + (dh2 * C) * (((p2cx[i] + at[0][i]) + r2dx[i]) + at[2][i])
+            nF[i] = np.float32(2) * lF[i] - lFm[i] + (dh2 * C) * (((p2cz[i] + r2cz[i]) + at[1][i]) + at[3][i])
+            r = res[nt - it - 1]
+            nP[shot.irz, shot.ir0:shot.ir0 + shot.nrec] += r
+            nF[shot.irz, shot.ir0:shot.ir0 + shot.nrec] += r
+            states[nt - it - 1] = (nP, nF)
+            j = nt - it - snap_lag
+            Pf, Ff = (fwd[j, 0], fwd[j, 1]) if j >= 0 else (Z(), Z())
+            d2p, d2f = Z(), Z()
+            L.orc_SecondDerivate_P(0, O.ptr(np.ascontiguousarray(Pf)), O.ptr(np.ascontiguousarray(Ff)), O.ptr(d2p),
+                                   O.ptr(d2f), dh2, nx, nz)
+            gh[i] = gh[i] + (dt * d2p[i]) * nP[i]
+            gp[i] = gp[i] + (dt * d2f[i]) * (nP[i] + nF[i])
+            gn[i] = gn[i] + (dt * d2p[i]) * nF[i]
+            lPm, lFm, lP, lF = lP, lF, nP, nF
+        return states, (gh, gp, gn)
+
+    states, g = run()
+    ref_states = out[False][6]
+    ref_g = out[False][2:5]
+    assert np.array_equal(states, ref_states)
+    for a, b in zip(g, ref_g):
+        assert np.abs(b).max() > 0
+        assert np.array_equal(a, b)
+    _, g_lag = run(snap_lag=1)
+    assert not np.array_equal(g_lag[0], ref_g[0])
+    s_swap, _ = run(swap=True)
+    assert not np.array_equal(s_swap, ref_states)
+
+
 def test_misfit_reference_float_sum_close_to_fp64():
     J, m0, phi, _, _ = _fd_check(True)
     nx, nz, cpml, nt = 48, 40, 10, 300
