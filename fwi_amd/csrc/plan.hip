@@ -972,9 +972,22 @@ struct fwi_plan {
       order_f = {opt.item_order};
       order_a = {b.grouped && opt.item_order == 1 ? 0 : opt.item_order};
     }
-    hipEvent_t e0, e1;
-    CK(hipEventCreate(&e0));
-    CK(hipEventCreate(&e1));
+    // the tuner times launches with events and waits for them: not possible while the caller captures `st`
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    CK(hipStreamIsCapturing(st, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+      return fail("the first call on a plan tunes its work items with timed launches and cannot run while the caller "
+                  "captures its stream: make one call outside the capture first");
+    struct Events {  // destroyed on every return path
+      hipEvent_t e[2] = {nullptr, nullptr};
+      ~Events() {
+        for (hipEvent_t x : e)
+          if (x) (void)hipEventDestroy(x);
+      }
+    } ev;
+    CK(hipEventCreate(&ev.e[0]));
+    CK(hipEventCreate(&ev.e[1]));
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     auto timed = [&](auto&& body, float& ms) -> int {
       CK(hipEventRecord(e0, st));
       if (int r = body()) return r;
@@ -1066,8 +1079,6 @@ struct fwi_plan {
         }
       }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     for (auto& bb : batches) {
       if (int r = build_items(bb, tz, order)) return r;
       if (int r = build_adj_items(bb, tz_adj, order_adj)) return r;
@@ -1345,7 +1356,10 @@ void fwi_plan_destroy(fwi_plan* p) { delete p; }
 int fwi_plan_set_wavelet(fwi_plan* p, const float* w) {
   if (check_host("fwi_plan_set_wavelet", {{p, "plan"}, {w, "wavelet"}})) return 1;
   p->wavelet.assign(w, w + p->nt);
-  hipError_t e = hipMemcpy(p->d_src, w, p->nt * sizeof(float), hipMemcpyHostToDevice);
+  // launches and graph replays of this plan may still run on the caller's (possibly non-blocking) streams and
+  // read the table: wait for the device before overwriting it
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(p->d_src, w, p->nt * sizeof(float), hipMemcpyHostToDevice);
   return e == hipSuccess ? 0 : p->fail(e, "set_wavelet");
 }
 
@@ -1353,7 +1367,8 @@ int fwi_plan_set_cpml(fwi_plan* p, const float* a, const float* b) {
   if (check_host("fwi_plan_set_cpml", {{p, "plan"}, {a, "cpml_a"}, {b, "cpml_b"}})) return 1;
   p->ta.assign(a, a + p->cpml);
   p->tb.assign(b, b + p->cpml);
-  hipError_t e = hipMemcpy(p->d_ta, a, p->cpml * sizeof(float), hipMemcpyHostToDevice);
+  hipError_t e = hipDeviceSynchronize();  // as in fwi_plan_set_wavelet
+  if (e == hipSuccess) e = hipMemcpy(p->d_ta, a, p->cpml * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_tb, b, p->cpml * sizeof(float), hipMemcpyHostToDevice);
   return e == hipSuccess ? 0 : p->fail(e, "set_cpml");
 }

@@ -426,6 +426,20 @@ def test_plan_retargets_to_another_stage_frequency(torch):
         torch.cuda.synchronize()
         assert not torch.equal(g10, g_fresh)
         assert torch.equal(g_moved, g_fresh)
+        # the same on a non-blocking side stream, with the retarget issued while a forward is still in flight
+        # there: the tables are replaced only after it has finished
+        moved.set_wavelet(src10)
+        moved.set_cpml(tab10)
+        side = torch.cuda.Stream()
+        with torch.cuda.stream(side):
+            moved.forward(model)
+            g10s = moved.forward(model).clone()
+            moved.set_wavelet(src15)
+            moved.set_cpml(tab15)
+            g15s = moved.forward(model).clone()
+        side.synchronize()
+        assert torch.equal(g10s, g10)
+        assert torch.equal(g15s, g_fresh)
     finally:
         moved.close()
         fresh.close()
