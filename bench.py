@@ -9,9 +9,13 @@ Beside it, on the same workload:
   * plain per-shot GPU path: the reference's per-shot flow on the legacy C ABI (launchPropagation with a
     pinned-host snapshot every step, launchBackPropagation), timed on one shot for a bounded number of
     steps and scaled to the full survey (it is linear in shots and steps);
-  * roofline of the dominant kernel (the adjoint step): algorithmic bytes per launch over its average
-    duration, measured with HIP events on the stream the kernel runs on during the timed region;
-  * cpu_baseline: the CPU oracle (f32i, exact gradient, OpenMP) on one shot for a bounded number of steps.
+  * roofline of every step kernel and of the dominant one (the largest share of the gradient's kernel time):
+    algorithmic bytes per launch (the survey's count, and this design's minimum) over its average duration,
+    measured with HIP events on the stream the kernel runs on during an instrumented gradient, beside the
+    counter-measured HBM bytes of the same kernel when profiles/ holds them for these kernel sources;
+  * the same gradient in the reference's arithmetic (FP64 stencil sums), timed apart;
+  * cpu_baseline: the CPU oracle (f32i, exact gradient, OpenMP) on one shot for a bounded number of steps, and
+    the reference's arithmetic and gradient on one thread, with the CPU model.
   * the device optimizer of one iteration (steepest-descent step, L-BFGS direction, trial), timed apart.
 """
 from __future__ import annotations
@@ -53,8 +57,11 @@ def parse():
                     help="0 FP32 two-level, 1 mixed (reference), 2 FP32 difference stencil, 3 FP32 increment form")
     ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")
     ap.add_argument("--order", type=int, default=-1, help="work-item order (-1 = tuned)")
-    ap.add_argument("--plain-steps", type=int, default=400)
+    ap.add_argument("--plain-steps", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=4000)
+    ap.add_argument("--cpu1-steps", type=int, default=200, help="steps of the single-thread reference-arithmetic leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the reference-arithmetic (mixed) GPU gradient")
+    ap.add_argument("--surface-rows", type=int, default=0, help="FP64 near-surface band rows (0 default, -1 none)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-stride", type=int, default=4)
@@ -133,24 +140,46 @@ def plain_path_seconds(w, model_sq_dev, src_dev, ab, nsteps):
     return sec
 
 
-def cpu_baseline(w, nsteps):
-    """Oracle (f32i arithmetic, exact gradient, OpenMP over all host threads) on one shot of the workload
-    for nsteps steps: forward + adjoint + imaging.  Returns (Gcell-updates/s, threads, seconds)."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, nsteps, mode="f32i", exact=True, threads=None):
+    """The oracle on one shot of the workload for nsteps steps (forward + adjoint + imaging): by default the
+    f32i arithmetic's exact gradient on all OpenMP threads; mode "mix" with exact=False is the reference's
+    arithmetic and gradient.  threads: OpenMP threads for this call (None: the runtime's default).  Returns
+    (Gcell-updates/s, threads used, seconds)."""
+    import ctypes
+
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    shot = w.shots[len(w.shots) // 2]
-    tab = O.cpml_table(w.cpml_params)
-    src = O.ricker(nsteps, w.f, w.dt)
-    sq = O.pre_variables(*[np.ascontiguousarray(x[:, shot.x0:shot.x0 + shot.nx]) for x in w.init_model])
-    obs = np.zeros((nsteps, shot.nrec), np.float32)
-    t0 = time.perf_counter()
-    O.gradient("f32i", True, sq, tab, src, obs, shot.nx, w.nz, nsteps, w.cpml, w.dh, w.dt, shot)
-    sec = time.perf_counter() - t0
-    return 2.0 * shot.nx * w.nz * nsteps / sec / 1e9, threads, sec
+    omp = ctypes.CDLL("libgomp.so.1")
+    before = omp.omp_get_max_threads()
+    if threads:
+        omp.omp_set_num_threads(threads)
+    used = omp.omp_get_max_threads()
+    try:
+        shot = w.shots[len(w.shots) // 2]
+        tab = O.cpml_table(w.cpml_params)
+        src = O.ricker(nsteps, w.f, w.dt)
+        sq = O.pre_variables(*[np.ascontiguousarray(x[:, shot.x0:shot.x0 + shot.nx]) for x in w.init_model])
+        obs = np.zeros((nsteps, shot.nrec), np.float32)
+        t0 = time.perf_counter()
+        O.gradient(mode, exact, sq, tab, src, obs, shot.nx, w.nz, nsteps, w.cpml, w.dh, w.dt, shot)
+        sec = time.perf_counter() - t0
+    finally:
+        omp.omp_set_num_threads(before)
+    return 2.0 * shot.nx * w.nz * nsteps / sec / 1e9, used, sec
 
 
 def optimizer_ms(w, model, grad):
@@ -181,6 +210,83 @@ def optimizer_ms(w, model, grad):
     finally:
         opt.close()
     return out
+
+
+# Step kernels of the batched gradient: (timing kind, kernel name).  Launches per gradient with nt steps of which
+# `kept` keep their imaging terms from the forward: the forward runs nt - kept plain steps and kept storing steps,
+# the recomputation nt - kept storing steps, the adjoint nt steps.
+STEP_KERNELS = [
+    (0, "k_fwd_main<0>", "forward step"),
+    (1, "k_fwd_main<1>", "forward step storing the imaging terms (forward's last segments and recomputation)"),
+    (2, "k_adj_ex_main", "exact adjoint step with imaging"),
+    (7, "k_fwd_band", "FP64 near-surface band of every forward-type step"),
+    (5, "k_fwd_strip", "forward C-PML memory variables psi"),
+    (6, "k_adj_ex_strip1+2", "adjoint C-PML memory variables (two kernels)"),
+]
+
+
+def kernel_bytes(kind, N, Nb, Nc, Nm, grouped, inc):
+    """Bytes per launch of a step kernel: (the survey's algorithmic count, this design's minimum).
+
+    Survey count per step (FP32): forward 24 B per cell + 32 per C-PML cell + 12 per model cell (read once per
+    batch); adjoint + imaging 56 B per cell + 32 per C-PML cell + 12 per model cell.  It is split over the kernels
+    that move it: the forward's C-PML bytes half to the strip kernel (psi), half to the main kernel (zeta); the
+    adjoint's to its strip kernels; the cells of the FP64 band (Nb) to the band kernel.
+    Design minimum: the increment form reads P, Q, F, R and writes them (32 B per cell; + 8 B when storing the
+    imaging terms); the band moves two FP64 levels of P and F (48 B per cell) + its float rounding (8 B) + the model
+    (12 B, per shot); the adjoint reads mu and mu^{k+2} (or D) and the imaging terms (24 B), writes mu (and D) and
+    read-modify-writes the FP32 gradient slot (24 B per cell, 6 with gradient groups of four)."""
+    Nf = N - Nb  # cells of the float rows
+    if kind in (0, 1):
+        survey = 24.0 * Nf + 16.0 * Nc + 12.0 * Nm
+        design = (32.0 + (8.0 if kind == 1 else 0.0)) * Nf + 16.0 * Nc + 12.0 * Nm
+    elif kind == 7:
+        survey = 24.0 * Nb
+        design = (48.0 + 8.0 + 12.0) * Nb
+    elif kind == 5:
+        survey = design = 16.0 * Nc
+    elif kind == 6:
+        survey = design = 32.0 * Nc
+    else:
+        survey = 56.0 * N + 12.0 * Nm
+        design = (24.0 + (16.0 if inc else 8.0) + (6.0 if grouped else 24.0)) * N + 12.0 * Nm
+    return survey, design
+
+
+def launches_per_gradient(kind, nt, kept, band):
+    return {0: nt - kept, 1: nt, 2: nt, 5: 2 * nt - kept, 6: nt, 7: (2 * nt - kept) if band else 0}.get(kind, 0)
+
+
+def load_traffic(w, nshots):
+    """Counter-measured HBM bytes per launch of each kernel (profiles/fused_steps_survey1600_hbm_traffic.json), if it
+    was measured on this workload with the kernel sources in this tree (its source hash); else None and why."""
+    from fwi_amd.pmc import source_hash
+
+    tfile = os.path.join(ROOT, "profiles", "fused_steps_survey1600_hbm_traffic.json")
+    if not (w.name == "survey1600" and nshots == 64):
+        return None, "counters measured on survey1600 only"
+    if not os.path.exists(tfile):
+        return None, "no counter profile"
+    with open(tfile) as fh:
+        t = json.load(fh)
+    if t.get("source_hash") != source_hash():
+        return None, f"counter profile is of other kernel sources ({t.get('source_hash')} vs {source_hash()})"
+    return t["kernels"], "profiles/fused_steps_survey1600_hbm_traffic.json (source hash " + t["source_hash"] + ")"
+
+
+TRAFFIC_KEYS = {"k_fwd_main<0>": ("k_fwd_main<0,",), "k_fwd_main<1>": ("k_fwd_main<1,",),
+                "k_adj_ex_main": ("k_adj_ex_main<",), "k_fwd_band": ("k_fwd_band<1>",),
+                "k_fwd_strip": ("k_fwd_strip<",), "k_adj_ex_strip1+2": ("k_adj_ex_strip1", "k_adj_ex_strip2<")}
+
+
+def traffic_of(traffic, name):
+    """Counter bytes (read + written) per launch of a step kernel, summed over the kernels it stands for."""
+    if not traffic:
+        return None
+    parts = [v for k, v in traffic.items() for p in TRAFFIC_KEYS.get(name, ()) if k.startswith(p)]
+    if not parts:
+        return None
+    return float(sum(v["read_bytes"] + v["write_bytes"] for v in parts))
 
 
 def main():
@@ -216,9 +322,9 @@ def main():
     plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
                 rows_per_item=args.rows, item_order=args.order, arith=args.arith,
                 ckpt_format=args.ckpt_format, use_graph=0 if args.no_graph else 1, store_segments=args.keep,
-                grad_group=args.grad_group, step_pairs=1 if args.pairs else 0)
+                grad_group=args.grad_group, step_pairs=1 if args.pairs else 0, surface_rows=args.surface_rows)
     log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, {plan.kept_steps} steps kept from the forward, "
-        f"workspace {plan.workspace_bytes / 1e9:.1f} GB")
+        f"workspace {plan.workspace_bytes / 1e9:.1f} GB, FP64 band rows 4..{plan.surface_rows - 1}")
     t0 = time.perf_counter()
     obs = plan.forward(true_sq)
     torch.cuda.synchronize()
@@ -246,9 +352,11 @@ def main():
     assert np.isfinite(phi) and torch.isfinite(grad).all().item()
     graphs = plan.graph_stats()
     log(f"graph captures {graphs[0]}, replays {graphs[1]}")
-    # kernel durations: one more gradient with every timing_stride-th launch bracketed by HIP events (the
-    # instrumented call launches eagerly; its kernels are the ones the timed calls replayed)
+    # kernel durations: one more gradient with every timing_stride-th launch of each step kernel bracketed by HIP
+    # events on the stream it runs on (the instrumented call launches eagerly; its kernels are the ones the timed
+    # calls replayed)
     sec_instr = None
+    kernels = []
     if args.timing_stride > 0:
         plan.set_timing(args.timing_stride)
         t0 = time.perf_counter()
@@ -257,39 +365,59 @@ def main():
         sec_instr = time.perf_counter() - t0
         assert phi_i == phi
         log(f"instrumented eager gradient: {sec_instr:.3f} s")
-
-    # roofline of the adjoint step kernel (dominant): algorithmic bytes per launch / mean event time
-    adj_ms, adj_n = plan.kernel_time(2)
-    fwd_ms, fwd_n = plan.kernel_time(0)
-    rec_ms, rec_n = plan.kernel_time(1)
-    nmodel = w.nx * w.nz
-    adj_bytes = 56.0 * ncell + 12.0 * nmodel
-    adj_avg = adj_ms / max(adj_n, 1) / 1e3
-    achieved = adj_bytes / adj_avg / 1e9 if adj_n else None
-    fwd_bytes = 24.0 * ncell + 12.0 * nmodel
-    fwd_avg = fwd_ms / max(fwd_n, 1) / 1e3
-    if adj_n:
-        log(f"adjoint step: {adj_n} timed launches, mean {adj_avg * 1e6:.1f} us -> {achieved:.0f} GB/s algorithmic")
-    pair_ms, pair_n = plan.kernel_time(3)
-    pairs_ms, pairs_n = plan.kernel_time(4)
-    if pair_n or pairs_n:
-        log(f"fused core pair: {pair_n} timed launches, mean {pair_ms / max(pair_n, 1) * 1e3:.1f} us; storing pair "
-            f"{pairs_n} launches, mean {pairs_ms / max(pairs_n, 1) * 1e3:.1f} us (rows per item {plan.pair_rows})")
-    if fwd_n:
-        log(f"forward step: {fwd_n} timed launches, mean {fwd_avg * 1e6:.1f} us -> "
-            f"{fwd_bytes / max(fwd_avg, 1e-12) / 1e9:.0f} GB/s; storing step mean "
-            f"{rec_ms / max(rec_n, 1) * 1e3:.1f} us")
-
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "fused_steps_survey1600_hbm_traffic.json")
-    kname = "k_adj_ex_main" if grad_mode == 1 else "k_adj_ref_main"
-    if os.path.exists(tfile) and w.name == "survey1600" and len(w.shots) == 64:
-        with open(tfile) as fh:
-            traffic = json.load(fh)["bytes_per_launch"].get(kname)
+        nmodel = w.nx * w.nz
+        band = plan.surface_rows
+        nband = sum((band - 4) * s.nx for s in w.shots) if band else 0
+        grouped = bool(plan.adj_tiling[2])
+        inc = args.arith == 3
+        traffic, tnote = load_traffic(w, len(w.shots))
+        for kind, name, what in STEP_KERNELS:
+            ms, n = plan.kernel_time(kind)
+            if not n:
+                continue
+            mean = ms / n / 1e3
+            survey, design = kernel_bytes(kind, ncell, nband, nstrip, nmodel, grouped, inc)
+            per_grad = launches_per_gradient(kind, nt, plan.kept_steps, band)
+            tb = traffic_of(traffic, name)
+            k = {"kernel": name, "does": what, "launches_per_gradient": per_grad, "mean_launch_us": round(mean * 1e6, 1),
+                 "s_per_gradient": round(per_grad * mean, 4),
+                 "bytes_survey": survey, "frac_survey": round(survey / mean / 1e9 / HBM_PEAK_GBS, 4),
+                 "bytes_design": design, "frac_design": round(design / mean / 1e9 / HBM_PEAK_GBS, 4),
+                 "traffic": tb, "traffic_frac": round(tb / mean / 1e9 / HBM_PEAK_GBS, 4) if tb else None}
+            kernels.append(k)
+            log(f"{name}: {n} timed launches, mean {mean * 1e6:.1f} us, {per_grad} per gradient "
+                f"({k['s_per_gradient']:.3f} s); frac of 8 TB/s: survey bytes {k['frac_survey']:.3f}, design bytes "
+                f"{k['frac_design']:.3f}, counters {k['traffic_frac']}")
+        ksum = sum(k["s_per_gradient"] for k in kernels)
+        for k in kernels:
+            k["share"] = round(k["s_per_gradient"] / ksum, 4) if ksum else None
+        log(f"step kernels {ksum:.3f} s of the {sec_instr:.3f} s instrumented gradient; traffic: {tnote}")
+    dom = max(kernels, key=lambda k: k["s_per_gradient"]) if kernels else None
 
     opt_ms = optimizer_ms(w, init_sq, grad)
     log(f"optimizer: SD step {opt_ms['sd_step_ms']} ms, L-BFGS direction {opt_ms['lbfgs_direction_ms']} ms, "
         f"trial {opt_ms['trial_ms']} ms")
+    surface_rows = plan.surface_rows
+    kept, K, wsb, bsz = plan.kept_steps, plan.ckpt_interval, plan.workspace_bytes, plan.batch_size
+    tiling = (plan.rows_per_item, plan.item_order, plan.adj_tiling)
+    plan.close()
+    del plan, grad
+    torch.cuda.empty_cache()
+
+    # the reference's arithmetic (double-literal stencil sums) on the same batched path, timed apart
+    sec_mixed = None
+    if not args.no_mixed:
+        pm = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, arith=1)
+        try:
+            pm.gradient(init_sq, obs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pm.gradient(init_sq, obs)
+            torch.cuda.synchronize()
+            sec_mixed = time.perf_counter() - t0
+        finally:
+            pm.close()
+        log(f"reference arithmetic (mixed: FP64 stencil sums), same path: {sec_mixed:.3f} s per gradient")
 
     plain = None
     if not args.no_plain:
@@ -304,9 +432,17 @@ def main():
         ns = min(args.cpu_steps, nt)
         cg, threads, csec = cpu_baseline(w, ns)
         cpu = {"value": round(cg, 4), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
-               "sample": f"oracle f32i exact gradient, 1 shot of {w.name} ({w.nx}x{w.nz}), {ns} of {nt} steps, "
-                         f"fwd+adj, {csec:.1f} s"}
+               "sample": f"oracle f32i exact gradient on {threads} OpenMP threads, 1 shot of {w.name} "
+                         f"({w.nx}x{w.nz}), {ns} of {nt} steps, fwd+adj, {csec:.1f} s",
+               "cpu_model": cpu_model()}
         log(f"cpu baseline: {cg:.3f} Gcell/s on {threads} threads ({csec:.1f} s)")
+        ns1 = min(args.cpu1_steps, nt)
+        c1, _, c1sec = cpu_baseline(w, ns1, mode="mix", exact=False, threads=1)
+        cpu["single_thread_reference_arith"] = {
+            "value": round(c1, 4), "unit": "Gcell-updates/s", "cores": 1,
+            "sample": f"oracle in the reference's arithmetic (FP64 stencil sums, FP32 storage) and gradient "
+                      f"formula, 1 thread, 1 shot, {ns1} of {nt} steps, fwd+adj, {c1sec:.1f} s"}
+        log(f"cpu baseline, reference arithmetic, 1 thread: {c1:.4f} Gcell/s ({c1sec:.1f} s)")
 
     nrecs = [sh.nrec for sh in w.shots]
     widths = [sh.nx for sh in w.shots]
@@ -330,34 +466,39 @@ def main():
                         f"nt={nt}, dt={w.dt} s, {w.f} Hz Ricker, cpml={w.cpml}",
             "grad_mode": args.mode,
             "arith": {0: "fp32", 1: "mixed", 2: "fp32-diff", 3: "fp32-increment"}[args.arith],
-            "ckpt_interval": plan.ckpt_interval,
-            "kept_steps": plan.kept_steps,
-            "step_pair_rows": plan.pair_rows,
+            "fp64_surface_rows": surface_rows,
+            "ckpt_interval": K,
+            "kept_steps": kept,
+            "step_pairs": bool(args.pairs),
             "ckpt_format": {0: "fp32", 1: "bf16-scaled", 2: "fp16-scaled"}[args.ckpt_format],
-            "workspace_gb": round(plan.workspace_bytes / 1e9, 2),
-            "batch_shots": plan.batch_size,
-            "rows_per_item": plan.rows_per_item,
-            "item_order": plan.item_order,
-            "adjoint_tiling": dict(zip(("rows_per_item", "item_order", "grouped"), plan.adj_tiling)),
+            "workspace_gb": round(wsb / 1e9, 2),
+            "batch_shots": bsz,
+            "rows_per_item": tiling[0],
+            "item_order": tiling[1],
+            "adjoint_tiling": dict(zip(("rows_per_item", "item_order", "grouped"), tiling[2])),
             "s_per_gradient_batched": round(sec, 4),
             "s_per_gradient_median": round(float(np.median(times)), 4),
+            "s_per_gradient_reference_arith": round(sec_mixed, 4) if sec_mixed else None,
             "optimizer_ms": opt_ms,
             "graphs": {"captures": graphs[0], "replays": graphs[1]},
             "s_per_gradient_eager_instrumented": round(sec_instr, 4) if sec_instr else None,
             "s_per_gradient_plain": round(plain, 3) if plain else None,
+            "plain_steps_timed": min(args.plain_steps, nt) if plain else None,
             "speedup_vs_plain": round(plain / sec, 2) if plain else None,
             "misfit": phi,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": kname,
-            "achieved": round(achieved, 1) if achieved else None,
+            "kernel": dom["kernel"] if dom else None,
+            "achieved": round(dom["bytes_survey"] / dom["mean_launch_us"] / 1e3, 1) if dom else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": traffic,
-            "bytes_per_launch": adj_bytes,
-            "mean_launch_us": round(adj_avg * 1e6, 2) if adj_n else None,
+            "frac": dom["frac_survey"] if dom else None,
+            "traffic": dom["traffic"] if dom else None,
+            "bytes_per_launch": dom["bytes_survey"] if dom else None,
+            "mean_launch_us": dom["mean_launch_us"] if dom else None,
+            "share_of_step_kernels": dom["share"] if dom else None,
+            "kernels": kernels,
         },
         "cpu_baseline": cpu,
     }

@@ -29,11 +29,15 @@ class FwiSurvey(ctypes.Structure):
                 ("cpml_a", ctypes.POINTER(c_float)), ("cpml_b", ctypes.POINTER(c_float))]
 
 
+class FwiLineSearch(ctypes.Structure):
+    _fields_ = [("alpha", c_double * 3), ("count", c_int), ("max_failures", c_int)]
+
+
 class FwiOptions(ctypes.Structure):
     _fields_ = [("grad_mode", c_int), ("arith", c_int), ("ckpt_interval", c_int), ("max_batch", c_int),
                 ("mem_budget", c_double), ("rows_per_item", c_int), ("item_order", c_int),
                 ("grad_group", c_int), ("ckpt_format", c_int), ("store_segments", c_int),
-                ("use_graph", c_int), ("step_pairs", c_int)]
+                ("use_graph", c_int), ("step_pairs", c_int), ("surface_rows", c_int)]
 
 
 # Every exported symbol with its signature (argument types, return type).
@@ -96,7 +100,11 @@ BATCHED = {
     "fwi_plan_item_order": ([vp_], c_int),
     "fwi_plan_kept_steps": ([vp_], c_int),
     "fwi_plan_pair_rows": ([vp_], c_int),
+    "fwi_plan_surface_rows": ([vp_], c_int),
     "fwi_plan_graph_stats": ([vp_, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)], c_int),
+    "fwi_line_search_init": ([ctypes.POINTER(FwiLineSearch)], None),
+    "fwi_line_search_step": ([ctypes.POINTER(FwiLineSearch), c_double, ctypes.POINTER(c_double)], c_int),
+    "fwi_plan_graph_nodes": ([vp_, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)], c_int),
     "fwi_plan_adj_tiling": ([vp_, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)], c_int),
 }
 SYMBOLS = {**LEGACY, **BATCHED}

@@ -21,6 +21,7 @@ struct ShotDev {
   int gx0, gnx;     // the slot covers model columns [gx0, gx0 + gnx) (the union of the group's windows)
   int gps;          // row pitch of the slot (gnx rounded up to 32)
   int lead;         // 1 if the shot leads its gradient group (its slot is summed into the model gradient)
+  long long boff;   // offset of the shot's band rows (band x ps cells) in the FP64 near-surface band arrays
 };
 
 // C-PML cells of a window: the side strips Omega & (ix < cpml | ix > nx-cpml-1) for rows 4..nz-5, then
@@ -117,6 +118,32 @@ struct FwdPairArgs {
   int it;            // n
 };
 
+// FP64 near-surface band of the increment-form forward: rows [4, band) of every shot advance in FP64 (two levels
+// of P and F and of psi_x by step parity, zeta_x on the side C-PML cells); one workgroup per (256-column chunk,
+// band row, shot).
+struct BandArgs {
+  GridArgs g;
+  ModelArgs m;
+  int band;           // first row of the float state
+  double dt;          // the float time step, promoted
+  const float* src;
+  int it;
+  const double* P;    // P^n, F^n (band arrays of this step's parity)
+  double* Pm;         // P^{n-1}, F^{n-1}: receive P^{n+1}, F^{n+1}
+  const double* F;
+  double* Fm;
+  const double* psx;  // psi_x^n; psi_x^{n+1} goes to psxn (two arrays by step parity)
+  double* psxn;
+  double* zx;
+  const float* Fc;    // float F^n (rows band.. are the float state's)
+  const float* Pcf;   // float P^n (reference-compatible imaging terms)
+  float* Po;          // float P^{n+1}, F^{n+1}: the band's values rounded
+  float* Fo;
+  float* gather;
+  float* qa;
+  float* qb;
+};
+
 // Strip (C-PML) blocks: (shot, first strip cell) per 256-thread block.
 struct StripArgs {
   GridArgs g;
@@ -175,6 +202,9 @@ struct AdjExArgs {
   float* ptz;
   float* gbx;  // gbar_x, gbar_z of this step on the updated C-PML cells (zero on every other cell)
   float* gbz;
+  float* DP;   // increment form (rows above `band`): D = m^{k+1} - m^{k+2}, updated in place to m^k - m^{k+1}
+  float* DF;
+  int band;    // rows [4, band) take the increment form (items are cut at band); 0: none
   const float* res;
   int rit;
   const float* qa;  // forward a^n, b^n of the paired step, or null

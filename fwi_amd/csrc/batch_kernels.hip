@@ -54,9 +54,13 @@ __global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __r
 // and loads each row two rows ahead, before the current row's stores: the vector memory counter retires in
 // issue order, so a load issued after a store could only be waited for together with that store.  The
 // stored values are those of k_fwd_main, expression by expression (its C-PML terms are +0 here).
-template <int QM, class S, bool INC>
+// SHM: the four waves of the workgroup are four shots on the same model tile (same model columns and rows, the
+// same row count): per row, wave w < 3 loads the row of model field w (v_hor, v_po, v_nmo) and the waves share
+// the three rows through LDS (double-buffered by row parity, one workgroup barrier per row), so the model is read
+// once per four shots.
+template <int QM, class S, bool INC, bool SHM = false>
 __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, const ShotDev& sd, int lane,
-                                             float* buf) {
+                                             float* buf, float (*mshare)[4][64] = nullptr) {
   const int nx = sd.nx, ps = sd.ps, nxm = a.m.nxm;
   const int ix = itm.y + lane;
   const int hx = lane < 8 ? clampi(itm.y + halo_col(lane), 0, nx - 1) : ix;
@@ -72,6 +76,8 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
   const float* vh = a.m.vh + sd.x0;
   const float* vp = a.m.vp + sd.x0;
   const float* vn = a.m.vn + sd.x0;
+  const int wave = SHM ? wave_id() : 0;
+  const float* vmine = wave == 0 ? vh : wave == 1 ? vp : vn;  // SHM: the model field this wave loads (wave 3: a copy)
   const float srcv = a.src[a.it];
   const int z0 = itm.z, z1 = itm.w;
   float fq[9];  // F along z (rows z-4..z+4)
@@ -79,7 +85,7 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
     float f4;                // F at row z+4
     float c, h;              // P at row z: the lane's column, the halo column
     float pold, fold;        // two-level form: P^{n-1}, F^{n-1}; increment form: Q^n, R^n
-    float vhv, vpv, vnv;     // model at row z
+    float vhv, vpv, vnv;     // model at row z (SHM: vhv holds this wave's field)
   };
   auto load_row = [&](int z, Row& r) __attribute__((always_inline)) {
     const unsigned o = (unsigned)(ix + ps * z);
@@ -89,13 +95,27 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
     r.h = ldu(P, (unsigned)(hx + ps * z));
     r.pold = INC ? ldn(Qc, o) : ldn(Po, o);
     r.fold = INC ? ldn(Rc, o) : ldn(Fo, o);
-    r.vhv = ldu(vh, mo);
-    r.vpv = ldu(vp, mo);
-    r.vnv = ldu(vn, mo);
+    if (SHM) {
+      r.vhv = ldu(vmine, mo);
+    } else {
+      r.vhv = ldu(vh, mo);
+      r.vpv = ldu(vp, mo);
+      r.vnv = ldu(vn, mo);
+    }
   };
   auto step = [&](int z, const Row& cur, Row& nxt) __attribute__((always_inline)) {
     const unsigned o = (unsigned)(ix + ps * z);
     fq[8] = cur.f4;
+    float vhv = cur.vhv, vpv = cur.vpv, vnv = cur.vnv;
+    if (SHM) {
+      // a wave can only overwrite this buffer two rows later, after every wave has passed the next row's barrier
+      float(*ms)[64] = mshare[z & 1];
+      ms[wave][lane] = cur.vhv;
+      __syncthreads();
+      vhv = ms[0][lane];
+      vpv = ms[1][lane];
+      vnv = ms[2][lane];
+    }
     float u[9];
     xrow9(buf, lane, cur.c, cur.h, u);
     load_row(min(z + 2, z1 - 1), nxt);  // two rows ahead; unused repeats on the last rows keep one path
@@ -106,8 +126,8 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
     const float tf = dh2 * (r2 + atf);
     float pn, fn;
     if (INC) {
-      float dq = cur.vhv * C * tp + cur.vpv * C * tf;
-      float dr = cur.vnv * C * tp + cur.vpv * C * tf;
+      float dq = vhv * C * tp + vpv * C * tf;
+      float dr = vnv * C * tp + vpv * C * tf;
       if (ix == sd.isx && z == sd.isz) {
         dq += srcv;
         dr += srcv;
@@ -118,8 +138,8 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
       stn(Qo, o, qn);
       stn(Ro, o, rn);
     } else {
-      pn = 2.f * cur.c - cur.pold + cur.vhv * C * tp + cur.vpv * C * tf;
-      fn = 2.f * fq[4] - cur.fold + cur.vnv * C * tp + cur.vpv * C * tf;
+      pn = 2.f * cur.c - cur.pold + vhv * C * tp + vpv * C * tf;
+      fn = 2.f * fq[4] - cur.fold + vnv * C * tp + vpv * C * tf;
       if (ix == sd.isx && z == sd.isz) {
         pn += srcv;
         fn += srcv;
@@ -172,17 +192,28 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
 // arithmetic.  In FP32 the stored Q keeps its own relative precision, whereas the difference of two
 // stored levels loses it: with s samples per period the rounding of the two-level form is amplified
 // about (s/2pi)^2 times (about 600 at 8 Hz, dt = 0.8 ms), that of the increment form about s/2pi.
+// Work items: (shot, band x0, z0, z1); bit kShareModel of the shot field marks the four items of a workgroup as
+// four interior items on one model tile (the host sets it on all four or none), which share the model rows.
+constexpr int kShareModel = 1 << 30;
+
 template <int QM, class S, bool INC = false>
 __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
   __shared__ float xbuf[4][72];
+  __shared__ float mshare[2][4][64];
   const int wave = wave_id(), lane = threadIdx.x & 63;
   const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
   if (item >= a.nitems) return;
-  const int4 itm = a.items[item];
+  int4 itm = a.items[item];
+  const bool share = (itm.x & kShareModel) != 0;
+  itm.x &= ~kShareModel;
   const ShotDev sd = a.g.shots[itm.x];
   const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
   // Interior items (no C-PML cell; the items are cut at row nz-e, e = max(cpml,4)) take the pipelined body.
   // Items with C-PML cells take the general body below, interleaved with them in the launch.
+  if (share) {
+    fwd_interior<QM, S, INC, true>(a, itm, sd, lane, xbuf[wave], mshare);
+    return;
+  }
   const int e = cpml > kRim ? cpml : kRim;
   if (!a.xr && itm.y >= e && itm.y + 64 <= nx - e && itm.w <= nz - e) {
     fwd_interior<QM, S, INC>(a, itm, sd, lane, xbuf[wave]);
@@ -289,6 +320,95 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) fq[k] = fq[k + 1];
   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 near-surface band of the increment-form forward (this project's arithmetic): rows [4, band) of every
+// shot, where the source, the receivers and the largest near-source fields lie, advance in FP64 with the
+// two-level leapfrog (P^{n+1} = 2P^n - P^{n-1} + ...; in FP64 the time difference keeps its precision), from the
+// float model, C-PML table, wavelet and dt promoted exactly.  The band lies above the bottom C-PML, so only the
+// side C-PML cells carry terms (psi_x, zeta_x).  F couples band and float rows through the z stencil: rows
+// band..band+3 of the float F^n enter the band's D2z, and the band writes its rows of the float P, F rounded, which
+// the float rows' D2z reads.  Also written: the band's imaging terms (rounded once) and its receivers (P + F
+// rounded once).  The expressions are those of the oracle's f32i band, in the same order.
+// Grid (256-column chunks, band - 4, shots).  psi_x is kept by step parity (read psx, write psxn): a chunk first
+// computes psi_x^{n+1} of the side cells within 4 columns of it -- neighbouring chunks compute the cells they share
+// from the same inputs and store the same bits -- then, after a workgroup barrier, its own cells.  (A tiled variant
+// staging 64 x 20 cells of P, F and psi in LDS was slower: 46.9 against 41.2 us per step on survey1600.)
+template <int QM>
+__global__ void __launch_bounds__(256) k_fwd_band(BandArgs a) {
+  const ShotDev sd = a.g.shots[blockIdx.z];
+  const int nx = sd.nx, ps = sd.ps, cpml = a.g.cpml, nxm = a.m.nxm, B = a.band;
+  const int x0 = blockIdx.x * 256;
+  if (x0 >= nx - kRim) return;  // the whole workgroup: past this shot's window
+  const int z = kRim + blockIdx.y;
+  const double dh = (double)a.g.dh, dh2 = dh * dh, Cd = (a.dt * a.dt) / dh2;
+  const double* P = a.P + sd.boff;
+  double* Pm = a.Pm + sd.boff;
+  const double* Fb = a.F + sd.boff;
+  double* Fm = a.Fm + sd.boff;
+  const double* psx = a.psx + sd.boff;
+  double* psxn = a.psxn + sd.boff;
+  double* zx = a.zx + sd.boff;
+  const float* Fc = a.Fc + sd.off;
+  const int w = cpml - kRim;  // side C-PML cells per row and side
+  const long row = (long)ps * z;
+  // psi_x^{n+1} of the side C-PML cells in [x0-4, x0+260) (PSI_F)
+  for (int j = threadIdx.x; j < 2 * w; j += 256) {
+    const int ix = j < w ? kRim + j : nx - cpml + (j - w);
+    if (ix < x0 - 4 || ix >= x0 + 260) continue;
+    const long i = row + ix;
+    const double t = d1d(P[i - 4], P[i - 3], P[i - 2], P[i - 1], P[i + 1], P[i + 2], P[i + 3], P[i + 4], dh);
+    psxn[i] = (double)coef_x(a.g.tb, ix, nx, cpml) * psx[i] + (double)coef_x(a.g.ta, ix, nx, cpml) * t;
+  }
+  __syncthreads();
+  const int ix = x0 + threadIdx.x;
+  if (ix < kRim || ix >= nx - kRim) return;
+  const long i = row + ix;
+  const double p2 = d2d(P[i - 4], P[i - 3], P[i - 2], P[i - 1], P[i], P[i + 1], P[i + 2], P[i + 3], P[i + 4], dh2);
+  double f[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int r = z + k - 4;
+    const long j = (long)ps * r + ix;
+    f[k] = r < B ? Fb[j] : (double)Fc[j];
+  }
+  const double r2 = d2d(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], dh2);
+  double atp = 0.0;
+  const double atf = 0.0;
+  if (ix < cpml || ix > nx - cpml - 1) {
+    const double tx = d1d(psxn[i - 4], psxn[i - 3], psxn[i - 2], psxn[i - 1], psxn[i + 1], psxn[i + 2], psxn[i + 3],
+                          psxn[i + 4], dh);
+    const double tpx = p2 + tx;
+    const double zn = (double)coef_x(a.g.tb, ix, nx, cpml) * zx[i] + (double)coef_x(a.g.ta, ix, nx, cpml) * tpx;
+    zx[i] = zn;
+    atp = tx + zn;
+  }
+  const double tp = dh2 * (p2 + atp);
+  const double tf = dh2 * (r2 + atf);
+  const long mi = sd.x0 + ix + (long)nxm * z;
+  const double vh = (double)a.m.vh[mi], vp = (double)a.m.vp[mi], vn = (double)a.m.vn[mi];
+  double pn = 2.0 * P[i] - Pm[i] + vh * Cd * tp + vp * Cd * tf;
+  double fn = 2.0 * Fb[i] - Fm[i] + vn * Cd * tp + vp * Cd * tf;
+  if (ix == sd.isx && z == sd.isz) {
+    const double srcv = (double)a.src[a.it];
+    pn += srcv;
+    fn += srcv;
+  }
+  Pm[i] = pn;
+  Fm[i] = fn;
+  a.Po[sd.off + i] = (float)pn;
+  a.Fo[sd.off + i] = (float)fn;
+  if (QM == 1) {
+    a.qa[sd.off + i] = (float)(p2 + atp);
+    a.qb[sd.off + i] = (float)(r2 + atf);
+  } else if (QM == 2) {  // the reference-compatible terms come from the float state, as in the float rows
+    const float* Pf = a.Pcf + sd.off;
+    a.qa[sd.off + i] = d2_at<float>(Pf, i, 1, a.g.dh2);
+    a.qb[sd.off + i] = d2_at<float>(Fc, i, ps, a.g.dh2);
+  }
+  if (a.gather && z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec)
+    a.gather[sd.goff + (long)a.it * sd.nrec + (ix - sd.ir0)] = (float)(pn + fn);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -704,7 +824,12 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
 // that need C-PML terms take a general body per row, interleaved with them in the launch.  Every path of a
 // wave passes one barrier per row, so a workgroup may mix them.
 // IMG: the step adds imaging terms (a.qa, a.qb are set), a launch constant so that the body has one path.
-template <class S, bool GROUP, bool IMG>
+// INC: increment form of the adjoint leapfrog on the rows of the forward's FP64 band, [4, a.band) (the FP32
+// increment-form arithmetic): the state there is m^{k+1} and D = m^{k+1} - m^{k+2} (a.DP, a.DF, updated in place)
+// instead of m^{k+2}; D^k = D^{k+1} + (the step's terms) (+ the residual at the receivers), m^k = m^{k+1} + D^k --
+// the same recursion in exact arithmetic, with the time difference stored at its own precision.  Near the
+// receivers, where the residual enters, the two-level form's rounding is what limits the gradient; below, it is not.
+template <class S, bool GROUP, bool IMG, bool INC = false>
 __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   __shared__ float xbuf[4][72];
   __shared__ float gsum[GROUP ? 2 : 1][4][3][64];
@@ -757,6 +882,8 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const float* M = a.Mc + sd.off;
   float* Lo = a.Lo + sd.off;
   float* Mo = a.Mo + sd.off;
+  float* DP = a.DP + sd.off;  // increment-form rows: D
+  float* DF = a.DF + sd.off;
   const float* vh = a.m.vh + sd.x0;
   const float* vp = a.m.vp + sd.x0;
   const float* vn = a.m.vn + sd.x0;
@@ -791,11 +918,11 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     float lc, mc, vhc, vnc;                // row z at the lane's column: mP, mF (an L2 hit), v_hor, v_nmo
     float vhh, vnh, lh, mh;                // row z at the halo column: v_hor, v_nmo, mP, mF
     float xc, xh, pc, ph;                  // zt_x and a_x psi-bar_tot at the lane's and the halo column (XN)
-    float lold, mold, qa, qb, g0, g1, g2;  // centre-only values: m^{k+2}, imaging terms, gradient
+    float lold, mold, qa, qb, g0, g1, g2;  // centre-only values: m^{k+2} (INC: D), imaging terms, gradient
   };
-  // XN: band near a side strip; ZB: rows z >= zc (bottom C-PML terms, zt_z of row z+4)
-  auto load_row = [&](auto xn_t, auto zb_t, int z, Row& r) __attribute__((always_inline)) {
-    constexpr bool XN = decltype(xn_t)::value, ZB = decltype(zb_t)::value;
+  // XN: band near a side strip; ZB: rows z >= zc (bottom C-PML terms, zt_z of row z+4); IN: increment form
+  auto load_row = [&](auto xn_t, auto zb_t, auto in_t, int z, Row& r) __attribute__((always_inline)) {
+    constexpr bool XN = decltype(xn_t)::value, ZB = decltype(zb_t)::value, IN = decltype(in_t)::value;
     const unsigned o = (unsigned)(cx + ps * z);
     const unsigned mo = (unsigned)(cx + nxm * z);
     const unsigned ho = (unsigned)(hx + ps * z), hmo = (unsigned)(hx + nxm * z);
@@ -818,8 +945,8 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       r.pc = ldu(ptx, o);
       r.ph = ldu(ptx, ho);
     }
-    r.lold = ldn(Lo, o);
-    r.mold = ldn(Mo, o);
+    r.lold = ldn(IN ? DP : Lo, o);
+    r.mold = ldn(IN ? DF : Mo, o);
     if (img) {
       r.qa = ldn(qa, o);
       r.qb = ldn(qb, o);
@@ -835,9 +962,10 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   // Row z from cur.  PIPE (interior items): row min(z+2, ze-1) is loaded into nxt once cur's stencil inputs
   // are consumed and before the row's stores (unused repeats on the last rows keep one path through the
   // body).  The edge kernel loads each row at its start: its C-PML inputs leave no registers for a second row.
-  auto step = [&](auto xn_t, auto zb_t, auto pipe_t, int z, int ze, const Row& cur, Row& nxt)
+  auto step = [&](auto xn_t, auto zb_t, auto pipe_t, auto in_t, int z, int ze, const Row& cur, Row& nxt)
                   __attribute__((always_inline)) {
     constexpr bool XN = decltype(xn_t)::value, ZB = decltype(zb_t)::value, PIPE = decltype(pipe_t)::value;
+    constexpr bool IN = decltype(in_t)::value;
     const unsigned o = (unsigned)(cx + ps * z);
     {
       const int r = z + 4;
@@ -866,7 +994,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       pz[6] = ldu(ptz, o + 3 * snx);
       pz[7] = ldu(ptz, o + 4 * snx);
     }
-    if (PIPE) load_row(xn_t, zb_t, min(z + 2, ze - 1), nxt);
+    if (PIPE) load_row(xn_t, zb_t, in_t, min(z + 2, ze - 1), nxt);
     float gx[9];
     xrow9(buf, lane, go, gh, gx);
     float ap = d2s<S>(gx[0], gx[1], gx[2], gx[3], gx[4], gx[5], gx[6], gx[7], gx[8], dh2);
@@ -877,8 +1005,9 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       ap = ap - d1s<S>(px[0], px[1], px[2], px[3], px[5], px[6], px[7], px[8], dh);
     }
     if (ZB) af = af - d1s<S>(pz[0], pz[1], pz[2], pz[3], pz[4], pz[5], pz[6], pz[7], dh);
-    float ln = 2.f * cur.lc + ap - cur.lold;
-    float mn = 2.f * cur.mc + af - cur.mold;
+    // IN: ln, mn first hold D^k, then m^k
+    float ln = IN ? cur.lold + ap : 2.f * cur.lc + ap - cur.lold;
+    float mn = IN ? cur.mold + af : 2.f * cur.mc + af - cur.mold;
     if (z == sd.irz && sd.nrec > 0) {
       const bool rec = ix >= sd.ir0 && ix < sd.ir0 + sd.nrec;
       const float r = a.res[sd.goff + (long)a.rit * sd.nrec + (rec ? ix - sd.ir0 : 0)];
@@ -888,9 +1017,18 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       }
     }
     // Every lane stores: a lane off the updated cells (the rim, or a column past the shot clamped onto the
-    // rim) writes back the value it read, which no other wave changes.
-    stn(Lo, o, act ? ln : cur.lold);
-    stn(Mo, o, act ? mn : cur.mold);
+    // rim) writes back the value it read, which no other wave changes (IN: m^{k+1} there, zero as m^{k+2} is).
+    if (IN) {
+      stn(DP, o, act ? ln : cur.lold);
+      stn(DF, o, act ? mn : cur.mold);
+      ln = cur.lc + ln;
+      mn = cur.mc + mn;
+      stn(Lo, o, act ? ln : cur.lc);
+      stn(Mo, o, act ? mn : cur.mc);
+    } else {
+      stn(Lo, o, act ? ln : cur.lold);
+      stn(Mo, o, act ? mn : cur.mold);
+    }
     float ch = 0.f, cn = 0.f, cp = 0.f;  // this shot's imaging contributions (GROUP)
     if (img) {
       if (GROUP) {
@@ -917,44 +1055,48 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       bst<kBufNT>(grs, gcol + grow * z, cur.g0 + sum);
     }
   };
-  auto rows = [&](auto xn_t, auto zb_t, auto pipe_t, int za, int ze) __attribute__((always_inline)) {
+  auto rows = [&](auto xn_t, auto zb_t, auto pipe_t, auto in_t, int za, int ze) __attribute__((always_inline)) {
     if (za >= ze) return;
     Row ra, rb;
     if (decltype(pipe_t)::value) {
       // two rows in flight, three buffers in rotation unrolled by three (no register copy waits for a load),
       // the first three rows peeled so the loop head sees the same memory-counter state from both paths
       Row rc;
-      load_row(xn_t, zb_t, za, ra);
-      load_row(xn_t, zb_t, min(za + 1, ze - 1), rb);
+      load_row(xn_t, zb_t, in_t, za, ra);
+      load_row(xn_t, zb_t, in_t, min(za + 1, ze - 1), rb);
       int z = za;
       if (z + 2 < ze) {
-        step(xn_t, zb_t, pipe_t, z, ze, ra, rc);
-        step(xn_t, zb_t, pipe_t, z + 1, ze, rb, ra);
-        step(xn_t, zb_t, pipe_t, z + 2, ze, rc, rb);
+        step(xn_t, zb_t, pipe_t, in_t, z, ze, ra, rc);
+        step(xn_t, zb_t, pipe_t, in_t, z + 1, ze, rb, ra);
+        step(xn_t, zb_t, pipe_t, in_t, z + 2, ze, rc, rb);
         for (z += 3; z + 2 < ze; z += 3) {
-          step(xn_t, zb_t, pipe_t, z, ze, ra, rc);
-          step(xn_t, zb_t, pipe_t, z + 1, ze, rb, ra);
-          step(xn_t, zb_t, pipe_t, z + 2, ze, rc, rb);
+          step(xn_t, zb_t, pipe_t, in_t, z, ze, ra, rc);
+          step(xn_t, zb_t, pipe_t, in_t, z + 1, ze, rb, ra);
+          step(xn_t, zb_t, pipe_t, in_t, z + 2, ze, rc, rb);
         }
       }
-      if (z < ze) step(xn_t, zb_t, pipe_t, z, ze, ra, rc);
-      if (z + 1 < ze) step(xn_t, zb_t, pipe_t, z + 1, ze, rb, ra);
+      if (z < ze) step(xn_t, zb_t, pipe_t, in_t, z, ze, ra, rc);
+      if (z + 1 < ze) step(xn_t, zb_t, pipe_t, in_t, z + 1, ze, rb, ra);
     } else {
       for (int z = za; z < ze; ++z) {
-        load_row(xn_t, zb_t, z, ra);
-        step(xn_t, zb_t, pipe_t, z, ze, ra, rb);
+        load_row(xn_t, zb_t, in_t, z, ra);
+        step(xn_t, zb_t, pipe_t, in_t, z, ze, ra, rb);
       }
     }
   };
-  // an item lies entirely above row zc or entirely at or below it (the items are split there)
+  // an item lies entirely above row zc or entirely at or below it, and entirely above a.band or at or below it
+  // (the items are split at both rows; the band ends above zc)
   using T = std::true_type;
   using F = std::false_type;
+  const bool inc = INC && z1 <= a.band;
   if (xnear) {
-    if (z0 >= zc) rows(T{}, T{}, F{}, z0, z1);
-    else rows(T{}, F{}, F{}, z0, z1);
+    if (z0 >= zc) rows(T{}, T{}, F{}, F{}, z0, z1);
+    else if (inc) rows(T{}, F{}, F{}, std::integral_constant<bool, INC>{}, z0, z1);
+    else rows(T{}, F{}, F{}, F{}, z0, z1);
   } else {
-    if (z0 >= zc) rows(F{}, T{}, F{}, z0, z1);
-    else rows(F{}, F{}, T{}, z0, z1);
+    if (z0 >= zc) rows(F{}, T{}, F{}, F{}, z0, z1);
+    else if (inc) rows(F{}, F{}, T{}, std::integral_constant<bool, INC>{}, z0, z1);
+    else rows(F{}, F{}, T{}, F{}, z0, z1);
   }
 }
 

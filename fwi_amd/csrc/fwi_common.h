@@ -59,6 +59,20 @@ __device__ __forceinline__ float d1s(float m4, float m3, float m2, float m1, flo
   return (float)(r / (S)h);
 }
 
+// The same stencils on FP64 samples (the near-surface band): pair sums and the weighted sum in double.
+__device__ __forceinline__ double d2d(double m4, double m3, double m2, double m1, double c, double p1, double p2,
+                                      double p3, double p4, double h2) {
+  const double c0 = 205.0 / 72.0, c1 = 8.0 / 5.0, c2 = 1.0 / 5.0, c3 = 8.0 / 315.0, c4 = 1.0 / 560.0;
+  const double r = -c0 * c + c1 * (p1 + m1) - c2 * (p2 + m2) + c3 * (p3 + m3) - c4 * (p4 + m4);
+  return r / h2;
+}
+__device__ __forceinline__ double d1d(double m4, double m3, double m2, double m1, double p1, double p2, double p3,
+                                      double p4, double h) {
+  const double c1 = 4.0 / 5.0, c2 = 1.0 / 5.0, c3 = 4.0 / 105.0, c4 = 1.0 / 280.0;
+  const double r = c1 * (p1 - m1) - c2 * (p2 - m2) + c3 * (p3 - m3) - c4 * (p4 - m4);
+  return r / h;
+}
+
 // Strided loads of the 9-point stencil around index i.
 template <class S>
 __device__ __forceinline__ float d2_at(const float* __restrict__ u, long i, long st, float h2) {

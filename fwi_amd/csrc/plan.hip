@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <array>
@@ -92,6 +93,16 @@ struct fwi_plan {
   int tz_adj = 32;   // rows per work item and order of the adjoint kernels
   int order_adj = 0;
   int tz_pair = 32;  // rows per fused core item (step pairs)
+  // FP64 near-surface band of the increment form: rows [4, band) of every shot (0: none).  State: seven FP64 arrays
+  // of cap_band cells (band rows x row pitch of every shot): P, F and psi_x by step parity, zeta_x.
+  int band = 0;
+  long long cap_band = 0;
+  double* d_band = nullptr;
+  double* bP(int n) const { return d_band + (size_t)(n & 1) * cap_band; }
+  double* bF(int n) const { return d_band + (size_t)(2 + (n & 1)) * cap_band; }
+  double* bpsx(int n) const { return d_band + (size_t)(4 + (n & 1)) * cap_band; }
+  double* bzx() const { return d_band + 6 * (size_t)cap_band; }
+  static constexpr int kBandF = 7;
   bool tuned = false;
   std::vector<Batch> batches;
   long long gather_total = 0;
@@ -109,9 +120,10 @@ struct fwi_plan {
   int max_count = 0;                // shots in the largest batch
   unsigned* d_ck_amax = nullptr;    // 16-bit checkpoints: max-abs bits per (checkpoint, shot, field)
   // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute),
-  // 2 = adjoint step, 3 / 4 = fused core step pair without / with store; every `timing_stride`-th launch of a
-  // kind is bracketed by events on its stream
-  static constexpr int kKinds = 5;
+  // 2 = adjoint step, 3 / 4 = fused core step pair without / with store, 5 = forward C-PML strip, 6 = the two
+  // adjoint C-PML strip kernels, 7 = FP64 near-surface band; every `timing_stride`-th launch of a kind is
+  // bracketed by events on its stream
+  static constexpr int kKinds = 8;
   int timing_stride = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_open;  // (kind, event index of the start; end = start + 1)
@@ -151,7 +163,7 @@ struct fwi_plan {
   // forward fields: the state of step n in set n mod 3 in the increment form (P 0..2, F 3..5, Q = P^n - P^{n-1}
   // 6..8, R = F^n - F^{n-1} 9..11; three sets so that fused step pairs can keep step n while writing n+1 and
   // n+2), in set n mod 2 of P 0..1 and F 3..4 in the two-level form; C-PML memory psx 12, psz 13, zx 14, zz 15
-  static constexpr int kFwdF = 16, kAdjF = 14, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
+  static constexpr int kFwdF = 16, kAdjF = 16, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
   int fset(int n) const { return inc() ? n % 3 : (n & 1); }
   float* fP(int n) const { return fwd(fset(n)); }
   float* fF(int n) const { return fwd(3 + fset(n)); }
@@ -162,11 +174,13 @@ struct fwi_plan {
   float* fzx() const { return fwd(14); }
   float* fzz() const { return fwd(15); }
   float* fwd(int k) const { return d_work + (size_t)k * cap_cell; }
-  float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..11
+  float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..15
   float* sgrad(int k) const { return d_work + (size_t)(kFwdF + kAdjF + k) * cap_cell; }      // 0..2
   float* qbuf() const { return d_work + (size_t)kFixedF * cap_cell; }                      // qslots * 2 * cap_cell
   // checkpoints j = 1..J-1: 4 cell fields then 4 strip fields, FP32 or 16-bit (ck_esize bytes per value)
-  size_t ck_bytes() const { return (4 * (size_t)cap_cell + 4 * (size_t)cap_strip) * ck_esize; }
+  size_t ck_bytes() const {
+    return (4 * (size_t)cap_cell + 4 * (size_t)cap_strip) * ck_esize + kBandF * (size_t)cap_band * sizeof(double);
+  }
   char* ckpt(int j) const {
     return (char*)(d_work + (size_t)(kFixedF + 2 * (size_t)qslots()) * cap_cell) + (size_t)(j - 1) * ck_bytes();
   }
@@ -217,18 +231,54 @@ struct fwi_plan {
   }
 
   // ---------------------------------------------------------------------------------------------
-  // Graphs: the device work of one batch of one entry point is captured once, on the plan's own stream,
-  // and its executable graph is cached under the entry point, the batch and the caller's buffers; later
-  // calls with the same buffers (an inversion updates its model in place) replay it on the caller's
-  // stream.  Not used while kernel timing is on.
+  // Graphs: the device work of one batch of one entry point is captured once, on the plan's own stream, and
+  // its executable graphs are cached under the entry point, the batch and the caller's buffers; later calls
+  // with the same buffers (an inversion updates its model in place) replay them on the caller's stream.  A
+  // capture is cut into consecutive graphs of at most kGraphSteps time steps (forward, recomputation or adjoint;
+  // 2-4 kernels each), replayed in order, so no graph grows with nt: a 4000-step survey1600 gradient is three
+  // graphs of ~12 000 kernel nodes instead of one of ~35 000.  Not used while kernel timing is on.
   struct GraphEntry {
     std::array<const void*, 10> key;
-    hipGraphExec_t exec;
+    std::vector<hipGraphExec_t> execs;
   };
   static constexpr size_t kMaxGraphs = 8;
+  static constexpr int kGraphSteps = 4096;
   std::vector<GraphEntry> graphs;
   hipStream_t cap_stream = nullptr;
   long long graph_captures = 0, graph_replays = 0;
+  long long graph_nodes_largest = 0, graph_nodes_last = 0;  // kernel nodes: largest graph, last capture's total
+  bool capturing = false;
+  int cap_steps = 0;                   // time steps in the graph being captured
+  std::vector<hipGraphExec_t> cap_execs;  // graphs of the capture so far
+
+  int end_capture() {
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(cap_stream, &graph);
+    if (e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(e, "hipStreamEndCapture");
+    }
+    size_t nodes = 0;
+    (void)hipGraphGetNodes(graph, nullptr, &nodes);
+    graph_nodes_largest = std::max(graph_nodes_largest, (long long)nodes);
+    graph_nodes_last += (long long)nodes;
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(ei, "hipGraphInstantiate");
+    cap_execs.push_back(exec);
+    return 0;
+  }
+  // Called between time steps: while capturing, a graph that holds kGraphSteps steps is closed and the capture
+  // continues into the next one on the same stream (the work is one stream's sequence, so cutting it anywhere
+  // between launches keeps its order).
+  int step_done() {
+    if (!capturing || ++cap_steps < kGraphSteps) return 0;
+    if (int r = end_capture()) return r;
+    CK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+    cap_steps = 0;
+    return 0;
+  }
 
   template <class F>
   int run_graph(const std::array<const void*, 10>& key, hipStream_t st, F&& body) {
@@ -236,39 +286,46 @@ struct fwi_plan {
     for (size_t i = 0; i < graphs.size(); ++i)
       if (graphs[i].key == key) {
         std::rotate(graphs.begin() + i, graphs.begin() + i + 1, graphs.end());  // most recently used last
-        CK(hipGraphLaunch(graphs.back().exec, st));
+        for (hipGraphExec_t x : graphs.back().execs) CK(hipGraphLaunch(x, st));
         ++graph_replays;
         return 0;
       }
     if (!cap_stream) CK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+    cap_execs.clear();
+    graph_nodes_last = 0;
+    cap_steps = 0;
     CK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
-    const int r = body(cap_stream);
-    hipGraph_t graph = nullptr;
-    const hipError_t e = hipStreamEndCapture(cap_stream, &graph);
+    capturing = true;
+    int r = body(cap_stream);
+    capturing = false;
     if (r) {
-      if (graph) (void)hipGraphDestroy(graph);
+      hipGraph_t graph = nullptr;
+      if (hipStreamEndCapture(cap_stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+    } else {
+      r = end_capture();
+    }
+    if (r) {
+      for (hipGraphExec_t x : cap_execs) (void)hipGraphExecDestroy(x);
+      cap_execs.clear();
       return r;
     }
-    if (e != hipSuccess) return fail(e, "hipStreamEndCapture");
-    hipGraphExec_t exec = nullptr;
-    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ei != hipSuccess) return fail(ei, "hipGraphInstantiate");
     if (graphs.size() >= kMaxGraphs) {  // least recently used first
       // its last launch may still run on the stream it was given: wait for the device before destroying it
       (void)hipDeviceSynchronize();
-      (void)hipGraphExecDestroy(graphs.front().exec);
+      for (hipGraphExec_t x : graphs.front().execs) (void)hipGraphExecDestroy(x);
       graphs.erase(graphs.begin());
     }
-    graphs.push_back({key, exec});
+    graphs.push_back({key, cap_execs});
+    cap_execs.clear();
     ++graph_captures;
-    CK(hipGraphLaunch(exec, st));
+    for (hipGraphExec_t x : graphs.back().execs) CK(hipGraphLaunch(x, st));
     return 0;
   }
 
   ~fwi_plan() {
     (void)hipDeviceSynchronize();  // launches of this plan may still be running on the caller's streams
-    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    for (auto& g : graphs)
+      for (hipGraphExec_t x : g.execs) (void)hipGraphExecDestroy(x);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (auto& b : batches) {
       (void)hipFree(b.d_shots);
@@ -287,16 +344,17 @@ struct fwi_plan {
     (void)hipFree(d_misfit);
     (void)hipFree(d_shot_misfit);
     (void)hipFree(d_ck_amax);
+    (void)hipFree(d_band);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
   }
 
   // ---------------------------------------------------------------------------------------------
   // Work items: (shot, 64-column band, first row, end row) per wave, four per workgroup.
   // Row runs of `rows` rows over [kRim, nz-kRim); a run is cut at every split row strictly inside it.
-  static std::vector<std::pair<int, int>> row_runs(int nz, int rows, std::vector<int> splits) {
+  static std::vector<std::pair<int, int>> row_runs(int nz, int rows, std::vector<int> splits, int zfirst = kRim) {
     std::sort(splits.begin(), splits.end());
     std::vector<std::pair<int, int>> runs;
-    for (int z0 = kRim; z0 < nz - kRim; z0 += rows) {
+    for (int z0 = zfirst; z0 < nz - kRim; z0 += rows) {
       const int z1 = std::min(z0 + rows, nz - kRim);
       int a = z0;
       for (int sp : splits)
@@ -310,16 +368,16 @@ struct fwi_plan {
   }
 
   static std::vector<int4> ungrouped_items(const Batch& b, int nz, int rows, int order,
-                                           const std::vector<int>& zsplit = {}) {
+                                           const std::vector<int>& zsplit = {}, int zfirst = kRim) {
     std::vector<int4> items;
     int maxnx = 0;
     for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
-    const std::vector<std::pair<int, int>> runs = row_runs(nz, rows, zsplit);
+    const std::vector<std::pair<int, int>> runs = row_runs(nz, rows, zsplit, zfirst);
     if (order == 1) {
       for (int s = 0; s < b.count; ++s)
         for (auto& r : runs)
           for (int x0 = 0; x0 < b.h_shots[s].nx; x0 += 64) items.push_back(make_int4(s, x0, r.first, r.second));
-    } else if (order == 3) {
+    } else if (order == 3 || order == 4) {
       for (int g = 0; g < b.count; g += 4) {
         const int ge = std::min(b.count, g + 4);
         int gnx = 0;
@@ -375,7 +433,7 @@ struct fwi_plan {
         for (auto& g : b.groups)
           if (x < g.unx)
             for (auto& r : runs) push(g, g.ux0 + x, r);
-    } else if (order == 3) {
+    } else if (order == 3 || order == 4) {
       for (auto& g : b.groups)
         for (auto& r : runs)
           for (int x = 0; x < g.unx; x += 64) push(g, g.ux0 + x, r);
@@ -425,7 +483,7 @@ struct fwi_plan {
     return 0;
   }
 
-  // Work items of the forward kernels in one of four orders (the tuner picks one per plan):
+  // Work items of the forward kernels in one of five orders (the tuner picks one per plan):
   //   0  row run -> band -> shot: the four waves of a workgroup are four shots on one tile (model rows
   //      shared through the L1), x-neighbour tiles are a few workgroups apart;
   //   1  shot -> row run -> band: a workgroup is four adjacent bands of one shot (x halos shared through
@@ -434,16 +492,53 @@ struct fwi_plan {
   //      so z halos are reused from the L2 while the neighbour is still resident;
   //   3  shot group of 4 -> row run -> band -> shot: a workgroup is four shots on one tile (model rows shared
   //      through the L1) and consecutive workgroups are its x neighbours, with the z neighbours a row of bands
-  //      later, so both halos are read while the neighbour tile is resident in the L2.
+  //      later, so both halos are read while the neighbour tile is resident in the L2;
+  //   4  as 3 with every workgroup one tile of four shots with the same window: the four waves of an interior
+  //      tile share the model rows through LDS (kShareModel).  Offered when the batch's shots, ordered by window,
+  //      form such groups of four (full-width surveys).
   // Workgroup ids are remapped so each XCD receives a contiguous range of this order.
   // Forward items, cut at row nz-e (e = max(cpml,4), the first bottom C-PML row): the kernel runs items without
   // C-PML cells with its pipelined interior body; the others go first in each XCD's range.
+  // Shot groups of order 4: the batch's shots ordered by window (x0, width, then index) in runs of four with
+  // identical windows; empty if the batch does not split that way.
+  static std::vector<std::array<int, 4>> share_groups(const Batch& b) {
+    std::vector<int> ord(b.count);
+    for (int k = 0; k < b.count; ++k) ord[k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) {
+      const ShotDev &x = b.h_shots[u], &y = b.h_shots[v];
+      return x.x0 != y.x0 ? x.x0 < y.x0 : x.nx < y.nx;
+    });
+    std::vector<std::array<int, 4>> g;
+    if (b.count % 4) return g;
+    for (int k = 0; k < b.count; k += 4) {
+      for (int j = 1; j < 4; ++j)
+        if (b.h_shots[ord[k + j]].x0 != b.h_shots[ord[k]].x0 || b.h_shots[ord[k + j]].nx != b.h_shots[ord[k]].nx)
+          return {};
+      g.push_back({ord[k], ord[k + 1], ord[k + 2], ord[k + 3]});
+    }
+    return g;
+  }
   int build_items(Batch& b, int rows, int order) {
     const int e = std::max(cpml, kRim);
-    std::vector<int4> items = ungrouped_items(b, nz, rows, order, {nz - e});
-    edge_first(items, 1, [&](const int4& it) {
-      return !(it.y >= e && it.y + 64 <= b.h_shots[it.x].nx - e && it.w <= nz - e);
-    });
+    auto interior = [&](const int4& it) {
+      return it.y >= e && it.y + 64 <= b.h_shots[it.x & ~kShareModel].nx - e && it.w <= nz - e;
+    };
+    std::vector<int4> items;
+    const auto groups = order == 4 ? share_groups(b) : std::vector<std::array<int, 4>>{};
+    const int zfirst = std::max(kRim, band);  // rows above are the FP64 band's
+    if (!groups.empty()) {
+      const std::vector<std::pair<int, int>> runs = row_runs(nz, rows, {nz - e}, zfirst);
+      for (const auto& g : groups)
+        for (auto& r : runs)
+          for (int x0 = 0; x0 < b.h_shots[g[0]].nx; x0 += 64) {
+            const bool sh = interior(make_int4(g[0], x0, r.first, r.second));
+            for (int k = 0; k < 4; ++k) items.push_back(make_int4(g[k] | (sh ? kShareModel : 0), x0, r.first, r.second));
+          }
+      edge_first(items, 4, [&](const int4& it) { return !interior(it); });
+    } else {
+      items = ungrouped_items(b, nz, rows, order == 4 ? 3 : order, {nz - e}, zfirst);
+      edge_first(items, 1, [&](const int4& it) { return !interior(it); });
+    }
     return upload_items(b.d_items, b.nitems, items);
   }
   // Step-pair items: the fused core items (56 output columns, runs of `rows`, row run -> band -> shot) and the
@@ -498,12 +593,15 @@ struct fwi_plan {
   }
 
   // Work items of the adjoint kernels: grouped (orders 0, 2 and 3) when the batch has gradient groups.  No item
-  // crosses row nz-cpml-4, the first row whose z stencil reaches the bottom C-PML strip: the exact adjoint
-  // kernel runs the rows of an item with or without the bottom C-PML terms, by its first row.
+  // crosses row nz-cpml-4, the first row whose z stencil reaches the bottom C-PML strip, nor the end row of the
+  // FP64 band: the exact adjoint kernel runs the rows of an item with or without the bottom C-PML terms, by its
+  // first row, and in the increment form when it lies in the band.
   int build_adj_items(Batch& b, int rows, int order) {
     const int zsplit = nz - cpml - 4;
-    std::vector<int4> items = b.grouped ? grouped_items(b, nz, rows, order, {zsplit})
-                                        : ungrouped_items(b, nz, rows, order, {zsplit});
+    std::vector<int> splits = {zsplit};
+    if (band > kRim) splits.push_back(band);  // the band's rows take the increment form
+    std::vector<int4> items = b.grouped ? grouped_items(b, nz, rows, order, splits)
+                                        : ungrouped_items(b, nz, rows, order, splits);
     if (opt.grad_mode == FWI_GRAD_EXACT)
       edge_first(items, b.grouped ? 4 : 1, [&](const int4& it) {
         if (it.x < 0) return false;  // a placeholder wave computes nothing
@@ -531,12 +629,14 @@ struct fwi_plan {
     if (b.nsblocks == 0) return 0;
     StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
     const int am = opt.arith;
+    const int tev = timed_begin(5, st);
     if (am == FWI_ARITH_MIXED)
       hipLaunchKernelGGL(k_fwd_strip<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, fP(n), fF(n), fpsx(), fpsz());
     else if (am == FWI_ARITH_FP32D)
       hipLaunchKernelGGL(k_fwd_strip<F32D>, dim3(b.nsblocks), dim3(256), 0, st, sa, fP(n), fF(n), fpsx(), fpsz());
     else
       hipLaunchKernelGGL(k_fwd_strip<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, fP(n), fF(n), fpsx(), fpsz());
+    timed_end(tev, st);
     CK(hipGetLastError());
     return 0;
   }
@@ -598,10 +698,51 @@ struct fwi_plan {
     return 0;
   }
 
+  // the FP64 near-surface band of forward step n (rows [4, band) of every shot)
+  int fwd_band(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
+               hipStream_t st) {
+    if (band <= kRim) return 0;
+    BandArgs a;
+    a.g = grid_args(b);
+    a.m = m;
+    a.band = band;
+    a.dt = (double)dt;
+    a.src = d_src;
+    a.it = n;
+    a.P = bP(n);
+    a.Pm = bP(n + 1);
+    a.F = bF(n);
+    a.Fm = bF(n + 1);
+    a.psx = bpsx(n);
+    a.psxn = bpsx(n + 1);
+    a.zx = bzx();
+    a.Fc = fF(n);
+    a.Pcf = fP(n);
+    a.Po = fP(n + 1);
+    a.Fo = fF(n + 1);
+    a.gather = gather;
+    a.qa = qa;
+    a.qb = qb;
+    int maxnx = 0;
+    for (const ShotDev& d : b.h_shots) maxnx = std::max(maxnx, d.nx);
+    const dim3 grid((maxnx + 255) / 256, band - kRim, b.count);
+    const int tev = timed_begin(7, st);
+    if (qm == 0)
+      hipLaunchKernelGGL(k_fwd_band<0>, grid, dim3(256), 0, st, a);
+    else if (qm == 1)
+      hipLaunchKernelGGL(k_fwd_band<1>, grid, dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL(k_fwd_band<2>, grid, dim3(256), 0, st, a);
+    timed_end(tev, st);
+    CK(hipGetLastError());
+    return 0;
+  }
+
   // one forward step of a batch
   int fwd_step(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
                hipStream_t st) {
     if (int r = fwd_strip(b, m, n, st)) return r;
+    if (int r = fwd_band(b, m, n, qm, qa, qb, gather, st)) return r;
     return fwd_main(b, m, n, qm, qa, qb, gather, b.d_items, b.nitems, nullptr, qm == 0 ? 0 : 1, st);
   }
 
@@ -659,6 +800,8 @@ struct fwi_plan {
         hipLaunchKernelGGL((k_adj_ex_main<double, true, IMG>), grid, dim3(256), 0, st, a);
       else if (am == FWI_ARITH_FP32D)
         hipLaunchKernelGGL((k_adj_ex_main<F32D, true, IMG>), grid, dim3(256), 0, st, a);
+      else if (am == FWI_ARITH_FP32I)  // increment form
+        hipLaunchKernelGGL((k_adj_ex_main<float, true, IMG, true>), grid, dim3(256), 0, st, a);
       else
         hipLaunchKernelGGL((k_adj_ex_main<float, true, IMG>), grid, dim3(256), 0, st, a);
     } else {
@@ -666,6 +809,8 @@ struct fwi_plan {
         hipLaunchKernelGGL((k_adj_ex_main<double, false, IMG>), grid, dim3(256), 0, st, a);
       else if (am == FWI_ARITH_FP32D)
         hipLaunchKernelGGL((k_adj_ex_main<F32D, false, IMG>), grid, dim3(256), 0, st, a);
+      else if (am == FWI_ARITH_FP32I)
+        hipLaunchKernelGGL((k_adj_ex_main<float, false, IMG, true>), grid, dim3(256), 0, st, a);
       else
         hipLaunchKernelGGL((k_adj_ex_main<float, false, IMG>), grid, dim3(256), 0, st, a);
     }
@@ -749,6 +894,9 @@ struct fwi_plan {
       a.ptz = adj(11);
       a.gbx = adj(12);
       a.gbz = adj(13);
+      a.DP = adj(14);
+      a.DF = adj(15);
+      a.band = opt.arith == FWI_ARITH_FP32I ? band : 0;
       a.res = resid();
       a.rit = k - 1;
       a.qa = qa;
@@ -757,9 +905,11 @@ struct fwi_plan {
       a.gp = sgrad(1);
       a.gn = sgrad(2);
       if (b.nsblocks > 0) {
+        const int tes = timed_begin(6, st);
         hipLaunchKernelGGL(k_adj_ex_strip1, dim3(b.nsblocks), dim3(256), 0, st, sa, a.dt2, Lc, Mc, a.zx, a.zz, a.ztx,
                            a.ztz, a.gbx, a.gbz);
         ARITH_LAUNCH(k_adj_ex_strip2, dim3(b.nsblocks), sa, a);
+        timed_end(tes, st);
       }
       const int tev = timed_begin(2, st);
       if (int r = adj_ex_launch(b.grouped, am, grid, a, st)) return r;
@@ -781,9 +931,13 @@ struct fwi_plan {
 
   int zero_fwd(const Batch& b, hipStream_t st) {
     if (int r = zero(fwd(0), kFwdF * (size_t)cap_cell, st)) return r;
+    if (band > kRim)
+      if (int r = zero((float*)d_band, 2 * kBandF * (size_t)cap_band, st)) return r;
     (void)b;
     return 0;
   }
+  // the band state of a checkpoint: after its cell and strip fields
+  float* ckpt_band(int j) const { return (float*)(ckpt(j) + (4 * (size_t)cap_cell + 4 * (size_t)cap_strip) * ck_esize); }
 
   int save_ckpt(const Batch& b, int j, hipStream_t st) {
     char* c = ckpt(j);
@@ -818,6 +972,8 @@ struct fwi_plan {
     float* cf = (float*)c;
     for (int f = 0; f < 4; ++f)
       if (int r = copy(cf + (size_t)f * b.ncell, fwd(ckpt_field(f, n)), b.ncell, st)) return r;
+    if (band > kRim)
+      if (int r = copy(ckpt_band(j), (const float*)d_band, 2 * kBandF * (size_t)cap_band, st)) return r;
     return strip_copy(b, 0, cf + 4 * (size_t)b.ncell, st);
   }
   int load_ckpt(const Batch& b, int j, hipStream_t st) {
@@ -849,6 +1005,8 @@ struct fwi_plan {
     float* cf = (float*)c;
     for (int f = 0; f < 4; ++f)
       if (int r = copy(fwd(ckpt_field(f, n)), cf + (size_t)f * b.ncell, b.ncell, st)) return r;
+    if (band > kRim)
+      if (int r = copy((float*)d_band, ckpt_band(j), 2 * kBandF * (size_t)cap_band, st)) return r;
     return strip_copy(b, 1, cf + 4 * (size_t)b.ncell, st);
   }
 
@@ -873,6 +1031,7 @@ struct fwi_plan {
         if (int r = fwd_step(b, m, n, 0, nullptr, nullptr, gather, st)) return r;
         n += 1;
       }
+      if (int r = step_done()) return r;
     }
     return 0;
   }
@@ -905,6 +1064,7 @@ struct fwi_plan {
         if (int r = fwd_step(b, m, n, n >= s0 ? qm : 0, q0, q0 ? q0 + cap_cell : nullptr, gather, st)) return r;
         n += 1;
       }
+      if (int r = step_done()) return r;
     }
     if (adj_src) {
       if (int r = copy(resid(), adj_src + b.gather_off, b.ngather, st)) return r;
@@ -943,12 +1103,14 @@ struct fwi_plan {
               if (int r = fwd_step(b, m, nn, qm, wa, wa + cap_cell, nullptr, st)) return r;
               nn += 1;
             }
+            if (int r = step_done()) return r;
           }
         }
         qa = qbuf() + (size_t)(n - seg * K) * slot;
         qb = qa + cap_cell;
       }
       if (int r = adj_step(b, m, k, qa, qb, st)) return r;
+      if (int r = step_done()) return r;
     }
     const long long nmodel = (long long)nx * nz;
     hipLaunchKernelGGL(k_cross_shot_sum, dim3((unsigned)((nmodel + 255) / 256)), dim3(256), 0, st, b.d_shots, b.count,
@@ -967,10 +1129,11 @@ struct fwi_plan {
     if (opt.rows_per_item > 0) rows_c = {opt.rows_per_item};
     Batch& b = batches[0];
     std::vector<int> order_f = {0, 1, 2, 3};
+    if (!share_groups(b).empty()) order_f.push_back(4);
     std::vector<int> order_a = b.grouped ? std::vector<int>{0, 2, 3} : std::vector<int>{0, 1, 2, 3};
     if (opt.item_order >= 0) {
       order_f = {opt.item_order};
-      order_a = {b.grouped && opt.item_order == 1 ? 0 : opt.item_order};
+      order_a = {b.grouped && opt.item_order == 1 ? 0 : opt.item_order == 4 ? 3 : opt.item_order};
     }
     // the tuner times launches with events and waits for them: not possible while the caller captures `st`
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -988,6 +1151,8 @@ struct fwi_plan {
     CK(hipEventCreate(&ev.e[0]));
     CK(hipEventCreate(&ev.e[1]));
     hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
+    const char* tlog = std::getenv("FWI_TUNE_LOG");  // "1": each candidate's time to stderr
+    const bool log = tlog && tlog[0] == '1';
     auto timed = [&](auto&& body, float& ms) -> int {
       CK(hipEventRecord(e0, st));
       if (int r = body()) return r;
@@ -1017,6 +1182,7 @@ struct fwi_plan {
               return 0;
             }, ms))
           return r;
+        if (log) std::fprintf(stderr, "[fwi tune] forward  order %d rows %2d: %.1f us/step\n", o, c, ms * 1e3 / 6);
         if (ms < best) {
           best = ms;
           tz = c;
@@ -1046,6 +1212,7 @@ struct fwi_plan {
               return 0;
             }, ms))
           return r;
+        if (log) std::fprintf(stderr, "[fwi tune] adjoint  order %d rows %2d: %.1f us/step\n", o, c, ms * 1e3 / 6);
         if (ms < best) {
           best = ms;
           tz_adj = c;
@@ -1175,6 +1342,21 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     return 1;
   }
   p->ck_esize = p->opt.ckpt_format == FWI_CKPT_FP32 ? 4 : 2;
+  {  // FP64 near-surface band
+    const int k = p->opt.surface_rows;
+    const bool allowed = p->opt.arith == FWI_ARITH_FP32I && p->opt.ckpt_format == FWI_CKPT_FP32 && p->opt.step_pairs == 0;
+    if (k == 0) {
+      p->band = allowed && s->nz - s->cpml - 4 >= 24 ? 24 : 0;
+    } else if (k > 0) {
+      if (!allowed || k <= kRim || k > s->nz - s->cpml - 4) {
+        delete p;
+        set_message("surface_rows needs FWI_ARITH_FP32I, FP32 checkpoints, no step pairs and 4 < surface_rows <= "
+                    "nz - cpml - 4");
+        return 1;
+      }
+      p->band = k;
+    }
+  }
   if (p->opt.grad_group < 0 || p->opt.grad_group > 4) {
     delete p;
     set_message("grad_group must be 0..4");
@@ -1231,6 +1413,7 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     p->cap_strip = std::max(p->cap_strip, b.nstrip);
     p->cap_gather = std::max(p->cap_gather, b.ngather);
   }
+  p->cap_band = (long long)p->band * (p->cap_cell / p->nz);  // band rows x the row pitches of the largest batch
   // trailing segments kept from the forward: as many as the budget allows with FP32 checkpoints (recomputation
   // reproduces them exactly, so the result does not change), one with 16-bit checkpoints (whose recomputed
   // segments start from rounded states), or exactly store_segments
@@ -1249,6 +1432,8 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     if (e != hipSuccess && rc == 0) rc = p->fail(e, w);
   };
   ck(hipMalloc(&p->d_work, p->work_bytes), "hipMalloc(workspace)");
+  if (p->cap_band > 0)
+    ck(hipMalloc(&p->d_band, fwi_plan::kBandF * (size_t)p->cap_band * sizeof(double)), "hipMalloc(band)");
   ck(hipMalloc(&p->d_src, nt * sizeof(float)), "hipMalloc(src)");
   ck(hipMalloc(&p->d_ta, p->cpml * sizeof(float)), "hipMalloc(ta)");
   ck(hipMalloc(&p->d_tb, p->cpml * sizeof(float)), "hipMalloc(tb)");
@@ -1266,7 +1451,7 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
   // per-batch tables
   for (auto& b : p->batches) {
     if (rc) break;
-    long long off = 0, soff = 0, goff = 0;
+    long long off = 0, soff = 0, goff = 0, boff = 0;
     for (int k = 0; k < b.count; ++k) {
       const fwi_shot& h = p->shots[b.s0 + k];
       ShotDev d;
@@ -1287,6 +1472,8 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
       d.gnx = h.nx;
       d.gps = d.ps;
       d.lead = 1;
+      d.boff = boff;
+      boff += (long long)d.ps * p->band;
       off += (long long)d.ps * p->nz;
       b.nreal += (long long)h.nx * p->nz;
       soff += d.nstrip;
@@ -1340,7 +1527,7 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
            "copy sblocks");
     }
     if (rc == 0) rc = p->build_items(b, p->tz, p->opt.item_order > 0 ? p->opt.item_order : 0);
-    if (rc == 0) rc = p->build_adj_items(b, p->tz, p->opt.item_order == 2 || p->opt.item_order == 3 ? p->opt.item_order : 0);
+    if (rc == 0) rc = p->build_adj_items(b, p->tz, p->opt.item_order >= 2 ? std::min(p->opt.item_order, 3) : 0);
     if (rc == 0 && p->inc() && p->opt.step_pairs != 0) rc = p->build_pair_items(b, p->tz_pair);
   }
   if (rc) {
@@ -1491,9 +1678,15 @@ int fwi_plan_graph_stats(const fwi_plan* p, long long* captures, long long* repl
   if (replays) *replays = p->graph_replays;
   return 0;
 }
+int fwi_plan_graph_nodes(const fwi_plan* p, long long* largest, long long* last) {
+  if (largest) *largest = p->graph_nodes_largest;
+  if (last) *last = p->graph_nodes_last;
+  return 0;
+}
 int fwi_plan_pair_rows(const fwi_plan* p) {
   return (p->inc() && p->opt.step_pairs != 0 && !p->batches.empty() && p->batches[0].nitems_core > 0) ? p->tz_pair : 0;
 }
+int fwi_plan_surface_rows(const fwi_plan* p) { return p->band > kRim ? p->band : 0; }
 int fwi_plan_adj_tiling(const fwi_plan* p, int* rows, int* order, int* grouped) {
   if (rows) *rows = p->tz_adj;
   if (order) *order = p->order_adj;

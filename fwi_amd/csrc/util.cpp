@@ -3,6 +3,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/fwi.h"
 #include "fwi_internal.h"
 
 namespace fwi {
@@ -75,6 +76,33 @@ int fwi_last_error(void) {
 }
 
 const char* fwi_error_message(void) { return fwi::g_message.c_str(); }
+
+// The reference's backtracking line search (src/main.c:690-786) as a state machine, shared by the C and Python
+// drivers: step lengths start at 1 per stage (main.c:308-310) and are only ever halved; failures accumulate over
+// the stage; a NaN misfit counts as 1e30 and as a failure.
+void fwi_line_search_init(fwi_line_search* ls) {
+  if (!ls) return;
+  ls->alpha[0] = ls->alpha[1] = ls->alpha[2] = 1.0;
+  ls->count = 0;
+  ls->max_failures = 5;
+}
+
+int fwi_line_search_step(fwi_line_search* ls, double phi_n, double* trial) {
+  if (!ls || !trial) {
+    fwi::set_message("fwi_line_search_step: null argument");
+    return -1;
+  }
+  if (std::isnan(*trial)) {  // main.c:756-763
+    *trial = 1e30;
+    ls->count += 1;
+    for (double& a : ls->alpha) a /= 2.0;
+  } else if (*trial > phi_n) {  // main.c:764-771
+    for (double& a : ls->alpha) a /= 2.0;
+    ls->count += 1;
+  }
+  if (ls->count > ls->max_failures) return 2;  // main.c:773-784: restore the backup, end the stage
+  return *trial <= phi_n ? 0 : 1;             // the loop runs while trial > phi_n (main.c:690)
+}
 
 // Host evaluation of the C-PML table (see cpml_table_host); writes cpml entries to a and b.
 void fwi_cpml_table(int cpml, float vmax, float dt, float dh, float f, float d0, float L, float* a, float* b) {

@@ -176,12 +176,12 @@ int main(int argc, char** argv) {
     /* observed data (main.c:334-366) */
     if (fwi_forward(plan, t_hor, t_po, t_nmo, obs, NULL)) die("fwi_forward");
     printf("stage %d: f = %g Hz, %d shots, nt = %d\n", scale, f, nshots + 1, nt);
-    double alpha[3] = {1, 1, 1}; /* main.c:308-310: reset per stage */
-    int count = 0;
+    fwi_line_search ls; /* main.c:306-311: step lengths and failure count reset per stage */
+    fwi_line_search_init(&ls);
     for (int it_n = 0; it_n < iterations; ++it_n) {
       double phi = 0;
       if (fwi_gradient(plan, v_hor, v_po, v_nmo, obs, grad, &phi, NULL, NULL)) die("fwi_gradient");
-      printf("Error total : %.9g\n", phi);
+      printf("Error total : %.17g\n", phi);
       fflush(stdout);
       if (it_n < 1) { /* main.c:539-568 */
         const double sd[3] = {3e5, 2e5, 2.5e5};
@@ -192,27 +192,19 @@ int main(int argc, char** argv) {
       }
       if (fwi_lbfgs_direction(opt, v_hor, v_po, v_nmo, grad, NULL)) die("fwi_lbfgs_direction");
       /* line search (main.c:690-786) */
-      double trial = 1e10;
-      int stop = 0;
-      while (trial > phi) {
-        if (fwi_lbfgs_trial(opt, v_hor, v_po, v_nmo, alpha, NULL)) die("fwi_lbfgs_trial");
+      int verdict = 1, stop = 0;
+      while (verdict == 1) {
+        double trial = 0;
+        if (fwi_lbfgs_trial(opt, v_hor, v_po, v_nmo, ls.alpha, NULL)) die("fwi_lbfgs_trial");
         if (vmax > 0 && fwi_model_clamp(v_hor, v_po, v_nmo, n, vmin > 0 ? vmin * vmin : 0.f, vmax * vmax, NULL))
           die("fwi_model_clamp");
         if (fwi_misfit(plan, v_hor, v_po, v_nmo, obs, &trial, NULL)) die("fwi_misfit");
-        if (isnan(trial)) {
-          trial = 1e30;
-          count++;
-          for (int k = 0; k < 3; ++k) alpha[k] /= 2.0;
-          if (count <= 5) printf("new alpha = %f ....\n", alpha[0]);
-        } else if (trial > phi) {
-          for (int k = 0; k < 3; ++k) alpha[k] /= 2.0;
-          count++;
-          if (count <= 5) printf("new alpha = %f .... \n", alpha[0]);
-        }
-        if (count > 5) {
+        verdict = fwi_line_search_step(&ls, phi, &trial);
+        printf("trial misfit : %.17g\n", trial);
+        if (verdict == 1) printf("new alpha = %f ....\n", ls.alpha[0]);
+        if (verdict == 2) {
           if (fwi_lbfgs_restore(opt, v_hor, v_po, v_nmo, NULL)) die("fwi_lbfgs_restore");
           stop = 1;
-          break;
         }
       }
       if (stop) break;

@@ -1,8 +1,9 @@
 """Steepest descent, L-BFGS and the line search of src/main.c:539-800 on the device optimizer.
 
 `LBFGS` wraps fwi_lbfgs_* (include/fwi.h): FP64 two-loop recursion with device dot products.
-`LineSearch` is the reference's backtracking state machine (main.c:690-786) as a pure host object, so
-its behaviour can be tested on scripted misfits without a GPU:
+`LineSearch` is the reference's backtracking state machine (main.c:690-786) on the library's host-only
+fwi_line_search_* functions, which the C driver runs too, so both drivers' behaviour is tested on scripted
+misfits without a GPU:
   * a trial is accepted when its misfit <= the misfit of the current iteration;
   * otherwise, or when the misfit is NaN (then taken as 1e30), every step length is halved and `count`
     grows; when count > 5 the backed-up model is restored and the stage ends;
@@ -12,7 +13,6 @@ its behaviour can be tested on scripted misfits without a GPU:
 from __future__ import annotations
 
 import ctypes
-import math
 
 import torch
 
@@ -87,12 +87,22 @@ class LBFGS:
 
 
 class LineSearch:
-    """Backtracking line search of main.c:690-786 (per stage state: step lengths and failure count)."""
+    """Backtracking line search of main.c:690-786 (per stage state: step lengths and failure count), on the
+    library's state machine (fwi_line_search_init / fwi_line_search_step, host code that the C driver runs too)."""
 
     def __init__(self, max_failures: int = 5):
-        self.alpha = [1.0, 1.0, 1.0]
-        self.count = 0
-        self.max_failures = max_failures
+        self._L = _lib.load()
+        self._ls = _lib.FwiLineSearch()
+        self._L.fwi_line_search_init(ctypes.byref(self._ls))
+        self._ls.max_failures = max_failures
+
+    @property
+    def alpha(self):
+        return [float(a) for a in self._ls.alpha]
+
+    @property
+    def count(self) -> int:
+        return int(self._ls.count)
 
     def run(self, phi_n: float, evaluate, restore):
         """evaluate(alpha) sets the trial model and returns its misfit; restore() restores the backed-up
@@ -101,15 +111,12 @@ class LineSearch:
         while True:
             phi = evaluate(tuple(self.alpha))
             trials += 1
-            if phi is None or math.isnan(phi):
-                phi = 1e30
-                self.count += 1
-                self.alpha = [a / 2.0 for a in self.alpha]
-            elif phi > phi_n:
-                self.alpha = [a / 2.0 for a in self.alpha]
-                self.count += 1
-            if self.count > self.max_failures:
+            t = ctypes.c_double(float("nan") if phi is None else float(phi))
+            r = self._L.fwi_line_search_step(ctypes.byref(self._ls), float(phi_n), ctypes.byref(t))
+            if r < 0:
+                raise RuntimeError(self._L.fwi_error_message().decode())
+            if r == 2:
                 restore()
                 return None, trials
-            if phi <= phi_n:
-                return phi, trials
+            if r == 0:
+                return t.value, trials

@@ -65,7 +65,7 @@ class Plan:
                  wavelet: np.ndarray, cpml_ab, grad_mode: int = GRAD_EXACT, arith: int = ARITH_FP32I,
                  ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0,
                  item_order: int = -1, grad_group: int = 0, ckpt_format: int = 0, use_graph: int = 1,
-                 store_segments: int = 0, step_pairs: int = 0):
+                 store_segments: int = 0, step_pairs: int = 0, surface_rows: int = 0):
         L = _lib.load()
         self.nx, self.nz, self.nt, self.dh, self.dt, self.cpml = nx, nz, nt, dh, dt, cpml
         self.shots = list(shots)
@@ -93,6 +93,7 @@ class Plan:
         opt.use_graph = use_graph
         opt.store_segments = store_segments
         opt.step_pairs = step_pairs
+        opt.surface_rows = surface_rows
         h = ctypes.c_void_p()
         _lib.check(L.fwi_plan_create(ctypes.byref(sv), ctypes.byref(opt), ctypes.byref(h)), "fwi_plan_create")
         self._h = h
@@ -136,6 +137,11 @@ class Plan:
         return int(_lib.load().fwi_plan_kept_steps(self._h))
 
     @property
+    def surface_rows(self) -> int:
+        """End row of the FP64 near-surface band of the forward (0: none)."""
+        return int(_lib.load().fwi_plan_surface_rows(self._h))
+
+    @property
     def rows_per_item(self) -> int:
         return int(_lib.load().fwi_plan_rows_per_item(self._h))
 
@@ -155,6 +161,12 @@ class Plan:
         c, r = ctypes.c_longlong(), ctypes.c_longlong()
         _lib.load().fwi_plan_graph_stats(self._h, ctypes.byref(c), ctypes.byref(r))
         return c.value, r.value
+
+    def graph_nodes(self):
+        """(kernel nodes of the largest captured graph, of all graphs of the last capture)."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        _lib.load().fwi_plan_graph_nodes(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def cells(self):
         """(sum of nx_s*nz over shots, C-PML cells of all windows)."""

@@ -50,10 +50,11 @@ enum { FWI_GRAD_REFCOMPAT = 0, FWI_GRAD_EXACT = 1 };
  * reference's double literals make them; FP32D = FP32 with second derivatives in difference form
  * sum_k c_k((u_{+k}-u_0)+(u_{-k}-u_0)), which avoids the cancellation of the reference form (up to 30x
  * smaller round-off in isotropic media; little change in strongly anisotropic ones); FP32I (default) =
- * FP32 with the forward leapfrog in increment form: the state is (P^n, P^n - P^{n-1}) instead of two
- * levels, so the time difference keeps FP32 relative precision.  At 150+ samples per period the
- * two-level form amplifies rounding several hundred times; FP32I cuts the gradient error against FP64
- * 20-50x for 8 more bytes per cell on the forward and recomputation steps. */
+ * FP32 with the leapfrog in increment form, forward and exact adjoint: the state is (P^n, P^n - P^{n-1})
+ * instead of two levels, so the time difference keeps FP32 relative precision (at 150+ samples per period
+ * the two-level form amplifies rounding several hundred times), for 8 more bytes per cell on every step;
+ * and, by default, the near-surface band of the forward (surface_rows) in FP64.  On the 1600x400 survey at
+ * 8 Hz this takes the exact gradient from ~3e-3 to ~2e-5 (relative L2 against FP64, full grid). */
 enum { FWI_ARITH_FP32 = 0, FWI_ARITH_MIXED = 1, FWI_ARITH_FP32D = 2, FWI_ARITH_FP32I = 3 };
 
 /* Checkpoint storage: FP32, or 16 bits after an exact power-of-two scale per field, shot and snapshot (half
@@ -69,7 +70,7 @@ typedef struct fwi_options {
   int max_batch;       /* shots per batch, 0 = as many as mem_budget allows */
   double mem_budget;   /* bytes of device memory the plan may use, 0 = 85 % of free memory */
   int rows_per_item;   /* z rows per wave in the stencil kernels, 0 = chosen by the tuner */
-  int item_order;      /* work-item order 0..3 (see plan.hip build_items), -1 = chosen by the tuner */
+  int item_order;      /* work-item order 0..4 (see plan.hip build_items), -1 = chosen by the tuner */
   int grad_group;      /* exact mode: up to this many consecutive shots with identical windows share one
                           FP32 gradient slot, their per-step imaging terms added in shot order before one
                           read-modify-write (0 = 4, the default; 1 = one slot per shot) */
@@ -84,6 +85,10 @@ typedef struct fwi_options {
                           per launch and the rest by single steps; 0 (default): single steps everywhere.  The
                           result is the same either way; on survey1600 the pairs are not faster yet (the fused
                           kernel gains ~10 % per cell, the separate shell launches lose more). */
+  int surface_rows;    /* FWI_ARITH_FP32I: rows 4..surface_rows-1 of every shot (the source, receivers and near-surface
+                          gradient) advance in FP64 in the forward and recomputation passes.  0 (default) = 24 rows
+                          where nz - cpml - 4 >= 24 and the plan allows it (FP32 checkpoints, no step pairs), none
+                          otherwise; -1 = none; k > 4 = k rows (k <= nz - cpml - 4; FP32 checkpoints, no step pairs). */
 } fwi_options;
 
 typedef struct fwi_plan fwi_plan;
@@ -177,8 +182,8 @@ int fwi_dot3(const double* a, const double* b, long long n, double* out, void* s
  * main.c:323,368-372).  With stride > 0, every stride-th launch of each fused step kernel is bracketed by
  * HIP events on the stream it runs on; fwi_plan_kernel_time returns the summed event time and the
  * number of timed launches of a kind (0 forward step, 1 forward step storing imaging terms, 2 adjoint
- * step, 3 / 4 fused core step pair without / with stored imaging terms) since the last fwi_plan_set_timing
- * call. */
+ * step, 3 / 4 fused core step pair without / with stored imaging terms, 5 forward C-PML strip, 6 the two
+ * adjoint C-PML strip kernels together, 7 FP64 near-surface band) since the last fwi_plan_set_timing call. */
 void fwi_plan_set_timing(fwi_plan* plan, int stride);
 int fwi_plan_kernel_time(fwi_plan* plan, int kind, double* total_ms, long long* count);
 /* Cells of all shot windows (nx_s*nz summed over shots) and, in *strip_cells, their C-PML cells. */
@@ -194,8 +199,28 @@ int fwi_plan_item_order(const fwi_plan* plan);
 int fwi_plan_adj_tiling(const fwi_plan* plan, int* rows_per_item, int* item_order, int* grouped);
 /* Rows per fused core item of the step pairs in use (0 when the plan takes single steps only). */
 int fwi_plan_pair_rows(const fwi_plan* plan);
+/* End row of the FP64 near-surface band (rows 4..end-1 of every shot) of the forward in use (0: none). */
+int fwi_plan_surface_rows(const fwi_plan* plan);
 /* Graph captures and replays so far (use_graph). */
 int fwi_plan_graph_stats(const fwi_plan* plan, long long* captures, long long* replays);
+/* Kernel nodes of the captured graphs: the largest graph so far, and all graphs of the last capture (a capture is
+ * cut into consecutive graphs of at most 4096 time steps each). */
+int fwi_plan_graph_nodes(const fwi_plan* plan, long long* largest, long long* last);
+
+/* The backtracking line search of the reference (main.c:690-786) as a host state machine, shared by the drivers
+ * (no device work).  fwi_line_search_init starts a stage: step lengths (v_hor, v_po, v_nmo) = 1, no failures, at
+ * most 5.  After each trial model's misfit, fwi_line_search_step judges it against the iteration's misfit phi_n:
+ * a NaN trial counts as 1e30 (written back) and as a failure; a trial above phi_n halves every step length and
+ * counts as a failure; the step lengths and the failure count carry over to the stage's later iterations.
+ * Returns 0 (accept: trial <= phi_n), 1 (try again with the halved step lengths), 2 (more than max_failures
+ * failures in the stage: restore the backed-up model and end the stage), -1 on a null argument. */
+typedef struct fwi_line_search {
+  double alpha[3];
+  int count;
+  int max_failures;
+} fwi_line_search;
+void fwi_line_search_init(fwi_line_search* ls);
+int fwi_line_search_step(fwi_line_search* ls, double phi_n, double* trial);
 
 /* Host helpers. */
 void fwi_cpml_table(int cpml, float vmax, float dt, float dh, float f, float d0, float L, float* a, float* b);

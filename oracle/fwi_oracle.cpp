@@ -364,6 +364,25 @@ struct ShotGeo {
 // One forward time step in the reference's order of launchPropagation (serialcalls.cu:55-66):
 // SecondDerivate_P, PSI_F, ZETA_F, propagator_F.  Receivers are written to gather row `it`.
 // Buffers: P (P^n), Pm (P^{n-1}, overwritten with P^{n+1}); likewise F.
+// Rows of the FP64 near-surface band of the f32i forward (see band_step): 0 = the library's default (24 rows when
+// nz - cpml - 4 >= 24, none otherwise), -1 = none, k > 4 = rows 4..k-1.  g_band_force >= -1 overrides the setting
+// inside a composition that runs without the band (16-bit checkpoints).
+int g_band_rows = 0;
+thread_local int g_band_force = -2;
+struct BandForce {  // within its scope the band setting is forced to v (>= -1), or left as it is (v = -2)
+  int old;
+  explicit BandForce(int v) : old(g_band_force) {
+    if (v >= -1) g_band_force = v;
+  }
+  ~BandForce() { g_band_force = old; }
+};
+inline int band_rows(int nz, int cpml) {
+  const int k = g_band_force >= -1 ? g_band_force : g_band_rows;
+  if (k < 0) return 0;
+  if (k == 0) return nz - cpml - 4 >= 24 ? 24 : 0;
+  return k;
+}
+
 template <class T, class S>
 struct FwdState {
   Geo g;
@@ -373,12 +392,96 @@ struct FwdState {
   Fields<T> f;
   std::vector<T> shotbuf;
   int cur = 0;  // parity of the ping-pong
-  FwdState(const Geo& g_) : g(g_), n((long)g_.nx * g_.nz), f(n, 14) {}
+  // f32i near-surface band: rows [kRim, band) in FP64, two levels of P and F by parity plus psi_x, zeta_x
+  int band = 0;
+  double dtd = 0;  // the float time step, promoted (set by the composition that runs the steps)
+  std::vector<double> bP[2], bF[2], bpsx, bzx, bFz;
+  FwdState(const Geo& g_) : g(g_), n((long)g_.nx * g_.nz), f(n, 14) {
+    if constexpr (std::is_same<S, F32I>::value) band = band_rows(g.nz, g.cpml);
+    if (band)
+      for (auto* v : {&bP[0], &bP[1], &bF[0], &bF[1], &bpsx, &bzx, &bFz}) v->assign((size_t)n, 0.0);
+  }
   T* Pcur() { return cur == 0 ? f[0] : f[1]; }
   T* Pold() { return cur == 0 ? f[1] : f[0]; }
   T* Fcur() { return cur == 0 ? f[2] : f[3]; }
   T* Fold() { return cur == 0 ? f[3] : f[2]; }
 };
+
+// FP64 near-surface band of the f32i forward (this project's arithmetic, not the reference's): the rows
+// [kRim, band), where the source, the receivers and the near-surface gradient lie, advance in FP64 with the
+// two-level leapfrog of the f64 arithmetic (C-PML on the side columns; the band lies above the bottom C-PML, so
+// its z terms vanish), from the float model, tables, wavelet, dh and dt promoted exactly.  F couples the band and
+// the float rows through the z stencil: the band reads rows band..band+3 of the float F^n, and the float rows
+// read rows band-4..band-1 of the float arrays, which hold the band's values rounded.  The band writes its rows
+// of the float P, F (rounded), the imaging terms (rounded) and the receivers in it (P + F rounded once).
+template <class T, class S>
+void band_step(FwdState<T, S>& st, const T* vh, const T* vp, const T* vn, const T* ax, const T* bx, const T* src,
+               T C, T dh, const ShotGeo& sg, int it, T* gather) {
+  (void)C;
+  const Geo& g = st.g;
+  const int nx = g.nx, B = st.band, cpml = g.cpml;
+  const double dhd = (double)dh, dh2d = dhd * dhd;
+  const double dtd = st.dtd, Cd = (dtd * dtd) / dh2d;
+  const int c = st.cur;
+  const double* P = st.bP[c].data();
+  double* Pm = st.bP[c ^ 1].data();
+  const double* Fb = st.bF[c].data();
+  double* Fm = st.bF[c ^ 1].data();
+  double* Fz = st.bFz.data();  // F^n over rows 0..band+3: the band's, then the float rows
+  const T* Ff = st.Fcur();
+  for (int iz = 0; iz < B + 4 && iz < g.nz; ++iz)
+    for (int ix = 0; ix < nx; ++ix) {
+      long i = ix + (long)nx * iz;
+      Fz[i] = iz < B ? Fb[i] : (double)Ff[i];
+    }
+  auto side = [&](int ix) { return ix < cpml || ix > nx - cpml - 1; };
+  // psi_x on the side C-PML cells (PSI_F)
+  for (int iz = kRim; iz < B; ++iz)
+    for (int ix = kRim; ix < nx - kRim; ++ix) {
+      if (!side(ix)) continue;
+      long i = ix + (long)nx * iz;
+      double tpx = d1<double, double>(P, i, 1, dhd);
+      st.bpsx[i] = (double)bx[ix] * st.bpsx[i] + (double)ax[ix] * tpx;
+    }
+  T* Pf = st.Pold();
+  T* Ffo = st.Fold();
+  const long is = sg.isx + (long)nx * sg.isz;
+  for (int iz = kRim; iz < B; ++iz)
+    for (int ix = kRim; ix < nx - kRim; ++ix) {
+      long i = ix + (long)nx * iz;
+      double p2 = d2<double, double>(P, i, 1, dh2d);
+      double r2 = d2<double, double>(Fz, i, nx, dh2d);
+      double atp = 0.0, atf = 0.0;
+      if (side(ix)) {
+        double tx = d1<double, double>(st.bpsx.data(), i, 1, dhd);
+        double tpx = p2 + tx;
+        st.bzx[i] = (double)bx[ix] * st.bzx[i] + (double)ax[ix] * tpx;
+        atp = tx + st.bzx[i];
+      }
+      double tp = dh2d * (p2 + atp);
+      double tf = dh2d * (r2 + atf);
+      double pn = 2.0 * P[i] - Pm[i] + (double)vh[i] * Cd * tp + (double)vp[i] * Cd * tf;
+      double fn = 2.0 * Fb[i] - Fm[i] + (double)vn[i] * Cd * tp + (double)vp[i] * Cd * tf;
+      if (i == is) {
+        pn += (double)src[it];
+        fn += (double)src[it];
+      }
+      Pm[i] = pn;
+      Fm[i] = fn;
+      Pf[i] = (T)pn;
+      Ffo[i] = (T)fn;
+      // the imaging terms the caller forms as f[4] + f[10], f[5] + f[11]: the band's, rounded once
+      st.f[4][i] = (T)(p2 + atp);
+      st.f[10][i] = (T)0;
+      st.f[5][i] = (T)(r2 + atf);
+      st.f[11][i] = (T)0;
+    }
+  if (gather && sg.irz < B)
+    for (int r = 0; r < sg.nrec; ++r) {
+      long i = sg.ir0 + r + (long)nx * sg.irz;
+      gather[(long)it * sg.nrec + r] = (T)(Pm[i] + Fm[i]);
+    }
+}
 
 template <class T, class S>
 void fwd_step(FwdState<T, S>& st, const T* vh, const T* vp, const T* vn, const T* ax, const T* bx, const T* az,
@@ -400,11 +503,12 @@ void fwd_step(FwdState<T, S>& st, const T* vh, const T* vp, const T* vn, const T
   long is = sg.isx + (long)nx * sg.isz;
   if constexpr (std::is_same<S, F32I>::value) {
     // increment form (the batched kernel's FP32I): Q += d (+ source), P^{n+1} = P^n + Q; the source
-    // cell is interior (the batched API validates it)
+    // cell is interior (the batched API validates it).  Rows above st.band are the FP64 band's.
+    if (st.band) band_step<T, S>(st, vh, vp, vn, ax, bx, src, C, dh, sg, it, gather);
     T* Q = st.f[12];
     T* R = st.f[13];
 #pragma omp parallel for
-    for (int iz = kRim; iz < nz - kRim; ++iz)
+    for (int iz = std::max(kRim, st.band); iz < nz - kRim; ++iz)
       for (int ix = kRim; ix < nx - kRim; ++ix) {
         long i = ix + (long)nx * iz;
         T tp = dh2 * (p2[i] + atp[i]);
@@ -433,7 +537,7 @@ void fwd_step(FwdState<T, S>& st, const T* vh, const T* vp, const T* vn, const T
     Pm[is] += src[it];
     Fm[is] += src[it];
   }
-  if (gather)
+  if (gather && sg.irz >= st.band)  // receivers in the band: written by band_step
     for (int r = 0; r < sg.nrec; ++r) {
       long i = sg.ir0 + r + (long)nx * sg.irz;
       gather[(long)it * sg.nrec + r] = Pm[i] + Fm[i];
@@ -684,6 +788,7 @@ void forward_shot(const T* vh, const T* vp, const T* vn, const Tables<T>& tb, co
                   const ShotGeo& sg, int nt, Scal sc, T* gather, int qmode, std::vector<T>* q, T* rec = nullptr) {
   FwdState<T, S> st(g);
   const T dh = (T)sc.dh, dt = (T)sc.dt, dh2 = dh * dh, C = (dt * dt) / dh2;
+  st.dtd = (double)dt;
   const long n = st.n;
   if (q) q->assign((size_t)nt * 2 * n, (T)0);
   // rec (optional): the state P^n, F^n that step n starts from, n = 0..nt (2 fields per n)
@@ -794,11 +899,11 @@ struct AdjOp {
   const T *vh, *vp, *vn;
   T dh, dh2, dt2;
   long n;
-  std::vector<T> abar, bbar, gx, gz, wx, wz, px, pz;
+  std::vector<T> abar, bbar, gx, gz, wx, wz, px, pz, ap, af;
   AdjOp(const Geo& g_, const Tables<T>& tb_, const T* vh_, const T* vp_, const T* vn_, T dh_, T dt_)
       : g(g_), tb(tb_), vh(vh_), vp(vp_), vn(vn_), dh(dh_), dh2(dh_ * dh_), dt2(dt_ * dt_),
         n((long)g_.nx * g_.nz) {
-    for (auto* v : {&abar, &bbar, &gx, &gz, &wx, &wz, &px, &pz}) v->assign((size_t)n, (T)0);
+    for (auto* v : {&abar, &bbar, &gx, &gz, &wx, &wz, &px, &pz, &ap, &af}) v->assign((size_t)n, (T)0);
   }
   inline bool chix(int ix, int iz) const { return g.omega(ix, iz) && (ix < g.cpml || ix > g.nx - g.cpml - 1); }
   inline bool chiz(int ix, int iz) const { return g.omega(ix, iz) && iz > g.nz - g.cpml - 1; }
@@ -808,6 +913,25 @@ struct AdjOp {
   // the adjoints of the old memory).  Out: PbarN, FbarN (adjoint of P^n, F^n), PbarO, FbarO (of P^{n-1}).
   void apply(const T* Pn, const T* Fn, const T* Ps, const T* Fs, T* sx, T* zx, T* sz, T* zz, T* PbarN, T* FbarN,
              T* PbarO, T* FbarO) {
+    terms(Pn, Fn, sx, zx, sz, zz);
+    const int nx = g.nx, nz = g.nz;
+#pragma omp parallel for
+    for (int iz = 0; iz < nz; ++iz)
+      for (int ix = 0; ix < nx; ++ix) {
+        long i = ix + (long)nx * iz;
+        if (!g.omega(ix, iz)) {
+          PbarN[i] = FbarN[i] = PbarO[i] = FbarO[i] = (T)0;
+          continue;
+        }
+        PbarN[i] = (T)2 * Pn[i] + ap[i] + Ps[i];
+        FbarN[i] = (T)2 * Fn[i] + af[i] + Fs[i];
+        PbarO[i] = -Pn[i];
+        FbarO[i] = -Fn[i];
+      }
+  }
+  // ap = D2x gbar_x - D1x(a_x pt_x), af likewise in z, on Omega (the new-state terms of the transposed step);
+  // updates the memory adjoints sx, zx, sz, zz in place.
+  void terms(const T* Pn, const T* Fn, T* sx, T* zx, T* sz, T* zz) {
     const int nx = g.nx, nz = g.nz;
     const T* ax = tb.ax.data();
     const T* bx = tb.bx.data();
@@ -852,18 +976,76 @@ struct AdjOp {
       for (int ix = 0; ix < nx; ++ix) {
         long i = ix + (long)nx * iz;
         if (!g.omega(ix, iz)) {
-          PbarN[i] = FbarN[i] = PbarO[i] = FbarO[i] = (T)0;
+          ap[i] = af[i] = (T)0;
           continue;
         }
-        T ap = d2<T, S>(gx.data(), i, 1, dh2) - d1<T, S>(px.data(), i, 1, dh);
-        T af = d2<T, S>(gz.data(), i, nx, dh2) - d1<T, S>(pz.data(), i, nx, dh);
-        PbarN[i] = (T)2 * Pn[i] + ap + Ps[i];
-        FbarN[i] = (T)2 * Fn[i] + af + Fs[i];
-        PbarO[i] = -Pn[i];
-        FbarO[i] = -Fn[i];
+        ap[i] = d2<T, S>(gx.data(), i, 1, dh2) - d1<T, S>(px.data(), i, 1, dh);
+        af[i] = d2<T, S>(gz.data(), i, nx, dh2) - d1<T, S>(pz.data(), i, nx, dh);
       }
   }
 };
+
+// One exact adjoint step: mu^k from mu^{k+1} with the residual res (one time sample of the gather) injected at
+// the receivers; on return muP, muF hold mu^k.  Two-level form: shP, shF hold -mu^{k+2}, the adjoint of the
+// shifted copy.  f32i: the rows of the forward's FP64 band ([kRim, band), band_rows) take the increment form
+// instead, with D = mu^{k+1} - mu^{k+2} in DP, DF updated to mu^k - mu^{k+1} (the residual, injected on the
+// band's receiver row, goes into D); near the receivers the two-level rounding is what limits the gradient.
+template <class T, class S>
+void adjoint_step_exact(AdjOp<T, S>& op, const Geo& g, const ShotGeo& sg, const T* res, T*& muP, T*& muF, T*& shP,
+                        T*& shF, T* sx, T* zx, T* sz, T* zz, T*& nP, T*& nF, T*& oP, T*& oF, T* DP = nullptr,
+                        T* DF = nullptr) {
+  const int B = std::is_same<S, F32I>::value ? band_rows(g.nz, g.cpml) : 0;
+  if (B > 0) {
+    op.terms(muP, muF, sx, zx, sz, zz);
+#pragma omp parallel for
+    for (int iz = 0; iz < g.nz; ++iz)
+      for (int ix = 0; ix < g.nx; ++ix) {
+        long i = ix + (long)g.nx * iz;
+        if (!g.omega(ix, iz)) {
+          nP[i] = nF[i] = oP[i] = oF[i] = (T)0;
+          continue;
+        }
+        if (iz < B) {
+          DP[i] = DP[i] + op.ap[i];
+          DF[i] = DF[i] + op.af[i];
+        } else {
+          nP[i] = (T)2 * muP[i] + op.ap[i] + shP[i];
+          nF[i] = (T)2 * muF[i] + op.af[i] + shF[i];
+        }
+        oP[i] = -muP[i];
+        oF[i] = -muF[i];
+      }
+    for (int r = 0; r < sg.nrec; ++r) {
+      long i = sg.ir0 + r + (long)g.nx * sg.irz;
+      T* tP = sg.irz < B ? DP : nP;
+      T* tF = sg.irz < B ? DF : nF;
+      tP[i] += res[r];
+      tF[i] += res[r];
+    }
+#pragma omp parallel for
+    for (int iz = kRim; iz < B; ++iz)
+      for (int ix = kRim; ix < g.nx - kRim; ++ix) {
+        long i = ix + (long)g.nx * iz;
+        nP[i] = muP[i] + DP[i];
+        nF[i] = muF[i] + DF[i];
+      }
+    std::swap(muP, nP);
+    std::swap(muF, nF);
+    std::swap(shP, oP);
+    std::swap(shF, oF);
+  } else {
+    op.apply(muP, muF, shP, shF, sx, zx, sz, zz, nP, nF, oP, oF);
+    for (int r = 0; r < sg.nrec; ++r) {
+      long i = sg.ir0 + r + (long)g.nx * sg.irz;
+      nP[i] += res[r];
+      nF[i] += res[r];
+    }
+    std::swap(muP, nP);
+    std::swap(muF, nF);
+    std::swap(shP, oP);
+    std::swap(shF, oF);
+  }
+}
 
 // Exact gradient of J = 1/2 sum (d - obs)^2 with respect to (v_hor, v_po, v_nmo):
 //   g_hor += dt^2 mu_P^{k} a^{k-1},  g_nmo += dt^2 mu_F^{k} a^{k-1},  g_po += dt^2 (mu_P^{k}+mu_F^{k}) b^{k-1}
@@ -875,23 +1057,15 @@ void adjoint_exact_shot(const T* vh, const T* vp, const T* vn, const Tables<T>& 
   const long n = (long)g.nx * g.nz;
   const T dh = (T)sc.dh, dt = (T)sc.dt, dt2 = dt * dt;
   AdjOp<T, S> op(g, tb, vh, vp, vn, dh, dt);
-  Fields<T> f(n, 12);
+  Fields<T> f(n, 14);
   T *muP = f[0], *muF = f[1], *shP = f[2], *shF = f[3];  // adjoint of u^{k+1}; shifted adjoint (= -mu^{k+2})
   T *sx = f[4], *zx = f[5], *sz = f[6], *zz = f[7];
   T *nP = f[8], *nF = f[9], *oP = f[10], *oF = f[11];
+  T *DP = f[12], *DF = f[13];  // f32i band rows: mu^{k+1} - mu^{k+2}
   for (int k = nt; k >= 1; --k) {
     // mu^k = transpose-step(mu^{k+1}) + R^T res^{k-1}; mu^{nt+1} = 0
-    op.apply(muP, muF, shP, shF, sx, zx, sz, zz, nP, nF, oP, oF);
-    for (int r = 0; r < sg.nrec; ++r) {
-      long i = sg.ir0 + r + (long)g.nx * sg.irz;
-      T v = res_g[(size_t)(k - 1) * sg.nrec + r];
-      nP[i] += v;
-      nF[i] += v;
-    }
-    std::swap(muP, nP);
-    std::swap(muF, nF);
-    std::swap(shP, oP);
-    std::swap(shF, oF);
+    adjoint_step_exact<T, S>(op, g, sg, res_g + (size_t)(k - 1) * sg.nrec, muP, muF, shP, shF, sx, zx, sz, zz, nP, nF,
+                             oP, oF, DP, DF);
     if (rec) {  // (mu_P^k, mu_F^k) at k - 1
       std::memcpy(rec + (size_t)(k - 1) * 2 * n, muP, n * sizeof(T));
       std::memcpy(rec + (size_t)(k - 1) * 2 * n + n, muF, n * sizeof(T));
@@ -985,6 +1159,8 @@ inline void ck_round_field(float* f, long n, int fmt) {
 template <class T, class S>
 void forward_shot_ck(const T* vh, const T* vp, const T* vn, const Tables<T>& tb, const T* src, const Geo& g,
                      const ShotGeo& sg, int nt, Scal sc, T* gather, std::vector<T>* q, int K, int fmt) {
+  // 16-bit checkpoints run without the FP64 band (the library refuses the combination)
+  BandForce force(fmt ? -1 : -2);
   forward_shot<T, S>(vh, vp, vn, tb, src, g, sg, nt, sc, gather, 1, q);
   if (fmt == 0) return;
   static_assert(std::is_same<T, float>::value, "BF16 checkpoints of the float state");
@@ -1049,8 +1225,9 @@ void group_gradient(const T* vh_m, const T* vp_m, const T* vn_m, const T* tab_a,
     Tables<T>* tb;
     AdjOp<T, S>* op;
     Fields<T>* f;
-    T *muP, *muF, *shP, *shF, *sx, *zx, *sz, *zz, *nP, *nF, *oP, *oF;
+    T *muP, *muF, *shP, *shF, *sx, *zx, *sz, *zz, *nP, *nF, *oP, *oF, *DP, *DF;
   };
+  BandForce force(ckfmt ? -1 : -2);  // 16-bit checkpoints run without the FP64 band
   std::vector<Member> M((size_t)ns);
   int ux0 = 1 << 30, ux1 = 0;
   long goff = 0;
@@ -1089,26 +1266,19 @@ void group_gradient(const T* vh_m, const T* vp_m, const T* vn_m, const T* tab_a,
     misfits[s] = phi;
     goff += (long)nt * m.sg.nrec;
     m.op = new AdjOp<T, S>(g, *m.tb, m.vh.data(), m.vp.data(), m.vn.data(), dh, dt);
-    m.f = new Fields<T>(m.n, 12);
+    m.f = new Fields<T>(m.n, 14);
     Fields<T>& F = *m.f;
     m.muP = F[0]; m.muF = F[1]; m.shP = F[2]; m.shF = F[3];
     m.sx = F[4]; m.zx = F[5]; m.sz = F[6]; m.zz = F[7];
     m.nP = F[8]; m.nF = F[9]; m.oP = F[10]; m.oF = F[11];
+    m.DP = F[12]; m.DF = F[13];
   }
   const int unx = ux1 - ux0;
   for (int k = nt; k >= 1; --k) {
     for (auto& m : M) {
-      m.op->apply(m.muP, m.muF, m.shP, m.shF, m.sx, m.zx, m.sz, m.zz, m.nP, m.nF, m.oP, m.oF);
-      for (int r = 0; r < m.sg.nrec; ++r) {
-        long i = m.sg.ir0 + r + (long)m.nx * m.sg.irz;
-        T v = m.res[(size_t)(k - 1) * m.sg.nrec + r];
-        m.nP[i] += v;
-        m.nF[i] += v;
-      }
-      std::swap(m.muP, m.nP);
-      std::swap(m.muF, m.nF);
-      std::swap(m.shP, m.oP);
-      std::swap(m.shF, m.oF);
+      Geo g{m.nx, nz, cpml};
+      adjoint_step_exact<T, S>(*m.op, g, m.sg, m.res.data() + (size_t)(k - 1) * m.sg.nrec, m.muP, m.muF, m.shP, m.shF,
+                               m.sx, m.zx, m.sz, m.zz, m.nP, m.nF, m.oP, m.oF, m.DP, m.DF);
     }
 #pragma omp parallel for
     for (int iz = kRim; iz < nz - kRim; ++iz)
@@ -1205,6 +1375,11 @@ ORC_GROUP(f32, float, float)
 ORC_GROUP(f64, double, double)
 ORC_GROUP(f32d, float, F32D)
 ORC_GROUP(f32i, float, F32I)
+
+// FP64 near-surface band of the f32i arithmetic: k = 0 the library's default (24 rows where nz - cpml - 4 >= 24),
+// k < 0 none, k > 4 rows 4..k-1; and the rows it gives a grid.
+void orc_set_surface_rows(int k) { g_band_rows = k; }
+int orc_surface_rows(int nz, int cpml) { return band_rows(nz, cpml); }
 
 // One checkpoint field rounded as a 16-bit checkpoint stores it (fmt 1 BF16, 2 FP16; scale from the
 // field's max-abs).

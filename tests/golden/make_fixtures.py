@@ -15,7 +15,8 @@ Contents (golden_small64.npz, read with numpy.load(allow_pickle=False)):
   sq_init_{vh,vp,vn}        pre_variables of the initial model (float32)
   obs                       observed gather: the FP64 forward of the true model, rounded to float32 [nt, nrec]
   gather_{mix,f32i,f64}     forward gathers of the true model in the reference's mixed, FP32 increment and
-                            FP64 arithmetic
+                            FP64 arithmetic (f32i: increment-form leapfrog in the forward and the exact adjoint,
+                            rows 4..23 of the forward in FP64 -- the library's default)
   grad_exact_{f32i,f64}     exact gradient (v_hor, v_po, v_nmo) [3, nz, nx] at the initial model against obs
                             (float32 and float64: each mode's own storage type)
   grad_ref_mix              reference-compatible gradient in the reference's arithmetic (float32)
@@ -107,7 +108,8 @@ def compute():
     meta = {"seed": SEED, "nx": w.nx, "nz": w.nz, "dh": w.dh, "dt": w.dt, "nt": w.nt, "f": w.f, "cpml": w.cpml,
             "shot": {"isx": sh.isx, "isz": sh.isz, "irz": sh.irz, "ir0": sh.ir0, "nrec": sh.nrec},
             "sd_alpha": SD_ALPHA, "lbfgs_trial_alpha": 1.0, "lbfgs_memory": 5,
-            "obs": "FP64 oracle forward of the true model, rounded to float32"}
+            "obs": "FP64 oracle forward of the true model, rounded to float32",
+            "f32i_surface_rows": int(O.lib().orc_surface_rows(w.nz, w.cpml))}
     return out, meta
 
 

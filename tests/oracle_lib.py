@@ -80,8 +80,28 @@ def _declare(L):
     L.orc_group_gradient_ck_f32i.restype = None
     L.orc_ck_round.argtypes = [_F, ctypes.c_long, _i]
     L.orc_ck_round.restype = None
+    L.orc_set_surface_rows.argtypes = [_i]
+    L.orc_set_surface_rows.restype = None
+    L.orc_surface_rows.argtypes = [_i, _i]
+    L.orc_surface_rows.restype = _i
     L.orc_step_fwd_f64.argtypes = [_D] * 5 + [_i, _i, _i, _d, _d, _D, _D]
     L.orc_step_adj_f64.argtypes = [_D] * 5 + [_i, _i, _i, _d, _d, _D, _D]
+
+
+class surface_rows:
+    """Within the block, the f32i arithmetic's FP64 near-surface band is k rows (0: the library's default, -1:
+    none), as a plan's surface_rows option sets it; the default is restored on exit."""
+
+    def __init__(self, k: int):
+        self.k = k
+
+    def __enter__(self):
+        lib().orc_set_surface_rows(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_surface_rows(0)
+        return False
 
 
 def ptr(a: np.ndarray):

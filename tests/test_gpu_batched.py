@@ -142,6 +142,86 @@ def test_tiling_invariance(torch):
                 assert np.array_equal(out[3], ref[3]), (exact, rows, order)
 
 
+def test_shared_model_tiles_invariance(torch):
+    """Item order 4 (four shots with the same window per workgroup, the model rows of interior tiles shared
+    through LDS) changes only the schedule: on eight full-width shots its gathers, misfit and gradient equal
+    those of order 1 bit for bit, and the gradient equals the oracle's."""
+    w = small_workload(nx=200, nz=72, nt=90, cpml=12, nshots=8)
+    say("shared-model tiles: order 4 vs order 1, 8 full-width shots")
+    tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+    obs_np = oracle_gathers(w, "f32i", sq_true, tab, src)
+    obs = torch.from_numpy(_cat(obs_np)).cuda()
+    model = _dev_model(torch, sq_init)
+    outs = []
+    for order in (1, 4):
+        plan = _plan(w, tab, src, rows_per_item=16, item_order=order, ckpt_interval=9, store_segments=1)
+        try:
+            g = plan.forward(_dev_model(torch, sq_true))
+            phi, G, syn = plan.gradient(model, obs, want_syn=True)
+            torch.cuda.synchronize()
+            outs.append((g.cpu().numpy(), phi, G.cpu().numpy(), syn.cpu().numpy()))
+        finally:
+            plan.close()
+    a, b = outs
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    assert np.array_equal(b[0], _cat(obs_np))
+    phi_ref, G_ref, _ = oracle_gradient(w, "f32i", True, sq_init, tab, src, obs_np)
+    for k in range(3):
+        assert np.array_equal(b[2][k], G_ref[k])
+
+
+@pytest.mark.parametrize("rows", [-1, 6, 7, 30])
+def test_surface_band_matches_oracle(torch, rows):
+    """The FP64 near-surface band of the increment form at several depths -- none, ending on the source and
+    receiver row 6 (both in the float rows), just below it (both in the band), deep -- gives the oracle's
+    gathers and exact gradient bit for bit, with checkpoints every 9 steps (the band state is checkpointed and
+    recomputed), ragged windows and the side C-PML crossing the band; the reference-compatible gradient too."""
+    w = small_workload(nx=150, nz=72, nt=100, cpml=12, nshots=3)
+    w.shots = [M.Shot(x0=0, nx=110, isx=50, isz=6, irz=6, ir0=12, nrec=86),
+               M.Shot(x0=20, nx=130, isx=70, isz=6, irz=6, ir0=14, nrec=100),
+               M.Shot(x0=90, nx=60, isx=30, isz=6, irz=6, ir0=12, nrec=36)]
+    say(f"surface band rows={rows}: 3 ragged shots, K=9")
+    with O.surface_rows(rows):
+        tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+        obs = oracle_gathers(w, "f32i", sq_true, tab, src)
+        phi_ref, G_ref, syn_ref = oracle_gradient(w, "f32i", True, sq_init, tab, src, obs)
+        _, Gr_ref, _ = oracle_gradient(w, "f32i", False, sq_init, tab, src, obs)
+    obs_d = torch.from_numpy(_cat(obs)).cuda()
+    model = _dev_model(torch, sq_init)
+    for exact, ref in ((1, G_ref), (0, Gr_ref)):
+        plan = _plan(w, tab, src, ckpt_interval=9, store_segments=1, surface_rows=rows, grad_mode=exact)
+        try:
+            assert plan.surface_rows == (0 if rows < 0 else rows)
+            g = plan.forward(_dev_model(torch, sq_true))
+            phi, G, syn = plan.gradient(model, obs_d, want_syn=True)
+            torch.cuda.synchronize()
+            assert np.array_equal(g.cpu().numpy(), _cat(obs))
+            if exact:
+                assert np.array_equal(syn.cpu().numpy(), _cat(syn_ref))
+                assert abs(phi - phi_ref) <= 1e-9 * phi_ref
+            G = G.cpu().numpy()
+            for k in range(3):
+                assert np.array_equal(G[k], ref[k]), (exact, k, rel_l2(G[k], ref[k]))
+        finally:
+            plan.close()
+
+
+def test_surface_band_option_checks(torch):
+    """An explicit band needs the increment form, FP32 checkpoints and single steps, and must end above the
+    bottom C-PML; the default switches itself off where it cannot run."""
+    w = small_workload(nshots=1, nt=20)
+    tab, src, _, _ = oracle_setup(w, "f32i")
+    for kw in ({"surface_rows": 24, "step_pairs": 1}, {"surface_rows": 24, "ckpt_format": 1},
+               {"surface_rows": 24, "arith": 0}, {"surface_rows": 4}, {"surface_rows": w.nz - w.cpml - 3}):
+        with pytest.raises(RuntimeError, match="surface_rows"):
+            _plan(w, tab, src, **kw)
+    for kw, rows in (({}, 24), ({"step_pairs": 1}, 0), ({"ckpt_format": 2}, 0), ({"arith": 1}, 0)):
+        plan = _plan(w, tab, src, **kw)
+        assert plan.surface_rows == rows, kw
+        plan.close()
+
+
 def test_checkpoint_interval_and_batching_invariance(torch):
     """Recomputation from checkpoints replays the same kernels: every interval K (including storing all
     steps) gives bit-identical gradients, in the two-level and the increment form; batch splits and group
@@ -356,7 +436,8 @@ def test_step_pairs_identical_to_single_steps(torch):
     for exact in (1, 0):
         res = []
         for pairs in (1, 0):
-            plan = _plan(w, tab, src, ckpt_interval=7, store_segments=1, grad_mode=exact, step_pairs=pairs)
+            plan = _plan(w, tab, src, ckpt_interval=7, store_segments=1, grad_mode=exact, step_pairs=pairs,
+                         surface_rows=-1)  # step pairs run without the FP64 band
             assert (plan.pair_rows > 0) == bool(pairs)
             g = plan.forward(model)
             phi, G, syn = plan.gradient(model, obs, want_syn=True)

@@ -80,3 +80,37 @@ def test_graph_cache_eviction():
     print("graph captures, replays:", captures, replays)
     # one batch: 11 distinct keys per round and a cache of 8, visited in order, so every call recaptures
     assert captures == 22 and replays == 0
+
+
+def test_long_capture_is_cut_into_bounded_graphs():
+    """A gradient of 3000 steps (3000 forward, ~2900 recomputed, 3000 adjoint steps) is captured as consecutive
+    graphs of at most 4096 steps each, replayed in order: three replays with the model changed in place in
+    between equal the eager plan bit for bit, and no graph holds more than 4096 steps' kernels."""
+    import torch
+
+    from fwi_amd.survey import Plan
+
+    w = small_workload(nshots=2, nt=3000, nx=64, nz=48, cpml=10)
+    say("long capture: 3000 steps, graphs of at most 4096 steps, 3 replays vs eager")
+    tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+    obs = torch.from_numpy(np.concatenate([o.reshape(-1) for o in oracle_gathers(w, "f32i", sq_true, tab, src)])).cuda()
+    mk = lambda g: Plan(w.nx, w.nz, w.nt, w.dh, w.dt, w.cpml, w.shots, src, tab, use_graph=g, store_segments=1)
+    pg, pe = mk(1), mk(0)
+    model = [torch.from_numpy(np.ascontiguousarray(x, np.float32)).cuda() for x in sq_init]
+    for rep in range(3):
+        phi_g, g_g, _ = pg.gradient(model, obs)
+        g_g = g_g.clone()
+        torch.cuda.synchronize()
+        phi_e, g_e, _ = pe.gradient(model, obs)
+        torch.cuda.synchronize()
+        assert phi_g == phi_e and torch.equal(g_g, g_e), rep
+        with torch.no_grad():
+            for t in model:
+                t.mul_(1.0 + 1e-3 * (rep + 1))
+        torch.cuda.synchronize()
+    largest, total = pg.graph_nodes()
+    steps = 3000 + (3000 - pg.kept_steps) + 3000
+    print(f"graph nodes: largest {largest}, capture total {total} for {steps} steps")
+    assert pg.graph_stats() == (1, 2)
+    assert total > largest  # the capture was cut
+    assert largest <= 4 * 4096 + 64  # 2-4 kernels per step, plus the checkpoint copies of a graph
