@@ -91,6 +91,7 @@ BATCHED = {
     "fwi_lbfgs_pairs": ([vp_], c_int),
     "fwi_model_clamp": ([vp_, vp_, vp_, c_ll, ctypes.c_float, ctypes.c_float, vp_], c_int),
     "fwi_dot3": ([vp_, vp_, c_ll, ctypes.POINTER(c_double), vp_], c_int),
+    "fwi_dot3_noalloc": ([vp_, vp_, c_ll, vp_, vp_], c_int),
     "fwi_plan_set_timing": ([vp_, c_int], None),
     "fwi_plan_kernel_time": ([vp_, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll)], c_int),
     "fwi_plan_cells": ([vp_, ctypes.POINTER(c_ll)], c_ll),

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -127,6 +128,40 @@ __global__ void __launch_bounds__(256) k_clamp3(float* __restrict__ vh, float* _
   vh[i] = h < m ? m : h;
   vp[i] = p;
   vn[i] = m;
+}
+
+// Scratch of fwi_dot3_noalloc: the 512 x 3 FP64 partial sums of k_dot3 (12 KiB), in a file-scope device array that
+// every GPU the library runs on holds until the process ends.  Its host-side name is not a device address: the
+// address on each device comes from hipGetSymbolAddress, taken once per device with that device current.
+constexpr int kScratchDoubles = kDotBlocks * 3;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr long long kNoAllocMaxN = 1LL << 40;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, resolved on the first call on that device.
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess && (dev < 0 || dev >= kMaxDevices)) {
+    set_message("fwi_dot3_noalloc: device index beyond the library's table of 64 devices");
+    return 1;
+  }
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void* p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e == hipSuccess) g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+  }
+  if (e != hipSuccess) {
+    set_message(std::string("fwi_dot3_noalloc: scratch address: ") + hipGetErrorString(e));
+    return (int)e;
+  }
+  return 0;
 }
 
 }  // namespace
@@ -358,6 +393,27 @@ int fwi_dot3(const double* a, const double* b, long long n, double* out, void* s
   if (e == hipSuccess) e = es;
   if (e != hipSuccess) {
     set_message(std::string("fwi_dot3: ") + hipGetErrorString(e));
+    return (int)e;
+  }
+  return 0;
+}
+
+int fwi_dot3_noalloc(const double* a, const double* b, long long n, double* out, void* stream) {
+  if (check_host("fwi_dot3_noalloc", {{a, "a"}, {b, "b"}, {out, "out"}})) return 1;
+  if (n < 0 || n > kNoAllocMaxN) {
+    set_message("fwi_dot3_noalloc: n must be in [0, 2^40] (the blocks are 3 x n doubles)");
+    return 1;
+  }
+  if (check_arrays("fwi_dot3_noalloc", {{a, "a"}, {b, "b"}, {out, "out"}})) return 1;
+  double* scr = nullptr;
+  if (int r = noalloc_scratch(&scr)) return r;
+  hipStream_t st = (hipStream_t)stream;
+  // every block writes its three partial sums and the reduction writes out: nothing needs clearing
+  hipLaunchKernelGGL(k_dot3, dim3(kDotBlocks), dim3(256), 0, st, a, b, n, scr);
+  hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, st, scr, kDotBlocks, out);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_message(std::string("fwi_dot3_noalloc: ") + hipGetErrorString(e));
     return (int)e;
   }
   return 0;

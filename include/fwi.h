@@ -177,6 +177,13 @@ int fwi_lbfgs_pairs(const fwi_lbfgs* h);
 int fwi_model_clamp(float* v_hor, float* v_po, float* v_nmo, long long n, float lo, float hi, void* stream);
 /* Deterministic FP64 dot products of three blocks of n: out (host double[3]). */
 int fwi_dot3(const double* a, const double* b, long long n, double* out, void* stream);
+/* fwi_dot3 without any allocation -- no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  out is device or
+ * pinned host memory (double[3]), written by the stream's last kernel; the call does not synchronize.  The same
+ * block partition as fwi_dot3, so the same bits.  Scratch: a fixed 12 KiB device array per GPU (512 x 3 partial
+ * sums, enough for any n), held until the process ends and shared by every call on that GPU: calls that overlap
+ * on one GPU (other streams or threads) race on it.  n is refused beyond 2^40. */
+int fwi_dot3_noalloc(const double* a, const double* b, long long n, double* out, void* stream);
 
 /* Instrumentation (no counterpart in the reference, whose only timer is clock() around whole phases,
  * main.c:323,368-372).  With stride > 0, every stride-th launch of each fused step kernel is bracketed by
