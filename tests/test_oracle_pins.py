@@ -677,6 +677,27 @@ def test_group_gradient_equals_per_shot_sum():
     assert mis[0] == ph and np.array_equal(gh, h1) and np.array_equal(gp, p1_) and np.array_equal(gn, n1)
 
 
+def test_group_fp32_sum_is_a_small_share_of_the_fp32_error():
+    """The gradient groups add four shots' imaging terms in FP32 before the FP32 slot update (the cross-shot sum over
+    slots is FP64).  Against one FP32 accumulator per shot summed in FP64 this moves the gradient by less than 5 % of
+    the f32i gradient's own distance to FP64 -- the in-group FP32 sum is not what limits the accuracy."""
+    from helpers import oracle_gathers, oracle_gradient, oracle_setup, rel_l2, small_workload
+
+    w = small_workload(nshots=4, nt=300, f=12.0)
+    tab64, src, st64, si64 = oracle_setup(w, "f64")
+    obs64 = oracle_gathers(w, "f64", st64, tab64, src.astype(np.float64))
+    _, G64, _ = oracle_gradient(w, "f64", True, si64, tab64, src.astype(np.float64), obs64)
+    tab, src32, _, si = oracle_setup(w, "f32i")
+    obs = [o.astype(np.float32) for o in obs64]
+    _, G1, _ = oracle_gradient(w, "f32i", True, si, tab, src32, obs, group=1)
+    _, G4, _ = oracle_gradient(w, "f32i", True, si, tab, src32, obs, group=4)
+    for k in range(3):
+        e_group = rel_l2(G4[k], G1[k])
+        e_fp32 = rel_l2(G1[k], G64[k])
+        print(f"grad[{k}]: in-group FP32 sum {e_group:.2e}, FP32 against FP64 {e_fp32:.2e}")
+        assert e_group < 0.05 * e_fp32
+
+
 def test_16bit_checkpoint_rounding_matches_torch_and_numpy():
     """The oracle's 16-bit checkpoint rounding (exact power-of-two scale from the max-abs, then round to
     nearest even) equals torch's float32 -> bfloat16 and numpy's float32 -> float16 conversions of the scaled
