@@ -220,7 +220,7 @@ STEP_KERNELS = [
     (1, "k_fwd_main<1>", "forward step storing the imaging terms (forward's last segments and recomputation)"),
     (2, "k_adj_ex_main", "exact adjoint step with imaging"),
     (7, "k_fwd_band", "FP64 near-surface band of every forward-type step"),
-    (5, "k_fwd_strip", "forward C-PML memory variables psi"),
+    (5, "k_fwd_strip", "forward C-PML memory variables psi (with the FP64 band in one launch when the plan has it)"),
     (6, "k_adj_ex_strip1+2", "adjoint C-PML memory variables (two kernels)"),
 ]
 
@@ -243,8 +243,9 @@ def kernel_bytes(kind, N, Nb, Nc, Nm, grouped, inc):
     elif kind == 7:
         survey = 24.0 * Nb
         design = (48.0 + 8.0 + 12.0) * Nb
-    elif kind == 5:
-        survey = design = 16.0 * Nc
+    elif kind == 5:  # with the band's cells when they share the launch (Nb > 0)
+        survey = 16.0 * Nc + 24.0 * Nb
+        design = 16.0 * Nc + (48.0 + 8.0 + 12.0) * Nb
     elif kind == 6:
         survey = design = 32.0 * Nc
     else:
@@ -254,7 +255,7 @@ def kernel_bytes(kind, N, Nb, Nc, Nm, grouped, inc):
 
 
 def launches_per_gradient(kind, nt, kept, band):
-    return {0: nt - kept, 1: nt, 2: nt, 5: 2 * nt - kept, 6: nt, 7: (2 * nt - kept) if band else 0}.get(kind, 0)
+    return {0: nt - kept, 1: nt, 2: nt, 5: 2 * nt - kept, 6: nt, 7: 2 * nt - kept}.get(kind, 0)
 
 
 def load_traffic(w, nshots):
@@ -276,7 +277,8 @@ def load_traffic(w, nshots):
 
 TRAFFIC_KEYS = {"k_fwd_main<0>": ("k_fwd_main<0,",), "k_fwd_main<1>": ("k_fwd_main<1,",),
                 "k_adj_ex_main": ("k_adj_ex_main<",), "k_fwd_band": ("k_fwd_band<1>",),
-                "k_fwd_strip": ("k_fwd_strip<",), "k_adj_ex_strip1+2": ("k_adj_ex_strip1", "k_adj_ex_strip2<")}
+                "k_fwd_strip": ("k_fwd_strip<",), "k_fwd_strip_band": ("k_fwd_strip_band<",),
+                "k_adj_ex_strip1+2": ("k_adj_ex_strip1", "k_adj_ex_strip2<")}
 
 
 def traffic_of(traffic, name):
@@ -376,7 +378,12 @@ def main():
             if not n:
                 continue
             mean = ms / n / 1e3
-            survey, design = kernel_bytes(kind, ncell, nband, nstrip, nmodel, grouped, inc)
+            merged = kind == 5 and nband > 0  # the strip kernel and the band share one launch
+            if merged:
+                name = "k_fwd_strip_band"
+                what = "forward C-PML memory variables psi and the FP64 near-surface band, one launch"
+            nb = nband if (kind in (0, 1, 7) or merged) else 0
+            survey, design = kernel_bytes(kind, ncell, nb, nstrip, nmodel, grouped, inc)
             per_grad = launches_per_gradient(kind, nt, plan.kept_steps, band)
             tb = traffic_of(traffic, name)
             k = {"kernel": name, "does": what, "launches_per_gradient": per_grad, "mean_launch_us": round(mean * 1e6, 1),

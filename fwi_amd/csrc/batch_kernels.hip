@@ -28,10 +28,10 @@ namespace fwi {
 // psi_px, psi_fz on the C-PML cells (parallelcalls.cu:48-63).  Cells of the side strips carry
 // psi_px; cells of the bottom strip carry psi_fz (the other variable keeps a = b = 0 there and stays 0).
 template <class S>
-__global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __restrict__ Pc,
-                                                   const float* __restrict__ Fc, float* __restrict__ psx,
-                                                   float* __restrict__ psz) {
-  const int2 blk = a.blocks[blockIdx.x];
+__device__ __forceinline__ void fwd_strip_block(const StripArgs& a, int block, const float* __restrict__ Pc,
+                                                const float* __restrict__ Fc, float* __restrict__ psx,
+                                                float* __restrict__ psz) {
+  const int2 blk = a.blocks[block];
   const ShotDev sd = a.g.shots[blk.x];
   const int k = blk.y + threadIdx.x;
   if (k >= sd.nstrip) return;
@@ -47,6 +47,12 @@ __global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __r
     const float t = d1_at<S>(Fc, i, ps, a.g.dh);
     psz[i] = coef_z(a.g.tb, iz, nz, cpml) * psz[i] + coef_z(a.g.ta, iz, nz, cpml) * t;
   }
+}
+template <class S>
+__global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __restrict__ Pc,
+                                                   const float* __restrict__ Fc, float* __restrict__ psx,
+                                                   float* __restrict__ psz) {
+  fwd_strip_block<S>(a, blockIdx.x, Pc, Fc, psx, psz);
 }
 
 // The same step on an interior work item: a band within [cpml, nx-cpml) and rows above nz-cpml, where no cell
@@ -336,12 +342,12 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
 // from the same inputs and store the same bits -- then, after a workgroup barrier, its own cells.  (A tiled variant
 // staging 64 x 20 cells of P, F and psi in LDS was slower: 46.9 against 41.2 us per step on survey1600.)
 template <int QM>
-__global__ void __launch_bounds__(256) k_fwd_band(BandArgs a) {
-  const ShotDev sd = a.g.shots[blockIdx.z];
+__device__ __forceinline__ void fwd_band_block(const BandArgs& a, int chunk, int brow, int shot) {
+  const ShotDev sd = a.g.shots[shot];
   const int nx = sd.nx, ps = sd.ps, cpml = a.g.cpml, nxm = a.m.nxm, B = a.band;
-  const int x0 = blockIdx.x * 256;
+  const int x0 = chunk * 256;
   if (x0 >= nx - kRim) return;  // the whole workgroup: past this shot's window
-  const int z = kRim + blockIdx.y;
+  const int z = kRim + brow;
   const double dh = (double)a.g.dh, dh2 = dh * dh, Cd = (a.dt * a.dt) / dh2;
   const double* P = a.P + sd.boff;
   double* Pm = a.Pm + sd.boff;
@@ -409,6 +415,28 @@ __global__ void __launch_bounds__(256) k_fwd_band(BandArgs a) {
   }
   if (a.gather && z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec)
     a.gather[sd.goff + (long)a.it * sd.nrec + (ix - sd.ir0)] = (float)(pn + fn);
+}
+template <int QM>
+__global__ void __launch_bounds__(256) k_fwd_band(BandArgs a) {
+  fwd_band_block<QM>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// The forward C-PML strip kernel and the FP64 band in one launch: they read step n and write disjoint arrays, and
+// both are small and latency-bound, so their workgroups fill each other's gaps.  Blocks [0, s.nblocks) are strip
+// blocks, the rest band workgroups in (chunk, band row, shot) order with nbc chunks per row.
+template <class S, int QM>
+__global__ void __launch_bounds__(256) k_fwd_strip_band(StripArgs s, BandArgs a, int nbc, const float* __restrict__ Pc,
+                                                        const float* __restrict__ Fc, float* __restrict__ psx,
+                                                        float* __restrict__ psz) {
+  if ((int)blockIdx.x < s.nblocks) {
+    fwd_strip_block<S>(s, blockIdx.x, Pc, Fc, psx, psz);
+    return;
+  }
+  int b = (int)blockIdx.x - s.nblocks;
+  const int rows = a.band - kRim;
+  const int chunk = b % nbc;
+  b /= nbc;
+  fwd_band_block<QM>(a, chunk, b % rows, b / rows);
 }
 
 // ---------------------------------------------------------------------------------------------

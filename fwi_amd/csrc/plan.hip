@@ -120,9 +120,9 @@ struct fwi_plan {
   int max_count = 0;                // shots in the largest batch
   unsigned* d_ck_amax = nullptr;    // 16-bit checkpoints: max-abs bits per (checkpoint, shot, field)
   // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute),
-  // 2 = adjoint step, 3 / 4 = fused core step pair without / with store, 5 = forward C-PML strip, 6 = the two
-  // adjoint C-PML strip kernels, 7 = FP64 near-surface band; every `timing_stride`-th launch of a kind is
-  // bracketed by events on its stream
+  // 2 = adjoint step, 3 / 4 = fused core step pair without / with store, 5 = forward C-PML strip (with the FP64
+  // near-surface band when both run), 6 = the two adjoint C-PML strip kernels, 7 = FP64 near-surface band alone;
+  // every `timing_stride`-th launch of a kind is bracketed by events on its stream
   static constexpr int kKinds = 8;
   int timing_stride = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -698,7 +698,8 @@ struct fwi_plan {
     return 0;
   }
 
-  // the FP64 near-surface band of forward step n (rows [4, band) of every shot)
+  // the FP64 near-surface band of forward step n (rows [4, band) of every shot), launched with the C-PML strip
+  // kernel of the same step when the batch has strip cells (one launch, timing kind 5)
   int fwd_band(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
                hipStream_t st) {
     if (band <= kRim) return 0;
@@ -725,7 +726,25 @@ struct fwi_plan {
     a.qb = qb;
     int maxnx = 0;
     for (const ShotDev& d : b.h_shots) maxnx = std::max(maxnx, d.nx);
-    const dim3 grid((maxnx + 255) / 256, band - kRim, b.count);
+    const int nbc = (maxnx + 255) / 256;
+    if (b.nsblocks > 0) {
+      StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
+      const long long nblk = (long long)b.nsblocks + (long long)nbc * (band - kRim) * b.count;
+      const int tev = timed_begin(5, st);
+      if (qm == 0)
+        hipLaunchKernelGGL((k_fwd_strip_band<float, 0>), dim3((unsigned)nblk), dim3(256), 0, st, sa, a, nbc, fP(n), fF(n),
+                           fpsx(), fpsz());
+      else if (qm == 1)
+        hipLaunchKernelGGL((k_fwd_strip_band<float, 1>), dim3((unsigned)nblk), dim3(256), 0, st, sa, a, nbc, fP(n), fF(n),
+                           fpsx(), fpsz());
+      else
+        hipLaunchKernelGGL((k_fwd_strip_band<float, 2>), dim3((unsigned)nblk), dim3(256), 0, st, sa, a, nbc, fP(n), fF(n),
+                           fpsx(), fpsz());
+      timed_end(tev, st);
+      CK(hipGetLastError());
+      return 0;
+    }
+    const dim3 grid(nbc, band - kRim, b.count);
     const int tev = timed_begin(7, st);
     if (qm == 0)
       hipLaunchKernelGGL(k_fwd_band<0>, grid, dim3(256), 0, st, a);
@@ -741,8 +760,11 @@ struct fwi_plan {
   // one forward step of a batch
   int fwd_step(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
                hipStream_t st) {
-    if (int r = fwd_strip(b, m, n, st)) return r;
-    if (int r = fwd_band(b, m, n, qm, qa, qb, gather, st)) return r;
+    if (band > kRim) {
+      if (int r = fwd_band(b, m, n, qm, qa, qb, gather, st)) return r;  // with the strip kernel
+    } else if (int r = fwd_strip(b, m, n, st)) {
+      return r;
+    }
     return fwd_main(b, m, n, qm, qa, qb, gather, b.d_items, b.nitems, nullptr, qm == 0 ? 0 : 1, st);
   }
 

@@ -189,8 +189,9 @@ int fwi_dot3_noalloc(const double* a, const double* b, long long n, double* out,
  * main.c:323,368-372).  With stride > 0, every stride-th launch of each fused step kernel is bracketed by
  * HIP events on the stream it runs on; fwi_plan_kernel_time returns the summed event time and the
  * number of timed launches of a kind (0 forward step, 1 forward step storing imaging terms, 2 adjoint
- * step, 3 / 4 fused core step pair without / with stored imaging terms, 5 forward C-PML strip, 6 the two
- * adjoint C-PML strip kernels together, 7 FP64 near-surface band) since the last fwi_plan_set_timing call. */
+ * step, 3 / 4 fused core step pair without / with stored imaging terms, 5 forward C-PML strip -- one launch with
+ * the FP64 near-surface band when the plan has both --, 6 the two adjoint C-PML strip kernels together, 7 FP64
+ * near-surface band alone) since the last fwi_plan_set_timing call. */
 void fwi_plan_set_timing(fwi_plan* plan, int stride);
 int fwi_plan_kernel_time(fwi_plan* plan, int kind, double* total_ms, long long* count);
 /* Cells of all shot windows (nx_s*nz summed over shots) and, in *strip_cells, their C-PML cells. */
