@@ -12,8 +12,10 @@ Beside it, on the same workload:
   * roofline of every step kernel and of the dominant one (the largest share of the gradient's kernel time):
     algorithmic bytes per launch (the survey's count, and this design's minimum) over its average duration,
     measured with HIP events on the stream the kernel runs on during an instrumented gradient, beside the
-    counter-measured HBM bytes of the same kernel when profiles/ holds them for these kernel sources;
-  * the same gradient in the reference's arithmetic (FP64 stencil sums), timed apart;
+    counter-measured HBM bytes of the same kernel when profiles/ holds them for these kernel sources (the run
+    then uses the work-item tiling those counters were measured with, unless --retune or a tiling is given);
+  * the same gradient in the reference's arithmetic (FP64 stencil sums), and without the FP64 near-surface
+    band, each timed apart;
   * cpu_baseline: the CPU oracle (f32i, exact gradient, OpenMP) on one shot for a bounded number of steps, and
     the reference's arithmetic and gradient on one thread, with the CPU model.
   * the device optimizer of one iteration (steepest-descent step, L-BFGS direction, trial), timed apart.
@@ -57,10 +59,15 @@ def parse():
                     help="0 FP32 two-level, 1 mixed (reference), 2 FP32 difference stencil, 3 FP32 increment form")
     ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")
     ap.add_argument("--order", type=int, default=-1, help="work-item order (-1 = tuned)")
+    ap.add_argument("--adj-rows", type=int, default=0, help="rows per adjoint work item (0 = --rows or tuned)")
+    ap.add_argument("--adj-order", type=int, default=-1, help="adjoint work-item order (-1 = from --order or tuned)")
+    ap.add_argument("--retune", action="store_true",
+                    help="let the tuner choose the tiling even where the counter profile names one")
     ap.add_argument("--plain-steps", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=4000)
     ap.add_argument("--cpu1-steps", type=int, default=200, help="steps of the single-thread reference-arithmetic leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the reference-arithmetic (mixed) GPU gradient")
+    ap.add_argument("--no-band-leg", action="store_true", help="skip the gradient timed without the FP64 band")
     ap.add_argument("--surface-rows", type=int, default=0, help="FP64 near-surface band rows (0 default, -1 none)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -258,9 +265,10 @@ def launches_per_gradient(kind, nt, kept, band):
     return {0: nt - kept, 1: nt, 2: nt, 5: 2 * nt - kept, 6: nt, 7: 2 * nt - kept}.get(kind, 0)
 
 
-def load_traffic(w, nshots):
+def load_traffic(w, nshots, tiling):
     """Counter-measured HBM bytes per launch of each kernel (profiles/fused_steps_survey1600_hbm_traffic.json), if it
-    was measured on this workload with the kernel sources in this tree (its source hash); else None and why."""
+    was measured on this workload with the kernel sources in this tree (its source hash) and the work-item tiling
+    this plan runs (the tiling decides the halo and model re-reads); else None and why."""
     from fwi_amd.pmc import source_hash
 
     tfile = os.path.join(ROOT, "profiles", "fused_steps_survey1600_hbm_traffic.json")
@@ -272,7 +280,30 @@ def load_traffic(w, nshots):
         t = json.load(fh)
     if t.get("source_hash") != source_hash():
         return None, f"counter profile is of other kernel sources ({t.get('source_hash')} vs {source_hash()})"
+    if tiling is not None and t.get("tiling") != tiling:
+        return None, f"counter profile is of another work-item tiling ({t.get('tiling')} vs {tiling})"
     return t["kernels"], "profiles/fused_steps_survey1600_hbm_traffic.json (source hash " + t["source_hash"] + ")"
+
+
+def profiled_tiling(w, args):
+    """The work-item tiling of the committed counter profile, when this run measures its workload with its kernel
+    sources and no tiling was asked for: the bench then runs that tiling (the tuner's candidates near the top
+    time within about 1 % of each other, so its own choice varies from run to run) and the counters describe the
+    kernels it times."""
+    if (args.rows, args.order, args.adj_rows, args.adj_order) != (0, -1, 0, -1) or args.retune:
+        return None
+    t, _ = load_traffic(w, len(w.shots), None)
+    if t is None:
+        return None
+    with open(os.path.join(ROOT, "profiles", "fused_steps_survey1600_hbm_traffic.json")) as fh:
+        return json.load(fh).get("tiling")
+
+
+def tiling_of(plan):
+    """The plan's work-item tiling as the bench line reports it."""
+    r, o, (ar, ao, g) = plan.rows_per_item, plan.item_order, plan.adj_tiling
+    return {"rows_per_item": r, "item_order": o,
+            "adjoint_tiling": {"rows_per_item": ar, "item_order": ao, "grouped": bool(g)}}
 
 
 TRAFFIC_KEYS = {"k_fwd_main<0>": ("k_fwd_main<0,",), "k_fwd_main<1>": ("k_fwd_main<1,",),
@@ -321,8 +352,15 @@ def main():
     true_sq = pre_variables(*(dev(x) for x in w.true_model))
     init_sq = pre_variables(*(dev(x) for x in w.init_model))
     grad_mode = 1 if args.mode == "exact" else 0
+    pinned = profiled_tiling(w, args)
+    if pinned:
+        args.rows, args.order = pinned["rows_per_item"], pinned["item_order"]
+        args.adj_rows, args.adj_order = pinned["adjoint_tiling"]["rows_per_item"], pinned["adjoint_tiling"]["item_order"]
+        log(f"tiling of the counter profile: forward {args.rows} rows order {args.order}, adjoint {args.adj_rows} rows "
+            f"order {args.adj_order}")
     plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
-                rows_per_item=args.rows, item_order=args.order, arith=args.arith,
+                rows_per_item=args.rows, item_order=args.order, adj_rows_per_item=args.adj_rows,
+                adj_item_order=args.adj_order, arith=args.arith,
                 ckpt_format=args.ckpt_format, use_graph=0 if args.no_graph else 1, store_segments=args.keep,
                 grad_group=args.grad_group, step_pairs=1 if args.pairs else 0, surface_rows=args.surface_rows)
     log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, {plan.kept_steps} steps kept from the forward, "
@@ -372,7 +410,7 @@ def main():
         nband = sum((band - 4) * s.nx for s in w.shots) if band else 0
         grouped = bool(plan.adj_tiling[2])
         inc = args.arith == 3
-        traffic, tnote = load_traffic(w, len(w.shots))
+        traffic, tnote = load_traffic(w, len(w.shots), tiling_of(plan))
         for kind, name, what in STEP_KERNELS:
             ms, n = plan.kernel_time(kind)
             if not n:
@@ -406,7 +444,7 @@ def main():
         f"trial {opt_ms['trial_ms']} ms")
     surface_rows = plan.surface_rows
     kept, K, wsb, bsz = plan.kept_steps, plan.ckpt_interval, plan.workspace_bytes, plan.batch_size
-    tiling = (plan.rows_per_item, plan.item_order, plan.adj_tiling)
+    tiling = tiling_of(plan)
     plan.close()
     del plan, grad
     torch.cuda.empty_cache()
@@ -425,6 +463,23 @@ def main():
         finally:
             pm.close()
         log(f"reference arithmetic (mixed: FP64 stencil sums), same path: {sec_mixed:.3f} s per gradient")
+
+    # what the FP64 near-surface band costs: the same gradient with every row in FP32
+    sec_noband = None
+    if surface_rows > 0 and not args.no_band_leg:
+        pb = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, arith=args.arith,
+                  ckpt_interval=args.ckpt, ckpt_format=args.ckpt_format, grad_group=args.grad_group,
+                  surface_rows=-1)
+        try:
+            pb.gradient(init_sq, obs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pb.gradient(init_sq, obs)
+            torch.cuda.synchronize()
+            sec_noband = time.perf_counter() - t0
+        finally:
+            pb.close()
+        log(f"without the FP64 surface band: {sec_noband:.3f} s per gradient")
 
     plain = None
     if not args.no_plain:
@@ -480,12 +535,12 @@ def main():
             "ckpt_format": {0: "fp32", 1: "bf16-scaled", 2: "fp16-scaled"}[args.ckpt_format],
             "workspace_gb": round(wsb / 1e9, 2),
             "batch_shots": bsz,
-            "rows_per_item": tiling[0],
-            "item_order": tiling[1],
-            "adjoint_tiling": dict(zip(("rows_per_item", "item_order", "grouped"), tiling[2])),
+            **tiling,
+            "tiling_from": "counter profile" if pinned else ("options" if (args.rows or args.order >= 0 or args.adj_rows or args.adj_order >= 0) else "tuner"),
             "s_per_gradient_batched": round(sec, 4),
             "s_per_gradient_median": round(float(np.median(times)), 4),
             "s_per_gradient_reference_arith": round(sec_mixed, 4) if sec_mixed else None,
+            "s_per_gradient_without_surface_band": round(sec_noband, 4) if sec_noband else None,
             "optimizer_ms": opt_ms,
             "graphs": {"captures": graphs[0], "replays": graphs[1]},
             "s_per_gradient_eager_instrumented": round(sec_instr, 4) if sec_instr else None,

@@ -37,7 +37,8 @@ class FwiOptions(ctypes.Structure):
     _fields_ = [("grad_mode", c_int), ("arith", c_int), ("ckpt_interval", c_int), ("max_batch", c_int),
                 ("mem_budget", c_double), ("rows_per_item", c_int), ("item_order", c_int),
                 ("grad_group", c_int), ("ckpt_format", c_int), ("store_segments", c_int),
-                ("use_graph", c_int), ("step_pairs", c_int), ("surface_rows", c_int)]
+                ("use_graph", c_int), ("step_pairs", c_int), ("surface_rows", c_int),
+                ("adj_rows_per_item", c_int), ("adj_item_order", c_int)]
 
 
 # Every exported symbol with its signature (argument types, return type).

@@ -52,7 +52,7 @@ template <class S>
 __global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __restrict__ Pc,
                                                    const float* __restrict__ Fc, float* __restrict__ psx,
                                                    float* __restrict__ psz) {
-  fwd_strip_block<S>(a, blockIdx.x, Pc, Fc, psx, psz);
+  fwd_strip_block<S>(a, xcd_logical_block(blockIdx.x, gridDim.x), Pc, Fc, psx, psz);
 }
 
 // The same step on an interior work item: a band within [cpml, nx-cpml) and rows above nz-cpml, where no cell
@@ -416,23 +416,31 @@ __device__ __forceinline__ void fwd_band_block(const BandArgs& a, int chunk, int
   if (a.gather && z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec)
     a.gather[sd.goff + (long)a.it * sd.nrec + (ix - sd.ir0)] = (float)(pn + fn);
 }
+// Grid: nbc chunks x (band - 4) rows x shots workgroups, in that order.
 template <int QM>
-__global__ void __launch_bounds__(256) k_fwd_band(BandArgs a) {
-  fwd_band_block<QM>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ void __launch_bounds__(256) k_fwd_band(BandArgs a, int nbc) {
+  int b = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int rows = a.band - kRim;
+  const int chunk = b % nbc;
+  b /= nbc;
+  fwd_band_block<QM>(a, chunk, b % rows, b / rows);
 }
 
 // The forward C-PML strip kernel and the FP64 band in one launch: they read step n and write disjoint arrays, and
 // both are small and latency-bound, so their workgroups fill each other's gaps.  Blocks [0, s.nblocks) are strip
-// blocks, the rest band workgroups in (chunk, band row, shot) order with nbc chunks per row.
+// blocks, the rest band workgroups in (chunk, band row, shot) order with nbc chunks per row.  Each part is dealt
+// to the XCDs in contiguous logical ranges of its own, so every XCD gets an eighth of the strip work and an eighth
+// of the band work, and the rows a z stencil reads come from workgroups on the same XCD (the same L2).  (One
+// contiguous range over both parts put the FP64 band on four XCDs only: 67 instead of 55 us per step.)
 template <class S, int QM>
 __global__ void __launch_bounds__(256) k_fwd_strip_band(StripArgs s, BandArgs a, int nbc, const float* __restrict__ Pc,
                                                         const float* __restrict__ Fc, float* __restrict__ psx,
                                                         float* __restrict__ psz) {
   if ((int)blockIdx.x < s.nblocks) {
-    fwd_strip_block<S>(s, blockIdx.x, Pc, Fc, psx, psz);
+    fwd_strip_block<S>(s, xcd_logical_block(blockIdx.x, s.nblocks), Pc, Fc, psx, psz);
     return;
   }
-  int b = (int)blockIdx.x - s.nblocks;
+  int b = xcd_logical_block((int)blockIdx.x - s.nblocks, (int)gridDim.x - s.nblocks);
   const int rows = a.band - kRim;
   const int chunk = b % nbc;
   b /= nbc;
@@ -595,7 +603,7 @@ __global__ void __launch_bounds__(256) k_adj_ref_strip(StripArgs a, const float*
                                                        const float* __restrict__ Mc, float* __restrict__ s1,
                                                        float* __restrict__ s2, float* __restrict__ s3,
                                                        float* __restrict__ s4) {
-  const int2 blk = a.blocks[blockIdx.x];
+  const int2 blk = a.blocks[xcd_logical_block(blockIdx.x, gridDim.x)];
   const ShotDev sd = a.g.shots[blk.x];
   const int k = blk.y + threadIdx.x;
   if (k >= sd.nstrip) return;
@@ -774,7 +782,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip1(StripArgs a, float dt2, c
                                                        float* __restrict__ zz, float* __restrict__ ztx,
                                                        float* __restrict__ ztz, float* __restrict__ gbx,
                                                        float* __restrict__ gbz) {
-  const int2 blk = a.blocks[blockIdx.x];
+  const int2 blk = a.blocks[xcd_logical_block(blockIdx.x, gridDim.x)];
   const ShotDev sd = a.g.shots[blk.x];
   const int k = blk.y + threadIdx.x;
   if (k >= sd.nstrip) return;
@@ -805,7 +813,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip1(StripArgs a, float dt2, c
 // Gamma-masked gbar of phase 1 at the 8 neighbours (zero outside the window).
 template <class S>
 __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a) {
-  const int2 blk = s.blocks[blockIdx.x];
+  const int2 blk = s.blocks[xcd_logical_block(blockIdx.x, gridDim.x)];
   const ShotDev sd = a.g.shots[blk.x];
   const int k = blk.y + threadIdx.x;
   if (k >= sd.nstrip) return;

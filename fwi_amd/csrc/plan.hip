@@ -744,14 +744,14 @@ struct fwi_plan {
       CK(hipGetLastError());
       return 0;
     }
-    const dim3 grid(nbc, band - kRim, b.count);
+    const dim3 grid((unsigned)((long long)nbc * (band - kRim) * b.count));
     const int tev = timed_begin(7, st);
     if (qm == 0)
-      hipLaunchKernelGGL(k_fwd_band<0>, grid, dim3(256), 0, st, a);
+      hipLaunchKernelGGL(k_fwd_band<0>, grid, dim3(256), 0, st, a, nbc);
     else if (qm == 1)
-      hipLaunchKernelGGL(k_fwd_band<1>, grid, dim3(256), 0, st, a);
+      hipLaunchKernelGGL(k_fwd_band<1>, grid, dim3(256), 0, st, a, nbc);
     else
-      hipLaunchKernelGGL(k_fwd_band<2>, grid, dim3(256), 0, st, a);
+      hipLaunchKernelGGL(k_fwd_band<2>, grid, dim3(256), 0, st, a, nbc);
     timed_end(tev, st);
     CK(hipGetLastError());
     return 0;
@@ -1148,7 +1148,9 @@ struct fwi_plan {
   int tune(const ModelArgs& m, hipStream_t st) {
     if (tuned) return 0;
     std::vector<int> rows_c = {16, 8, 4, 24, 32, 48, 64, 96};  // 8, 4: small grids, whose few items are latency-bound
-    if (opt.rows_per_item > 0) rows_c = {opt.rows_per_item};
+    std::vector<int> rows_a = rows_c;
+    if (opt.rows_per_item > 0) rows_c = rows_a = {opt.rows_per_item};
+    if (opt.adj_rows_per_item > 0) rows_a = {opt.adj_rows_per_item};
     Batch& b = batches[0];
     std::vector<int> order_f = {0, 1, 2, 3};
     if (!share_groups(b).empty()) order_f.push_back(4);
@@ -1157,6 +1159,19 @@ struct fwi_plan {
       order_f = {opt.item_order};
       order_a = {b.grouped && opt.item_order == 1 ? 0 : opt.item_order == 4 ? 3 : opt.item_order};
     }
+    if (opt.adj_item_order >= 0) order_a = {b.grouped && opt.adj_item_order == 1 ? 0 : opt.adj_item_order};
+    // Each candidate is timed twice over 6 steps and keeps the shorter time; the fastest candidate is taken.
+    struct Cand {
+      float ms;
+      int order, rows;
+    };
+    auto pick = [&](const std::vector<Cand>& cs, int& rows, int& ord) {
+      const Cand* sel = &cs[0];
+      for (const Cand& c : cs)
+        if (c.ms < sel->ms) sel = &c;
+      rows = sel->rows;
+      ord = sel->order;
+    };
     // the tuner times launches with events and waits for them: not possible while the caller captures `st`
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     CK(hipStreamIsCapturing(st, &cap));
@@ -1184,68 +1199,68 @@ struct fwi_plan {
       return 0;
     };
     // forward
-    float best = 1e30f;
+    std::vector<Cand> cands;
     for (int o : order_f)
       for (int c : rows_c) {
         if (c > nz - 8 && c != rows_c[0]) continue;
         if (int r = build_items(b, c, o)) return r;
         if (rows_c.size() == 1 && order_f.size() == 1) {
-          tz = c;
-          order = o;
+          cands.push_back({0.f, o, c});
           break;
         }
         if (int r = zero_fwd(b, st)) return r;
         for (int w = 0; w < 2; ++w)
           if (int r = fwd_step(b, m, w, 0, nullptr, nullptr, nullptr, st)) return r;
-        float ms = 0;
-        if (int r = timed([&]() -> int {
-              for (int n = 2; n < 8; ++n)
-                if (int r2 = fwd_step(b, m, n, 0, nullptr, nullptr, nullptr, st)) return r2;
-              return 0;
-            }, ms))
-          return r;
-        if (log) std::fprintf(stderr, "[fwi tune] forward  order %d rows %2d: %.1f us/step\n", o, c, ms * 1e3 / 6);
-        if (ms < best) {
-          best = ms;
-          tz = c;
-          order = o;
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+          float t = 0;
+          if (int r = timed([&]() -> int {
+                for (int n = 2; n < 8; ++n)
+                  if (int r2 = fwd_step(b, m, n, 0, nullptr, nullptr, nullptr, st)) return r2;
+                return 0;
+              }, t))
+            return r;
+          ms = std::min(ms, t);
         }
+        if (log) std::fprintf(stderr, "[fwi tune] forward  order %d rows %2d: %.1f us/step\n", o, c, ms * 1e3 / 6);
+        cands.push_back({ms, o, c});
       }
+    pick(cands, tz, order);
     // adjoint, imaging into zeroed scratch slots
     float* qa = qbuf();
     CK(hipMemsetAsync(qa, 0, 2 * (size_t)cap_cell * sizeof(float), st));
-    best = 1e30f;
+    cands.clear();
     for (int o : order_a)
-      for (int c : rows_c) {
-        if (c > nz - 8 && c != rows_c[0]) continue;
+      for (int c : rows_a) {
+        if (c > nz - 8 && c != rows_a[0]) continue;
         if (int r = build_adj_items(b, c, o)) return r;
-        if (rows_c.size() == 1 && order_a.size() == 1) {
-          tz_adj = c;
-          order_adj = o;
+        if (rows_a.size() == 1 && order_a.size() == 1) {
+          cands.push_back({0.f, o, c});
           break;
         }
         CK(hipMemsetAsync(adj(0), 0, kAdjF * (size_t)cap_cell * sizeof(float), st));
         for (int w = 0; w < 2; ++w)
           if (int r = adj_step(b, m, nt - w, qa, qa + cap_cell, st)) return r;
-        float ms = 0;
-        if (int r = timed([&]() -> int {
-              for (int n = 2; n < 8; ++n)
-                if (int r2 = adj_step(b, m, nt - n, qa, qa + cap_cell, st)) return r2;
-              return 0;
-            }, ms))
-          return r;
-        if (log) std::fprintf(stderr, "[fwi tune] adjoint  order %d rows %2d: %.1f us/step\n", o, c, ms * 1e3 / 6);
-        if (ms < best) {
-          best = ms;
-          tz_adj = c;
-          order_adj = o;
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+          float t = 0;
+          if (int r = timed([&]() -> int {
+                for (int n = 2; n < 8; ++n)
+                  if (int r2 = adj_step(b, m, nt - n, qa, qa + cap_cell, st)) return r2;
+                return 0;
+              }, t))
+            return r;
+          ms = std::min(ms, t);
         }
+        if (log) std::fprintf(stderr, "[fwi tune] adjoint  order %d rows %2d: %.1f us/step\n", o, c, ms * 1e3 / 6);
+        cands.push_back({ms, o, c});
       }
+    pick(cands, tz_adj, order_adj);
     // fused core step pairs: rows per item
     if (inc() && opt.step_pairs != 0) {
       std::vector<int> rows_p = {16, 32, 48, 64};
       if (opt.rows_per_item > 0) rows_p = {opt.rows_per_item};
-      best = 1e30f;
+      float best = 1e30f;
       for (int c : rows_p) {
         if (c > nz - 8 && c != rows_p[0]) continue;
         if (int r = build_pair_items(b, c)) return r;
@@ -1313,6 +1328,7 @@ void fwi_options_default(fwi_options* o) {
   o->grad_mode = FWI_GRAD_EXACT;
   o->arith = FWI_ARITH_FP32I;
   o->item_order = -1;
+  o->adj_item_order = -1;
   o->use_graph = 1;
   o->step_pairs = 0;
 }
@@ -1384,10 +1400,17 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     set_message("grad_group must be 0..4");
     return 1;
   }
+  if (p->opt.item_order < -1 || p->opt.item_order > 4 || p->opt.adj_item_order < -1 || p->opt.adj_item_order > 3 ||
+      p->opt.rows_per_item < 0 || p->opt.adj_rows_per_item < 0) {
+    delete p;
+    set_message("item_order must be -1..4, adj_item_order -1..3, rows_per_item and adj_rows_per_item >= 0");
+    return 1;
+  }
   const int nt = p->nt;
   p->K = p->opt.ckpt_interval > 0 ? std::min(p->opt.ckpt_interval, nt) : (int)std::ceil(std::sqrt((double)nt));
   p->J = (nt + p->K - 1) / p->K;
   p->tz = p->tz_adj = p->opt.rows_per_item > 0 ? p->opt.rows_per_item : 32;
+  if (p->opt.adj_rows_per_item > 0) p->tz_adj = p->opt.adj_rows_per_item;
 
   // per-shot sizes
   const int ns = s->nshots;
@@ -1548,8 +1571,14 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
         ck(hipMemcpy(b.d_sblocks, b.h_sblocks.data(), b.h_sblocks.size() * sizeof(int2), hipMemcpyHostToDevice),
            "copy sblocks");
     }
-    if (rc == 0) rc = p->build_items(b, p->tz, p->opt.item_order > 0 ? p->opt.item_order : 0);
-    if (rc == 0) rc = p->build_adj_items(b, p->tz, p->opt.item_order >= 2 ? std::min(p->opt.item_order, 3) : 0);
+    // until the first call tunes (or applies the pinned tiling), the items are a placeholder tiling
+    p->order = p->opt.item_order > 0 ? p->opt.item_order : 0;
+    const int ao = p->opt.adj_item_order >= 0 ? p->opt.adj_item_order
+                   : p->opt.item_order >= 2    ? std::min(p->opt.item_order, 3)
+                                               : 0;
+    p->order_adj = b.grouped && ao == 1 ? 0 : ao;
+    if (rc == 0) rc = p->build_items(b, p->tz, p->order);
+    if (rc == 0) rc = p->build_adj_items(b, p->tz_adj, p->order_adj);
     if (rc == 0 && p->inc() && p->opt.step_pairs != 0) rc = p->build_pair_items(b, p->tz_pair);
   }
   if (rc) {

@@ -1,6 +1,7 @@
 """HBM traffic per kernel launch from two rocprofv3 counter runs (FETCH_SIZE, then WRITE_SIZE).
 
-    python -m fwi_amd.pmc FETCH_DIR WRITE_DIR --workload NAME --command "..." > traffic.json
+    python -m fwi_amd.pmc FETCH_DIR WRITE_DIR --workload NAME --command "..." \
+        --bench-log FETCH_RUN_OUTPUT --bench-log WRITE_RUN_OUTPUT > traffic.json
 
 Each directory is the -d output of one `rocprofv3 --pmc <counter> --output-format csv` run of the same program,
 which first copies 1 GiB device to device (`bench.py --calibrate-copy`): the first `__amd_rocclr_copyBuffer`
@@ -91,9 +92,23 @@ def main(argv=None):
     ap.add_argument("write_dir")
     ap.add_argument("--workload", default="")
     ap.add_argument("--command", default="")
+    ap.add_argument("--bench-log", action="append", default=[],
+                    help="output of a profiled bench.py run (once per counter run): its JSON line gives the work-item "
+                         "tiling, which every run must share")
     a = ap.parse_args(argv)
+    tilings = []
+    for f in a.bench_log:
+        with open(f) as fh:
+            lines = [ln for ln in fh if ln.startswith('{"metric"')]
+        if not lines:
+            sys.exit(f"{f}: no bench JSON line")
+        c = json.loads(lines[-1])["config"]
+        tilings.append({k: c[k] for k in ("rows_per_item", "item_order", "adjoint_tiling")})
+    if any(t != tilings[0] for t in tilings):
+        sys.exit(f"the counter runs used different tilings: {tilings}")
     res = summarize(a.fetch_dir, a.write_dir)
-    res = {"source_hash": source_hash(), "workload": a.workload, "command": a.command, **res}
+    res = {"source_hash": source_hash(), "workload": a.workload, "command": a.command,
+           "tiling": tilings[0] if tilings else None, **res}
     json.dump(res, sys.stdout, indent=1)
     print()
 

@@ -89,6 +89,9 @@ typedef struct fwi_options {
                           gradient) advance in FP64 in the forward and recomputation passes.  0 (default) = 24 rows
                           where nz - cpml - 4 >= 24 and the plan allows it (FP32 checkpoints, no step pairs), none
                           otherwise; -1 = none; k > 4 = k rows (k <= nz - cpml - 4; FP32 checkpoints, no step pairs). */
+  int adj_rows_per_item; /* z rows per wave in the adjoint kernels: 0 = rows_per_item when that is set, else tuned */
+  int adj_item_order;    /* adjoint work-item order 0..3 (1 means 0 with gradient groups), -1 = derived from item_order
+                            when that is set (1 -> 0 with groups, 4 -> 3), else tuned */
 } fwi_options;
 
 typedef struct fwi_plan fwi_plan;

@@ -145,7 +145,8 @@ def test_tiling_invariance(torch):
 def test_shared_model_tiles_invariance(torch):
     """Item order 4 (four shots with the same window per workgroup, the model rows of interior tiles shared
     through LDS) changes only the schedule: on eight full-width shots its gathers, misfit and gradient equal
-    those of order 1 bit for bit, and the gradient equals the oracle's."""
+    those of order 1 bit for bit, and the gradient equals the oracle's; so does an adjoint tiling pinned apart
+    from the forward's (adj_rows_per_item, adj_item_order)."""
     w = small_workload(nx=200, nz=72, nt=90, cpml=12, nshots=8)
     say("shared-model tiles: order 4 vs order 1, 8 full-width shots")
     tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
@@ -153,18 +154,21 @@ def test_shared_model_tiles_invariance(torch):
     obs = torch.from_numpy(_cat(obs_np)).cuda()
     model = _dev_model(torch, sq_init)
     outs = []
-    for order in (1, 4):
-        plan = _plan(w, tab, src, rows_per_item=16, item_order=order, ckpt_interval=9, store_segments=1)
+    for order, adj in ((1, {}), (4, {}), (4, {"adj_rows_per_item": 8, "adj_item_order": 2})):
+        plan = _plan(w, tab, src, rows_per_item=16, item_order=order, ckpt_interval=9, store_segments=1, **adj)
         try:
             g = plan.forward(_dev_model(torch, sq_true))
+            if adj:
+                assert plan.adj_tiling == (8, 2, True) and (plan.rows_per_item, plan.item_order) == (16, 4)
             phi, G, syn = plan.gradient(model, obs, want_syn=True)
             torch.cuda.synchronize()
             outs.append((g.cpu().numpy(), phi, G.cpu().numpy(), syn.cpu().numpy()))
         finally:
             plan.close()
-    a, b = outs
-    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
-    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    a, b, c = outs
+    for x in (b, c):
+        assert np.array_equal(a[0], x[0]) and a[1] == x[1]
+        assert np.array_equal(a[2], x[2]) and np.array_equal(a[3], x[3])
     assert np.array_equal(b[0], _cat(obs_np))
     phi_ref, G_ref, _ = oracle_gradient(w, "f32i", True, sq_init, tab, src, obs_np)
     for k in range(3):
@@ -209,7 +213,7 @@ def test_surface_band_matches_oracle(torch, rows):
 
 def test_surface_band_option_checks(torch):
     """An explicit band needs the increment form, FP32 checkpoints and single steps, and must end above the
-    bottom C-PML; the default switches itself off where it cannot run."""
+    bottom C-PML; the default switches itself off where it cannot run.  Out-of-range work-item tilings are refused."""
     w = small_workload(nshots=1, nt=20)
     tab, src, _, _ = oracle_setup(w, "f32i")
     for kw in ({"surface_rows": 24, "step_pairs": 1}, {"surface_rows": 24, "ckpt_format": 1},
@@ -220,6 +224,9 @@ def test_surface_band_option_checks(torch):
         plan = _plan(w, tab, src, **kw)
         assert plan.surface_rows == rows, kw
         plan.close()
+    for kw in ({"item_order": 5}, {"adj_item_order": 4}, {"adj_rows_per_item": -2}):
+        with pytest.raises(RuntimeError, match="item_order"):
+            _plan(w, tab, src, **kw)
 
 
 def test_checkpoint_interval_and_batching_invariance(torch):
