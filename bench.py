@@ -306,16 +306,25 @@ def tiling_of(plan):
             "adjoint_tiling": {"rows_per_item": ar, "item_order": ao, "grouped": bool(g)}}
 
 
-TRAFFIC_KEYS = {"k_fwd_main<0>": ("k_fwd_main<0,",), "k_fwd_main<1>": ("k_fwd_main<1,",),
-                "k_adj_ex_main": ("k_adj_ex_main<",), "k_fwd_band": ("k_fwd_band<1>",),
-                "k_fwd_strip": ("k_fwd_strip<",), "k_fwd_strip_band": ("k_fwd_strip_band<",),
+TRAFFIC_KEYS = {"k_fwd_main<0>": ("k_fwd_main<0, float",), "k_fwd_main<1>": ("k_fwd_main<1, float",),
+                "k_adj_ex_main": ("k_adj_ex_main<float",), "k_fwd_band": ("k_fwd_band<1>",),
+                "k_fwd_strip": ("k_fwd_strip<float",),
                 "k_adj_ex_strip1+2": ("k_adj_ex_strip1", "k_adj_ex_strip2<")}
 
 
-def traffic_of(traffic, name):
-    """Counter bytes (read + written) per launch of a step kernel, summed over the kernels it stands for."""
+def traffic_of(traffic, name, nt=0, kept=0):
+    """Counter bytes (read + written) per launch of a step kernel, summed over the kernels it stands for; for the
+    merged strip + band launch, whose two variants (without and with stored imaging terms) share one timing, the
+    average over a gradient's launches of each (nt - kept and nt)."""
     if not traffic:
         return None
+    if name == "k_fwd_strip_band":
+        v0, v1 = (traffic.get(f"k_fwd_strip_band<float, {q}>") for q in (0, 1))
+        if not (v0 and v1):
+            return None
+        n0, n1 = nt - kept, nt
+        return float(((v0["read_bytes"] + v0["write_bytes"]) * n0 + (v1["read_bytes"] + v1["write_bytes"]) * n1)
+                     / (n0 + n1))
     parts = [v for k, v in traffic.items() for p in TRAFFIC_KEYS.get(name, ()) if k.startswith(p)]
     if not parts:
         return None
@@ -423,7 +432,7 @@ def main():
             nb = nband if (kind in (0, 1, 7) or merged) else 0
             survey, design = kernel_bytes(kind, ncell, nb, nstrip, nmodel, grouped, inc)
             per_grad = launches_per_gradient(kind, nt, plan.kept_steps, band)
-            tb = traffic_of(traffic, name)
+            tb = traffic_of(traffic, name, nt, plan.kept_steps)
             k = {"kernel": name, "does": what, "launches_per_gradient": per_grad, "mean_launch_us": round(mean * 1e6, 1),
                  "s_per_gradient": round(per_grad * mean, 4),
                  "bytes_survey": survey, "frac_survey": round(survey / mean / 1e9 / HBM_PEAK_GBS, 4),
