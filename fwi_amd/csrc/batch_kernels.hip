@@ -337,18 +337,28 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
 // band..band+3 of the float F^n enter the band's D2z, and the band writes its rows of the float P, F rounded, which
 // the float rows' D2z reads.  Also written: the band's imaging terms (rounded once) and its receivers (P + F
 // rounded once).  The expressions are those of the oracle's f32i band, in the same order.
-// Grid (256-column chunks, band - 4, shots).  psi_x is kept by step parity (read psx, write psxn): a chunk first
-// computes psi_x^{n+1} of the side cells within 4 columns of it -- neighbouring chunks compute the cells they share
-// from the same inputs and store the same bits -- then, after a workgroup barrier, its own cells.  (A tiled variant
-// staging 64 x 20 cells of P, F and psi in LDS was slower: 46.9 against 41.2 us per step on survey1600.)
+// Grid (256-column chunks, groups of kBandRows band rows, shots): a thread updates one column of the group's rows,
+// taking the x stencil of P from the rows staged in LDS and the z stencil of F from a register queue of the
+// column (kBandRows + 8 loads instead of 9 per row).  psi_x is kept by step parity (read psx, write psxn): a chunk
+// first computes psi_x^{n+1} of the side cells within 4 columns of it -- neighbouring chunks compute the cells
+// they share from the same inputs and store the same bits -- then, after a workgroup barrier, its own cells.
+// Measured per strip + band launch on survey1600: one row per workgroup 57.7 us, 2 rows 53.8, 3 rows 53.1, 4 rows
+// 54.2, 5 rows 55.7, 10 rows 104 (LDS and registers limit the waves); a tile of 64 x 20 cells staged in LDS was
+// slower still.
+constexpr int kBandRows = 4;  // band rows per workgroup
 template <int QM>
-__device__ __forceinline__ void fwd_band_block(const BandArgs& a, int chunk, int brow, int shot) {
+__device__ __forceinline__ void fwd_band_block(const BandArgs& a, int chunk, int rgroup, int shot) {
+  constexpr int R = kBandRows;
+  __shared__ double sp[R][264];  // P^n of the rows, columns x0-4 .. x0+259
+  __shared__ double sx[R][264];  // psi_x^{n+1} of the side C-PML cells among them, 0 elsewhere (as in psxn)
   const ShotDev sd = a.g.shots[shot];
   const int nx = sd.nx, ps = sd.ps, cpml = a.g.cpml, nxm = a.m.nxm, B = a.band;
   const int x0 = chunk * 256;
   if (x0 >= nx - kRim) return;  // the whole workgroup: past this shot's window
-  const int z = kRim + brow;
+  const int z0 = kRim + rgroup * R;
+  const int nr = B - z0 < R ? B - z0 : R;  // rows of this group inside the band
   const double dh = (double)a.g.dh, dh2 = dh * dh, Cd = (a.dt * a.dt) / dh2;
+  // per-shot bases (wave-uniform) and 32-bit element offsets
   const double* P = a.P + sd.boff;
   double* Pm = a.Pm + sd.boff;
   const double* Fb = a.F + sd.boff;
@@ -358,72 +368,101 @@ __device__ __forceinline__ void fwd_band_block(const BandArgs& a, int chunk, int
   double* zx = a.zx + sd.boff;
   const float* Fc = a.Fc + sd.off;
   const int w = cpml - kRim;  // side C-PML cells per row and side
-  const long row = (long)ps * z;
-  // psi_x^{n+1} of the side C-PML cells in [x0-4, x0+260) (PSI_F)
-  for (int j = threadIdx.x; j < 2 * w; j += 256) {
-    const int ix = j < w ? kRim + j : nx - cpml + (j - w);
-    if (ix < x0 - 4 || ix >= x0 + 260) continue;
-    const long i = row + ix;
-    const double t = d1d(P[i - 4], P[i - 3], P[i - 2], P[i - 1], P[i + 1], P[i + 2], P[i + 3], P[i + 4], dh);
-    psxn[i] = (double)coef_x(a.g.tb, ix, nx, cpml) * psx[i] + (double)coef_x(a.g.ta, ix, nx, cpml) * t;
+  for (int t = threadIdx.x; t < 264; t += 256) {
+    const int c = x0 - 4 + t;
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+      sp[rr][t] = (rr < nr && c >= 0 && c < nx) ? ldud(P, (unsigned)(ps * (z0 + rr) + c)) : 0.0;  // else never used
+      sx[rr][t] = 0.0;
+    }
+  }
+  const bool sides = x0 - 4 < cpml || x0 + 260 > nx - cpml;  // workgroup-uniform
+  if (sides) {
+    __syncthreads();
+    // psi_x^{n+1} of the side C-PML cells in [x0-4, x0+260) (PSI_F); neighbouring chunks compute the cells they
+    // share from the same inputs and store the same bits
+    for (int j = threadIdx.x; j < 2 * w * nr; j += 256) {
+      const int rr = j / (2 * w), jj = j - rr * 2 * w;
+      const int ix = jj < w ? kRim + jj : nx - cpml + (jj - w);
+      if (ix < x0 - 4 || ix >= x0 + 260) continue;
+      const unsigned i = (unsigned)(ps * (z0 + rr) + ix);
+      const double t = d1d(ldud(P, i - 4), ldud(P, i - 3), ldud(P, i - 2), ldud(P, i - 1), ldud(P, i + 1),
+                           ldud(P, i + 2), ldud(P, i + 3), ldud(P, i + 4), dh);
+      const double v = (double)coef_x(a.g.tb, ix, nx, cpml) * ldud(psx, i) + (double)coef_x(a.g.ta, ix, nx, cpml) * t;
+      stud(psxn, i, v);
+      sx[rr][ix - (x0 - 4)] = v;
+    }
   }
   __syncthreads();
   const int ix = x0 + threadIdx.x;
   if (ix < kRim || ix >= nx - kRim) return;
-  const long i = row + ix;
-  const double p2 = d2d(P[i - 4], P[i - 3], P[i - 2], P[i - 1], P[i], P[i + 1], P[i + 2], P[i + 3], P[i + 4], dh2);
-  double f[9];
+  // F^n at rows z0-4 .. z0+R+3 of this column: FP64 in the band, the float rows below it
+  double f[R + 8];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    const int r = z + k - 4;
-    const long j = (long)ps * r + ix;
-    f[k] = r < B ? Fb[j] : (double)Fc[j];
+  for (int k = 0; k < R + 8; ++k) {
+    const int r = z0 + k - 4;
+    const unsigned j = (unsigned)(ps * r + ix);
+    f[k] = (k < nr + 8) ? (r < B ? ldud(Fb, j) : (double)ldu(Fc, j)) : 0.0;
   }
-  const double r2 = d2d(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], dh2);
-  double atp = 0.0;
-  const double atf = 0.0;
-  if (ix < cpml || ix > nx - cpml - 1) {
-    const double tx = d1d(psxn[i - 4], psxn[i - 3], psxn[i - 2], psxn[i - 1], psxn[i + 1], psxn[i + 2], psxn[i + 3],
-                          psxn[i + 4], dh);
-    const double tpx = p2 + tx;
-    const double zn = (double)coef_x(a.g.tb, ix, nx, cpml) * zx[i] + (double)coef_x(a.g.ta, ix, nx, cpml) * tpx;
-    zx[i] = zn;
-    atp = tx + zn;
+  const bool xside = ix < cpml || ix > nx - cpml - 1;
+  const float axv = coef_x(a.g.ta, ix, nx, cpml), bxv = coef_x(a.g.tb, ix, nx, cpml);
+  const int t = threadIdx.x + 4;
+#pragma unroll
+  for (int rr = 0; rr < R; ++rr) {
+    if (rr >= nr) break;
+    const int z = z0 + rr;
+    const unsigned i = (unsigned)(ps * z + ix);
+    const double pc = sp[rr][t];
+    const double p2 = d2d(sp[rr][t - 4], sp[rr][t - 3], sp[rr][t - 2], sp[rr][t - 1], pc, sp[rr][t + 1],
+                          sp[rr][t + 2], sp[rr][t + 3], sp[rr][t + 4], dh2);
+    const double r2 = d2d(f[rr], f[rr + 1], f[rr + 2], f[rr + 3], f[rr + 4], f[rr + 5], f[rr + 6], f[rr + 7],
+                          f[rr + 8], dh2);
+    double atp = 0.0;
+    const double atf = 0.0;
+    if (xside) {
+      const double tx = d1d(sx[rr][t - 4], sx[rr][t - 3], sx[rr][t - 2], sx[rr][t - 1], sx[rr][t + 1], sx[rr][t + 2],
+                            sx[rr][t + 3], sx[rr][t + 4], dh);
+      const double tpx = p2 + tx;
+      const double zn = (double)bxv * ldud(zx, i) + (double)axv * tpx;
+      stud(zx, i, zn);
+      atp = tx + zn;
+    }
+    const double tp = dh2 * (p2 + atp);
+    const double tf = dh2 * (r2 + atf);
+    const unsigned mi = (unsigned)(ix + nxm * z);
+    const double vh = (double)ldu(a.m.vh + sd.x0, mi), vp = (double)ldu(a.m.vp + sd.x0, mi),
+                 vn = (double)ldu(a.m.vn + sd.x0, mi);
+    double pn = 2.0 * pc - ldud(Pm, i) + vh * Cd * tp + vp * Cd * tf;
+    double fn = 2.0 * f[rr + 4] - ldud(Fm, i) + vn * Cd * tp + vp * Cd * tf;
+    if (ix == sd.isx && z == sd.isz) {
+      const double srcv = (double)a.src[a.it];
+      pn += srcv;
+      fn += srcv;
+    }
+    stud(Pm, i, pn);
+    stud(Fm, i, fn);
+    stu(a.Po + sd.off, i, (float)pn);
+    stu(a.Fo + sd.off, i, (float)fn);
+    if (QM == 1) {
+      stu(a.qa + sd.off, i, (float)(p2 + atp));
+      stu(a.qb + sd.off, i, (float)(r2 + atf));
+    } else if (QM == 2) {  // the reference-compatible terms come from the float state, as in the float rows
+      const float* Pf = a.Pcf + sd.off;
+      a.qa[sd.off + i] = d2_at<float>(Pf, i, 1, a.g.dh2);
+      a.qb[sd.off + i] = d2_at<float>(Fc, i, ps, a.g.dh2);
+    }
+    if (a.gather && z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec)
+      a.gather[sd.goff + (long)a.it * sd.nrec + (ix - sd.ir0)] = (float)(pn + fn);
   }
-  const double tp = dh2 * (p2 + atp);
-  const double tf = dh2 * (r2 + atf);
-  const long mi = sd.x0 + ix + (long)nxm * z;
-  const double vh = (double)a.m.vh[mi], vp = (double)a.m.vp[mi], vn = (double)a.m.vn[mi];
-  double pn = 2.0 * P[i] - Pm[i] + vh * Cd * tp + vp * Cd * tf;
-  double fn = 2.0 * Fb[i] - Fm[i] + vn * Cd * tp + vp * Cd * tf;
-  if (ix == sd.isx && z == sd.isz) {
-    const double srcv = (double)a.src[a.it];
-    pn += srcv;
-    fn += srcv;
-  }
-  Pm[i] = pn;
-  Fm[i] = fn;
-  a.Po[sd.off + i] = (float)pn;
-  a.Fo[sd.off + i] = (float)fn;
-  if (QM == 1) {
-    a.qa[sd.off + i] = (float)(p2 + atp);
-    a.qb[sd.off + i] = (float)(r2 + atf);
-  } else if (QM == 2) {  // the reference-compatible terms come from the float state, as in the float rows
-    const float* Pf = a.Pcf + sd.off;
-    a.qa[sd.off + i] = d2_at<float>(Pf, i, 1, a.g.dh2);
-    a.qb[sd.off + i] = d2_at<float>(Fc, i, ps, a.g.dh2);
-  }
-  if (a.gather && z == sd.irz && ix >= sd.ir0 && ix < sd.ir0 + sd.nrec)
-    a.gather[sd.goff + (long)a.it * sd.nrec + (ix - sd.ir0)] = (float)(pn + fn);
 }
 // Grid: nbc chunks x (band - 4) rows x shots workgroups, in that order.
 template <int QM>
 __global__ void __launch_bounds__(256) k_fwd_band(BandArgs a, int nbc) {
   int b = xcd_logical_block(blockIdx.x, gridDim.x);
-  const int rows = a.band - kRim;
+  const int groups = (a.band - kRim + kBandRows - 1) / kBandRows;
   const int chunk = b % nbc;
   b /= nbc;
-  fwd_band_block<QM>(a, chunk, b % rows, b / rows);
+  fwd_band_block<QM>(a, chunk, b % groups, b / groups);
 }
 
 // The forward C-PML strip kernel and the FP64 band in one launch: they read step n and write disjoint arrays, and
@@ -441,10 +480,10 @@ __global__ void __launch_bounds__(256) k_fwd_strip_band(StripArgs s, BandArgs a,
     return;
   }
   int b = xcd_logical_block((int)blockIdx.x - s.nblocks, (int)gridDim.x - s.nblocks);
-  const int rows = a.band - kRim;
+  const int groups = (a.band - kRim + kBandRows - 1) / kBandRows;
   const int chunk = b % nbc;
   b /= nbc;
-  fwd_band_block<QM>(a, chunk, b % rows, b / rows);
+  fwd_band_block<QM>(a, chunk, b % groups, b / groups);
 }
 
 // ---------------------------------------------------------------------------------------------

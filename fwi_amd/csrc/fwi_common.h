@@ -134,6 +134,12 @@ __device__ __forceinline__ float ldu(const float* base, unsigned idx) {
 __device__ __forceinline__ void stu(float* base, unsigned idx, float v) {
   *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (idx << 2)) = v;
 }
+__device__ __forceinline__ double ldud(const double* base, unsigned idx) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + (idx << 3));
+}
+__device__ __forceinline__ void stud(double* base, unsigned idx, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + (idx << 3)) = v;
+}
 // Streaming (non-temporal) variants for values touched once per launch (centre-only state, imaging terms,
 // gradient slots), so the L2 keeps the rows that neighbouring work items read again (stencil halos, model).
 __device__ __forceinline__ float ldn(const float* base, unsigned idx) {

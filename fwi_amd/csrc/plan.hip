@@ -698,6 +698,9 @@ struct fwi_plan {
     return 0;
   }
 
+  // workgroups of the band per 256-column chunk and shot: kBandRows rows each
+  int band_groups() const { return (band - kRim + kBandRows - 1) / kBandRows; }
+
   // the FP64 near-surface band of forward step n (rows [4, band) of every shot), launched with the C-PML strip
   // kernel of the same step when the batch has strip cells (one launch, timing kind 5)
   int fwd_band(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
@@ -729,7 +732,7 @@ struct fwi_plan {
     const int nbc = (maxnx + 255) / 256;
     if (b.nsblocks > 0) {
       StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
-      const long long nblk = (long long)b.nsblocks + (long long)nbc * (band - kRim) * b.count;
+      const long long nblk = (long long)b.nsblocks + (long long)nbc * band_groups() * b.count;
       const int tev = timed_begin(5, st);
       if (qm == 0)
         hipLaunchKernelGGL((k_fwd_strip_band<float, 0>), dim3((unsigned)nblk), dim3(256), 0, st, sa, a, nbc, fP(n), fF(n),
@@ -744,7 +747,7 @@ struct fwi_plan {
       CK(hipGetLastError());
       return 0;
     }
-    const dim3 grid((unsigned)((long long)nbc * (band - kRim) * b.count));
+    const dim3 grid((unsigned)((long long)nbc * band_groups() * b.count));
     const int tev = timed_begin(7, st);
     if (qm == 0)
       hipLaunchKernelGGL(k_fwd_band<0>, grid, dim3(256), 0, st, a, nbc);
