@@ -192,17 +192,17 @@ int main(int argc, char** argv) {
       }
       if (fwi_lbfgs_direction(opt, v_hor, v_po, v_nmo, grad, NULL)) die("fwi_lbfgs_direction");
       /* line search (main.c:690-786) */
-      int verdict = 1, stop = 0;
-      while (verdict == 1) {
+      int decision = 1, stop = 0;
+      while (decision == 1) {
         double trial = 0;
         if (fwi_lbfgs_trial(opt, v_hor, v_po, v_nmo, ls.alpha, NULL)) die("fwi_lbfgs_trial");
         if (vmax > 0 && fwi_model_clamp(v_hor, v_po, v_nmo, n, vmin > 0 ? vmin * vmin : 0.f, vmax * vmax, NULL))
           die("fwi_model_clamp");
         if (fwi_misfit(plan, v_hor, v_po, v_nmo, obs, &trial, NULL)) die("fwi_misfit");
-        verdict = fwi_line_search_step(&ls, phi, &trial);
+        decision = fwi_line_search_step(&ls, phi, &trial);
         printf("trial misfit : %.17g\n", trial);
-        if (verdict == 1) printf("new alpha = %f ....\n", ls.alpha[0]);
-        if (verdict == 2) {
+        if (decision == 1) printf("new alpha = %f ....\n", ls.alpha[0]);
+        if (decision == 2) {
           if (fwi_lbfgs_restore(opt, v_hor, v_po, v_nmo, NULL)) die("fwi_lbfgs_restore");
           stop = 1;
         }

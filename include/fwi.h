@@ -220,7 +220,7 @@ int fwi_plan_graph_nodes(const fwi_plan* plan, long long* largest, long long* la
 
 /* The backtracking line search of the reference (main.c:690-786) as a host state machine, shared by the drivers
  * (no device work).  fwi_line_search_init starts a stage: step lengths (v_hor, v_po, v_nmo) = 1, no failures, at
- * most 5.  After each trial model's misfit, fwi_line_search_step judges it against the iteration's misfit phi_n:
+ * most 5.  After each trial model's misfit, fwi_line_search_step compares it with the iteration's misfit phi_n:
  * a NaN trial counts as 1e30 (written back) and as a failure; a trial above phi_n halves every step length and
  * counts as a failure; the step lengths and the failure count carry over to the stage's later iterations.
  * Returns 0 (accept: trial <= phi_n), 1 (try again with the halved step lengths), 2 (more than max_failures
