@@ -1326,6 +1326,56 @@ int validate(const fwi_survey* s, std::string& why) {
 
 extern "C" {
 
+// Gradient groups of up to gmax shots from `ord` (the batch's shots ordered by window); member lists in wave order.
+constexpr int kGroupLook = 12;
+static std::vector<std::vector<int>> group_shots(const std::vector<ShotDev>& sh, const std::vector<int>& ord,
+                                                 int gmax) {
+  // placeholder waves of a group: per 64-column band of its union window, the members whose window misses it
+  auto waste = [&](const std::vector<int>& m) {
+    int ux0 = 1 << 30, ux1 = 0;
+    for (int s : m) {
+      ux0 = std::min(ux0, sh[s].x0);
+      ux1 = std::max(ux1, sh[s].x0 + sh[s].nx);
+    }
+    int w = 0;
+    for (int c = ux0; c < ux1; c += 64)
+      for (int s : m)
+        if (sh[s].x0 >= c + 64 || sh[s].x0 + sh[s].nx <= c) ++w;
+    return w;
+  };
+  std::vector<int> rest = ord;
+  std::vector<std::vector<int>> out;
+  while (!rest.empty()) {
+    const int nc = std::min<int>(kGroupLook, (int)rest.size() - 1);
+    const int k = std::min(gmax - 1, nc);
+    std::vector<int> pick(k), best;
+    for (int j = 0; j < k; ++j) pick[j] = j + 1;  // positions in rest, lexicographic combinations of [1, nc]
+    int bw = -1;
+    while (true) {
+      std::vector<int> m = {rest[0]};
+      for (int j : pick) m.push_back(rest[j]);
+      const int w = waste(m);
+      if (bw < 0 || w < bw) {
+        bw = w;
+        best = pick;
+      }
+      int j = k - 1;
+      while (j >= 0 && pick[j] == nc - (k - 1 - j)) --j;
+      if (j < 0) break;
+      ++pick[j];
+      for (int t = j + 1; t < k; ++t) pick[t] = pick[t - 1] + 1;
+    }
+    std::vector<int> g = {rest[0]};
+    for (int j : best) g.push_back(rest[j]);
+    std::vector<int> left;
+    for (int j = 1; j < (int)rest.size(); ++j)
+      if (std::find(best.begin(), best.end(), j) == best.end()) left.push_back(rest[j]);
+    out.push_back(g);
+    rest.swap(left);
+  }
+  return out;
+}
+
 void fwi_options_default(fwi_options* o) {
   std::memset(o, 0, sizeof *o);
   o->grad_mode = FWI_GRAD_EXACT;
@@ -1530,9 +1580,12 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
       for (int k0 = 0; k0 < d.nstrip; k0 += 256) b.h_sblocks.push_back(make_int2(k, k0));
     }
     b.nsblocks = (int)b.h_sblocks.size();
-    // gradient groups (exact mode): the batch's shots ordered by window (x0, width, then batch order) in runs
-    // of up to grad_group; each group's FP32 slot spans the union of its windows, slots back to back (their
-    // total is at most the batch's cells)
+    // gradient groups (exact mode): the batch's shots ordered by window (x0, width, then batch order); each group
+    // is led by the first shot not yet grouped and completed by the up to grad_group-1 shots, among the next
+    // kGroupLook ungrouped ones in that order, whose union window leaves the fewest placeholder waves (64-column
+    // bands of the union a member's window does not reach; first such choice in lexicographic order, so shots
+    // with identical windows group in runs).  Each group's FP32 slot spans the union of its windows, slots back
+    // to back (their total is at most the batch's cells).
     const int gmax = p->opt.grad_mode == FWI_GRAD_EXACT ? (p->opt.grad_group == 0 ? 4 : p->opt.grad_group) : 1;
     std::vector<int> ord(b.count);
     for (int k = 0; k < b.count; ++k) ord[k] = k;
@@ -1540,14 +1593,15 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
       const ShotDev &a = b.h_shots[u], &c = b.h_shots[v];
       return a.x0 != c.x0 ? a.x0 < c.x0 : a.nx < c.nx;
     });
+    std::vector<std::vector<int>> gsets = group_shots(b.h_shots, ord, gmax);
     long long slot = 0;
-    for (int k = 0; k < b.count; k += gmax) {
+    for (const std::vector<int>& gm : gsets) {
       Batch::Group g;
-      g.n = std::min(gmax, b.count - k);
+      g.n = (int)gm.size();
       int ux1 = 0;
       g.ux0 = 1 << 30;
       for (int j = 0; j < g.n; ++j) {
-        g.m[j] = ord[k + j];
+        g.m[j] = gm[j];
         const ShotDev& d = b.h_shots[g.m[j]];
         g.ux0 = std::min(g.ux0, d.x0);
         ux1 = std::max(ux1, d.x0 + d.nx);

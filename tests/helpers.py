@@ -1,6 +1,7 @@
 """Shared test helpers: workload setup on the CPU oracle and on the GPU."""
 from __future__ import annotations
 
+import itertools
 import sys
 
 import numpy as np
@@ -54,15 +55,37 @@ def oracle_gathers(w, mode, sq, tab, src):
     return out
 
 
+GROUP_LOOK = 12  # ungrouped shots a group leader chooses its members among (plan.hip: kGroupLook)
+
+
+def _placeholder_waves(shots, members):
+    """Waves of a grouped workgroup whose shot's window misses the 64-column band of the group's union window."""
+    ux0 = min(shots[i].x0 for i in members)
+    ux1 = max(shots[i].x0 + shots[i].nx for i in members)
+    return sum(1 for c in range(ux0, ux1, 64) for i in members
+               if shots[i].x0 >= c + 64 or shots[i].x0 + shots[i].nx <= c)
+
+
 def gradient_groups(shots, group=4, max_batch=0):
-    """The batched path's gradient groups: within each batch (of `max_batch` shots, 0 = one batch) the shots
-    ordered by window (x0, width, then index) in runs of up to `group`; member lists in wave order."""
+    """The batched path's gradient groups (plan.hip: group_shots): within each batch (of `max_batch` shots, 0 = one
+    batch) the shots ordered by window (x0, width, then index); each group is led by the first ungrouped shot and
+    completed by the up to group-1 shots among the next GROUP_LOOK ungrouped ones whose union window leaves the
+    fewest placeholder waves, the first such choice in lexicographic order.  Member lists in wave order."""
     out, b0 = [], 0
     n = len(shots)
     while b0 < n:
         b1 = n if max_batch <= 0 else min(n, b0 + max_batch)
-        order = sorted(range(b0, b1), key=lambda i: (shots[i].x0, shots[i].nx, i))
-        out.extend(order[k:k + group] for k in range(0, len(order), group))
+        rest = sorted(range(b0, b1), key=lambda i: (shots[i].x0, shots[i].nx, i))
+        while rest:
+            nc = min(GROUP_LOOK, len(rest) - 1)
+            k = min(group - 1, nc)
+            best, bw = (), None
+            for pick in itertools.combinations(range(1, nc + 1), k):
+                w = _placeholder_waves(shots, [rest[0]] + [rest[j] for j in pick])
+                if bw is None or w < bw:
+                    best, bw = pick, w
+            out.append([rest[0]] + [rest[j] for j in best])
+            rest = [r for j, r in enumerate(rest) if j and j not in best]
         b0 = b1
     return out
 

@@ -273,7 +273,7 @@ def test_checkpoint_interval_and_batching_invariance(torch):
 @pytest.mark.parametrize("layout", ["six_full", "mixed_windows", "per_shot"])
 def test_gradient_groups_match_oracle(torch, layout):
     """Exact gradient with gradient groups: six full-width shots form groups of 4 and 2 (two placeholder
-    waves); with one narrower window in the middle the groups are 3, 1, 2 in one grouped launch; grad_group=1
+    waves); with one narrower window among them the groups span the union of their windows; grad_group=1
     keeps one accumulator per shot.  Each against the oracle with the same groups (bit-level bar)."""
     from helpers import gradient_groups
 
@@ -297,6 +297,35 @@ def test_gradient_groups_match_oracle(torch, layout):
         e = rel_l2(G[k], G_ref[k])
         print(f"groups {layout} grad[{k}] rel-L2 {e:.3e}")
         assert np.array_equal(G[k], G_ref[k])
+
+
+def test_ragged_window_groups_match_oracle(torch):
+    """Eight ragged windows: the plan's gradient groups follow the placeholder-minimising rule (here not runs of
+    four in window order), the same groups as the oracle's restatement of the rule, so the grouped gradient equals
+    the oracle's bit for bit."""
+    from helpers import _placeholder_waves, gradient_groups
+
+    w = small_workload(nx=200, nz=72, nt=100, cpml=12, nshots=8)
+    wins = [(0, 200), (0, 80), (4, 196), (60, 140), (0, 90), (100, 100), (10, 150), (120, 80)]
+    w.shots = [M.Shot(x0=x0, nx=wd, isx=wd // 2, isz=6, irz=6, ir0=12, nrec=wd - 24) for x0, wd in wins]
+    groups = gradient_groups(w.shots, 4)
+    order = sorted(range(8), key=lambda i: (w.shots[i].x0, w.shots[i].nx, i))
+    assert groups != [order[:4], order[4:]]
+    say(f"ragged groups {groups}, placeholder waves {[_placeholder_waves(w.shots, g) for g in groups]}")
+    tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+    obs = oracle_gathers(w, "f32i", sq_true, tab, src)
+    phi_ref, G_ref, _ = oracle_gradient(w, "f32i", True, sq_init, tab, src, obs)
+    plan = _plan(w, tab, src)
+    try:
+        phi, G, _ = plan.gradient(_dev_model(torch, sq_init), torch.from_numpy(_cat(obs)).cuda())
+        torch.cuda.synchronize()
+        assert plan.adj_tiling[2]
+        G = G.cpu().numpy()
+    finally:
+        plan.close()
+    assert abs(phi - phi_ref) <= 1e-9 * phi_ref
+    for k in range(3):
+        assert np.array_equal(G[k], G_ref[k]), k
 
 
 @pytest.mark.parametrize("fmt", [1, 2])

@@ -175,3 +175,32 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "measures one GPU" in r.stderr and "all-reduce" in r.stderr
     assert r.stdout == ""
+
+
+def test_gradient_groups_rule():
+    """The gradient-group rule (its restatement in tests/helpers.py, which the GPU tests hold the plan to): every
+    shot in exactly one group of at most four, leaders in window order; identical windows group in runs of four
+    (survey1600); on the ragged survey the rule leaves fewer placeholder waves than runs of four in window order."""
+    from helpers import _placeholder_waves, gradient_groups
+
+    from fwi_amd import models as M
+
+    full = M.workload("survey1600")
+    assert gradient_groups(full.shots, 4) == [list(range(k, k + 4)) for k in range(0, 64, 4)]
+    w = M.workload("uneven1600")
+    sh = w.shots
+    groups = gradient_groups(sh, 4)
+    assert sorted(i for g in groups for i in g) == list(range(len(sh)))
+    assert all(1 <= len(g) <= 4 for g in groups)
+    key = lambda i: (sh[i].x0, sh[i].nx, i)
+    assert [key(g[0]) for g in groups] == sorted(key(g[0]) for g in groups)
+    order = sorted(range(len(sh)), key=key)
+    runs = [order[k:k + 4] for k in range(0, len(order), 4)]
+    waves = lambda gs: sum(len(g) * len(range(min(sh[i].x0 for i in g), max(sh[i].x0 + sh[i].nx for i in g), 64))
+                           for g in gs)
+    share = sum(_placeholder_waves(sh, g) for g in groups) / waves(groups)
+    share_runs = sum(_placeholder_waves(sh, g) for g in runs) / waves(runs)
+    print(f"placeholder waves: {share:.3f} of the grouped waves, runs of four {share_runs:.3f}")
+    assert share < 0.6 * share_runs
+    # batches group separately
+    assert all(max(g) < 48 or min(g) >= 48 for g in gradient_groups(sh, 4, max_batch=48))
