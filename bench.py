@@ -54,7 +54,8 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying captured graphs")
     ap.add_argument("--keep", type=int, default=0, help="trailing segments kept from the forward (0 = memory allows)")
     ap.add_argument("--grad-group", type=int, default=0, help="shots per gradient group (0 = 4)")
-    ap.add_argument("--pairs", action="store_true", help="fused core step pairs (increment form)")
+    ap.add_argument("--pairs", type=int, nargs="?", const=1, default=0,
+                    help="step pairs: the core two steps per launch (1 = rows tuned, k > 1 = k rows per core item)")
     ap.add_argument("--arith", type=int, default=3,
                     help="0 FP32 two-level, 1 mixed (reference), 2 FP32 difference stencil, 3 FP32 increment form")
     ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")
@@ -226,6 +227,9 @@ STEP_KERNELS = [
     (0, "k_fwd_main<0>", "forward step"),
     (1, "k_fwd_main<1>", "forward step storing the imaging terms (forward's last segments and recomputation)"),
     (2, "k_adj_ex_main", "exact adjoint step with imaging"),
+    (3, "k_fwd_pair<0>", "two forward steps of the core with the shell's first step"),
+    (4, "k_fwd_pair<1>", "two forward steps of the core storing the imaging terms, with the shell's first step"),
+    (8, "k_fwd_main (shell)", "the shell's second step of a step pair"),
     (7, "k_fwd_band", "FP64 near-surface band of every forward-type step"),
     (5, "k_fwd_strip", "forward C-PML memory variables psi (with the FP64 band in one launch when the plan has it)"),
     (6, "k_adj_ex_strip1+2", "adjoint C-PML memory variables (two kernels)"),
@@ -244,7 +248,12 @@ def kernel_bytes(kind, N, Nb, Nc, Nm, grouped, inc):
     (12 B, per shot); the adjoint reads mu and mu^{k+2} (or D) and the imaging terms (24 B), writes mu (and D) and
     read-modify-writes the FP32 gradient slot (24 B per cell, 6 with gradient groups of four)."""
     Nf = N - Nb  # cells of the float rows
-    if kind in (0, 1):
+    if kind == 8:  # the shell's cells only: counted with the pair kernel
+        survey = design = 0.0
+    elif kind in (3, 4):  # two steps
+        survey = 2.0 * (24.0 * Nf + 16.0 * Nc + 12.0 * Nm)
+        design = (32.0 + (16.0 if kind == 4 else 0.0)) * Nf + 16.0 * Nc + 12.0 * Nm
+    elif kind in (0, 1):
         survey = 24.0 * Nf + 16.0 * Nc + 12.0 * Nm
         design = (32.0 + (8.0 if kind == 1 else 0.0)) * Nf + 16.0 * Nc + 12.0 * Nm
     elif kind == 7:
@@ -262,7 +271,8 @@ def kernel_bytes(kind, N, Nb, Nc, Nm, grouped, inc):
 
 
 def launches_per_gradient(kind, nt, kept, band):
-    return {0: nt - kept, 1: nt, 2: nt, 5: 2 * nt - kept, 6: nt, 7: 2 * nt - kept}.get(kind, 0)
+    return {0: nt - kept, 1: nt, 2: nt, 3: (nt - kept) // 2, 4: nt // 2, 5: 2 * nt - kept, 6: nt,
+            7: 2 * nt - kept, 8: (2 * nt - kept) // 2}.get(kind, 0)
 
 
 def load_traffic(w, nshots, tiling):
@@ -302,8 +312,11 @@ def profiled_tiling(w, args):
 def tiling_of(plan):
     """The plan's work-item tiling as the bench line reports it."""
     r, o, (ar, ao, g) = plan.rows_per_item, plan.item_order, plan.adj_tiling
-    return {"rows_per_item": r, "item_order": o,
-            "adjoint_tiling": {"rows_per_item": ar, "item_order": ao, "grouped": bool(g)}}
+    t = {"rows_per_item": r, "item_order": o,
+         "adjoint_tiling": {"rows_per_item": ar, "item_order": ao, "grouped": bool(g)}}
+    if plan.pair_rows:
+        t["pair_rows"] = plan.pair_rows
+    return t
 
 
 TRAFFIC_KEYS = {"k_fwd_main<0>": ("k_fwd_main<0, float",), "k_fwd_main<1>": ("k_fwd_main<1, float",),
@@ -371,7 +384,7 @@ def main():
                 rows_per_item=args.rows, item_order=args.order, adj_rows_per_item=args.adj_rows,
                 adj_item_order=args.adj_order, arith=args.arith,
                 ckpt_format=args.ckpt_format, use_graph=0 if args.no_graph else 1, store_segments=args.keep,
-                grad_group=args.grad_group, step_pairs=1 if args.pairs else 0, surface_rows=args.surface_rows)
+                grad_group=args.grad_group, step_pairs=args.pairs, surface_rows=args.surface_rows)
     log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, {plan.kept_steps} steps kept from the forward, "
         f"workspace {plan.workspace_bytes / 1e9:.1f} GB, FP64 band rows 4..{plan.surface_rows - 1}")
     t0 = time.perf_counter()

@@ -89,10 +89,10 @@ struct FwdArgs {
   const int2* xr; // per-item column range [x, y) the item updates, or null (its whole band)
 };
 
-// Two forward steps n, n+1 of the increment form on the core of every shot (the cells at least 4 columns and
-// rows away from the C-PML cells, where no C-PML term enters either step).  Items: (shot, first of 56 output
-// columns, z0, z1).  Reads the set of step n, writes the set of step n+2 and, on the 4-cell layer next to the
-// shell, P and F of step n+1 (which the shell's single steps read).
+// Two forward steps n, n+1 of the increment form.  Core items: (shot, first of up to 224 output columns, z0, z1)
+// within the core (columns [cpml+4, nx-cpml-4) rounded inward to multiples of 32, rows [zfirst+4, nz-cpml-4)); the launch reads the set of step n and
+// writes the set of step n+2 on the core and (blocks [0, nshell_wg), sh: single-step arguments with their items and
+// column ranges) the set of step n+1 on the shell and the core's cells within 4 of it.
 struct FwdPairArgs {
   GridArgs g;
   ModelArgs m;
@@ -104,18 +104,16 @@ struct FwdPairArgs {
   const float* F0;
   const float* Q0;
   const float* R0;
-  float* P1;
-  float* F1;
   float* P2;
   float* F2;
   float* Q2;
   float* R2;
   float* gather;     // batch gathers or null
-  float* qa0;        // imaging terms of step n (QM 1: a, b; QM 2: p2, r2) or null
-  float* qb0;
-  float* qa1;        // of step n+1
-  float* qb1;
+  float* qa0;        // imaging terms (QM 1: a, b; QM 2: p2, r2): a, b of step n at qa0, qa0 + qstride bytes, of
+  unsigned qstride;  // step n+1 at qa0 + 2 qstride, qa0 + 3 qstride; qa0 null when QM = 0
   int it;            // n
+  FwdArgs sh;        // step n of the shell and the core's cells next to it
+  int nshell_wg;     // shell workgroups (a multiple of 8)
 };
 
 // FP64 near-surface band of the increment-form forward: rows [4, band) of every shot advance in FP64 (two levels

@@ -202,29 +202,11 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
 // four interior items on one model tile (the host sets it on all four or none), which share the model rows.
 constexpr int kShareModel = 1 << 30;
 
-template <int QM, class S, bool INC = false>
-__global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
-  __shared__ float xbuf[4][72];
-  __shared__ float mshare[2][4][64];
-  const int wave = wave_id(), lane = threadIdx.x & 63;
-  const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
-  if (item >= a.nitems) return;
-  int4 itm = a.items[item];
-  const bool share = (itm.x & kShareModel) != 0;
-  itm.x &= ~kShareModel;
-  const ShotDev sd = a.g.shots[itm.x];
+// The general body: any item, C-PML terms per cell; with a.xr, only the item's column range is updated.
+template <int QM, class S, bool INC>
+__device__ __forceinline__ void fwd_general(const FwdArgs& a, int item, const int4 itm, const ShotDev& sd, int lane,
+                                            float* buf) {
   const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
-  // Interior items (no C-PML cell; the items are cut at row nz-e, e = max(cpml,4)) take the pipelined body.
-  // Items with C-PML cells take the general body below, interleaved with them in the launch.
-  if (share) {
-    fwd_interior<QM, S, INC, true>(a, itm, sd, lane, xbuf[wave], mshare);
-    return;
-  }
-  const int e = cpml > kRim ? cpml : kRim;
-  if (!a.xr && itm.y >= e && itm.y + 64 <= nx - e && itm.w <= nz - e) {
-    fwd_interior<QM, S, INC>(a, itm, sd, lane, xbuf[wave]);
-    return;
-  }
   const int ix = itm.y + lane;
   bool act = ix >= kRim && ix < nx - kRim;
   if (a.xr) {  // shell items of the step-pair scheme update only their column range
@@ -249,7 +231,6 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
   const float* vh = a.m.vh + sd.x0;
   const float* vp = a.m.vp + sd.x0;
   const float* vn = a.m.vn + sd.x0;
-  float* buf = xbuf[wave];
   const float srcv = a.src[a.it];
   const int z0 = itm.z, z1 = itm.w;
   float fq[9];
@@ -326,6 +307,32 @@ __global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) fq[k] = fq[k + 1];
   }
+}
+
+template <int QM, class S, bool INC = false>
+__global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
+  __shared__ float xbuf[4][72];
+  __shared__ float mshare[2][4][64];
+  const int wave = wave_id(), lane = threadIdx.x & 63;
+  const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
+  if (item >= a.nitems) return;
+  int4 itm = a.items[item];
+  const bool share = (itm.x & kShareModel) != 0;
+  itm.x &= ~kShareModel;
+  const ShotDev sd = a.g.shots[itm.x];
+  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
+  // Interior items (no C-PML cell; the items are cut at row nz-e, e = max(cpml,4)) take the pipelined body.
+  // Items with C-PML cells take the general body, interleaved with them in the launch.
+  if (share) {
+    fwd_interior<QM, S, INC, true>(a, itm, sd, lane, xbuf[wave], mshare);
+    return;
+  }
+  const int e = cpml > kRim ? cpml : kRim;
+  if (!a.xr && itm.y >= e && itm.y + 64 <= nx - e && itm.w <= nz - e) {
+    fwd_interior<QM, S, INC>(a, itm, sd, lane, xbuf[wave]);
+    return;
+  }
+  fwd_general<QM, S, INC>(a, item, itm, sd, lane, xbuf[wave]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -487,32 +494,62 @@ __global__ void __launch_bounds__(256) k_fwd_strip_band(StripArgs s, BandArgs a,
 }
 
 // ---------------------------------------------------------------------------------------------
-// Two increment-form forward steps per launch on the core of every shot: columns [cpml+4, nx-cpml-4) and
-// rows [4, nz-cpml-4), where neither step nor its 4-cell halo touches a C-PML cell, so no C-PML term enters.
-// A wave owns 56 output columns [x0, x0+56) and rows [z0, z1).  Its 64 lanes compute step n on columns
-// [x0-4, x0+60) and rows [z0-4, z1+4) (the halo that step n+1 needs), marching down: at row r it finishes
-// step n at row r and step n+1 at row r-4.  D2x acts on P only and D2z on F only, so step n+1 takes the x
-// neighbours of P^{n+1} from the wave's own lanes (LDS row) and the z neighbours of F^{n+1} from a register
-// queue.  Per output cell the state crosses HBM once for two steps.  Every cell is computed with the
-// single-step kernel's expressions in the same order, so results are identical to two single steps.
+// Two increment-form forward steps n, n+1 per launch (temporal blocking).  The core of a window -- columns
+// [cpml+4, nx-cpml-4) rounded inward to whole lines and rows [zfirst+4, nz-cpml-4), zfirst the first row of the
+// float state (below the FP64 band) -- advances two steps reading only the state of step n: neither step, nor the
+// halo of step n that step n+1 reads, touches a C-PML cell or the band.  A core workgroup owns up to 224 output columns [xt, xt+224) and a run of rows
+// [z0, z1); its lanes compute step n on columns [xt-4, xt+228) and rows [z0-4, z1+4), marching down: at row r
+// they finish step n at row r and step n+1 at row r-4.  D2x acts on P only and D2z on F only: step n+1 takes the x
+// neighbours of P^{n+1} from a six-row LDS ring shared by the four waves (one workgroup barrier per row), the z
+// neighbours of F^{n+1} from a register queue; Q^{n+1}, R^{n+1}, the model and the stored terms of step n wait four
+// rows in register delay lines.  Per output cell the four state fields cross HBM once for two steps.  Storing, both
+// steps' imaging terms are written.  Stores go through buffer resources with an out-of-range offset on the lanes that
+// do not own the cell, so the loop body has one path and its memory-counter waits cover only the oldest prefetched
+// row; four row buffers in rotation (two rows in flight) and the four-row delay lines give the unrolled body one
+// register assignment.  Blocks [0, nshell_wg) of the same launch run step n of the shell and of the core's cells
+// within 4 of it with the single-step general body (they read step n and write the set of step n+1 only).  Every
+// cell is computed with the single-step kernels' expressions in the same order, so the results are identical to
+// single steps.
 // ---------------------------------------------------------------------------------------------
+// Output columns of a core workgroup: 224 = 7 cache lines, so that, with the core's columns rounded inward to whole
+// 128-byte lines, every line a core workgroup writes is written whole by it (a line shared by two workgroups would
+// reach HBM as two partial writes); 232 of the 256 lanes compute step n.
+constexpr int kPairCols = 224;
+// The core's columns: [cpml+4, nx-cpml-4) rounded inward to multiples of 32.
+__host__ __device__ inline int pair_cx0(int cpml) { return (cpml + 4 + 31) & ~31; }
+__host__ __device__ inline int pair_cx1(int nx, int cpml) { return (nx - cpml - 4) & ~31; }
+
+// xrow9 without lane-dependent branches: every lane writes its value and its halo value, lanes >= 8 (whose halo
+// value is their own) to their own slot again.  hidx = xrow9_halo_slot(lane).
+__device__ __forceinline__ int xrow9_halo_slot(int lane) { return lane < 4 ? lane : lane < 8 ? 64 + lane : 4 + lane; }
+__device__ __forceinline__ void xrow9u(float* buf, int lane, int hidx, float v, float h, float (&u)[9]) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  buf[4 + lane] = v;
+  buf[hidx] = h;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int k = 0; k < 9; ++k) u[k] = buf[lane + k];
+}
+
 template <int QM>
-__global__ void __launch_bounds__(256) k_fwd_pair(FwdPairArgs a) {
-  __shared__ float xbuf[4][72];
-  const int wave = wave_id(), lane = threadIdx.x & 63;
-  const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
-  if (item >= a.nitems) return;
-  const int4 itm = a.items[item];
+__device__ __forceinline__ void fwd_pair_core(const FwdPairArgs& a, const int4 itm, float* xb, float (*ring)[264]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
   const ShotDev sd = a.g.shots[itm.x];
-  const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
-  const int cx0 = cpml + 4, cx1 = nx - cpml - 4, zc1 = nz - cpml - 4;  // the core
-  const int x0 = itm.y;
-  const int col = x0 - 4 + lane;                                  // this lane's column for step n
-  const int cc = clampi(col, 0, nx - 1);
-  const int hc = lane < 8 ? clampi(x0 - 4 + halo_col(lane), 0, nx - 1) : cc;
-  const bool own = lane >= 4 && lane < 60 && col < cx1;           // output columns of this wave
-  const bool thin = col < cx0 + 4 || col >= cx1 - 4;              // next to the side shell
+  const int nx = sd.nx, ps = sd.ps, nxm = a.m.nxm;
+  const int cxb = pair_cx1(nx, a.g.cpml);    // end of the core columns
+  const int xt = itm.y, z0 = itm.z, z1 = itm.w;
+  const int col = xt - 4 + tid;              // step-n column of this lane (>= cpml)
+  const int cc = col < nx ? col : nx - 1;    // its loads: a valid column (lanes past cxb+4 feed no output)
+  const int hx = lane < 8 ? clampi(xt - 4 + 64 * wave + halo_col(lane), 0, nx - 1) : cc;
+  const int hidx = xrow9_halo_slot(lane);
+  const bool own = tid >= 4 && tid < 4 + kPairCols && col < cxb;
+  const bool rec = a.gather != nullptr && col >= sd.ir0 && col < sd.ir0 + sd.nrec;
   const float dh2 = a.g.dh2, C = a.C;
+  const float src0 = a.src[a.it], src1 = a.src[a.it + 1];
   const float* P0 = a.P0 + sd.off;
   const float* F0 = a.F0 + sd.off;
   const float* Q0 = a.Q0 + sd.off;
@@ -520,115 +557,176 @@ __global__ void __launch_bounds__(256) k_fwd_pair(FwdPairArgs a) {
   const float* vh = a.m.vh + sd.x0;
   const float* vp = a.m.vp + sd.x0;
   const float* vn = a.m.vn + sd.x0;
-  float* buf = xbuf[wave];
-  const float src0 = a.src[a.it], src1 = a.src[a.it + 1];
-  const int z0 = itm.z, z1 = itm.w;
-  const bool rec = a.gather && col >= sd.ir0 && col < sd.ir0 + sd.nrec;
-  float f0q[9];                    // F^n at rows r-4..r+4
-  float f1q[9];                    // F^{n+1} at rows r-8..r
-  float p1q[5], q1q[5], r1q[5];    // P^{n+1}, Q^{n+1}, R^{n+1} at rows r-4..r
-#pragma unroll
-  for (int k = 0; k < 8; ++k) f0q[k] = ldu(F0, (unsigned)(cc + ps * max(z0 - 8 + k, 0)));
-#pragma unroll
-  for (int k = 0; k < 9; ++k) f1q[k] = 0.f;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) p1q[k] = q1q[k] = r1q[k] = 0.f;
-  for (int r = z0 - 4; r < z1 + 4; ++r) {
-    // ---- step n at row r (rows above 4 are the rim: zero at every step)
-    const unsigned o = (unsigned)(cc + ps * max(r, 0));
-    f0q[8] = ldu(F0, (unsigned)(cc + ps * max(r + 4, 0)));
-    const float c = ldu(P0, o);
-    const float h = ldu(P0, (unsigned)(hc + ps * max(r, 0)));
-    const float q0 = ldn(Q0, o), r0 = ldn(R0, o);
-    const unsigned mo = (unsigned)(cc + nxm * max(r, 0));
-    const float vhv = ldu(vh, mo), vpv = ldu(vp, mo), vnv = ldu(vn, mo);
+  const unsigned fbytes = (unsigned)ps * (unsigned)a.g.nz * 4u;
+  const brsrc sP2 = make_rsrc(a.P2 + sd.off, fbytes), sF2 = make_rsrc(a.F2 + sd.off, fbytes);
+  const brsrc sQ2 = make_rsrc(a.Q2 + sd.off, fbytes), sR2 = make_rsrc(a.R2 + sd.off, fbytes);
+  // the four imaging-term fields of the pair lie qstride bytes apart (a, b of step n, then of step n+1): one
+  // resource over all four (QM != 0; otherwise an empty one that is never used)
+  const unsigned qs = a.qstride;
+  const brsrc sq = make_rsrc(QM != 0 ? a.qa0 + sd.off : a.P2, QM != 0 ? 3u * qs + fbytes : 0u);
+  const int zend = z1 + 4;  // step n runs on rows [z0-4, zend)
+  struct Row {
+    float f4;          // F^n at row r+4
+    float c, h;        // P^n at row r: the lane's column, the halo column
+    float q, r;        // Q^n, R^n
+    float vh, vp, vn;  // model
+  };
+  auto load_row = [&](int z, Row& w) __attribute__((always_inline)) {
+    const unsigned o = (unsigned)(cc + ps * z);
+    const unsigned mo = (unsigned)(cc + nxm * z);
+    w.f4 = ldu(F0, o + 4u * ps);
+    w.c = ldu(P0, o);
+    w.h = ldu(P0, (unsigned)(hx + ps * z));
+    w.q = ldn(Q0, o);
+    w.r = ldn(R0, o);
+    w.vh = ldu(vh, mo);
+    w.vp = ldu(vp, mo);
+    w.vn = ldu(vn, mo);
+  };
+  float fq0[9];  // F^n, rows r-4..r+4
+  float fq1[9];  // F^{n+1}, rows r-8..r
+  float dQ[4], dR[4], dvh[4], dvp[4], dvn[4], dta[4], dtb[4];  // rows r-4..r-1
+  // MAIN: step n+1 at row r-4 and the stores of that row (rows r >= z0+4); otherwise step n only
+  auto iter = [&](int r, const Row& cur, Row& nxt, auto mainc) __attribute__((always_inline)) {
+    constexpr bool MAIN = decltype(mainc)::value;
+    fq0[8] = cur.f4;
     float u[9];
-    xrow9(buf, lane, c, h, u);
-    float p1 = 0.f, f1 = 0.f, q1 = 0.f, r1 = 0.f;
-    if (r >= kRim) {
-      const float p2 = d2s<float>(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8], dh2);
-      const float r2 = d2s<float>(f0q[0], f0q[1], f0q[2], f0q[3], f0q[4], f0q[5], f0q[6], f0q[7], f0q[8], dh2);
-      const float atp = 0.f, atf = 0.f;  // no C-PML term in the core and its halo
-      const float tp = dh2 * (p2 + atp);
-      const float tf = dh2 * (r2 + atf);
-      float dq = vhv * C * tp + vpv * C * tf;
-      float dr = vnv * C * tp + vpv * C * tf;
-      if (col == sd.isx && r == sd.isz) {
-        dq += src0;
-        dr += src0;
-      }
-      q1 = q0 + dq;
-      r1 = r0 + dr;
-      p1 = c + q1;
-      f1 = f0q[4] + r1;
-      if (own && r >= z0 && r < z1) {
-        const unsigned oo = (unsigned)(col + ps * r);
-        if (thin || r >= zc1 - 4) {
-          stn(a.P1 + sd.off, oo, p1);
-          stn(a.F1 + sd.off, oo, f1);
-        }
-        if (rec && r == sd.irz) a.gather[sd.goff + (long)a.it * sd.nrec + (col - sd.ir0)] = p1 + f1;
-        if (QM == 1) {
-          stn(a.qa0 + sd.off, oo, p2 + atp);
-          stn(a.qb0 + sd.off, oo, r2 + atf);
-        } else if (QM == 2) {
-          stn(a.qa0 + sd.off, oo, p2);
-          stn(a.qb0 + sd.off, oo, r2);
-        }
-      }
+    xrow9u(xb, lane, hidx, cur.c, cur.h, u);
+    load_row(min(r + 2, zend - 1), nxt);  // two rows ahead; unused repeats on the last rows keep one path
+    // step n at row r (fwd_interior's expressions; no C-PML term in the core and its halo)
+    const float p2 = d2s<float>(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8], dh2);
+    const float r2 = d2s<float>(fq0[0], fq0[1], fq0[2], fq0[3], fq0[4], fq0[5], fq0[6], fq0[7], fq0[8], dh2);
+    const float atp = 0.f, atf = 0.f;
+    const float tp = dh2 * (p2 + atp);
+    const float tf = dh2 * (r2 + atf);
+    float dq = cur.vh * C * tp + cur.vp * C * tf;
+    float dr = cur.vn * C * tp + cur.vp * C * tf;
+    if (col == sd.isx && r == sd.isz) {
+      dq += src0;
+      dr += src0;
     }
-    f1q[8] = f1;
-    p1q[4] = p1;
-    q1q[4] = q1;
-    r1q[4] = r1;
-    // ---- step n+1 at row rb = r - 4
-    const int rb = r - 4;
-    if (rb >= z0) {
+    const float q1 = cur.q + dq, r1 = cur.r + dr;
+    const float p1 = cur.c + q1, f1 = fq0[4] + r1;
+    ring[r % 6][tid + 4] = p1;
+    fq1[8] = f1;
+    if (MAIN) {
+      // the ring slot written above last held row r-6, which every wave read before the previous row's barrier
+      __syncthreads();
+      const int rb = r - 4;
+      const float* rw = &ring[rb % 6][tid];
       float w[9];
-      xrow9(buf, lane, p1q[0], p1q[0], w);  // x neighbours of P^{n+1}: valid on lanes 4..59
-      const unsigned mb = (unsigned)(cc + nxm * rb);
-      const float vhb = ldu(vh, mb), vpb = ldu(vp, mb), vnb = ldu(vn, mb);
-      if (own) {
-        const float p2 = d2s<float>(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], dh2);
-        const float r2 = d2s<float>(f1q[0], f1q[1], f1q[2], f1q[3], f1q[4], f1q[5], f1q[6], f1q[7], f1q[8], dh2);
-        const float atp = 0.f, atf = 0.f;
-        const float tp = dh2 * (p2 + atp);
-        const float tf = dh2 * (r2 + atf);
-        float dq = vhb * C * tp + vpb * C * tf;
-        float dr = vnb * C * tp + vpb * C * tf;
-        if (col == sd.isx && rb == sd.isz) {
-          dq += src1;
-          dr += src1;
-        }
-        const float q2 = q1q[0] + dq, r2n = r1q[0] + dr;
-        const float pn = p1q[0] + q2, fn = f1q[4] + r2n;
-        const unsigned ob = (unsigned)(col + ps * rb);
-        stn(a.P2 + sd.off, ob, pn);
-        stn(a.F2 + sd.off, ob, fn);
-        stn(a.Q2 + sd.off, ob, q2);
-        stn(a.R2 + sd.off, ob, r2n);
-        if (rec && rb == sd.irz) a.gather[sd.goff + (long)(a.it + 1) * sd.nrec + (col - sd.ir0)] = pn + fn;
-        if (QM == 1) {
-          stn(a.qa1 + sd.off, ob, p2 + atp);
-          stn(a.qb1 + sd.off, ob, r2 + atf);
-        } else if (QM == 2) {
-          stn(a.qa1 + sd.off, ob, p2);
-          stn(a.qb1 + sd.off, ob, r2);
-        }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) w[k] = rw[k];
+      const float p2b = d2s<float>(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], dh2);
+      const float r2b = d2s<float>(fq1[0], fq1[1], fq1[2], fq1[3], fq1[4], fq1[5], fq1[6], fq1[7], fq1[8], dh2);
+      const float tpb = dh2 * (p2b + atp);
+      const float tfb = dh2 * (r2b + atf);
+      float dq2 = dvh[0] * C * tpb + dvp[0] * C * tfb;
+      float dr2 = dvn[0] * C * tpb + dvp[0] * C * tfb;
+      if (col == sd.isx && rb == sd.isz) {
+        dq2 += src1;
+        dr2 += src1;
+      }
+      const float q2 = dQ[0] + dq2, r2n = dR[0] + dr2;
+      const float pn = w[4] + q2, fn = fq1[4] + r2n;
+      const unsigned ob = own ? (unsigned)(col + ps * rb) * 4u : kOutOfRange;
+      bst<kBufNT>(sP2, ob, pn);
+      bst<kBufNT>(sF2, ob, fn);
+      bst<kBufNT>(sQ2, ob, q2);
+      bst<kBufNT>(sR2, ob, r2n);
+      if (QM == 1) {
+        bst<kBufNT>(sq, ob, dta[0]);
+        bst<kBufNT>(sq, ob + qs, dtb[0]);
+        bst<kBufNT>(sq, ob + 2u * qs, p2b + atp);
+        bst<kBufNT>(sq, ob + 3u * qs, r2b + atf);
+      } else if (QM == 2) {
+        bst<kBufNT>(sq, ob, dta[0]);
+        bst<kBufNT>(sq, ob + qs, dtb[0]);
+        bst<kBufNT>(sq, ob + 2u * qs, p2b);
+        bst<kBufNT>(sq, ob + 3u * qs, r2b);
+      }
+      if (rb == sd.irz && rec && own) {
+        const long g = sd.goff + (long)a.it * sd.nrec + (col - sd.ir0);
+        a.gather[g] = w[4] + fq1[4];
+        a.gather[g + sd.nrec] = pn + fn;
       }
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      f0q[k] = f0q[k + 1];
-      f1q[k] = f1q[k + 1];
+      fq0[k] = fq0[k + 1];
+      fq1[k] = fq1[k + 1];
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      p1q[k] = p1q[k + 1];
-      q1q[k] = q1q[k + 1];
-      r1q[k] = r1q[k + 1];
+    for (int k = 0; k < 3; ++k) {
+      dQ[k] = dQ[k + 1];
+      dR[k] = dR[k + 1];
+      dvh[k] = dvh[k + 1];
+      dvp[k] = dvp[k + 1];
+      dvn[k] = dvn[k + 1];
+      dta[k] = dta[k + 1];
+      dtb[k] = dtb[k + 1];
+    }
+    dQ[3] = q1;
+    dR[3] = r1;
+    dvh[3] = cur.vh;
+    dvp[3] = cur.vp;
+    dvn[3] = cur.vn;
+    dta[3] = QM == 1 ? p2 + atp : p2;
+    dtb[3] = QM == 1 ? r2 + atf : r2;
+  };
+  using Pro = std::integral_constant<bool, false>;
+  using Main = std::integral_constant<bool, true>;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) fq0[k] = ldu(F0, (unsigned)(cc + ps * (z0 - 8 + k)));
+  // four row buffers in rotation, two rows in flight: the 8 rows of step n above the first output row, the first
+  // four main rows peeled (so the loop is entered as it repeats), the main rows unrolled by four, the last ones
+  Row b0, b1, b2, b3;
+  load_row(z0 - 4, b0);
+  load_row(z0 - 3, b1);
+  iter(z0 - 4, b0, b2, Pro{});
+  iter(z0 - 3, b1, b3, Pro{});
+  iter(z0 - 2, b2, b0, Pro{});
+  iter(z0 - 1, b3, b1, Pro{});
+  iter(z0, b0, b2, Pro{});
+  iter(z0 + 1, b1, b3, Pro{});
+  iter(z0 + 2, b2, b0, Pro{});
+  iter(z0 + 3, b3, b1, Pro{});
+  int z = z0 + 4;
+  if (z + 3 < zend) {
+    iter(z, b0, b2, Main{});
+    iter(z + 1, b1, b3, Main{});
+    iter(z + 2, b2, b0, Main{});
+    iter(z + 3, b3, b1, Main{});
+    for (z += 4; z + 3 < zend; z += 4) {
+      iter(z, b0, b2, Main{});
+      iter(z + 1, b1, b3, Main{});
+      iter(z + 2, b2, b0, Main{});
+      iter(z + 3, b3, b1, Main{});
     }
   }
+  if (z < zend) iter(z, b0, b2, Main{});
+  if (z + 1 < zend) iter(z + 1, b1, b3, Main{});
+  if (z + 2 < zend) iter(z + 2, b2, b0, Main{});
+}
+
+// Grid: [0, nshell_wg) shell workgroups (four single-step items each), then one workgroup per core item.  Each part
+// is dealt to the XCDs in contiguous logical ranges of its own (nshell_wg is a multiple of 8).
+template <int QM>
+__global__ void __launch_bounds__(256) k_fwd_pair(FwdPairArgs a) {
+  __shared__ float xbuf[4][72];
+  __shared__ float ring[6][264];
+  const int wave = wave_id(), lane = threadIdx.x & 63;
+  if ((int)blockIdx.x < a.nshell_wg) {  // step n of the shell and the core's cells next to it
+    const int item = xcd_logical_block(blockIdx.x, a.nshell_wg) * 4 + wave;
+    if (item >= a.sh.nitems) return;
+    const int4 itm = a.sh.items[item];
+    const ShotDev sd = a.g.shots[itm.x];
+    fwd_general<QM, float, true>(a.sh, item, itm, sd, lane, xbuf[wave]);
+    return;
+  }
+  const int b = xcd_logical_block((int)blockIdx.x - a.nshell_wg, (int)gridDim.x - a.nshell_wg);
+  if (b >= a.nitems) return;
+  fwd_pair_core<QM>(a, a.items[b], xbuf[wave], ring);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -46,11 +46,14 @@ struct Batch {
   int nitems = 0;
   int4* d_items_adj = nullptr;  // adjoint work items (grouped: whole workgroups per group and tile)
   int nitems_adj = 0;
-  int4* d_items_core = nullptr;  // step pairs: fused core items (56 output columns each)
+  int4* d_items_core = nullptr;  // step pairs: core items (up to kPairCols output columns each)
   int nitems_core = 0;
   int4* d_items_shell = nullptr; // step pairs: single-step items of the shell, with their column ranges
   int2* d_xr_shell = nullptr;
   int nitems_shell = 0;
+  int4* d_items_shell1 = nullptr;  // the same for the shell's first step, widened by the core's 4-cell layer
+  int2* d_xr_shell1 = nullptr;
+  int nitems_shell1 = 0;
   struct Group {
     int n = 0;
     int m[4] = {0, 0, 0, 0};  // members (batch indices) in wave order; m[0] leads
@@ -121,9 +124,10 @@ struct fwi_plan {
   unsigned* d_ck_amax = nullptr;    // 16-bit checkpoints: max-abs bits per (checkpoint, shot, field)
   // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute),
   // 2 = adjoint step, 3 / 4 = fused core step pair without / with store, 5 = forward C-PML strip (with the FP64
-  // near-surface band when both run), 6 = the two adjoint C-PML strip kernels, 7 = FP64 near-surface band alone;
-  // every `timing_stride`-th launch of a kind is bracketed by events on its stream
-  static constexpr int kKinds = 8;
+  // near-surface band when both run), 6 = the two adjoint C-PML strip kernels, 7 = FP64 near-surface band alone,
+  // 8 = the shell's second step of a step pair; every `timing_stride`-th launch of a kind is bracketed by events on
+  // its stream
+  static constexpr int kKinds = 9;
   int timing_stride = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_open;  // (kind, event index of the start; end = start + 1)
@@ -334,6 +338,8 @@ struct fwi_plan {
       (void)hipFree(b.d_items_core);
       (void)hipFree(b.d_items_shell);
       (void)hipFree(b.d_xr_shell);
+      (void)hipFree(b.d_items_shell1);
+      (void)hipFree(b.d_xr_shell1);
       (void)hipFree(b.d_sblocks);
     }
     (void)hipFree(d_src);
@@ -541,53 +547,58 @@ struct fwi_plan {
     }
     return upload_items(b.d_items, b.nitems, items);
   }
-  // Step-pair items: the fused core items (56 output columns, runs of `rows`, row run -> band -> shot) and the
-  // shell's single-step items with the column range each one updates.  The core of a window is columns
-  // [cpml+4, nx-cpml-4) and rows [4, nz-cpml-4); a window without a core is all shell.
+  // Step pairs.  The core of a window is columns [pair_cx0, pair_cx1) and rows [pair_za(), pair_zb()); the shell,
+  // everything else of the float state, takes single steps.  Core items: (shot, first of up to kPairCols output
+  // columns, row run of `rows`), shot -> row run -> tile, one workgroup each.  Shell items: 64-column bands with the
+  // column range each one updates, over the rows above and below the core and the side columns beside it (a window
+  // without a core is all shell).  The shell's first step also covers the core's cells within 4 of the shell, whose
+  // step n+1 values the shell's second step and the C-PML update read.
+  int pair_za() const { return std::max(kRim, band) + 4; }
+  int pair_zb() const { return nz - cpml - 4; }
   int build_pair_items(Batch& b, int rows) {
-    std::vector<int4> core, shell;
-    std::vector<int2> xr;
-    const int zc1 = nz - cpml - 4;
-    auto runs_of = [&](int za, int zb) {
+    std::vector<int4> core, shell[2];
+    std::vector<int2> xr[2];
+    const int zf = std::max(kRim, band), za = pair_za(), zb = pair_zb();
+    auto runs_of = [&](int z0, int z1, int step) {
       std::vector<std::pair<int, int>> r;
-      for (int z0 = za; z0 < zb; z0 += rows) r.push_back({z0, std::min(z0 + rows, zb)});
+      for (int z = z0; z < z1; z += step) r.push_back({z, std::min(z + step, z1)});
       return r;
     };
-    const auto core_runs = runs_of(kRim, zc1);
-    int maxnx = 0;
-    for (int s = 0; s < b.count; ++s) maxnx = std::max(maxnx, b.h_shots[s].nx);
-    for (auto& r : core_runs)
-      for (int xo = cpml + 4; xo < maxnx; xo += 56)
-        for (int s = 0; s < b.count; ++s) {
-          const int cx1 = b.h_shots[s].nx - cpml - 4;
-          if (xo < cx1 && zc1 > kRim) core.push_back(make_int4(s, xo, r.first, r.second));
-        }
-    const auto all_runs = runs_of(kRim, nz - kRim);
-    const auto bottom_runs = runs_of(std::max(zc1, kRim), nz - kRim);
+    const auto core_runs = runs_of(za, zb, rows);
     for (int s = 0; s < b.count; ++s) {
-      const int nxs = b.h_shots[s].nx, cx0 = cpml + 4, cx1 = nxs - cpml - 4;
-      auto band = [&](int x, int x1, const std::vector<std::pair<int, int>>& rr, int lo, int hi) {
+      const int nxs = b.h_shots[s].nx, cx0 = pair_cx0(cpml), cx1 = pair_cx1(nxs, cpml);
+      auto band_items = [&](int k, int x, int x1, int z0, int z1, int lo, int hi) {
         for (; x < x1; x += 64)
-          for (auto& r : rr) {
-            shell.push_back(make_int4(s, x, r.first, r.second));
-            xr.push_back(make_int2(lo, hi));
+          for (auto& r : runs_of(z0, z1, tz)) {
+            shell[k].push_back(make_int4(s, x, r.first, r.second));
+            xr[k].push_back(make_int2(lo, hi));
           }
       };
-      if (cx1 <= cx0 || zc1 <= kRim) {  // no core: the whole window is shell
-        band(0, nxs, all_runs, 0, nxs);
+      if (cx1 <= cx0 || zb <= za) {  // no core: the whole window is shell
+        for (int k = 0; k < 2; ++k) band_items(k, 0, nxs, zf, nz - kRim, 0, nxs);
         continue;
       }
-      band(0, cx0, all_runs, 0, cx0);             // left side
-      band(cx1, nxs, all_runs, cx1, nxs);         // right side
-      band(cx0, cx1, bottom_runs, cx0, cx1);      // bottom between the sides
+      for (auto& r : core_runs)
+        for (int x = cx0; x < cx1; x += kPairCols) core.push_back(make_int4(s, x, r.first, r.second));
+      for (int k = 0; k < 2; ++k) {
+        const int w = k == 0 ? 4 : 0;  // the first step's shell reaches 4 cells into the core
+        band_items(k, 0, nxs, zf, za + w, 0, nxs);                              // above the core
+        band_items(k, 0, nxs, zb - w, nz - kRim, 0, nxs);                       // below it
+        band_items(k, 0, ((cx0 + w + 63) / 64) * 64, za + w, zb - w, 0, cx0 + w);  // left of it
+        band_items(k, ((cx1 - w) / 64) * 64, nxs, za + w, zb - w, cx1 - w, nxs);   // right of it
+      }
     }
     if (int r = upload_items(b.d_items_core, b.nitems_core, core)) return r;
-    if (int r = upload_items(b.d_items_shell, b.nitems_shell, shell)) return r;
-    (void)hipFree(b.d_xr_shell);
-    b.d_xr_shell = nullptr;
-    if (!xr.empty()) {
-      CK(hipMalloc(&b.d_xr_shell, xr.size() * sizeof(int2)));
-      CK(hipMemcpy(b.d_xr_shell, xr.data(), xr.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (int r = upload_items(b.d_items_shell1, b.nitems_shell1, shell[0])) return r;
+    if (int r = upload_items(b.d_items_shell, b.nitems_shell, shell[1])) return r;
+    int2** dst[2] = {&b.d_xr_shell1, &b.d_xr_shell};
+    for (int k = 0; k < 2; ++k) {
+      (void)hipFree(*dst[k]);
+      *dst[k] = nullptr;
+      if (!xr[k].empty()) {
+        CK(hipMalloc(dst[k], xr[k].size() * sizeof(int2)));
+        CK(hipMemcpy(*dst[k], xr[k].data(), xr[k].size() * sizeof(int2), hipMemcpyHostToDevice));
+      }
     }
     return 0;
   }
@@ -642,9 +653,8 @@ struct fwi_plan {
   }
 
   // single-step main kernel over an item list (all items, or the shell's with column ranges)
-  int fwd_main(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
-               const int4* items, int nitems, const int2* xr, int kind, hipStream_t st) {
-    if (nitems == 0) return 0;
+  FwdArgs fwd_args(const Batch& b, const ModelArgs& m, int n, float* qa, float* qb, float* gather, const int4* items,
+                   int nitems, const int2* xr) const {
     FwdArgs a;
     a.g = grid_args(b);
     a.m = m;
@@ -669,6 +679,12 @@ struct fwi_plan {
     a.qb = qb;
     a.it = n;
     a.xr = xr;
+    return a;
+  }
+  int fwd_main(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
+               const int4* items, int nitems, const int2* xr, int kind, hipStream_t st) {
+    if (nitems == 0) return 0;
+    const FwdArgs a = fwd_args(b, m, n, qa, qb, gather, items, nitems, xr);
     dim3 grid((nitems + 3) / 4);
     const int am = opt.arith;
     const int tev = kind >= 0 ? timed_begin(kind, st) : -1;
@@ -771,15 +787,23 @@ struct fwi_plan {
     return fwd_main(b, m, n, qm, qa, qb, gather, b.d_items, b.nitems, nullptr, qm == 0 ? 0 : 1, st);
   }
 
-  // Two forward steps n, n+1 (increment form): the shell by single steps around the fused core pair.  The
-  // shell's step n and the core pair read only the set of step n and write disjoint cells; the C-PML update
-  // and the shell's step n+1 then read step n+1, which the core leaves on its 4-cell layer next to the shell.
+  // Two forward steps n, n+1 (increment form) in four launches: the C-PML strip (and the FP64 band) of step n; the
+  // core pair with step n of the shell and of the core's cells next to it (both read only step n and write disjoint
+  // sets); the strip (and band) of step n+1, which reads the set of step n+1 there; the shell's step n+1.  Storing,
+  // the imaging terms of step n+1 lie 2 fields after those of step n (qa1 = qa0 + 2 cap_cell, as every caller lays
+  // them out), and the four fields must be addressable with 31-bit byte offsets (pair_store_ok).
   bool pairs(const Batch& b) const { return inc() && opt.step_pairs != 0 && b.nitems_core > 0; }
+  bool pair_store_ok() const { return 4.0 * 4.0 * (double)cap_cell < 2147483648.0; }
   int fwd_pair(const Batch& b, const ModelArgs& m, int n, int qm, float* qa0, float* qb0, float* qa1, float* qb1,
                float* gather, hipStream_t st) {
-    if (int r = fwd_strip(b, m, n, st)) return r;
-    if (int r = fwd_main(b, m, n, qm, qa0, qb0, gather, b.d_items_shell, b.nitems_shell, b.d_xr_shell, -1, st))
-      return r;
+    if (qm != 0 && (!pair_store_ok() || qb0 != qa0 + cap_cell || qa1 != qa0 + 2 * cap_cell ||
+                    qb1 != qa0 + 3 * cap_cell))
+      return fail("step pair: imaging-term slots not laid out as the pair kernel addresses them");
+    auto strip = [&](int k, float* qa, float* qb) -> int {
+      if (band > kRim) return fwd_band(b, m, k, qm, qa, qb, gather, st);
+      return fwd_strip(b, m, k, st);
+    };
+    if (int r = strip(n, qa0, qb0)) return r;
     FwdPairArgs a;
     a.g = grid_args(b);
     a.m = m;
@@ -791,19 +815,17 @@ struct fwi_plan {
     a.F0 = fF(n);
     a.Q0 = fQ(n);
     a.R0 = fR(n);
-    a.P1 = fP(n + 1);
-    a.F1 = fF(n + 1);
     a.P2 = fP(n + 2);
     a.F2 = fF(n + 2);
     a.Q2 = fQ(n + 2);
     a.R2 = fR(n + 2);
     a.gather = gather;
-    a.qa0 = qa0;
-    a.qb0 = qb0;
-    a.qa1 = qa1;
-    a.qb1 = qb1;
+    a.qa0 = qm != 0 ? qa0 : nullptr;
+    a.qstride = (unsigned)(4 * cap_cell);
     a.it = n;
-    dim3 grid((b.nitems_core + 3) / 4);
+    a.sh = fwd_args(b, m, n, qa0, qb0, gather, b.d_items_shell1, b.nitems_shell1, b.d_xr_shell1);
+    a.nshell_wg = ((b.nitems_shell1 + 3) / 4 + 7) / 8 * 8;
+    dim3 grid((unsigned)(a.nshell_wg + b.nitems_core));
     const int tev = timed_begin(qm == 0 ? 3 : 4, st);
     if (qm == 0)
       hipLaunchKernelGGL(k_fwd_pair<0>, grid, dim3(256), 0, st, a);
@@ -813,10 +835,9 @@ struct fwi_plan {
       hipLaunchKernelGGL(k_fwd_pair<2>, grid, dim3(256), 0, st, a);
     timed_end(tev, st);
     CK(hipGetLastError());
-    if (int r = fwd_strip(b, m, n + 1, st)) return r;
-    return fwd_main(b, m, n + 1, qm, qa1, qb1, gather, b.d_items_shell, b.nitems_shell, b.d_xr_shell, -1, st);
+    if (int r = strip(n + 1, qa1, qb1)) return r;
+    return fwd_main(b, m, n + 1, qm, qa1, qb1, gather, b.d_items_shell, b.nitems_shell, b.d_xr_shell, 8, st);
   }
-
 
   template <bool IMG>
   static void adj_ex_launch_img(bool grouped, int am, dim3 grid, const AdjExArgs& a, hipStream_t st) {
@@ -1077,7 +1098,7 @@ struct fwi_plan {
       if (ckpt_at(n))
         if (int r = save_ckpt(b, n / K, st)) return r;
       // a pair may not straddle a checkpoint or the first kept step
-      if (pairs(b) && n + 1 < nt && !ckpt_at(n + 1) && (n >= s0) == (n + 1 >= s0)) {
+      if (pairs(b) && n + 1 < nt && !ckpt_at(n + 1) && (n >= s0) == (n + 1 >= s0) && (n < s0 || pair_store_ok())) {
         float* q0 = qslot(n);
         float* q1 = qslot(n + 1);
         if (int r = fwd_pair(b, m, n, n >= s0 ? qm : 0, q0, q0 ? q0 + cap_cell : nullptr, q1,
@@ -1120,7 +1141,7 @@ struct fwi_plan {
           const int e = std::min((seg + 1) * K, nt);
           for (int nn = seg * K; nn < e;) {
             float* wa = qbuf() + (size_t)(nn - seg * K) * slot;
-            if (pairs(b) && nn + 1 < e) {
+            if (pairs(b) && nn + 1 < e && pair_store_ok()) {
               if (int r = fwd_pair(b, m, nn, qm, wa, wa + cap_cell, wa + slot, wa + slot + cap_cell, nullptr, st))
                 return r;
               nn += 2;
@@ -1261,8 +1282,8 @@ struct fwi_plan {
     pick(cands, tz_adj, order_adj);
     // fused core step pairs: rows per item
     if (inc() && opt.step_pairs != 0) {
-      std::vector<int> rows_p = {16, 32, 48, 64};
-      if (opt.rows_per_item > 0) rows_p = {opt.rows_per_item};
+      std::vector<int> rows_p = {16, 24, 32, 48, 64, 96};
+      if (opt.step_pairs > 1) rows_p = {opt.step_pairs};
       float best = 1e30f;
       for (int c : rows_p) {
         if (c > nz - 8 && c != rows_p[0]) continue;
@@ -1280,6 +1301,7 @@ struct fwi_plan {
               return 0;
             }, ms))
           return r;
+        if (log) std::fprintf(stderr, "[fwi tune] pair     rows %2d: %.1f us/step\n", c, ms * 1e3 / 6);
         if (ms < best) {
           best = ms;
           tz_pair = c;
@@ -1435,14 +1457,13 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
   p->ck_esize = p->opt.ckpt_format == FWI_CKPT_FP32 ? 4 : 2;
   {  // FP64 near-surface band
     const int k = p->opt.surface_rows;
-    const bool allowed = p->opt.arith == FWI_ARITH_FP32I && p->opt.ckpt_format == FWI_CKPT_FP32 && p->opt.step_pairs == 0;
+    const bool allowed = p->opt.arith == FWI_ARITH_FP32I && p->opt.ckpt_format == FWI_CKPT_FP32;
     if (k == 0) {
       p->band = allowed && s->nz - s->cpml - 4 >= 24 ? 24 : 0;
     } else if (k > 0) {
       if (!allowed || k <= kRim || k > s->nz - s->cpml - 4) {
         delete p;
-        set_message("surface_rows needs FWI_ARITH_FP32I, FP32 checkpoints, no step pairs and 4 < surface_rows <= "
-                    "nz - cpml - 4");
+        set_message("surface_rows needs FWI_ARITH_FP32I, FP32 checkpoints and 4 < surface_rows <= nz - cpml - 4");
         return 1;
       }
       p->band = k;

@@ -81,14 +81,14 @@ typedef struct fwi_options {
   int use_graph;       /* 1 (default): each batch's device work of fwi_forward / fwi_misfit / fwi_gradient /
                           fwi_gradient_src is captured once as a HIP graph per set of caller buffers and
                           replayed on later calls with the same buffers; 0: launched eagerly */
-  int step_pairs;      /* increment form: 1 advances the core of every window (4+ cells from the C-PML) two steps
-                          per launch and the rest by single steps; 0 (default): single steps everywhere.  The
-                          result is the same either way; on survey1600 the pairs are not faster yet (the fused
-                          kernel gains ~10 % per cell, the separate shell launches lose more). */
+  int step_pairs;      /* increment form: the core of every window (4+ cells from the C-PML and below the FP64 band)
+                          advances two steps per launch, the rest by single steps; 1 = rows per core work item
+                          tuned, k > 1 = k rows; 0 (default): single steps everywhere.  The result is the same
+                          either way. */
   int surface_rows;    /* FWI_ARITH_FP32I: rows 4..surface_rows-1 of every shot (the source, receivers and near-surface
                           gradient) advance in FP64 in the forward and recomputation passes.  0 (default) = 24 rows
-                          where nz - cpml - 4 >= 24 and the plan allows it (FP32 checkpoints, no step pairs), none
-                          otherwise; -1 = none; k > 4 = k rows (k <= nz - cpml - 4; FP32 checkpoints, no step pairs). */
+                          where nz - cpml - 4 >= 24 and the plan allows it (FP32 checkpoints), none otherwise;
+                          -1 = none; k > 4 = k rows (k <= nz - cpml - 4; FP32 checkpoints). */
   int adj_rows_per_item; /* z rows per wave in the adjoint kernels: 0 = rows_per_item when that is set, else tuned */
   int adj_item_order;    /* adjoint work-item order 0..3 (1 means 0 with gradient groups), -1 = derived from item_order
                             when that is set (1 -> 0 with groups, 4 -> 3), else tuned */
@@ -194,7 +194,8 @@ int fwi_dot3_noalloc(const double* a, const double* b, long long n, double* out,
  * number of timed launches of a kind (0 forward step, 1 forward step storing imaging terms, 2 adjoint
  * step, 3 / 4 fused core step pair without / with stored imaging terms, 5 forward C-PML strip -- one launch with
  * the FP64 near-surface band when the plan has both --, 6 the two adjoint C-PML strip kernels together, 7 FP64
- * near-surface band alone) since the last fwi_plan_set_timing call. */
+ * near-surface band alone, 8 the single step of the shell that completes a step pair) since the last
+ * fwi_plan_set_timing call. */
 void fwi_plan_set_timing(fwi_plan* plan, int stride);
 int fwi_plan_kernel_time(fwi_plan* plan, int kind, double* total_ms, long long* count);
 /* Cells of all shot windows (nx_s*nz summed over shots) and, in *strip_cells, their C-PML cells. */

@@ -212,15 +212,15 @@ def test_surface_band_matches_oracle(torch, rows):
 
 
 def test_surface_band_option_checks(torch):
-    """An explicit band needs the increment form, FP32 checkpoints and single steps, and must end above the
+    """An explicit band needs the increment form and FP32 checkpoints, and must end above the
     bottom C-PML; the default switches itself off where it cannot run.  Out-of-range work-item tilings are refused."""
     w = small_workload(nshots=1, nt=20)
     tab, src, _, _ = oracle_setup(w, "f32i")
-    for kw in ({"surface_rows": 24, "step_pairs": 1}, {"surface_rows": 24, "ckpt_format": 1},
-               {"surface_rows": 24, "arith": 0}, {"surface_rows": 4}, {"surface_rows": w.nz - w.cpml - 3}):
+    for kw in ({"surface_rows": 24, "ckpt_format": 1}, {"surface_rows": 24, "arith": 0}, {"surface_rows": 4},
+               {"surface_rows": w.nz - w.cpml - 3}):
         with pytest.raises(RuntimeError, match="surface_rows"):
             _plan(w, tab, src, **kw)
-    for kw, rows in (({}, 24), ({"step_pairs": 1}, 0), ({"ckpt_format": 2}, 0), ({"arith": 1}, 0)):
+    for kw, rows in (({}, 24), ({"step_pairs": 1}, 24), ({"ckpt_format": 2}, 0), ({"arith": 1}, 0)):
         plan = _plan(w, tab, src, **kw)
         assert plan.surface_rows == rows, kw
         plan.close()
@@ -457,32 +457,38 @@ def test_adjoint128_checkpoint28_fp32(torch):
 
 
 def test_step_pairs_identical_to_single_steps(torch):
-    """Fused core step pairs (an option) against single steps everywhere (the default): gathers, misfit and exact and
-    reference-compatible gradients bit for bit, with an odd checkpoint interval (pairs stop at checkpoints and
-    at the first kept step), recomputed segments of odd length, and ragged windows, one of them too narrow to
-    have a core."""
-    w = small_workload(nx=200, nz=72, nt=150, cpml=12, nshots=3)
-    w.shots = [M.Shot(x0=0, nx=120, isx=60, isz=6, irz=6, ir0=20, nrec=70),
-               M.Shot(x0=30, nx=170, isx=90, isz=7, irz=6, ir0=12, nrec=170 - 24),
-               M.Shot(x0=150, nx=44, isx=20, isz=6, irz=8, ir0=14, nrec=16)]
-    say("step pairs vs single steps: 3 ragged shots (one without core), K=7, 1 kept segment")
+    """Step pairs (the core advanced two steps per launch) against single steps everywhere: gathers, misfit and
+    exact and reference-compatible gradients bit for bit, with and without the FP64 near-surface band, an odd
+    checkpoint interval (pairs stop at checkpoints and at the first kept step), recomputed segments of odd length,
+    ragged windows (one exactly 2 cpml + 8 wide, so without a core), a source and a receiver row inside the core,
+    and core tiles cut at several row-run lengths."""
+    w = small_workload(nx=420, nz=72, nt=150, cpml=12, nshots=4)
+    w.shots = [M.Shot(x0=0, nx=300, isx=60, isz=6, irz=6, ir0=20, nrec=250),
+               M.Shot(x0=30, nx=170, isx=90, isz=40, irz=33, ir0=12, nrec=170 - 24),
+               M.Shot(x0=150, nx=44, isx=20, isz=6, irz=8, ir0=14, nrec=16),
+               M.Shot(x0=388, nx=32, isx=16, isz=30, irz=6, ir0=6, nrec=20)]
     tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
     obs = torch.from_numpy(_cat(oracle_gathers(w, "f32i", sq_true, tab, src))).cuda()
     model = _dev_model(torch, sq_init)
-    for exact in (1, 0):
-        res = []
-        for pairs in (1, 0):
-            plan = _plan(w, tab, src, ckpt_interval=7, store_segments=1, grad_mode=exact, step_pairs=pairs,
-                         surface_rows=-1)  # step pairs run without the FP64 band
-            assert (plan.pair_rows > 0) == bool(pairs)
-            g = plan.forward(model)
-            phi, G, syn = plan.gradient(model, obs, want_syn=True)
-            torch.cuda.synchronize()
-            res.append((g.cpu().numpy(), phi, G.cpu().numpy(), syn.cpu().numpy()))
-        (g1, p1, G1, s1), (g0, p0, G0, s0) = res
-        assert np.array_equal(g1, g0) and np.array_equal(s1, s0)
-        assert p1 == p0
-        assert np.array_equal(G1, G0), exact
+    for surface, rows in ((0, 16), (-1, 24), (0, 9)):
+        for exact in (1, 0):
+            say(f"step pairs vs single steps: 4 ragged shots, K=7, 1 kept segment, surface_rows={surface}, "
+                f"pair rows {rows}, exact={exact}")
+            res = []
+            for pairs in (1, 0):
+                plan = _plan(w, tab, src, ckpt_interval=7, store_segments=1, grad_mode=exact, step_pairs=pairs,
+                             surface_rows=surface, rows_per_item=rows)
+                assert (plan.pair_rows > 0) == bool(pairs)
+                assert (plan.surface_rows > 0) == (surface == 0)
+                g = plan.forward(model)
+                phi, G, syn = plan.gradient(model, obs, want_syn=True)
+                torch.cuda.synchronize()
+                res.append((g.cpu().numpy(), phi, G.cpu().numpy(), syn.cpu().numpy()))
+                plan.close()
+            (g1, p1, G1, s1), (g0, p0, G0, s0) = res
+            assert np.array_equal(g1, g0) and np.array_equal(s1, s0)
+            assert p1 == p0
+            assert np.array_equal(G1, G0), exact
 
 
 def test_entry_points_refuse_null_and_pageable_arrays(torch):
