@@ -54,8 +54,9 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying captured graphs")
     ap.add_argument("--keep", type=int, default=0, help="trailing segments kept from the forward (0 = memory allows)")
     ap.add_argument("--grad-group", type=int, default=0, help="shots per gradient group (0 = 4)")
-    ap.add_argument("--pairs", type=int, nargs="?", const=1, default=0,
-                    help="step pairs: the core two steps per launch (1 = rows tuned, k > 1 = k rows per core item)")
+    ap.add_argument("--pairs", type=int, default=1,
+                    help="step pairs: the core two steps per launch (1 = rows tuned, k > 1 = k rows per core item, "
+                         "0 = single steps)")
     ap.add_argument("--arith", type=int, default=3,
                     help="0 FP32 two-level, 1 mixed (reference), 2 FP32 difference stencil, 3 FP32 increment form")
     ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")

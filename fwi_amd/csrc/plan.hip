@@ -581,11 +581,16 @@ struct fwi_plan {
       for (auto& r : core_runs)
         for (int x = cx0; x < cx1; x += kPairCols) core.push_back(make_int4(s, x, r.first, r.second));
       for (int k = 0; k < 2; ++k) {
-        const int w = k == 0 ? 4 : 0;  // the first step's shell reaches 4 cells into the core
-        band_items(k, 0, nxs, zf, za + w, 0, nxs);                              // above the core
-        band_items(k, 0, nxs, zb - w, nz - kRim, 0, nxs);                       // below it
-        band_items(k, 0, ((cx0 + w + 63) / 64) * 64, za + w, zb - w, 0, cx0 + w);  // left of it
-        band_items(k, ((cx1 - w) / 64) * 64, nxs, za + w, zb - w, cx1 - w, nxs);   // right of it
+        // The first step's shell reaches 4 cells into the core.  Its four parts must not overlap (a C-PML memory
+        // variable updated twice in one launch would be wrong): in a core thinner than 8 rows or columns the part
+        // above (left of) the core takes the rows (columns) both would reach.
+        const int w = k == 0 ? 4 : 0;
+        const int z1 = std::min(za + w, zb), z2 = std::max(zb - w, z1);
+        const int x1 = std::min(cx0 + w, cx1), x2 = std::max(cx1 - w, x1);
+        band_items(k, 0, nxs, zf, z1, 0, nxs);                      // above the core
+        band_items(k, 0, nxs, z2, nz - kRim, 0, nxs);               // below it
+        band_items(k, 0, ((x1 + 63) / 64) * 64, z1, z2, 0, x1);     // left of it
+        band_items(k, (x2 / 64) * 64, nxs, z1, z2, x2, nxs);        // right of it
       }
     }
     if (int r = upload_items(b.d_items_core, b.nitems_core, core)) return r;
@@ -1405,7 +1410,7 @@ void fwi_options_default(fwi_options* o) {
   o->item_order = -1;
   o->adj_item_order = -1;
   o->use_graph = 1;
-  o->step_pairs = 0;
+  o->step_pairs = 1;
 }
 
 int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out) {

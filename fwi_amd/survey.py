@@ -65,7 +65,7 @@ class Plan:
                  wavelet: np.ndarray, cpml_ab, grad_mode: int = GRAD_EXACT, arith: int = ARITH_FP32I,
                  ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0,
                  item_order: int = -1, grad_group: int = 0, ckpt_format: int = 0, use_graph: int = 1,
-                 store_segments: int = 0, step_pairs: int = 0, surface_rows: int = 0, adj_rows_per_item: int = 0,
+                 store_segments: int = 0, step_pairs: int = 1, surface_rows: int = 0, adj_rows_per_item: int = 0,
                  adj_item_order: int = -1):
         L = _lib.load()
         self.nx, self.nz, self.nt, self.dh, self.dt, self.cpml = nx, nz, nt, dh, dt, cpml

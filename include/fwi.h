@@ -82,9 +82,9 @@ typedef struct fwi_options {
                           fwi_gradient_src is captured once as a HIP graph per set of caller buffers and
                           replayed on later calls with the same buffers; 0: launched eagerly */
   int step_pairs;      /* increment form: the core of every window (4+ cells from the C-PML and below the FP64 band)
-                          advances two steps per launch, the rest by single steps; 1 = rows per core work item
-                          tuned, k > 1 = k rows; 0 (default): single steps everywhere.  The result is the same
-                          either way. */
+                          advances two steps per launch, the rest by single steps; 1 (default) = rows per core work
+                          item tuned, k > 1 = k rows; 0: single steps everywhere.  The result is the same either
+                          way. */
   int surface_rows;    /* FWI_ARITH_FP32I: rows 4..surface_rows-1 of every shot (the source, receivers and near-surface
                           gradient) advance in FP64 in the forward and recomputation passes.  0 (default) = 24 rows
                           where nz - cpml - 4 >= 24 and the plan allows it (FP32 checkpoints), none otherwise;

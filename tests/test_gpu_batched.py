@@ -456,30 +456,44 @@ def test_adjoint128_checkpoint28_fp32(torch):
     assert min(errs) < 2e-3
 
 
-def test_step_pairs_identical_to_single_steps(torch):
-    """Step pairs (the core advanced two steps per launch) against single steps everywhere: gathers, misfit and
-    exact and reference-compatible gradients bit for bit, with and without the FP64 near-surface band, an odd
-    checkpoint interval (pairs stop at checkpoints and at the first kept step), recomputed segments of odd length,
-    ragged windows (one exactly 2 cpml + 8 wide, so without a core), a source and a receiver row inside the core,
-    and core tiles cut at several row-run lengths."""
-    w = small_workload(nx=420, nz=72, nt=150, cpml=12, nshots=4)
-    w.shots = [M.Shot(x0=0, nx=300, isx=60, isz=6, irz=6, ir0=20, nrec=250),
-               M.Shot(x0=30, nx=170, isx=90, isz=40, irz=33, ir0=12, nrec=170 - 24),
-               M.Shot(x0=150, nx=44, isx=20, isz=6, irz=8, ir0=14, nrec=16),
-               M.Shot(x0=388, nx=32, isx=16, isz=30, irz=6, ir0=6, nrec=20)]
+def _pair_workloads():
+    """Ragged windows for the step-pair checks: a core several tiles wide, one 32 columns wide, one window exactly
+    2 cpml + 8 wide (no core), a source and a receiver row inside the core; nz chosen for cores 28, 1 and 4 rows tall
+    under the FP64 band (thin cores make the parts of the shell meet)."""
+    out = []
+    for nz, cpml in ((72, 12), (45, 12), (41, 5)):
+        w = small_workload(nx=420, nz=nz, nt=150, cpml=cpml, nshots=4)
+        zc = min(nz - cpml - 5, 30)  # a row inside the core without the band
+        w.shots = [M.Shot(x0=0, nx=300, isx=60, isz=6, irz=6, ir0=20, nrec=250),
+                   M.Shot(x0=30, nx=170, isx=90, isz=zc, irz=zc - 2, ir0=12, nrec=170 - 24),
+                   M.Shot(x0=150, nx=2 * cpml + 8 + 32, isx=20, isz=6, irz=8, ir0=14, nrec=16),
+                   M.Shot(x0=420 - 2 * cpml - 8, nx=2 * cpml + 8, isx=cpml + 4, isz=zc, irz=6, ir0=6,
+                          nrec=2 * cpml - 4)]
+        out.append(w)
+    return out
+
+
+@pytest.mark.parametrize("geometry", [0, 1, 2])
+def test_step_pairs_identical_to_single_steps(torch, geometry):
+    """Step pairs (the core advanced two steps per launch, the default) against single steps everywhere: gathers,
+    misfit and exact and reference-compatible gradients bit for bit, with and without the FP64 near-surface band, an
+    odd checkpoint interval (pairs stop at checkpoints and at the first kept step), recomputed segments of odd
+    length, ragged windows, a source and a receiver row inside the core, core tiles cut at several row-run lengths
+    and cores 1 and 4 rows tall."""
+    w = _pair_workloads()[geometry]
     tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
     obs = torch.from_numpy(_cat(oracle_gathers(w, "f32i", sq_true, tab, src))).cuda()
     model = _dev_model(torch, sq_init)
     for surface, rows in ((0, 16), (-1, 24), (0, 9)):
         for exact in (1, 0):
-            say(f"step pairs vs single steps: 4 ragged shots, K=7, 1 kept segment, surface_rows={surface}, "
-                f"pair rows {rows}, exact={exact}")
+            say(f"step pairs vs single steps: nz={w.nz} cpml={w.cpml}, 4 ragged shots, K=7, 1 kept segment, "
+                f"surface_rows={surface}, pair rows {rows}, exact={exact}")
             res = []
-            for pairs in (1, 0):
+            for pairs in (rows, 0):
                 plan = _plan(w, tab, src, ckpt_interval=7, store_segments=1, grad_mode=exact, step_pairs=pairs,
-                             surface_rows=surface, rows_per_item=rows)
+                             surface_rows=surface, rows_per_item=8)
                 assert (plan.pair_rows > 0) == bool(pairs)
-                assert (plan.surface_rows > 0) == (surface == 0)
+                assert (plan.surface_rows > 0) == (surface == 0 and w.nz - w.cpml - 4 >= 24)
                 g = plan.forward(model)
                 phi, G, syn = plan.gradient(model, obs, want_syn=True)
                 torch.cuda.synchronize()

@@ -8,8 +8,8 @@ The oracle cannot run 64 shots x 4000 steps in seconds, so the full size is cove
   * size-independent properties of the full 4000-step, 64-shot gradient in the benchmark's configuration
     (default options, captured graphs, the tuner's tiling, trailing segments kept to the memory budget):
     a source delayed by m steps delays every gather by m steps bit for bit, and a second plan with another
-    checkpoint interval, one kept segment, eager launches and another tiling returns the same misfit and the
-    same gradient bits (recomputation and the schedule change nothing).
+    checkpoint interval, one kept segment, eager launches, another tiling and single steps instead of step pairs
+    returns the same misfit and the same gradient bits (recomputation and the schedule change nothing).
 """
 import numpy as np
 import pytest
@@ -87,7 +87,7 @@ def test_survey1600_full_gradient_properties(torch):
         phi_a, G_a, _ = plan.gradient(init_d, obs)
         torch.cuda.synchronize()
         assert np.isfinite(phi_a) and phi_a > 0 and torch.isfinite(G_a).all().item()
-        kept, tiling = plan.kept_steps, (plan.rows_per_item, plan.item_order, plan.adj_tiling)
+        kept, tiling, plan_pairs = plan.kept_steps, (plan.rows_per_item, plan.item_order, plan.adj_tiling), plan.pair_rows
         print(f"misfit {phi_a:.9e}; kept steps {kept}, K {plan.ckpt_interval}, tiling {tiling}")
         # time shift: the whole survey's gathers move by m steps, nothing else changes
         m = 173
@@ -102,12 +102,14 @@ def test_survey1600_full_gradient_properties(torch):
         del shifted
     finally:
         plan.close()
-    other = _plan(w, tab, src, ckpt_interval=50, store_segments=1, use_graph=0, rows_per_item=32, item_order=0)
+    other = _plan(w, tab, src, ckpt_interval=50, store_segments=1, use_graph=0, rows_per_item=32, item_order=0,
+                  step_pairs=0)
     try:
         assert other.kept_steps != kept
         phi_b, G_b, _ = other.gradient(init_d, obs)
         torch.cuda.synchronize()
-        print(f"second plan: K {other.ckpt_interval}, kept {other.kept_steps}, misfit {phi_b:.9e}")
+        assert plan_pairs > 0 and other.pair_rows == 0
+        print(f"second plan: K {other.ckpt_interval}, kept {other.kept_steps}, single steps, misfit {phi_b:.9e}")
         assert phi_b == phi_a
         assert torch.equal(G_b, G_a)
     finally:
