@@ -60,62 +60,6 @@ struct GridArgs {
   const float* tb;  // C-PML b
 };
 
-// Forward step.  P/F "cur" hold step n, "old" hold n-1 and receive n+1.  In the increment form "old"
-// only receives n+1, and Q, R (the increments P^n - P^{n-1}, F^n - F^{n-1}) are read from Qc, Rc and
-// written to Qo, Ro.
-struct FwdArgs {
-  GridArgs g;
-  ModelArgs m;
-  const int4* items;  // (shot, band x0, z0, z1)
-  int nitems;
-  float C;            // dt^2/dh^2
-  const float* src;
-  const float* Pc;
-  float* Po;
-  const float* Fc;
-  float* Fo;
-  float* psx;
-  float* psz;
-  float* zx;
-  float* zz;
-  const float* Qc;  // increment form: Q, R of step n (read) and of step n+1 (written)
-  const float* Rc;
-  float* Qo;
-  float* Ro;
-  float* gather;  // batch gathers or null
-  float* qa;      // per-step store or null
-  float* qb;
-  int it;
-  const int2* xr; // per-item column range [x, y) the item updates, or null (its whole band)
-};
-
-// Two forward steps n, n+1 of the increment form.  Core items: (shot, first of up to 224 output columns, z0, z1)
-// within the core (columns [cpml+4, nx-cpml-4) rounded inward to multiples of 32, rows [zfirst+4, nz-cpml-4)); the launch reads the set of step n and
-// writes the set of step n+2 on the core and (blocks [0, nshell_wg), sh: single-step arguments with their items and
-// column ranges) the set of step n+1 on the shell and the core's cells within 4 of it.
-struct FwdPairArgs {
-  GridArgs g;
-  ModelArgs m;
-  const int4* items;
-  int nitems;
-  float C;
-  const float* src;
-  const float* P0;
-  const float* F0;
-  const float* Q0;
-  const float* R0;
-  float* P2;
-  float* F2;
-  float* Q2;
-  float* R2;
-  float* gather;     // batch gathers or null
-  float* qa0;        // imaging terms (QM 1: a, b; QM 2: p2, r2): a, b of step n at qa0, qa0 + qstride bytes, of
-  unsigned qstride;  // step n+1 at qa0 + 2 qstride, qa0 + 3 qstride; qa0 null when QM = 0
-  int it;            // n
-  FwdArgs sh;        // step n of the shell and the core's cells next to it
-  int nshell_wg;     // shell workgroups (a multiple of 8)
-};
-
 // FP64 near-surface band of the increment-form forward: rows [4, band) of every shot advance in FP64 (two levels
 // of P and F and of psi_x by step parity, zeta_x on the side C-PML cells); one workgroup per (256-column chunk,
 // band row, shot).
@@ -140,6 +84,71 @@ struct BandArgs {
   float* gather;
   float* qa;
   float* qb;
+};
+
+// Forward step.  P/F "cur" hold step n, "old" hold n-1 and receive n+1.  In the increment form "old"
+// only receives n+1, and Q, R (the increments P^n - P^{n-1}, F^n - F^{n-1}) are read from Qc, Rc and
+// written to Qo, Ro.
+// The C-PML memory variables psi_x, psi_z are kept by step parity: step n reads psi^n (psx, psz) and writes psi^{n+1}
+// (psxn, pszn) on the C-PML cells it updates.  Workgroups [0, nband_wg) of the launch advance the FP64 near-surface
+// band of the same step (band; nbc 256-column chunks per band row), the rest the items.
+struct FwdArgs {
+  GridArgs g;
+  ModelArgs m;
+  const int4* items;  // (shot, band x0, z0, z1)
+  int nitems;
+  float C;            // dt^2/dh^2
+  const float* src;
+  const float* Pc;
+  float* Po;
+  const float* Fc;
+  float* Fo;
+  const float* psx;
+  const float* psz;
+  float* psxn;
+  float* pszn;
+  float* zx;
+  float* zz;
+  const float* Qc;  // increment form: Q, R of step n (read) and of step n+1 (written)
+  const float* Rc;
+  float* Qo;
+  float* Ro;
+  float* gather;  // batch gathers or null
+  float* qa;      // per-step store or null
+  float* qb;
+  int it;
+  const int2* xr; // per-item column range [x, y) the item updates, or null (its whole band)
+  BandArgs band;  // the FP64 band's step (nband_wg = 0: none)
+  int nband_wg;   // a multiple of 8, of which nband_real band workgroups
+  int nband_real;
+  int nbc;
+};
+
+// Two forward steps n, n+1 of the increment form.  Core items: (shot, first of up to 224 output columns, z0, z1)
+// within the core (columns [cpml+4, nx-cpml-4) rounded inward to multiples of 32, rows [zfirst+4, nz-cpml-4)); the
+// launch reads the set of step n and writes the set of step n+2 on the core and (sh: single-step arguments with their
+// items and column ranges) the set of step n+1 on the FP64 band, the shell and the core's cells within 4 of it.
+struct FwdPairArgs {
+  GridArgs g;
+  ModelArgs m;
+  const int4* items;
+  int nitems;
+  float C;
+  const float* src;
+  const float* P0;
+  const float* F0;
+  const float* Q0;
+  const float* R0;
+  float* P2;
+  float* F2;
+  float* Q2;
+  float* R2;
+  float* gather;     // batch gathers or null
+  float* qa0;        // imaging terms (QM 1: a, b; QM 2: p2, r2): a, b of step n at qa0, qa0 + qstride bytes, of
+  unsigned qstride;  // step n+1 at qa0 + 2 qstride, qa0 + 3 qstride; qa0 null when QM = 0
+  int it;            // n
+  FwdArgs sh;        // step n of the shell and the core's cells next to it, and of the FP64 band
+  int nshell_wg;     // shell workgroups (a multiple of 8), after the sh.nband_wg band workgroups
 };
 
 // Strip (C-PML) blocks: (shot, first strip cell) per 256-thread block.

@@ -25,36 +25,6 @@ namespace fwi {
 // Forward
 // ---------------------------------------------------------------------------------------------
 
-// psi_px, psi_fz on the C-PML cells (parallelcalls.cu:48-63).  Cells of the side strips carry
-// psi_px; cells of the bottom strip carry psi_fz (the other variable keeps a = b = 0 there and stays 0).
-template <class S>
-__device__ __forceinline__ void fwd_strip_block(const StripArgs& a, int block, const float* __restrict__ Pc,
-                                                const float* __restrict__ Fc, float* __restrict__ psx,
-                                                float* __restrict__ psz) {
-  const int2 blk = a.blocks[block];
-  const ShotDev sd = a.g.shots[blk.x];
-  const int k = blk.y + threadIdx.x;
-  if (k >= sd.nstrip) return;
-  const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml;
-  int ix, iz;
-  strip_cell(k, nx, nz, cpml, ix, iz);
-  const long i = sd.off + ix + (long)ps * iz;
-  if (ix < cpml || ix > nx - cpml - 1) {
-    const float t = d1_at<S>(Pc, i, 1, a.g.dh);
-    psx[i] = coef_x(a.g.tb, ix, nx, cpml) * psx[i] + coef_x(a.g.ta, ix, nx, cpml) * t;
-  }
-  if (iz > nz - cpml - 1) {
-    const float t = d1_at<S>(Fc, i, ps, a.g.dh);
-    psz[i] = coef_z(a.g.tb, iz, nz, cpml) * psz[i] + coef_z(a.g.ta, iz, nz, cpml) * t;
-  }
-}
-template <class S>
-__global__ void __launch_bounds__(256) k_fwd_strip(StripArgs a, const float* __restrict__ Pc,
-                                                   const float* __restrict__ Fc, float* __restrict__ psx,
-                                                   float* __restrict__ psz) {
-  fwd_strip_block<S>(a, xcd_logical_block(blockIdx.x, gridDim.x), Pc, Fc, psx, psz);
-}
-
 // The same step on an interior work item: a band within [cpml, nx-cpml) and rows above nz-cpml, where no cell
 // carries a C-PML term and every lane updates its cell.  The body has one path (no lane-masked memory access)
 // and loads each row two rows ahead, before the current row's stores: the vector memory counter retires in
@@ -190,10 +160,7 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
   if (z + 1 < z1) step(z + 1, rb, ra);
 }
 
-// One forward step of every shot (parallelcalls.cu:120-130 SecondDerivate_P, :65-86 ZETA_F, :88-117
-// propagator_F).  QM selects what is kept for the adjoint pass: 0 nothing, 1 the model-multiplied terms
-// a = p2 + aten_p, b = r2 + aten_f (exact gradient), 2 p2, r2 (reference-compatible gradient).
-// INC: increment form of the same leapfrog.  The state is (P^n, Q^n = P^n - P^{n-1}) and the update is
+// INC: increment form of the forward leapfrog.  The state is (P^n, Q^n = P^n - P^{n-1}) and the update is
 // Q^{n+1} = Q^n + d, P^{n+1} = P^n + Q^{n+1}, which equals P^{n+1} = 2P^n - P^{n-1} + d in exact
 // arithmetic.  In FP32 the stored Q keeps its own relative precision, whereas the difference of two
 // stored levels loses it: with s samples per period the rounding of the two-level form is amplified
@@ -202,12 +169,20 @@ __device__ __forceinline__ void fwd_interior(const FwdArgs& a, const int4 itm, c
 // four interior items on one model tile (the host sets it on all four or none), which share the model rows.
 constexpr int kShareModel = 1 << 30;
 
-// The general body: any item, C-PML terms per cell; with a.xr, only the item's column range is updated.
-template <int QM, class S, bool INC>
-__device__ __forceinline__ void fwd_general(const FwdArgs& a, int item, const int4 itm, const ShotDev& sd, int lane,
-                                            float* buf) {
+// The general body: any item, with the C-PML terms of every cell (parallelcalls.cu:48-117); with a.xr, only the
+// item's column range is updated.  The C-PML memory variables psi of the step are computed in the item, as PSI_F
+// computes them (parallelcalls.cu:48-63), from psi^n (a.psx, a.psz) and P^n, F^n, and psi^{n+1} is stored on the
+// item's own C-PML cells (a.psxn, a.pszn): psi_x^{n+1} on the item's 64 columns and the 4 beyond each side, from an
+// exchange of P^n over 80 columns; psi_z^{n+1} 4 rows ahead of the current row, from F^n in a 13-row queue (items
+// that reach within 4 rows of the bottom C-PML).  Items that meet compute each other's border psi^{n+1} from the
+// same inputs, so every cell reads the same bits, and psi^n and psi^{n+1} are separate arrays, so no item reads a
+// value another one has already advanced.  A cell off the C-PML strips keeps psi = 0.
+// buf: this wave's 152 LDS floats (80 for P^n, 72 for psi_x^{n+1}).
+template <int QM, class S, bool INC, bool DEEPZ>
+__device__ __forceinline__ void fwd_general_rows(const FwdArgs& a, int item, const int4 itm, const ShotDev& sd,
+                                                 int lane, float* buf) {
   const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, cpml = a.g.cpml, nxm = a.m.nxm;
-  const int ix = itm.y + lane;
+  const int x0 = itm.y, ix = x0 + lane;
   bool act = ix >= kRim && ix < nx - kRim;
   if (a.xr) {  // shell items of the step-pair scheme update only their column range
     const int2 r = a.xr[item];
@@ -215,7 +190,21 @@ __device__ __forceinline__ void fwd_general(const FwdArgs& a, int item, const in
   }
   // every lane loads from a valid column; values of columns outside the shot are never consumed
   const int cx = clampi(ix, 0, nx - 1);
-  const int hx = lane < 8 ? clampi(itm.y + halo_col(lane), 0, nx - 1) : cx;
+  // P^n on columns x0-8 .. x0+71 (slot = column - x0 + 8): each lane its column, lanes 0..15 one more
+  const int hx = lane < 8 ? clampi(x0 - 8 + lane, 0, nx - 1) : lane < 16 ? clampi(x0 + 56 + lane, 0, nx - 1) : cx;
+  const int hslot = lane < 8 ? lane : lane < 16 ? 64 + lane : 8 + lane;
+  // psi_x^{n+1} on columns x0-4 .. x0+67: each lane its column, lanes 0..7 one more (lanes 8.. their own again)
+  const int pcol = lane < 4 ? x0 - 4 + lane : lane < 8 ? x0 + 60 + lane : ix;
+  const int pslot = pcol - x0 + 8;
+  const int pcx = clampi(pcol, 0, nx - 1);
+  const int hidx9 = xrow9_halo_slot(lane);
+  auto xs_col = [&](int c) { return (c >= kRim && c < cpml) || (c >= nx - cpml && c < nx - kRim); };
+  const bool xs_own = xs_col(ix), xs_h = xs_col(pcol);
+  const float pax = xs_own ? coef_x(a.g.ta, ix, nx, cpml) : 0.f, pbx = xs_own ? coef_x(a.g.tb, ix, nx, cpml) : 0.f;
+  const float hax = xs_h ? coef_x(a.g.ta, pcol, nx, cpml) : 0.f, hbx = xs_h ? coef_x(a.g.tb, pcol, nx, cpml) : 0.f;
+  // a side C-PML cell within 4 columns of the item: psi_x enters (wave-uniform); otherwise psi_x is 0 around it
+  const bool need_px = x0 - 4 < cpml || x0 + 68 > nx - cpml;
+  const bool zcol = ix >= kRim && ix < nx - kRim;  // psi_z is updated on the bottom strip rows of these columns
   const bool xstrip = ix < cpml || ix > nx - cpml - 1;
   const float axv = act ? coef_x(a.g.ta, ix, nx, cpml) : 0.f;
   const float bxv = act ? coef_x(a.g.tb, ix, nx, cpml) : 0.f;
@@ -226,6 +215,8 @@ __device__ __forceinline__ void fwd_general(const FwdArgs& a, int item, const in
   float* Fo = a.Fo + sd.off;
   const float* psx = a.psx + sd.off;
   const float* psz = a.psz + sd.off;
+  float* psxn = a.psxn + sd.off;
+  float* pszn = a.pszn + sd.off;
   float* zx = a.zx + sd.off;
   float* zz = a.zz + sd.off;
   const float* vh = a.m.vh + sd.x0;
@@ -233,32 +224,72 @@ __device__ __forceinline__ void fwd_general(const FwdArgs& a, int item, const in
   const float* vn = a.m.vn + sd.x0;
   const float srcv = a.src[a.it];
   const int z0 = itm.z, z1 = itm.w;
-  float fq[9];
+  auto zs_row = [&](int r) { return zcol && r >= nz - cpml && r < nz - kRim; };
+  // psi_z^{n+1} of row r from F^n at rows r-4..r+4 (f[0..8])
+  auto psiz = [&](int r, const float* f) -> float {
+    if (!zs_row(r)) return 0.f;
+    return coef_z(a.g.tb, r, nz, cpml) * ldu(psz, (unsigned)(cx + ps * r)) +
+           coef_z(a.g.ta, r, nz, cpml) * d1s<S>(f[0], f[1], f[2], f[3], f[5], f[6], f[7], f[8], dh);
+  };
+  constexpr int NF = DEEPZ ? 13 : 9;
+  float fq[NF];  // F^n, rows z-4 .. z+4 (+4 more with DEEPZ)
+  float zq[9];   // DEEPZ: psi_z^{n+1}, rows z-4 .. z+4
+  if (DEEPZ) {
+    float f16[16];  // rows z0-8 .. z0+7 (rows outside the grid feed no C-PML row)
 #pragma unroll
-  for (int k = 0; k < 8; ++k) fq[k] = ldu(F, (unsigned)(cx + ps * (z0 - 4 + k)));
+    for (int k = 0; k < 16; ++k) f16[k] = ldu(F, (unsigned)(cx + ps * clampi(z0 - 8 + k, 0, nz - 1)));
+#pragma unroll
+    for (int j = 0; j < 8; ++j) zq[j] = psiz(z0 - 4 + j, f16 + j);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) fq[k] = f16[k + 4];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) fq[k] = ldu(F, (unsigned)(cx + ps * (z0 - 4 + k)));
+  }
   for (int z = z0; z < z1; ++z) {
     const unsigned o = (unsigned)(cx + ps * z);
     const unsigned mo = (unsigned)(cx + nxm * z);
-    fq[8] = ldu(F, o + 4u * ps);
+    fq[NF - 1] = ldu(F, (unsigned)(cx + ps * min(z + NF - 5, nz - 1)));
     const float c = ldu(P, o);
     const float h = ldu(P, (unsigned)(hx + ps * z));
     // two-level form: P^{n-1}, F^{n-1}; increment form: Q^n, R^n
     const float pold = INC ? ldn(a.Qc + sd.off, o) : ldn(Po, o);
     const float fold = INC ? ldn(a.Rc + sd.off, o) : ldn(Fo, o);
     const float vhv = ldu(vh, mo), vpv = ldu(vp, mo), vnv = ldu(vn, mo);
+    float tz = 0.f;
+    if (DEEPZ) {
+      zq[8] = psiz(z + 4, fq + 4);
+      tz = d1s<S>(zq[0], zq[1], zq[2], zq[3], zq[5], zq[6], zq[7], zq[8], dh);
+    }
     float u[9];
-    xrow9(buf, lane, c, h, u);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    buf[8 + lane] = c;
+    buf[hslot] = h;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int k = 0; k < 9; ++k) u[k] = buf[lane + 4 + k];
+    float tx = 0.f, pxo = 0.f;
+    if (need_px) {
+      float v[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) v[k] = buf[pslot - 4 + k];
+      pxo = xs_own ? pbx * ldu(psx, o) + pax * d1s<S>(u[0], u[1], u[2], u[3], u[5], u[6], u[7], u[8], dh) : 0.f;
+      const float pxh = xs_h ? hbx * ldu(psx, (unsigned)(pcx + ps * z)) +
+                                   hax * d1s<S>(v[0], v[1], v[2], v[3], v[5], v[6], v[7], v[8], dh)
+                             : 0.f;
+      float w[9];
+      xrow9u(buf + 80, lane, hidx9, pxo, pxh, w);
+      tx = d1s<S>(w[0], w[1], w[2], w[3], w[5], w[6], w[7], w[8], dh);
+    }
     if (act) {
       const float p2 = d2s<S>(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8], dh2);
       const float r2 = d2s<S>(fq[0], fq[1], fq[2], fq[3], fq[4], fq[5], fq[6], fq[7], fq[8], dh2);
       float atp = 0.f, atf = 0.f;
       if (xstrip || z > nz - cpml - 1) {
-        const unsigned snx = (unsigned)ps;
-        const float tx = d1s<S>(ldu(psx, o - 4), ldu(psx, o - 3), ldu(psx, o - 2), ldu(psx, o - 1), ldu(psx, o + 1),
-                                ldu(psx, o + 2), ldu(psx, o + 3), ldu(psx, o + 4), dh);
-        const float tz = d1s<S>(ldu(psz, o - 4 * snx), ldu(psz, o - 3 * snx), ldu(psz, o - 2 * snx),
-                                ldu(psz, o - snx), ldu(psz, o + snx), ldu(psz, o + 2 * snx), ldu(psz, o + 3 * snx),
-                                ldu(psz, o + 4 * snx), dh);
         const float tpx = p2 + tx;
         const float tfz = r2 + tz;
         const float azv = coef_z(a.g.ta, z, nz, cpml), bzv = coef_z(a.g.tb, z, nz, cpml);
@@ -269,6 +300,8 @@ __device__ __forceinline__ void fwd_general(const FwdArgs& a, int item, const in
         atp = tx + zxn;
         atf = tz + zzn;
       }
+      if (xs_own) stu(psxn, o, pxo);
+      if (DEEPZ && zs_row(z)) stu(pszn, o, zq[4]);
       const float tp = dh2 * (p2 + atp);
       const float tf = dh2 * (r2 + atf);
       float pn, fn;
@@ -305,34 +338,20 @@ __device__ __forceinline__ void fwd_general(const FwdArgs& a, int item, const in
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) fq[k] = fq[k + 1];
+    for (int k = 0; k < NF - 1; ++k) fq[k] = fq[k + 1];
+    if (DEEPZ) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) zq[k] = zq[k + 1];
+    }
   }
 }
-
-template <int QM, class S, bool INC = false>
-__global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
-  __shared__ float xbuf[4][72];
-  __shared__ float mshare[2][4][64];
-  const int wave = wave_id(), lane = threadIdx.x & 63;
-  const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
-  if (item >= a.nitems) return;
-  int4 itm = a.items[item];
-  const bool share = (itm.x & kShareModel) != 0;
-  itm.x &= ~kShareModel;
-  const ShotDev sd = a.g.shots[itm.x];
-  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
-  // Interior items (no C-PML cell; the items are cut at row nz-e, e = max(cpml,4)) take the pipelined body.
-  // Items with C-PML cells take the general body, interleaved with them in the launch.
-  if (share) {
-    fwd_interior<QM, S, INC, true>(a, itm, sd, lane, xbuf[wave], mshare);
-    return;
-  }
-  const int e = cpml > kRim ? cpml : kRim;
-  if (!a.xr && itm.y >= e && itm.y + 64 <= nx - e && itm.w <= nz - e) {
-    fwd_interior<QM, S, INC>(a, itm, sd, lane, xbuf[wave]);
-    return;
-  }
-  fwd_general<QM, S, INC>(a, item, itm, sd, lane, xbuf[wave]);
+template <int QM, class S, bool INC>
+__device__ __forceinline__ void fwd_general(const FwdArgs& a, int item, const int4 itm, const ShotDev& sd, int lane,
+                                            float* buf) {
+  if (itm.w > a.g.nz - a.g.cpml - kRim)  // within 4 rows of the bottom C-PML: psi_z enters
+    fwd_general_rows<QM, S, INC, true>(a, item, itm, sd, lane, buf);
+  else
+    fwd_general_rows<QM, S, INC, false>(a, item, itm, sd, lane, buf);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -462,35 +481,52 @@ __device__ __forceinline__ void fwd_band_block(const BandArgs& a, int chunk, int
       a.gather[sd.goff + (long)a.it * sd.nrec + (ix - sd.ir0)] = (float)(pn + fn);
   }
 }
-// Grid: nbc chunks x (band - 4) rows x shots workgroups, in that order.
+// One band workgroup of a forward launch: workgroups [0, a.nband_wg) are (chunk, band row group, shot) in that order,
+// nbc chunks per row, dealt to the XCDs in contiguous ranges of their own (the rows a z stencil reads come from
+// workgroups on the same XCD); the last nband_wg - nband_real do nothing.
 template <int QM>
-__global__ void __launch_bounds__(256) k_fwd_band(BandArgs a, int nbc) {
-  int b = xcd_logical_block(blockIdx.x, gridDim.x);
-  const int groups = (a.band - kRim + kBandRows - 1) / kBandRows;
-  const int chunk = b % nbc;
-  b /= nbc;
-  fwd_band_block<QM>(a, chunk, b % groups, b / groups);
+__device__ __forceinline__ void fwd_band_wg(const FwdArgs& a) {
+  int b = xcd_logical_block(blockIdx.x, a.nband_wg);
+  if (b >= a.nband_real) return;
+  const int groups = (a.band.band - kRim + kBandRows - 1) / kBandRows;
+  const int chunk = b % a.nbc;
+  b /= a.nbc;
+  fwd_band_block<QM>(a.band, chunk, b % groups, b / groups);
 }
 
-// The forward C-PML strip kernel and the FP64 band in one launch: they read step n and write disjoint arrays, and
-// both are small and latency-bound, so their workgroups fill each other's gaps.  Blocks [0, s.nblocks) are strip
-// blocks, the rest band workgroups in (chunk, band row, shot) order with nbc chunks per row.  Each part is dealt
-// to the XCDs in contiguous logical ranges of its own, so every XCD gets an eighth of the strip work and an eighth
-// of the band work, and the rows a z stencil reads come from workgroups on the same XCD (the same L2).  (One
-// contiguous range over both parts put the FP64 band on four XCDs only: 67 instead of 55 us per step.)
-template <class S, int QM>
-__global__ void __launch_bounds__(256) k_fwd_strip_band(StripArgs s, BandArgs a, int nbc, const float* __restrict__ Pc,
-                                                        const float* __restrict__ Fc, float* __restrict__ psx,
-                                                        float* __restrict__ psz) {
-  if ((int)blockIdx.x < s.nblocks) {
-    fwd_strip_block<S>(s, xcd_logical_block(blockIdx.x, s.nblocks), Pc, Fc, psx, psz);
+// One forward step of every shot (parallelcalls.cu:120-130 SecondDerivate_P, :48-86 PSI_F / ZETA_F, :88-117
+// propagator_F) in one launch: the FP64 near-surface band's workgroups first (increment form), then the items --
+// interior items by the pipelined bodies, items with C-PML cells by the general body.  QM selects what is kept for
+// the adjoint pass: 0 nothing, 1 the model-multiplied terms a = p2 + aten_p, b = r2 + aten_f (exact gradient), 2 p2,
+// r2 (reference-compatible gradient).
+template <int QM, class S, bool INC = false>
+__global__ void __launch_bounds__(256) k_fwd_main(FwdArgs a) {
+  __shared__ float xbuf[4][152];
+  __shared__ float mshare[2][4][64];
+  if ((int)blockIdx.x < a.nband_wg) {
+    if constexpr (INC) fwd_band_wg<QM>(a);
     return;
   }
-  int b = xcd_logical_block((int)blockIdx.x - s.nblocks, (int)gridDim.x - s.nblocks);
-  const int groups = (a.band - kRim + kBandRows - 1) / kBandRows;
-  const int chunk = b % nbc;
-  b /= nbc;
-  fwd_band_block<QM>(a, chunk, b % groups, b / groups);
+  const int wave = wave_id(), lane = threadIdx.x & 63;
+  const int item = xcd_logical_block((int)blockIdx.x - a.nband_wg, (int)gridDim.x - a.nband_wg) * 4 + wave;
+  if (item >= a.nitems) return;
+  int4 itm = a.items[item];
+  const bool share = (itm.x & kShareModel) != 0;
+  itm.x &= ~kShareModel;
+  const ShotDev sd = a.g.shots[itm.x];
+  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
+  // Interior items (no C-PML cell; the items are cut at row nz-e, e = max(cpml,4)) take the pipelined body.
+  // Items with C-PML cells take the general body, interleaved with them in the launch.
+  if (share) {
+    fwd_interior<QM, S, INC, true>(a, itm, sd, lane, xbuf[wave], mshare);
+    return;
+  }
+  const int e = cpml > kRim ? cpml : kRim;
+  if (!a.xr && itm.y >= e && itm.y + 64 <= nx - e && itm.w <= nz - e) {
+    fwd_interior<QM, S, INC>(a, itm, sd, lane, xbuf[wave]);
+    return;
+  }
+  fwd_general<QM, S, INC>(a, item, itm, sd, lane, xbuf[wave]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -518,22 +554,6 @@ constexpr int kPairCols = 224;
 // The core's columns: [cpml+4, nx-cpml-4) rounded inward to multiples of 32.
 __host__ __device__ inline int pair_cx0(int cpml) { return (cpml + 4 + 31) & ~31; }
 __host__ __device__ inline int pair_cx1(int nx, int cpml) { return (nx - cpml - 4) & ~31; }
-
-// xrow9 without lane-dependent branches: every lane writes its value and its halo value, lanes >= 8 (whose halo
-// value is their own) to their own slot again.  hidx = xrow9_halo_slot(lane).
-__device__ __forceinline__ int xrow9_halo_slot(int lane) { return lane < 4 ? lane : lane < 8 ? 64 + lane : 4 + lane; }
-__device__ __forceinline__ void xrow9u(float* buf, int lane, int hidx, float v, float h, float (&u)[9]) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  buf[4 + lane] = v;
-  buf[hidx] = h;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-  for (int k = 0; k < 9; ++k) u[k] = buf[lane + k];
-}
 
 template <int QM>
 __device__ __forceinline__ void fwd_pair_core(const FwdPairArgs& a, const int4 itm, float* xb, float (*ring)[264]) {
@@ -709,22 +729,29 @@ __device__ __forceinline__ void fwd_pair_core(const FwdPairArgs& a, const int4 i
   if (z + 2 < zend) iter(z + 2, b2, b0, Main{});
 }
 
-// Grid: [0, nshell_wg) shell workgroups (four single-step items each), then one workgroup per core item.  Each part
-// is dealt to the XCDs in contiguous logical ranges of its own (nshell_wg is a multiple of 8).
+// Grid: [0, sh.nband_wg) FP64 band workgroups of step n, [.., + nshell_wg) shell workgroups (four single-step items
+// each), then one workgroup per core item.  Each part is dealt to the XCDs in contiguous logical ranges of its own
+// (the first two counts are multiples of 8).
 template <int QM>
 __global__ void __launch_bounds__(256) k_fwd_pair(FwdPairArgs a) {
-  __shared__ float xbuf[4][72];
+  __shared__ float xbuf[4][152];
   __shared__ float ring[6][264];
   const int wave = wave_id(), lane = threadIdx.x & 63;
-  if ((int)blockIdx.x < a.nshell_wg) {  // step n of the shell and the core's cells next to it
-    const int item = xcd_logical_block(blockIdx.x, a.nshell_wg) * 4 + wave;
+  const int nb = a.sh.nband_wg;
+  if ((int)blockIdx.x < nb) {
+    fwd_band_wg<QM>(a.sh);
+    return;
+  }
+  const int bid = (int)blockIdx.x - nb;
+  if (bid < a.nshell_wg) {  // step n of the shell and the core's cells next to it
+    const int item = xcd_logical_block(bid, a.nshell_wg) * 4 + wave;
     if (item >= a.sh.nitems) return;
     const int4 itm = a.sh.items[item];
     const ShotDev sd = a.g.shots[itm.x];
     fwd_general<QM, float, true>(a.sh, item, itm, sd, lane, xbuf[wave]);
     return;
   }
-  const int b = xcd_logical_block((int)blockIdx.x - a.nshell_wg, (int)gridDim.x - a.nshell_wg);
+  const int b = xcd_logical_block(bid - a.nshell_wg, (int)gridDim.x - nb - a.nshell_wg);
   if (b >= a.nitems) return;
   fwd_pair_core<QM>(a, a.items[b], xbuf[wave], ring);
 }

@@ -124,6 +124,22 @@ __device__ __forceinline__ void xrow9(float* buf, int lane, float v, float h, fl
 // Band column of the halo value a lane supplies to xrow9: lanes 0..3 -> -4..-1, lanes 4..7 -> 64..67.
 __device__ __forceinline__ int halo_col(int lane) { return lane < 4 ? lane - 4 : 60 + lane; }
 
+// xrow9 without lane-dependent branches: every lane writes its value and its halo value, lanes >= 8 (whose halo
+// value is their own) to their own slot again.  hidx = xrow9_halo_slot(lane).
+__device__ __forceinline__ int xrow9_halo_slot(int lane) { return lane < 4 ? lane : lane < 8 ? 64 + lane : 4 + lane; }
+__device__ __forceinline__ void xrow9u(float* buf, int lane, int hidx, float v, float h, float (&u)[9]) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  buf[4 + lane] = v;
+  buf[hidx] = h;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int k = 0; k < 9; ++k) u[k] = buf[lane + k];
+}
+
 // Load/store through a wave-uniform base pointer and a 32-bit element index: the address is
 // base + zext(4 * idx), which the compiler emits as one global access with the base in scalar registers
 // and a 32-bit vector offset (no per-lane 64-bit address arithmetic).  Per-shot fields are far below

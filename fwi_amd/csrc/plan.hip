@@ -122,11 +122,11 @@ struct fwi_plan {
   size_t ck_esize = 4;              // bytes per checkpoint value (4 FP32, 2 BF16 / FP16)
   int max_count = 0;                // shots in the largest batch
   unsigned* d_ck_amax = nullptr;    // 16-bit checkpoints: max-abs bits per (checkpoint, shot, field)
-  // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute),
-  // 2 = adjoint step, 3 / 4 = fused core step pair without / with store, 5 = forward C-PML strip (with the FP64
-  // near-surface band when both run), 6 = the two adjoint C-PML strip kernels, 7 = FP64 near-surface band alone,
-  // 8 = the shell's second step of a step pair; every `timing_stride`-th launch of a kind is bracketed by events on
-  // its stream
+  // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute), each one launch
+  // with the FP64 near-surface band, 2 = adjoint step, 3 / 4 = step pair launch without / with store (the core's two
+  // steps, the band's and the shell's first), 6 = the two adjoint C-PML strip kernels, 8 = the launch that completes
+  // a step pair (the band's and the shell's second step); every `timing_stride`-th launch of a kind is bracketed by
+  // events on its stream.  Kinds 5 and 7 (separate C-PML strip and band launches of earlier versions) are unused.
   static constexpr int kKinds = 9;
   int timing_stride = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -165,18 +165,19 @@ struct fwi_plan {
 
   // workspace views (sized for cap_cell)
   // forward fields: the state of step n in set n mod 3 in the increment form (P 0..2, F 3..5, Q = P^n - P^{n-1}
-  // 6..8, R = F^n - F^{n-1} 9..11; three sets so that fused step pairs can keep step n while writing n+1 and
-  // n+2), in set n mod 2 of P 0..1 and F 3..4 in the two-level form; C-PML memory psx 12, psz 13, zx 14, zz 15
-  static constexpr int kFwdF = 16, kAdjF = 16, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
+  // 6..8, R = F^n - F^{n-1} 9..11; three sets so that step pairs can keep step n while writing n+1 and n+2), in set
+  // n mod 2 of P 0..1 and F 3..4 in the two-level form; C-PML memory psi_x^n 12..13 and psi_z^n 14..15 by step
+  // parity (work items compute psi^{n+1} from psi^n, their neighbours' included), zx 16, zz 17
+  static constexpr int kFwdF = 18, kAdjF = 16, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
   int fset(int n) const { return inc() ? n % 3 : (n & 1); }
   float* fP(int n) const { return fwd(fset(n)); }
   float* fF(int n) const { return fwd(3 + fset(n)); }
   float* fQ(int n) const { return fwd(6 + n % 3); }
   float* fR(int n) const { return fwd(9 + n % 3); }
-  float* fpsx() const { return fwd(12); }
-  float* fpsz() const { return fwd(13); }
-  float* fzx() const { return fwd(14); }
-  float* fzz() const { return fwd(15); }
+  float* fpsx(int n) const { return fwd(12 + (n & 1)); }
+  float* fpsz(int n) const { return fwd(14 + (n & 1)); }
+  float* fzx() const { return fwd(16); }
+  float* fzz() const { return fwd(17); }
   float* fwd(int k) const { return d_work + (size_t)k * cap_cell; }
   float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..15
   float* sgrad(int k) const { return d_work + (size_t)(kFwdF + kAdjF + k) * cap_cell; }      // 0..2
@@ -640,26 +641,9 @@ struct fwi_plan {
   }
 
   // ---------------------------------------------------------------------------------------------
-  // one forward step of a batch
-  int fwd_strip(const Batch& b, const ModelArgs& m, int n, hipStream_t st) {
-    if (b.nsblocks == 0) return 0;
-    StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
-    const int am = opt.arith;
-    const int tev = timed_begin(5, st);
-    if (am == FWI_ARITH_MIXED)
-      hipLaunchKernelGGL(k_fwd_strip<double>, dim3(b.nsblocks), dim3(256), 0, st, sa, fP(n), fF(n), fpsx(), fpsz());
-    else if (am == FWI_ARITH_FP32D)
-      hipLaunchKernelGGL(k_fwd_strip<F32D>, dim3(b.nsblocks), dim3(256), 0, st, sa, fP(n), fF(n), fpsx(), fpsz());
-    else
-      hipLaunchKernelGGL(k_fwd_strip<float>, dim3(b.nsblocks), dim3(256), 0, st, sa, fP(n), fF(n), fpsx(), fpsz());
-    timed_end(tev, st);
-    CK(hipGetLastError());
-    return 0;
-  }
-
-  // single-step main kernel over an item list (all items, or the shell's with column ranges)
+  // Arguments of forward step n over an item list; with_band: the FP64 band's step n in the same launch.
   FwdArgs fwd_args(const Batch& b, const ModelArgs& m, int n, float* qa, float* qb, float* gather, const int4* items,
-                   int nitems, const int2* xr) const {
+                   int nitems, const int2* xr, bool with_band) const {
     FwdArgs a;
     a.g = grid_args(b);
     a.m = m;
@@ -671,8 +655,10 @@ struct fwi_plan {
     a.Po = fP(n + 1);
     a.Fc = fF(n);
     a.Fo = fF(n + 1);
-    a.psx = fpsx();
-    a.psz = fpsz();
+    a.psx = fpsx(n);
+    a.psz = fpsz(n);
+    a.psxn = fpsx(n + 1);
+    a.pszn = fpsz(n + 1);
     a.zx = fzx();
     a.zz = fzz();
     a.Qc = fQ(n);
@@ -684,13 +670,23 @@ struct fwi_plan {
     a.qb = qb;
     a.it = n;
     a.xr = xr;
+    a.nband_wg = a.nband_real = a.nbc = 0;
+    if (with_band && band > kRim) {
+      a.band = band_args(b, m, n, qa, qb, gather);
+      int maxnx = 0;
+      for (const ShotDev& d : b.h_shots) maxnx = std::max(maxnx, d.nx);
+      a.nbc = (maxnx + 255) / 256;
+      a.nband_real = a.nbc * band_groups() * b.count;
+      a.nband_wg = (a.nband_real + 7) / 8 * 8;
+    }
     return a;
   }
+  // one forward launch: the FP64 band's step n and the items
   int fwd_main(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
                const int4* items, int nitems, const int2* xr, int kind, hipStream_t st) {
-    if (nitems == 0) return 0;
-    const FwdArgs a = fwd_args(b, m, n, qa, qb, gather, items, nitems, xr);
-    dim3 grid((nitems + 3) / 4);
+    const FwdArgs a = fwd_args(b, m, n, qa, qb, gather, items, nitems, xr, true);
+    if (nitems == 0 && a.nband_wg == 0) return 0;
+    dim3 grid((unsigned)(a.nband_wg + (nitems + 3) / 4));
     const int am = opt.arith;
     const int tev = kind >= 0 ? timed_begin(kind, st) : -1;
 #define FWD_LAUNCH(QMV, SV) hipLaunchKernelGGL((k_fwd_main<QMV, SV>), grid, dim3(256), 0, st, a)
@@ -722,11 +718,8 @@ struct fwi_plan {
   // workgroups of the band per 256-column chunk and shot: kBandRows rows each
   int band_groups() const { return (band - kRim + kBandRows - 1) / kBandRows; }
 
-  // the FP64 near-surface band of forward step n (rows [4, band) of every shot), launched with the C-PML strip
-  // kernel of the same step when the batch has strip cells (one launch, timing kind 5)
-  int fwd_band(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
-               hipStream_t st) {
-    if (band <= kRim) return 0;
+  // the FP64 near-surface band of forward step n (rows [4, band) of every shot)
+  BandArgs band_args(const Batch& b, const ModelArgs& m, int n, float* qa, float* qb, float* gather) const {
     BandArgs a;
     a.g = grid_args(b);
     a.m = m;
@@ -748,55 +741,20 @@ struct fwi_plan {
     a.gather = gather;
     a.qa = qa;
     a.qb = qb;
-    int maxnx = 0;
-    for (const ShotDev& d : b.h_shots) maxnx = std::max(maxnx, d.nx);
-    const int nbc = (maxnx + 255) / 256;
-    if (b.nsblocks > 0) {
-      StripArgs sa{grid_args(b), m, b.d_sblocks, b.nsblocks};
-      const long long nblk = (long long)b.nsblocks + (long long)nbc * band_groups() * b.count;
-      const int tev = timed_begin(5, st);
-      if (qm == 0)
-        hipLaunchKernelGGL((k_fwd_strip_band<float, 0>), dim3((unsigned)nblk), dim3(256), 0, st, sa, a, nbc, fP(n), fF(n),
-                           fpsx(), fpsz());
-      else if (qm == 1)
-        hipLaunchKernelGGL((k_fwd_strip_band<float, 1>), dim3((unsigned)nblk), dim3(256), 0, st, sa, a, nbc, fP(n), fF(n),
-                           fpsx(), fpsz());
-      else
-        hipLaunchKernelGGL((k_fwd_strip_band<float, 2>), dim3((unsigned)nblk), dim3(256), 0, st, sa, a, nbc, fP(n), fF(n),
-                           fpsx(), fpsz());
-      timed_end(tev, st);
-      CK(hipGetLastError());
-      return 0;
-    }
-    const dim3 grid((unsigned)((long long)nbc * band_groups() * b.count));
-    const int tev = timed_begin(7, st);
-    if (qm == 0)
-      hipLaunchKernelGGL(k_fwd_band<0>, grid, dim3(256), 0, st, a, nbc);
-    else if (qm == 1)
-      hipLaunchKernelGGL(k_fwd_band<1>, grid, dim3(256), 0, st, a, nbc);
-    else
-      hipLaunchKernelGGL(k_fwd_band<2>, grid, dim3(256), 0, st, a, nbc);
-    timed_end(tev, st);
-    CK(hipGetLastError());
-    return 0;
+    return a;
   }
 
-  // one forward step of a batch
+  // one forward step of a batch: one launch
   int fwd_step(const Batch& b, const ModelArgs& m, int n, int qm, float* qa, float* qb, float* gather,
                hipStream_t st) {
-    if (band > kRim) {
-      if (int r = fwd_band(b, m, n, qm, qa, qb, gather, st)) return r;  // with the strip kernel
-    } else if (int r = fwd_strip(b, m, n, st)) {
-      return r;
-    }
     return fwd_main(b, m, n, qm, qa, qb, gather, b.d_items, b.nitems, nullptr, qm == 0 ? 0 : 1, st);
   }
 
-  // Two forward steps n, n+1 (increment form) in four launches: the C-PML strip (and the FP64 band) of step n; the
-  // core pair with step n of the shell and of the core's cells next to it (both read only step n and write disjoint
-  // sets); the strip (and band) of step n+1, which reads the set of step n+1 there; the shell's step n+1.  Storing,
-  // the imaging terms of step n+1 lie 2 fields after those of step n (qa1 = qa0 + 2 cap_cell, as every caller lays
-  // them out), and the four fields must be addressable with 31-bit byte offsets (pair_store_ok).
+  // Two forward steps n, n+1 (increment form) in two launches: the core pair with step n of the FP64 band, of the
+  // shell and of the core's cells next to it (they read only step n and write disjoint sets); then step n+1 of the
+  // band and the shell, which read the set of step n+1 there.  Storing, the imaging terms of step n+1 lie 2 fields
+  // after those of step n (qa1 = qa0 + 2 cap_cell, as every caller lays them out), and the four fields must be
+  // addressable with 31-bit byte offsets (pair_store_ok).
   bool pairs(const Batch& b) const { return inc() && opt.step_pairs != 0 && b.nitems_core > 0; }
   bool pair_store_ok() const { return 4.0 * 4.0 * (double)cap_cell < 2147483648.0; }
   int fwd_pair(const Batch& b, const ModelArgs& m, int n, int qm, float* qa0, float* qb0, float* qa1, float* qb1,
@@ -804,11 +762,6 @@ struct fwi_plan {
     if (qm != 0 && (!pair_store_ok() || qb0 != qa0 + cap_cell || qa1 != qa0 + 2 * cap_cell ||
                     qb1 != qa0 + 3 * cap_cell))
       return fail("step pair: imaging-term slots not laid out as the pair kernel addresses them");
-    auto strip = [&](int k, float* qa, float* qb) -> int {
-      if (band > kRim) return fwd_band(b, m, k, qm, qa, qb, gather, st);
-      return fwd_strip(b, m, k, st);
-    };
-    if (int r = strip(n, qa0, qb0)) return r;
     FwdPairArgs a;
     a.g = grid_args(b);
     a.m = m;
@@ -828,9 +781,9 @@ struct fwi_plan {
     a.qa0 = qm != 0 ? qa0 : nullptr;
     a.qstride = (unsigned)(4 * cap_cell);
     a.it = n;
-    a.sh = fwd_args(b, m, n, qa0, qb0, gather, b.d_items_shell1, b.nitems_shell1, b.d_xr_shell1);
+    a.sh = fwd_args(b, m, n, qa0, qb0, gather, b.d_items_shell1, b.nitems_shell1, b.d_xr_shell1, true);
     a.nshell_wg = ((b.nitems_shell1 + 3) / 4 + 7) / 8 * 8;
-    dim3 grid((unsigned)(a.nshell_wg + b.nitems_core));
+    dim3 grid((unsigned)(a.sh.nband_wg + a.nshell_wg + b.nitems_core));
     const int tev = timed_begin(qm == 0 ? 3 : 4, st);
     if (qm == 0)
       hipLaunchKernelGGL(k_fwd_pair<0>, grid, dim3(256), 0, st, a);
@@ -840,7 +793,6 @@ struct fwi_plan {
       hipLaunchKernelGGL(k_fwd_pair<2>, grid, dim3(256), 0, st, a);
     timed_end(tev, st);
     CK(hipGetLastError());
-    if (int r = strip(n + 1, qa1, qb1)) return r;
     return fwd_main(b, m, n + 1, qm, qa1, qb1, gather, b.d_items_shell, b.nitems_shell, b.d_xr_shell, 8, st);
   }
 
@@ -971,10 +923,11 @@ struct fwi_plan {
     return 0;
   }
 
-  int strip_copy(const Batch& b, int dir, float* packed, hipStream_t st) {
+  // the C-PML memory variables of step n (psi of its parity) to or from a checkpoint's packed strip cells
+  int strip_copy(const Batch& b, int dir, int n, float* packed, hipStream_t st) {
     if (b.nsblocks == 0) return 0;
     StripArgs sa{grid_args(b), ModelArgs{nullptr, nullptr, nullptr, nx}, b.d_sblocks, b.nsblocks};
-    hipLaunchKernelGGL(k_strip_copy, dim3(b.nsblocks), dim3(256), 0, st, sa, dir, fpsx(), fpsz(), fzx(), fzz(),
+    hipLaunchKernelGGL(k_strip_copy, dim3(b.nsblocks), dim3(256), 0, st, sa, dir, fpsx(n), fpsz(n), fzx(), fzz(),
                        packed, b.nstrip);
     CK(hipGetLastError());
     return 0;
@@ -1009,12 +962,13 @@ struct fwi_plan {
                          fwd(ckpt_field(3, n)), (unsigned short*)c, b.ncell, am);
       if (b.nsblocks > 0) {
         StripArgs sa{grid_args(b), ModelArgs{nullptr, nullptr, nullptr, nx}, b.d_sblocks, b.nsblocks};
-        hipLaunchKernelGGL(k_ck_absmax_strips, dim3(b.nsblocks), dim3(256), 0, st, sa, fpsx(), fpsz(), fzx(), fzz(), am);
+        hipLaunchKernelGGL(k_ck_absmax_strips, dim3(b.nsblocks), dim3(256), 0, st, sa, fpsx(n), fpsz(n), fzx(), fzz(),
+                           am);
         if (h)
-          hipLaunchKernelGGL(k_ck_pack_strips<2>, dim3(b.nsblocks), dim3(256), 0, st, sa, 0, fpsx(), fpsz(), fzx(),
+          hipLaunchKernelGGL(k_ck_pack_strips<2>, dim3(b.nsblocks), dim3(256), 0, st, sa, 0, fpsx(n), fpsz(n), fzx(),
                              fzz(), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
         else
-          hipLaunchKernelGGL(k_ck_pack_strips<1>, dim3(b.nsblocks), dim3(256), 0, st, sa, 0, fpsx(), fpsz(), fzx(),
+          hipLaunchKernelGGL(k_ck_pack_strips<1>, dim3(b.nsblocks), dim3(256), 0, st, sa, 0, fpsx(n), fpsz(n), fzx(),
                              fzz(), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
       }
       CK(hipGetLastError());
@@ -1025,7 +979,7 @@ struct fwi_plan {
       if (int r = copy(cf + (size_t)f * b.ncell, fwd(ckpt_field(f, n)), b.ncell, st)) return r;
     if (band > kRim)
       if (int r = copy(ckpt_band(j), (const float*)d_band, 2 * kBandF * (size_t)cap_band, st)) return r;
-    return strip_copy(b, 0, cf + 4 * (size_t)b.ncell, st);
+    return strip_copy(b, 0, n, cf + 4 * (size_t)b.ncell, st);
   }
   int load_ckpt(const Batch& b, int j, hipStream_t st) {
     char* c = ckpt(j);
@@ -1044,10 +998,10 @@ struct fwi_plan {
       if (b.nsblocks > 0) {
         StripArgs sa{grid_args(b), ModelArgs{nullptr, nullptr, nullptr, nx}, b.d_sblocks, b.nsblocks};
         if (h)
-          hipLaunchKernelGGL(k_ck_pack_strips<2>, dim3(b.nsblocks), dim3(256), 0, st, sa, 1, fpsx(), fpsz(), fzx(),
+          hipLaunchKernelGGL(k_ck_pack_strips<2>, dim3(b.nsblocks), dim3(256), 0, st, sa, 1, fpsx(n), fpsz(n), fzx(),
                              fzz(), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
         else
-          hipLaunchKernelGGL(k_ck_pack_strips<1>, dim3(b.nsblocks), dim3(256), 0, st, sa, 1, fpsx(), fpsz(), fzx(),
+          hipLaunchKernelGGL(k_ck_pack_strips<1>, dim3(b.nsblocks), dim3(256), 0, st, sa, 1, fpsx(n), fpsz(n), fzx(),
                              fzz(), (unsigned short*)c + 4 * (size_t)b.ncell, b.nstrip, am);
       }
       CK(hipGetLastError());
@@ -1058,7 +1012,7 @@ struct fwi_plan {
       if (int r = copy(fwd(ckpt_field(f, n)), cf + (size_t)f * b.ncell, b.ncell, st)) return r;
     if (band > kRim)
       if (int r = copy((float*)d_band, ckpt_band(j), 2 * kBandF * (size_t)cap_band, st)) return r;
-    return strip_copy(b, 1, cf + 4 * (size_t)b.ncell, st);
+    return strip_copy(b, 1, n, cf + 4 * (size_t)b.ncell, st);
   }
 
   int residual(const Batch& b, const float* syn, const float* obs, float* res, double* out, int accumulate,

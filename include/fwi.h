@@ -191,10 +191,10 @@ int fwi_dot3_noalloc(const double* a, const double* b, long long n, double* out,
 /* Instrumentation (no counterpart in the reference, whose only timer is clock() around whole phases,
  * main.c:323,368-372).  With stride > 0, every stride-th launch of each fused step kernel is bracketed by
  * HIP events on the stream it runs on; fwi_plan_kernel_time returns the summed event time and the
- * number of timed launches of a kind (0 forward step, 1 forward step storing imaging terms, 2 adjoint
- * step, 3 / 4 fused core step pair without / with stored imaging terms, 5 forward C-PML strip -- one launch with
- * the FP64 near-surface band when the plan has both --, 6 the two adjoint C-PML strip kernels together, 7 FP64
- * near-surface band alone, 8 the single step of the shell that completes a step pair) since the last
+ * number of timed launches of a kind (0 forward step, 1 forward step storing imaging terms -- each one launch with
+ * the FP64 near-surface band --, 2 adjoint step, 3 / 4 step-pair launch without / with stored imaging terms (two
+ * steps of the core, one of the band and the shell), 6 the two adjoint C-PML strip kernels together, 8 the launch
+ * that completes a step pair (the band's and the shell's second step); 5 and 7 are no longer used) since the last
  * fwi_plan_set_timing call. */
 void fwi_plan_set_timing(fwi_plan* plan, int stride);
 int fwi_plan_kernel_time(fwi_plan* plan, int kind, double* total_ms, long long* count);
