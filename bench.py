@@ -14,8 +14,8 @@ Beside it, on the same workload:
     measured with HIP events on the stream the kernel runs on during an instrumented gradient, beside the
     counter-measured HBM bytes of the same kernel when profiles/ holds them for these kernel sources (the run
     then uses the work-item tiling those counters were measured with, unless --retune or a tiling is given);
-  * the same gradient in the reference's arithmetic (FP64 stencil sums), and without the FP64 near-surface
-    band, each timed apart;
+  * the same gradient in the reference's arithmetic (FP64 stencil sums), without the FP64 near-surface band,
+    and with single steps only (no step pairs), each timed apart;
   * cpu_baseline: the CPU oracle (f32i, exact gradient, OpenMP) on one shot for a bounded number of steps, and
     the reference's arithmetic and gradient on one thread, with the CPU model.
   * the device optimizer of one iteration (steepest-descent step, L-BFGS direction, trial), timed apart.
@@ -70,6 +70,7 @@ def parse():
     ap.add_argument("--cpu1-steps", type=int, default=200, help="steps of the single-thread reference-arithmetic leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the reference-arithmetic (mixed) GPU gradient")
     ap.add_argument("--no-band-leg", action="store_true", help="skip the gradient timed without the FP64 band")
+    ap.add_argument("--no-single-leg", action="store_true", help="skip the gradient timed with single steps only")
     ap.add_argument("--surface-rows", type=int, default=0, help="FP64 near-surface band rows (0 default, -1 none)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -221,59 +222,56 @@ def optimizer_ms(w, model, grad):
     return out
 
 
-# Step kernels of the batched gradient: (timing kind, kernel name).  Launches per gradient with nt steps of which
-# `kept` keep their imaging terms from the forward: the forward runs nt - kept plain steps and kept storing steps,
-# the recomputation nt - kept storing steps, the adjoint nt steps.
+# Step launches of the batched gradient: (timing kind, kernel name, what it does).  With step pairs (the default)
+# a pair of forward steps is two launches: k_fwd_pair (two steps of every window's core, one step of the FP64
+# near-surface band, the C-PML shell and the core's cells next to it) and k_fwd_main completing the pair (the band's
+# and the shell's second step); single forward steps (where a pair would cross a checkpoint, or without pairs) are
+# one k_fwd_main launch with the band.  Launches per gradient are counted by the library.
 STEP_KERNELS = [
-    (0, "k_fwd_main<0>", "forward step"),
-    (1, "k_fwd_main<1>", "forward step storing the imaging terms (forward's last segments and recomputation)"),
+    (0, "k_fwd_main<0>", "forward step (one launch with the FP64 band)"),
+    (1, "k_fwd_main<1>", "forward step storing the imaging terms (one launch with the FP64 band)"),
     (2, "k_adj_ex_main", "exact adjoint step with imaging"),
-    (3, "k_fwd_pair<0>", "two forward steps of the core with the shell's first step"),
-    (4, "k_fwd_pair<1>", "two forward steps of the core storing the imaging terms, with the shell's first step"),
-    (8, "k_fwd_main (shell)", "the shell's second step of a step pair"),
-    (7, "k_fwd_band", "FP64 near-surface band of every forward-type step"),
-    (5, "k_fwd_strip", "forward C-PML memory variables psi (with the FP64 band in one launch when the plan has it)"),
+    (3, "k_fwd_pair<0>", "step pair: two steps of the core, one of the band and the shell"),
+    (4, "k_fwd_pair<1>", "step pair storing the imaging terms: two steps of the core, one of the band and the shell"),
+    (8, "k_fwd_main<0> (pair)", "the band's and the shell's second step of a step pair"),
+    (9, "k_fwd_main<1> (pair)", "the band's and the shell's second step of a step pair, storing"),
     (6, "k_adj_ex_strip1+2", "adjoint C-PML memory variables (two kernels)"),
 ]
 
 
-def kernel_bytes(kind, N, Nb, Nc, Nm, grouped, inc):
-    """Bytes per launch of a step kernel: (the survey's algorithmic count, this design's minimum).
+def kernel_bytes(kind, N, Nb, Nc, Nm, Ncore, grouped, inc):
+    """Bytes per launch of a step launch: (the survey's algorithmic count, this design's minimum).
 
-    Survey count per step (FP32): forward 24 B per cell + 32 per C-PML cell + 12 per model cell (read once per
-    batch); adjoint + imaging 56 B per cell + 32 per C-PML cell + 12 per model cell.  It is split over the kernels
-    that move it: the forward's C-PML bytes half to the strip kernel (psi), half to the main kernel (zeta); the
-    adjoint's to its strip kernels; the cells of the FP64 band (Nb) to the band kernel.
-    Design minimum: the increment form reads P, Q, F, R and writes them (32 B per cell; + 8 B when storing the
-    imaging terms); the band moves two FP64 levels of P and F (48 B per cell) + its float rounding (8 B) + the model
-    (12 B, per shot); the adjoint reads mu and mu^{k+2} (or D) and the imaging terms (24 B), writes mu (and D) and
-    read-modify-writes the FP32 gradient slot (24 B per cell, 6 with gradient groups of four)."""
-    Nf = N - Nb  # cells of the float rows
-    if kind == 8:  # the shell's cells only: counted with the pair kernel
-        survey = design = 0.0
-    elif kind in (3, 4):  # two steps
-        survey = 2.0 * (24.0 * Nf + 16.0 * Nc + 12.0 * Nm)
-        design = (32.0 + (16.0 if kind == 4 else 0.0)) * Nf + 16.0 * Nc + 12.0 * Nm
-    elif kind in (0, 1):
-        survey = 24.0 * Nf + 16.0 * Nc + 12.0 * Nm
-        design = (32.0 + (8.0 if kind == 1 else 0.0)) * Nf + 16.0 * Nc + 12.0 * Nm
-    elif kind == 7:
-        survey = 24.0 * Nb
-        design = (48.0 + 8.0 + 12.0) * Nb
-    elif kind == 5:  # with the band's cells when they share the launch (Nb > 0)
-        survey = 16.0 * Nc + 24.0 * Nb
-        design = 16.0 * Nc + (48.0 + 8.0 + 12.0) * Nb
+    Survey count per forward step (FP32): 24 B per cell + 32 per C-PML cell + 12 per model cell (read once per
+    batch); adjoint + imaging 56 B per cell + 32 per C-PML cell + 12 per model cell; the adjoint's C-PML bytes go to
+    its strip kernels.  A step-pair launch advances the Ncore core cells two steps and the other N - Ncore cells one
+    step, the launch completing the pair those N - Ncore cells one more step: together they count two steps.
+    Design minimum: the increment form reads P, Q, F, R and writes them (32 B per cell and step or per core cell and
+    step pair; + 8 B per cell and step when storing the imaging terms); the FP64 band's Nb cells move two FP64 levels
+    of P and F (48 B) + their float rounding (8 B) + the model (12 B, per shot); the adjoint reads mu and mu^{k+2} (or
+    D) and the imaging terms (24 B), writes mu (and D) and read-modify-writes the FP32 gradient slot (24 B per cell,
+    6 with gradient groups of four)."""
+    st = 8.0 if kind in (1, 4, 9) else 0.0
+    band = (48.0 + 8.0 + 12.0 + st) * Nb
+
+    def shell(cells):  # one step of `cells` cells, the band's among them
+        return (32.0 + st) * (cells - Nb) + band + 32.0 * Nc + 12.0 * Nm
+
+    if kind in (0, 1):
+        survey = 24.0 * N + 32.0 * Nc + 12.0 * Nm
+        design = shell(N)
+    elif kind in (3, 4):
+        survey = 24.0 * (2 * Ncore + (N - Ncore)) + 32.0 * Nc + 12.0 * Nm
+        design = (32.0 + 2 * st) * Ncore + shell(N - Ncore)
+    elif kind in (8, 9):
+        survey = 24.0 * (N - Ncore) + 32.0 * Nc + 12.0 * Nm
+        design = shell(N - Ncore)
     elif kind == 6:
         survey = design = 32.0 * Nc
     else:
         survey = 56.0 * N + 12.0 * Nm
         design = (24.0 + (16.0 if inc else 8.0) + (6.0 if grouped else 24.0)) * N + 12.0 * Nm
     return survey, design
-
-
-def launches_per_gradient(kind, nt, kept, band):
-    return {0: nt - kept, 1: nt, 2: nt, 3: (nt - kept) // 2, 4: nt // 2, 5: 2 * nt - kept, 6: nt,
-            7: 2 * nt - kept, 8: (2 * nt - kept) // 2}.get(kind, 0)
 
 
 def load_traffic(w, nshots, tiling):
@@ -321,24 +319,16 @@ def tiling_of(plan):
 
 
 TRAFFIC_KEYS = {"k_fwd_main<0>": ("k_fwd_main<0, float",), "k_fwd_main<1>": ("k_fwd_main<1, float",),
-                "k_adj_ex_main": ("k_adj_ex_main<float",), "k_fwd_band": ("k_fwd_band<1>",),
-                "k_fwd_strip": ("k_fwd_strip<float",),
+                "k_fwd_main<0> (pair)": ("k_fwd_main<0, float",), "k_fwd_main<1> (pair)": ("k_fwd_main<1, float",),
+                "k_fwd_pair<0>": ("k_fwd_pair<0>",), "k_fwd_pair<1>": ("k_fwd_pair<1>",),
+                "k_adj_ex_main": ("k_adj_ex_main<float",),
                 "k_adj_ex_strip1+2": ("k_adj_ex_strip1", "k_adj_ex_strip2<")}
 
 
-def traffic_of(traffic, name, nt=0, kept=0):
-    """Counter bytes (read + written) per launch of a step kernel, summed over the kernels it stands for; for the
-    merged strip + band launch, whose two variants (without and with stored imaging terms) share one timing, the
-    average over a gradient's launches of each (nt - kept and nt)."""
+def traffic_of(traffic, name):
+    """Counter bytes (read + written) per launch of a step kernel, summed over the kernels it stands for."""
     if not traffic:
         return None
-    if name == "k_fwd_strip_band":
-        v0, v1 = (traffic.get(f"k_fwd_strip_band<float, {q}>") for q in (0, 1))
-        if not (v0 and v1):
-            return None
-        n0, n1 = nt - kept, nt
-        return float(((v0["read_bytes"] + v0["write_bytes"]) * n0 + (v1["read_bytes"] + v1["write_bytes"]) * n1)
-                     / (n0 + n1))
     parts = [v for k, v in traffic.items() for p in TRAFFIC_KEYS.get(name, ()) if k.startswith(p)]
     if not parts:
         return None
@@ -379,6 +369,8 @@ def main():
     if pinned:
         args.rows, args.order = pinned["rows_per_item"], pinned["item_order"]
         args.adj_rows, args.adj_order = pinned["adjoint_tiling"]["rows_per_item"], pinned["adjoint_tiling"]["item_order"]
+        if args.pairs == 1 and pinned.get("pair_rows"):
+            args.pairs = pinned["pair_rows"]
         log(f"tiling of the counter profile: forward {args.rows} rows order {args.order}, adjoint {args.adj_rows} rows "
             f"order {args.adj_order}")
     plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
@@ -431,6 +423,7 @@ def main():
         nmodel = w.nx * w.nz
         band = plan.surface_rows
         nband = sum((band - 4) * s.nx for s in w.shots) if band else 0
+        ncore = plan.core_cells()
         grouped = bool(plan.adj_tiling[2])
         inc = args.arith == 3
         traffic, tnote = load_traffic(w, len(w.shots), tiling_of(plan))
@@ -439,14 +432,9 @@ def main():
             if not n:
                 continue
             mean = ms / n / 1e3
-            merged = kind == 5 and nband > 0  # the strip kernel and the band share one launch
-            if merged:
-                name = "k_fwd_strip_band"
-                what = "forward C-PML memory variables psi and the FP64 near-surface band, one launch"
-            nb = nband if (kind in (0, 1, 7) or merged) else 0
-            survey, design = kernel_bytes(kind, ncell, nb, nstrip, nmodel, grouped, inc)
-            per_grad = launches_per_gradient(kind, nt, plan.kept_steps, band)
-            tb = traffic_of(traffic, name, nt, plan.kept_steps)
+            survey, design = kernel_bytes(kind, ncell, nband, nstrip, nmodel, ncore, grouped, inc)
+            per_grad = plan.kernel_launches(kind)
+            tb = traffic_of(traffic, name)
             k = {"kernel": name, "does": what, "launches_per_gradient": per_grad, "mean_launch_us": round(mean * 1e6, 1),
                  "s_per_gradient": round(per_grad * mean, 4),
                  "bytes_survey": survey, "frac_survey": round(survey / mean / 1e9 / HBM_PEAK_GBS, 4),
@@ -504,6 +492,23 @@ def main():
             pb.close()
         log(f"without the FP64 surface band: {sec_noband:.3f} s per gradient")
 
+    # the same gradient with single steps only (no temporal blocking), timed apart
+    sec_single = None
+    if args.pairs and args.arith == 3 and not args.no_single_leg:
+        p1 = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, arith=args.arith,
+                  ckpt_interval=args.ckpt, ckpt_format=args.ckpt_format, grad_group=args.grad_group,
+                  surface_rows=args.surface_rows, step_pairs=0)
+        try:
+            p1.gradient(init_sq, obs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            p1.gradient(init_sq, obs)
+            torch.cuda.synchronize()
+            sec_single = time.perf_counter() - t0
+        finally:
+            p1.close()
+        log(f"single steps only: {sec_single:.3f} s per gradient")
+
     plain = None
     if not args.no_plain:
         ns = min(args.plain_steps, nt)
@@ -554,7 +559,7 @@ def main():
             "fp64_surface_rows": surface_rows,
             "ckpt_interval": K,
             "kept_steps": kept,
-            "step_pairs": bool(args.pairs),
+            "step_pairs": bool(tiling.get("pair_rows")),
             "ckpt_format": {0: "fp32", 1: "bf16-scaled", 2: "fp16-scaled"}[args.ckpt_format],
             "workspace_gb": round(wsb / 1e9, 2),
             "batch_shots": bsz,
@@ -564,6 +569,7 @@ def main():
             "s_per_gradient_median": round(float(np.median(times)), 4),
             "s_per_gradient_reference_arith": round(sec_mixed, 4) if sec_mixed else None,
             "s_per_gradient_without_surface_band": round(sec_noband, 4) if sec_noband else None,
+            "s_per_gradient_single_steps": round(sec_single, 4) if sec_single else None,
             "optimizer_ms": opt_ms,
             "graphs": {"captures": graphs[0], "replays": graphs[1]},
             "s_per_gradient_eager_instrumented": round(sec_instr, 4) if sec_instr else None,

@@ -95,6 +95,8 @@ BATCHED = {
     "fwi_dot3_noalloc": ([vp_, vp_, c_ll, vp_, vp_], c_int),
     "fwi_plan_set_timing": ([vp_, c_int], None),
     "fwi_plan_kernel_time": ([vp_, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_ll)], c_int),
+    "fwi_plan_kernel_launches": ([vp_, c_int], c_ll),
+    "fwi_plan_core_cells": ([vp_], c_ll),
     "fwi_plan_cells": ([vp_, ctypes.POINTER(c_ll)], c_ll),
     "fwi_plan_shot_misfits": ([vp_, ctypes.POINTER(c_double), vp_], c_int),
     "fwi_plan_ckpt_interval": ([vp_], c_int),

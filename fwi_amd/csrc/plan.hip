@@ -124,10 +124,11 @@ struct fwi_plan {
   unsigned* d_ck_amax = nullptr;    // 16-bit checkpoints: max-abs bits per (checkpoint, shot, field)
   // sampled kernel timing: kinds 0 = forward step (no store), 1 = forward step (store / recompute), each one launch
   // with the FP64 near-surface band, 2 = adjoint step, 3 / 4 = step pair launch without / with store (the core's two
-  // steps, the band's and the shell's first), 6 = the two adjoint C-PML strip kernels, 8 = the launch that completes
-  // a step pair (the band's and the shell's second step); every `timing_stride`-th launch of a kind is bracketed by
-  // events on its stream.  Kinds 5 and 7 (separate C-PML strip and band launches of earlier versions) are unused.
-  static constexpr int kKinds = 9;
+  // steps, the band's and the shell's first), 6 = the two adjoint C-PML strip kernels, 8 / 9 = the launch that
+  // completes a step pair without / with store (the band's and the shell's second step); every `timing_stride`-th
+  // launch of a kind is bracketed by events on its stream.  Kinds 5 and 7 (separate C-PML strip and band launches of
+  // earlier versions) are unused.
+  static constexpr int kKinds = 10;
   int timing_stride = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_open;  // (kind, event index of the start; end = start + 1)
@@ -793,7 +794,8 @@ struct fwi_plan {
       hipLaunchKernelGGL(k_fwd_pair<2>, grid, dim3(256), 0, st, a);
     timed_end(tev, st);
     CK(hipGetLastError());
-    return fwd_main(b, m, n + 1, qm, qa1, qb1, gather, b.d_items_shell, b.nitems_shell, b.d_xr_shell, 8, st);
+    return fwd_main(b, m, n + 1, qm, qa1, qb1, gather, b.d_items_shell, b.nitems_shell, b.d_xr_shell, qm == 0 ? 8 : 9,
+                    st);
   }
 
   template <bool IMG>
@@ -1737,6 +1739,23 @@ void fwi_plan_set_timing(fwi_plan* p, int stride) {
     p->kt_count[k] = 0;
     p->launch_no[k] = 0;
   }
+}
+
+long long fwi_plan_kernel_launches(const fwi_plan* p, int kind) {
+  if (kind < 0 || kind >= fwi_plan::kKinds) return -1;
+  return p->launch_no[kind];
+}
+
+long long fwi_plan_core_cells(const fwi_plan* p) {
+  if (!p->inc() || p->opt.step_pairs == 0) return 0;
+  long long c = 0;
+  const int za = p->pair_za(), zb = p->pair_zb();
+  for (auto& b : p->batches)
+    for (const ShotDev& d : b.h_shots) {
+      const int x0 = pair_cx0(p->cpml), x1 = pair_cx1(d.nx, p->cpml);
+      if (x1 > x0 && zb > za) c += (long long)(x1 - x0) * (zb - za);
+    }
+  return c;
 }
 
 int fwi_plan_kernel_time(fwi_plan* p, int kind, double* total_ms, long long* count) {

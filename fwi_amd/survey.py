@@ -187,6 +187,14 @@ class Plan:
     def set_timing(self, stride: int):
         _lib.load().fwi_plan_set_timing(self._h, stride)
 
+    def kernel_launches(self, kind: int) -> int:
+        """Launches of a timing kind since the last set_timing (counted while timing is on)."""
+        return int(_lib.load().fwi_plan_kernel_launches(self._h, kind))
+
+    def core_cells(self) -> int:
+        """Cells the step pairs advance two steps per launch (0 without step pairs)."""
+        return int(_lib.load().fwi_plan_core_cells(self._h))
+
     def kernel_time(self, kind: int):
         """(total ms, launches timed) of kind 0 forward step, 1 forward step storing, 2 adjoint step, 3 / 4 fused
         core step pair without / with store."""
