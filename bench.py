@@ -406,7 +406,9 @@ def main():
     gcells = work / sec / 1e9
     assert np.isfinite(phi) and torch.isfinite(grad).all().item()
     graphs = plan.graph_stats()
-    log(f"graph captures {graphs[0]}, replays {graphs[1]}")
+    gnodes = plan.graph_nodes()
+    log(f"graph captures {graphs[0]}, replays {graphs[1]}; kernel nodes: largest graph {gnodes[0]}, the gradient's "
+        f"capture {gnodes[1]}")
     # kernel durations: one more gradient with every timing_stride-th launch of each step kernel bracketed by HIP
     # events on the stream it runs on (the instrumented call launches eagerly; its kernels are the ones the timed
     # calls replayed)
@@ -571,7 +573,8 @@ def main():
             "s_per_gradient_without_surface_band": round(sec_noband, 4) if sec_noband else None,
             "s_per_gradient_single_steps": round(sec_single, 4) if sec_single else None,
             "optimizer_ms": opt_ms,
-            "graphs": {"captures": graphs[0], "replays": graphs[1]},
+            "graphs": {"captures": graphs[0], "replays": graphs[1], "nodes_largest_graph": gnodes[0],
+                       "nodes_gradient": gnodes[1]},
             "s_per_gradient_eager_instrumented": round(sec_instr, 4) if sec_instr else None,
             "s_per_gradient_plain": round(plain, 3) if plain else None,
             "plain_steps_timed": min(args.plain_steps, nt) if plain else None,

@@ -96,6 +96,7 @@ struct fwi_plan {
   int tz_adj = 32;   // rows per work item and order of the adjoint kernels
   int order_adj = 0;
   int tz_pair = 32;  // rows per fused core item (step pairs)
+  bool pairs_off = false;  // step_pairs = 1 and the tuner timed single steps faster (small, latency-bound grids)
   // FP64 near-surface band of the increment form: rows [4, band) of every shot (0: none).  State: seven FP64 arrays
   // of cap_band cells (band rows x row pitch of every shot): P, F and psi_x by step parity, zeta_x.
   int band = 0;
@@ -756,7 +757,7 @@ struct fwi_plan {
   // band and the shell, which read the set of step n+1 there.  Storing, the imaging terms of step n+1 lie 2 fields
   // after those of step n (qa1 = qa0 + 2 cap_cell, as every caller lays them out), and the four fields must be
   // addressable with 31-bit byte offsets (pair_store_ok).
-  bool pairs(const Batch& b) const { return inc() && opt.step_pairs != 0 && b.nitems_core > 0; }
+  bool pairs(const Batch& b) const { return inc() && opt.step_pairs != 0 && !pairs_off && b.nitems_core > 0; }
   bool pair_store_ok() const { return 4.0 * 4.0 * (double)cap_cell < 2147483648.0; }
   int fwd_pair(const Batch& b, const ModelArgs& m, int n, int qm, float* qa0, float* qb0, float* qa1, float* qb1,
                float* gather, hipStream_t st) {
@@ -1255,18 +1256,44 @@ struct fwi_plan {
         }
         if (int r = zero_fwd(b, st)) return r;
         if (int r = fwd_pair(b, m, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, st)) return r;
-        float ms = 0;
-        if (int r = timed([&]() -> int {
-              for (int n = 2; n < 8; n += 2)
-                if (int r2 = fwd_pair(b, m, n, 0, nullptr, nullptr, nullptr, nullptr, nullptr, st)) return r2;
-              return 0;
-            }, ms))
-          return r;
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+          float t = 0;
+          if (int r = timed([&]() -> int {
+                for (int n = 2; n < 8; n += 2)
+                  if (int r2 = fwd_pair(b, m, n, 0, nullptr, nullptr, nullptr, nullptr, nullptr, st)) return r2;
+                return 0;
+              }, t))
+            return r;
+          ms = std::min(ms, t);
+        }
         if (log) std::fprintf(stderr, "[fwi tune] pair     rows %2d: %.1f us/step\n", c, ms * 1e3 / 6);
         if (ms < best) {
           best = ms;
           tz_pair = c;
         }
+      }
+      // tuned pairs are kept only where they beat single steps (six steps each, the chosen tilings)
+      if (opt.step_pairs == 1 && best < 1e29f) {
+        if (int r = build_items(b, tz, order)) return r;
+        if (int r = zero_fwd(b, st)) return r;
+        for (int w = 0; w < 2; ++w)
+          if (int r = fwd_step(b, m, w, 0, nullptr, nullptr, nullptr, st)) return r;
+        float single_ms = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+          float t = 0;
+          if (int r = timed([&]() -> int {
+                for (int n = 2; n < 8; ++n)
+                  if (int r2 = fwd_step(b, m, n, 0, nullptr, nullptr, nullptr, st)) return r2;
+                return 0;
+              }, t))
+            return r;
+          single_ms = std::min(single_ms, t);
+        }
+        pairs_off = best >= single_ms;
+        if (log)
+          std::fprintf(stderr, "[fwi tune] pairs %s (%.1f vs %.1f us/step single)\n", pairs_off ? "off" : "on",
+                       best * 1e3 / 6, single_ms * 1e3 / 6);
       }
     }
     for (auto& bb : batches) {
@@ -1747,7 +1774,7 @@ long long fwi_plan_kernel_launches(const fwi_plan* p, int kind) {
 }
 
 long long fwi_plan_core_cells(const fwi_plan* p) {
-  if (!p->inc() || p->opt.step_pairs == 0) return 0;
+  if (p->batches.empty() || !p->pairs(p->batches[0])) return 0;
   long long c = 0;
   const int za = p->pair_za(), zb = p->pair_zb();
   for (auto& b : p->batches)
@@ -1791,7 +1818,7 @@ int fwi_plan_graph_nodes(const fwi_plan* p, long long* largest, long long* last)
   return 0;
 }
 int fwi_plan_pair_rows(const fwi_plan* p) {
-  return (p->inc() && p->opt.step_pairs != 0 && !p->batches.empty() && p->batches[0].nitems_core > 0) ? p->tz_pair : 0;
+  return (!p->batches.empty() && p->pairs(p->batches[0])) ? p->tz_pair : 0;
 }
 int fwi_plan_surface_rows(const fwi_plan* p) { return p->band > kRim ? p->band : 0; }
 int fwi_plan_adj_tiling(const fwi_plan* p, int* rows, int* order, int* grouped) {
