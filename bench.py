@@ -235,7 +235,6 @@ STEP_KERNELS = [
     (4, "k_fwd_pair<1>", "step pair storing the imaging terms: two steps of the core, one of the band and the shell"),
     (8, "k_fwd_main<0> (pair)", "the band's and the shell's second step of a step pair"),
     (9, "k_fwd_main<1> (pair)", "the band's and the shell's second step of a step pair, storing"),
-    (6, "k_adj_ex_strip1+2", "adjoint C-PML memory variables (two kernels)"),
 ]
 
 
@@ -243,14 +242,14 @@ def kernel_bytes(kind, N, Nb, Nc, Nm, Ncore, grouped, inc):
     """Bytes per launch of a step launch: (the survey's algorithmic count, this design's minimum).
 
     Survey count per forward step (FP32): 24 B per cell + 32 per C-PML cell + 12 per model cell (read once per
-    batch); adjoint + imaging 56 B per cell + 32 per C-PML cell + 12 per model cell; the adjoint's C-PML bytes go to
-    its strip kernels.  A step-pair launch advances the Ncore core cells two steps and the other N - Ncore cells one
+    batch); adjoint + imaging 56 B per cell + 32 per C-PML cell + 12 per model cell.  A step-pair launch advances the Ncore core cells two steps and the other N - Ncore cells one
     step, the launch completing the pair those N - Ncore cells one more step: together they count two steps.
     Design minimum: the increment form reads P, Q, F, R and writes them (32 B per cell and step or per core cell and
     step pair; + 8 B per cell and step when storing the imaging terms); the FP64 band's Nb cells move two FP64 levels
     of P and F (48 B) + their float rounding (8 B) + the model (12 B, per shot); the adjoint reads mu and mu^{k+2} (or
     D) and the imaging terms (24 B), writes mu (and D) and read-modify-writes the FP32 gradient slot (24 B per cell,
-    6 with gradient groups of four)."""
+    6 with gradient groups of four), and reads and writes the two C-PML memory variables of its direction on each
+    C-PML cell (16 B; both directions on the corners)."""
     st = 8.0 if kind in (1, 4, 9) else 0.0
     band = (48.0 + 8.0 + 12.0 + st) * Nb
 
@@ -266,11 +265,9 @@ def kernel_bytes(kind, N, Nb, Nc, Nm, Ncore, grouped, inc):
     elif kind in (8, 9):
         survey = 24.0 * (N - Ncore) + 32.0 * Nc + 12.0 * Nm
         design = shell(N - Ncore)
-    elif kind == 6:
-        survey = design = 32.0 * Nc
     else:
-        survey = 56.0 * N + 12.0 * Nm
-        design = (24.0 + (16.0 if inc else 8.0) + (6.0 if grouped else 24.0)) * N + 12.0 * Nm
+        survey = 56.0 * N + 32.0 * Nc + 12.0 * Nm
+        design = (24.0 + (16.0 if inc else 8.0) + (6.0 if grouped else 24.0)) * N + 16.0 * Nc + 12.0 * Nm
     return survey, design
 
 
@@ -321,8 +318,7 @@ def tiling_of(plan):
 TRAFFIC_KEYS = {"k_fwd_main<0>": ("k_fwd_main<0, float",), "k_fwd_main<1>": ("k_fwd_main<1, float",),
                 "k_fwd_main<0> (pair)": ("k_fwd_main<0, float",), "k_fwd_main<1> (pair)": ("k_fwd_main<1, float",),
                 "k_fwd_pair<0>": ("k_fwd_pair<0>",), "k_fwd_pair<1>": ("k_fwd_pair<1>",),
-                "k_adj_ex_main": ("k_adj_ex_main<float",),
-                "k_adj_ex_strip1+2": ("k_adj_ex_strip1", "k_adj_ex_strip2<")}
+                "k_adj_ex_main": ("k_adj_ex_main<float",)}
 
 
 def traffic_of(traffic, name):

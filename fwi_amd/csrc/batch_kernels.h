@@ -199,16 +199,14 @@ struct AdjExArgs {
   float* Lo;
   const float* Mc;
   float* Mo;
-  float* sx;   // psi-bar x
-  float* zx;   // zeta-bar x
-  float* sz;
-  float* zz;
-  float* ztx;  // chi_x (zeta-bar_in + a-bar) of this step
-  float* ztz;
-  float* ptx;  // a_x * psi-bar_tot of this step
-  float* ptz;
-  float* gbx;  // gbar_x, gbar_z of this step on the updated C-PML cells (zero on every other cell)
-  float* gbz;
+  const float* sxo;  // C-PML memory variables of step k+1: psi-bar x, zeta-bar x, psi-bar z, zeta-bar z
+  const float* zxo;
+  const float* szo;
+  const float* zzo;
+  float* sxn;        // of step k (written on the C-PML cells only; zero elsewhere since the pass began)
+  float* zxn;
+  float* szn;
+  float* zzn;
   float* DP;   // increment form (rows above `band`): D = m^{k+1} - m^{k+2}, updated in place to m^k - m^{k+1}
   float* DF;
   int band;    // rows [4, band) take the increment form (items are cut at band); 0: none

@@ -931,84 +931,19 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Exact discrete adjoint.  With chi_x / chi_z the side / bottom C-PML cells and Gamma = chi_x U chi_z:
+// Exact discrete adjoint.  With chi_x / chi_z the side / bottom C-PML cells of the updated region and
+// Gamma = chi_x U chi_z:
 //   abar = dt^2 (v_hor mP + v_nmo mF), bbar = dt^2 v_po (mP + mF)
 //   zt_x = chi_x (zbar_x + abar), gbar_x = abar + a_x zt_x, zbar_x <- b_x zt_x
 //   pt_x = chi_x (sbar_x - D1x(Gamma gbar_x)), sbar_x <- b_x pt_x
 //   m^k = 2 m^{k+1} + D2x gbar_x - D1x(a_x pt_x) - m^{k+2} (+ residual at the receivers), likewise z.
+// The work items compute these C-PML terms themselves from the memory variables zbar, sbar of step k+1 (a.zxo,
+// a.sxo, a.zzo, a.szo) and store those of step k on their own C-PML cells (a.zxn, a.sxn, a.zzn, a.szn).  a_x pt_x
+// enters D1x 4 columns beyond an item and is itself a D1x of gbar_x, so an item near a side strip forms gbar_x on
+// the 80 columns around its 64 (exchanged through LDS); a bottom item keeps gbar_z of 13 rows and a_z pt_z of 9 in
+// queues.  Neighbouring items form each other's border terms from the same inputs, and the memory variables of
+// steps k+1 and k live in separate arrays, so every item reads the same bits whatever the launch order.
 // ---------------------------------------------------------------------------------------------
-
-// Phase 1 on the C-PML cells: zt (kept for this step), the new zeta-bar, and the cell's gbar_x, gbar_z, which
-// phase 2 reads at the neighbours of each C-PML cell (the C-PML cells are Gamma without the rim, so the arrays,
-// zero elsewhere, hold the Gamma-masked gbar).
-__global__ void __launch_bounds__(256) k_adj_ex_strip1(StripArgs a, float dt2, const float* __restrict__ Lc,
-                                                       const float* __restrict__ Mc, float* __restrict__ zx,
-                                                       float* __restrict__ zz, float* __restrict__ ztx,
-                                                       float* __restrict__ ztz, float* __restrict__ gbx,
-                                                       float* __restrict__ gbz) {
-  const int2 blk = a.blocks[xcd_logical_block(blockIdx.x, gridDim.x)];
-  const ShotDev sd = a.g.shots[blk.x];
-  const int k = blk.y + threadIdx.x;
-  if (k >= sd.nstrip) return;
-  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
-  int ix, iz;
-  strip_cell(k, nx, nz, cpml, ix, iz);
-  const long i = sd.off + ix + (long)sd.ps * iz;
-  const long mi = sd.x0 + ix + (long)a.m.nxm * iz;
-  const float mP = Lc[i], mF = Mc[i];
-  const float ab = dt2 * (a.m.vh[mi] * mP + a.m.vn[mi] * mF);
-  const float bb = dt2 * (a.m.vp[mi] * (mP + mF));
-  float tx = 0.f, tz = 0.f;  // zt off chi_x / chi_z: never written, zero since the pass began
-  if (ix < cpml || ix > nx - cpml - 1) {
-    tx = zx[i] + ab;
-    ztx[i] = tx;
-    zx[i] = coef_x(a.g.tb, ix, nx, cpml) * tx;
-  }
-  if (iz > nz - cpml - 1) {
-    tz = zz[i] + bb;
-    ztz[i] = tz;
-    zz[i] = coef_z(a.g.tb, iz, nz, cpml) * tz;
-  }
-  gbx[i] = ab + coef_x(a.g.ta, ix, nx, cpml) * tx;
-  gbz[i] = bb + coef_z(a.g.ta, iz, nz, cpml) * tz;
-}
-
-// Phase 2 on the C-PML cells: psi-bar_tot -> a*psi-bar_tot (kept for this step) and the new psi-bar, from the
-// Gamma-masked gbar of phase 1 at the 8 neighbours (zero outside the window).
-template <class S>
-__global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a) {
-  const int2 blk = s.blocks[xcd_logical_block(blockIdx.x, gridDim.x)];
-  const ShotDev sd = a.g.shots[blk.x];
-  const int k = blk.y + threadIdx.x;
-  if (k >= sd.nstrip) return;
-  const int nx = sd.nx, nz = a.g.nz, cpml = a.g.cpml;
-  int ix, iz;
-  strip_cell(k, nx, nz, cpml, ix, iz);
-  const long i = sd.off + ix + (long)sd.ps * iz;
-  const float dh = a.g.dh;
-  if (ix < cpml || ix > nx - cpml - 1) {
-    float w[9];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) {
-      const int jx = ix + j - 4;
-      w[j] = (j == 4 || jx < 0 || jx >= nx) ? 0.f : a.gbx[i + j - 4];
-    }
-    const float t = a.sx[i] - d1s<S>(w[0], w[1], w[2], w[3], w[5], w[6], w[7], w[8], dh);
-    a.sx[i] = coef_x(a.g.tb, ix, nx, cpml) * t;
-    a.ptx[i] = coef_x(a.g.ta, ix, nx, cpml) * t;
-  }
-  if (iz > nz - cpml - 1) {
-    float w[9];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) {
-      const int jz = iz + j - 4;
-      w[j] = (j == 4 || jz < 0 || jz >= nz) ? 0.f : a.gbz[i + (long)(j - 4) * sd.ps];
-    }
-    const float t = a.sz[i] - d1s<S>(w[0], w[1], w[2], w[3], w[5], w[6], w[7], w[8], dh);
-    a.sz[i] = coef_z(a.g.tb, iz, nz, cpml) * t;
-    a.ptz[i] = coef_z(a.g.ta, iz, nz, cpml) * t;
-  }
-}
 
 // Main exact adjoint step with the exact imaging condition
 //   g_hor += dt^2 mP^k a^{k-1}, g_nmo += dt^2 mF^k a^{k-1}, g_po += dt^2 (mP^k + mF^k) b^{k-1}.
@@ -1031,7 +966,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_strip2(StripArgs s, AdjExArgs a)
 // receivers, where the residual enters, the two-level form's rounding is what limits the gradient; below, it is not.
 template <class S, bool GROUP, bool IMG, bool INC = false>
 __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
-  __shared__ float xbuf[4][72];
+  __shared__ float xbuf[4][152];
   __shared__ float gsum[GROUP ? 2 : 1][4][3][64];
   const int wave = wave_id(), lane = threadIdx.x & 63;
   const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
@@ -1075,7 +1010,25 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const int hx = lane < 8 ? clampi(hxr, 0, nx - 1) : cx;
   // bands within reach of a side strip need the C-PML terms in x
   const bool xnear = itm.y < cpml + 4 || itm.y + 64 > nx - cpml - 4;
+  // There gbar_x is formed on the 80 columns x0-8 .. x0+71 (LDS slot = column - x0 + 8): each lane at its column
+  // and at a second one, x0-8+lane for lanes 0..7, x0+56+lane for lanes 8..15, its own again for the others
+  const int wxr = lane < 8 ? itm.y - 8 + lane : lane < 16 ? itm.y + 56 + lane : ix;
+  const bool wact = wxr >= kRim && wxr < nx - kRim;
+  const int wx = clampi(wxr, 0, nx - 1);
+  const int wslot = wxr - itm.y + 8;
+  auto chix = [&](int c) { return (c >= kRim && c < cpml) || (c >= nx - cpml && c < nx - kRim); };
+  const bool xso = chix(ix), xsw = chix(wxr), xsh = lane < 8 && chix(hxr);
+  // Gamma membership above the bottom strip of the 9 columns around the lane's column (bit k: column ix-4+k) and
+  // around its halo column, as bits of a vector register (tested per row, not held as loop-invariant lane masks)
+  unsigned gmo = 0, gmh = 0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    gmo |= chix(ix - 4 + k) ? 1u << k : 0u;
+    gmh |= chix(hxr - 4 + k) ? 1u << k : 0u;
+  }
   const float axo = act ? coef_x(a.g.ta, ix, nx, cpml) : 0.f;
+  const float bxo = xso ? coef_x(a.g.tb, ix, nx, cpml) : 0.f;
+  const float axw = wact ? coef_x(a.g.ta, wx, nx, cpml) : 0.f;
   const float axh = hact ? coef_x(a.g.ta, hx, nx, cpml) : 0.f;
   const float dh = a.g.dh, dh2 = a.g.dh2, dt2 = a.dt2;
   const float* L = a.Lc + sd.off;
@@ -1087,10 +1040,14 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const float* vh = a.m.vh + sd.x0;
   const float* vp = a.m.vp + sd.x0;
   const float* vn = a.m.vn + sd.x0;
-  const float* ztz = a.ztz + sd.off;
-  const float* ztx = a.ztx + sd.off;
-  const float* ptx = a.ptx + sd.off;
-  const float* ptz = a.ptz + sd.off;
+  const float* zxo = a.zxo + sd.off;
+  const float* sxo = a.sxo + sd.off;
+  const float* zzo = a.zzo + sd.off;
+  const float* szo = a.szo + sd.off;
+  float* zxn = a.zxn + sd.off;
+  float* sxn = a.sxn + sd.off;
+  float* zzn = a.zzn + sd.off;
+  float* szn = a.szn + sd.off;
   const float* qa = img ? a.qa + sd.off : L;
   const float* qb = img ? a.qb + sd.off : L;
   float* buf = xbuf[wave];
@@ -1100,37 +1057,73 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   float* gn_ = a.gn + sd.gslot;
   float* gp_ = a.gp + sd.gslot;
 
-  float gq[9];  // gbar_z along z (rows z-4..z+4)
+  // gbar_z along z: rows z-4..z+4 in gq[0..8]; in the bottom items rows z-4..z+8 in gq[0..12], with a_z pt_z of
+  // rows z-4..z+4 in pq[0..8]
+  float gq[13], pq[9];
+  // C-PML coefficients a_z, b_z of row r (zero above the strip), read without a branch so that the scalar loads
+  // issue with the row's vector loads (rows past the grid read the outermost entry and are masked by the caller)
+  auto coefs_z = [&](int r) __attribute__((always_inline)) {
+    const int e = clampi(nz - 1 - r, 0, cpml - 1);
+    const __attribute__((address_space(4))) float* ta = (const __attribute__((address_space(4))) float*)a.g.ta;
+    const __attribute__((address_space(4))) float* tb = (const __attribute__((address_space(4))) float*)a.g.tb;
+    const float av = ta[e], bv = tb[e];
+    return make_float2(r >= zb ? av : 0.f, r >= zb ? bv : 0.f);
+  };
+  // gbar_z of row r of a bottom item from its mP, mF, v_po, zeta-bar_z^{k+1} and c = coefs_z(r); stores zeta-bar_z^k
+  // of an own cell
+  auto gbar_z = [&](int r, float l, float m, float vpv, float zo, float2 c) __attribute__((always_inline)) {
+    const float v = dt2 * (vpv * (l + m));
+    const bool zs = act && r >= zb && r < nz - kRim;
+    const float tz = zs ? zo + v : 0.f;
+    if (zs && r >= z0 && r < z1) stu(zzn, (unsigned)(cx + ps * r), c.y * tz);
+    return (act && r >= kRim && r < nz - kRim) ? v + c.x * tz : 0.f;
+  };
+  // a_z pt_z of row r from gbar_z of rows r-4..r+4 (g[0..8], Gamma-masked here: rows above the strip only in the
+  // side strips) and psi-bar_z^{k+1} (loaded here if so is null); stores psi-bar_z^k of an own cell
+  auto psi_z = [&](int r, const float* g, const float* so, float2 c) __attribute__((always_inline)) {
+    if (!(act && r >= zb && r < nz - kRim)) return 0.f;
+    const float sv = so ? *so : ldu(szo, (unsigned)(cx + ps * r));
+    float d;
+    if (r >= zb + 4) {
+      d = d1s<S>(g[0], g[1], g[2], g[3], g[5], g[6], g[7], g[8], dh);
+    } else {
+      float w[9];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int r = z0 - 4 + k;
-    const unsigned ir = (unsigned)(cx + ps * r);
-    float v = dt2 * (ldu(vp, (unsigned)(cx + nxm * r)) * (ldu(L, ir) + ldu(M, ir)));
-    if (r >= zb) v = v + coef_z(a.g.ta, r, nz, cpml) * ldu(ztz, ir);
-    gq[k] = (act && r >= kRim) ? v : 0.f;
-  }
+      for (int k = 0; k < 9; ++k) w[k] = (xso || r - 4 + k >= zb) ? g[k] : 0.f;
+      d = d1s<S>(w[0], w[1], w[2], w[3], w[5], w[6], w[7], w[8], dh);
+    }
+    const float t = sv - d;
+    if (r >= z0 && r < z1) stu(szn, (unsigned)(cx + ps * r), c.y * t);
+    return c.x * t;
+  };
   // The loads of a row are issued two rows ahead, before the current row's stores (the vector memory counter
   // retires in issue order, so a load issued after a store can only be waited for together with it), into
   // three row buffers used in rotation, so no register copy waits for them (interior items; the general body
   // loads each row at its start).
   struct Row {
-    float l4, m4, vp4, zt4;                // row z+4: mP, mF, v_po, zt_z (bottom C-PML rows)
+    float l4, m4, vp4, zt4, s4;            // row z+4 (ZB: z+8): mP, mF, v_po, zeta-bar_z; ZB: psi-bar_z of row z+4
     float lc, mc, vhc, vnc;                // row z at the lane's column: mP, mF (an L2 hit), v_hor, v_nmo
-    float vhh, vnh, lh, mh;                // row z at the halo column: v_hor, v_nmo, mP, mF
-    float xc, xh, pc, ph;                  // zt_x and a_x psi-bar_tot at the lane's and the halo column (XN)
+    float vhh, vnh, lh, mh;                // row z at the halo column (XN: the lane's second gbar_x column)
+    float xc, xh, pc, ph;                  // XN: zeta-bar_x at the lane's and second column, psi-bar_x at the
+                                           // lane's and the halo column
     float lold, mold, qa, qb, g0, g1, g2;  // centre-only values: m^{k+2} (INC: D), imaging terms, gradient
   };
-  // XN: band near a side strip; ZB: rows z >= zc (bottom C-PML terms, zt_z of row z+4); IN: increment form
+  // XN: band near a side strip; ZB: rows z >= zc (bottom C-PML terms); IN: increment form
   auto load_row = [&](auto xn_t, auto zb_t, auto in_t, int z, Row& r) __attribute__((always_inline)) {
     constexpr bool XN = decltype(xn_t)::value, ZB = decltype(zb_t)::value, IN = decltype(in_t)::value;
     const unsigned o = (unsigned)(cx + ps * z);
     const unsigned mo = (unsigned)(cx + nxm * z);
-    const unsigned ho = (unsigned)(hx + ps * z), hmo = (unsigned)(hx + nxm * z);
-    const unsigned ir = o + 4u * ps;
+    const int hc = XN ? wx : hx;
+    const unsigned ho = (unsigned)(hc + ps * z), hmo = (unsigned)(hc + nxm * z);
+    const int r4 = ZB ? min(z + 8, nz - 1) : z + 4;
+    const unsigned ir = (unsigned)(cx + ps * r4);
     r.l4 = ldu(L, ir);
     r.m4 = ldu(M, ir);
-    r.vp4 = ldu(vp, mo + 4u * nxm);
-    if (ZB) r.zt4 = ldu(ztz, ir);
+    r.vp4 = ldu(vp, (unsigned)(cx + nxm * r4));
+    if (ZB) {
+      r.zt4 = ldu(zzo, ir);
+      r.s4 = ldu(szo, o + 4u * ps);
+    }
     r.lc = ldu(L, o);
     r.mc = ldu(M, o);
     r.vhc = ldu(vh, mo);
@@ -1140,10 +1133,10 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     r.vnh = ldu(vn, hmo);
     r.mh = ldu(M, ho);
     if (XN) {
-      r.xc = ldu(ztx, o);
-      r.xh = ldu(ztx, ho);
-      r.pc = ldu(ptx, o);
-      r.ph = ldu(ptx, ho);
+      r.xc = ldu(zxo, o);
+      r.xh = ldu(zxo, ho);
+      r.pc = ldu(sxo, o);
+      r.ph = ldu(sxo, (unsigned)(hx + ps * z));
     }
     r.lold = ldn(IN ? DP : Lo, o);
     r.mold = ldn(IN ? DF : Mo, o);
@@ -1159,6 +1152,50 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       }
     }
   };
+  // gq (and pq) at the first row of the item
+  auto prologue = [&](auto zb_t) __attribute__((always_inline)) {
+    if constexpr (!decltype(zb_t)::value) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int r = z0 - 4 + k;
+        const unsigned ir = (unsigned)(cx + ps * r);
+        const float v = dt2 * (ldu(vp, (unsigned)(cx + nxm * r)) * (ldu(L, ir) + ldu(M, ir)));
+        gq[k] = (act && r >= kRim) ? v : 0.f;
+      }
+    } else if (z0 <= zb - 4) {
+      // an item starting 4 rows or more above the strip: pq of rows z0-4..z0+3 is zero, gq of rows z0-4..z0+7 is
+      // loaded in one round trip
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        const int r = z0 - 4 + k;
+        const unsigned ir = (unsigned)(cx + ps * r);
+        const float zo = r >= zb ? ldu(zzo, ir) : 0.f;
+        gq[k] = gbar_z(r, ldu(L, ir), ldu(M, ir), ldu(vp, (unsigned)(cx + nxm * r)), zo, coefs_z(r));
+      }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) pq[k] = 0.f;
+    } else {
+      // the queues' advance over the rows before z0, as the step makes it, from zeros: gq needs rows from z0-4, pq
+      // rows from z0-4, which take gbar_z from row z0-8 -- unless they lie above the strip, where pq is zero, as it
+      // is up to row z0+3 when the item starts at or above the strip (then gq needs rows from z0-4 only)
+#pragma unroll
+      for (int k = 0; k < 13; ++k) gq[k] = 0.f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) pq[k] = 0.f;
+#pragma unroll 4
+      for (int z = z0 <= zb ? z0 - 12 : z0 - 16; z < z0; ++z) {
+        const int r = clampi(z + 8, 0, nz - 1);
+        const unsigned ir = (unsigned)(cx + ps * r);
+        const float2 c8 = coefs_z(z + 8), c4 = coefs_z(z + 4);
+        gq[12] = gbar_z(z + 8, ldu(L, ir), ldu(M, ir), ldu(vp, (unsigned)(cx + nxm * r)), ldu(zzo, ir), c8);
+        pq[8] = psi_z(z + 4, gq + 4, nullptr, c4);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) gq[k] = gq[k + 1];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) pq[k] = pq[k + 1];
+      }
+    }
+  };
   // Row z from cur.  PIPE (interior items): row min(z+2, ze-1) is loaded into nxt once cur's stencil inputs
   // are consumed and before the row's stores (unused repeats on the last rows keep one path through the
   // body).  The edge kernel loads each row at its start: its C-PML inputs leave no registers for a second row.
@@ -1167,44 +1204,76 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     constexpr bool XN = decltype(xn_t)::value, ZB = decltype(zb_t)::value, PIPE = decltype(pipe_t)::value;
     constexpr bool IN = decltype(in_t)::value;
     const unsigned o = (unsigned)(cx + ps * z);
-    {
+    if (ZB) {
+      const float2 c8 = coefs_z(z + 8), c4 = coefs_z(z + 4);
+      gq[12] = gbar_z(z + 8, cur.l4, cur.m4, cur.vp4, cur.zt4, c8);
+      pq[8] = psi_z(z + 4, gq + 4, &cur.s4, c4);
+    } else {
       const int r = z + 4;
-      float v = dt2 * (cur.vp4 * (cur.l4 + cur.m4));
-      if (ZB) v = v + coef_z_uniform(a.g.ta, r, nz, cpml) * cur.zt4;
+      const float v = dt2 * (cur.vp4 * (cur.l4 + cur.m4));
       gq[8] = (act && r < nz - kRim) ? v : 0.f;
     }
-    // gbar_x of the band row (zero outside Omega) and of the halo columns
+    // gbar_x of the band row (zero outside Omega) and of the halo (XN: second) columns
     float go = dt2 * (cur.vhc * cur.lc + cur.vnc * cur.mc);
     float gh = dt2 * (cur.vhh * cur.lh + cur.vnh * cur.mh);
-    if (XN) {
-      go = go + axo * cur.xc;
-      gh = gh + axh * cur.xh;
-    }
-    go = act ? go : 0.f;
-    gh = hact ? gh : 0.f;
-    float pz[8];
-    if (ZB) {
-      const unsigned snx = (unsigned)ps;
-      pz[0] = ldu(ptz, o - 4 * snx);
-      pz[1] = ldu(ptz, o - 3 * snx);
-      pz[2] = ldu(ptz, o - 2 * snx);
-      pz[3] = ldu(ptz, o - snx);
-      pz[4] = ldu(ptz, o + snx);
-      pz[5] = ldu(ptz, o + 2 * snx);
-      pz[6] = ldu(ptz, o + 3 * snx);
-      pz[7] = ldu(ptz, o + 4 * snx);
-    }
     if (PIPE) load_row(xn_t, zb_t, in_t, min(z + 2, ze - 1), nxt);
-    float gx[9];
-    xrow9(buf, lane, go, gh, gx);
-    float ap = d2s<S>(gx[0], gx[1], gx[2], gx[3], gx[4], gx[5], gx[6], gx[7], gx[8], dh2);
-    float af = d2s<S>(gq[0], gq[1], gq[2], gq[3], gq[4], gq[5], gq[6], gq[7], gq[8], dh2);
+    float ap;
     if (XN) {
+      // zt_x = zeta-bar_x^{k+1} + abar on chi_x; zeta-bar_x^k of the lane's cell
+      const float tx = xso ? cur.xc + go : 0.f;
+      const float tw = xsw ? cur.xh + gh : 0.f;
+      if (xso) stu(zxn, o, bxo * tx);
+      go = act ? go + axo * tx : 0.f;
+      gh = wact ? gh + axw * tw : 0.f;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      buf[8 + lane] = go;
+      buf[wslot] = gh;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      float u[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) u[k] = buf[lane + 4 + k];
+      ap = d2s<S>(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8], dh2);
+      // a_x pt_x at the lane's column and (lanes 0..7) at its halo column, from the Gamma-masked gbar_x; on a
+      // bottom strip row every cell of Omega is in Gamma
+      const unsigned mo = z >= zb ? 0x1ffu : gmo, mh = z >= zb ? 0x1ffu : gmh;
+      float pto = 0.f, pth = 0.f;
+      if (xso) {
+        float w[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = ((mo >> k) & 1u) ? u[k] : 0.f;
+        const float t = cur.pc - d1s<S>(w[0], w[1], w[2], w[3], w[5], w[6], w[7], w[8], dh);
+        stu(sxn, o, bxo * t);
+        pto = axo * t;
+      }
+      if (xsh) {
+        const int hs = hxr - itm.y + 8;
+        float w[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = ((mh >> k) & 1u) ? buf[hs - 4 + k] : 0.f;
+        const float t = cur.ph - d1s<S>(w[0], w[1], w[2], w[3], w[5], w[6], w[7], w[8], dh);
+        pth = axh * t;
+      }
       float px[9];
-      xrow9(buf, lane, cur.pc, cur.ph, px);
+      xrow9(buf + 80, lane, pto, pth, px);
       ap = ap - d1s<S>(px[0], px[1], px[2], px[3], px[5], px[6], px[7], px[8], dh);
+    } else {
+      go = act ? go : 0.f;
+      gh = hact ? gh : 0.f;
+      float gx[9];
+      xrow9(buf, lane, go, gh, gx);
+      ap = d2s<S>(gx[0], gx[1], gx[2], gx[3], gx[4], gx[5], gx[6], gx[7], gx[8], dh2);
     }
-    if (ZB) af = af - d1s<S>(pz[0], pz[1], pz[2], pz[3], pz[4], pz[5], pz[6], pz[7], dh);
+    float af;
+    if (ZB) {
+      af = d2s<S>(gq[0], gq[1], gq[2], gq[3], gq[4], gq[5], gq[6], gq[7], gq[8], dh2);
+      af = af - d1s<S>(pq[0], pq[1], pq[2], pq[3], pq[5], pq[6], pq[7], pq[8], dh);
+    } else {
+      af = d2s<S>(gq[0], gq[1], gq[2], gq[3], gq[4], gq[5], gq[6], gq[7], gq[8], dh2);
+    }
     // IN: ln, mn first hold D^k, then m^k
     float ln = IN ? cur.lold + ap : 2.f * cur.lc + ap - cur.lold;
     float mn = IN ? cur.mold + af : 2.f * cur.mc + af - cur.mold;
@@ -1241,8 +1310,15 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
         stn(gp_, o, act ? cur.g2 + dt2 * (ln + mn) * cur.qb : cur.g2);
       }
     }
+    if (ZB) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) gq[k] = gq[k + 1];
+      for (int k = 0; k < 12; ++k) gq[k] = gq[k + 1];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) pq[k] = pq[k + 1];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) gq[k] = gq[k + 1];
+    }
     if (GROUP && img) {
       // double-buffered by row parity: one barrier per row suffices, since a wave can only overwrite this
       // buffer two rows later, after every wave has passed the next row's barrier
@@ -1257,6 +1333,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   };
   auto rows = [&](auto xn_t, auto zb_t, auto pipe_t, auto in_t, int za, int ze) __attribute__((always_inline)) {
     if (za >= ze) return;
+    prologue(zb_t);
     Row ra, rb;
     if (decltype(pipe_t)::value) {
       // two rows in flight, three buffers in rotation unrolled by three (no register copy waits for a load),

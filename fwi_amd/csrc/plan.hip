@@ -170,7 +170,10 @@ struct fwi_plan {
   // 6..8, R = F^n - F^{n-1} 9..11; three sets so that step pairs can keep step n while writing n+1 and n+2), in set
   // n mod 2 of P 0..1 and F 3..4 in the two-level form; C-PML memory psi_x^n 12..13 and psi_z^n 14..15 by step
   // parity (work items compute psi^{n+1} from psi^n, their neighbours' included), zx 16, zz 17
-  static constexpr int kFwdF = 18, kAdjF = 16, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
+  // adjoint fields: mu_P 0..1 and mu_F 2..3 by step parity; the exact adjoint's C-PML memory variables psi-bar_x
+  // 4..5, zeta-bar_x 6..7, psi-bar_z 8..9, zeta-bar_z 10..11 by step parity (the reference-compatible adjoint's
+  // eight in 4..11) and the increment form's D_P 12, D_F 13
+  static constexpr int kFwdF = 18, kAdjF = 14, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
   int fset(int n) const { return inc() ? n % 3 : (n & 1); }
   float* fP(int n) const { return fwd(fset(n)); }
   float* fF(int n) const { return fwd(3 + fset(n)); }
@@ -181,7 +184,7 @@ struct fwi_plan {
   float* fzx() const { return fwd(16); }
   float* fzz() const { return fwd(17); }
   float* fwd(int k) const { return d_work + (size_t)k * cap_cell; }
-  float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..15
+  float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..13
   float* sgrad(int k) const { return d_work + (size_t)(kFwdF + kAdjF + k) * cap_cell; }      // 0..2
   float* qbuf() const { return d_work + (size_t)kFixedF * cap_cell; }                      // qslots * 2 * cap_cell
   // checkpoints j = 1..J-1: 4 cell fields then 4 strip fields, FP32 or 16-bit (ck_esize bytes per value)
@@ -890,18 +893,17 @@ struct fwi_plan {
       a.Lo = Lo;
       a.Mc = Mc;
       a.Mo = Mo;
-      a.sx = adj(4);
-      a.zx = adj(5);
-      a.sz = adj(6);
-      a.zz = adj(7);
-      a.ztx = adj(8);
-      a.ztz = adj(9);
-      a.ptx = adj(10);
-      a.ptz = adj(11);
-      a.gbx = adj(12);
-      a.gbz = adj(13);
-      a.DP = adj(14);
-      a.DF = adj(15);
+      // C-PML memory variables of step k+1 in set (k+1) mod 2, of step k in set k mod 2
+      a.sxo = adj(4 + (cur ^ 1));
+      a.sxn = adj(4 + cur);
+      a.zxo = adj(6 + (cur ^ 1));
+      a.zxn = adj(6 + cur);
+      a.szo = adj(8 + (cur ^ 1));
+      a.szn = adj(8 + cur);
+      a.zzo = adj(10 + (cur ^ 1));
+      a.zzn = adj(10 + cur);
+      a.DP = adj(12);
+      a.DF = adj(13);
       a.band = opt.arith == FWI_ARITH_FP32I ? band : 0;
       a.res = resid();
       a.rit = k - 1;
@@ -910,13 +912,6 @@ struct fwi_plan {
       a.gh = sgrad(0);
       a.gp = sgrad(1);
       a.gn = sgrad(2);
-      if (b.nsblocks > 0) {
-        const int tes = timed_begin(6, st);
-        hipLaunchKernelGGL(k_adj_ex_strip1, dim3(b.nsblocks), dim3(256), 0, st, sa, a.dt2, Lc, Mc, a.zx, a.zz, a.ztx,
-                           a.ztz, a.gbx, a.gbz);
-        ARITH_LAUNCH(k_adj_ex_strip2, dim3(b.nsblocks), sa, a);
-        timed_end(tes, st);
-      }
       const int tev = timed_begin(2, st);
       if (int r = adj_ex_launch(b.grouped, am, grid, a, st)) return r;
       timed_end(tev, st);

@@ -193,9 +193,9 @@ int fwi_dot3_noalloc(const double* a, const double* b, long long n, double* out,
  * HIP events on the stream it runs on; fwi_plan_kernel_time returns the summed event time and the
  * number of timed launches of a kind (0 forward step, 1 forward step storing imaging terms -- each one launch with
  * the FP64 near-surface band --, 2 adjoint step, 3 / 4 step-pair launch without / with stored imaging terms (two
- * steps of the core, one of the band and the shell), 6 the two adjoint C-PML strip kernels together, 8 / 9 the
- * launch that completes a step pair without / with stored imaging terms (the band's and the shell's second step);
- * 5 and 7 are no longer used) since the last fwi_plan_set_timing call; fwi_plan_kernel_launches the number of
+ * steps of the core, one of the band and the shell), 8 / 9 the launch that completes a step pair without / with
+ * stored imaging terms (the band's and the shell's second step); 5, 6 and 7 are no longer used: the exact adjoint
+ * computes its C-PML terms inside the adjoint step) since the last fwi_plan_set_timing call; fwi_plan_kernel_launches the number of
  * launches of a kind since then (timed or not; counted while stride > 0). */
 void fwi_plan_set_timing(fwi_plan* plan, int stride);
 int fwi_plan_kernel_time(fwi_plan* plan, int kind, double* total_ms, long long* count);
