@@ -96,6 +96,7 @@ struct fwi_plan {
   int tz_adj = 32;   // rows per work item and order of the adjoint kernels
   int order_adj = 0;
   int tz_pair = 32;  // rows per fused core item (step pairs)
+  int tz_shell = 0;  // rows per shell item of the launch completing a step pair (0: tz)
   bool pairs_off = false;  // step_pairs = 1 and the tuner timed single steps faster (small, latency-bound grids)
   // FP64 near-surface band of the increment form: rows [4, band) of every shot (0: none).  State: seven FP64 arrays
   // of cap_band cells (band rows x row pitch of every shot): P, F and psi_x by step parity, zeta_x.
@@ -558,7 +559,10 @@ struct fwi_plan {
   // columns, row run of `rows`), shot -> row run -> tile, one workgroup each.  Shell items: 64-column bands with the
   // column range each one updates, over the rows above and below the core and the side columns beside it (a window
   // without a core is all shell).  The shell's first step also covers the core's cells within 4 of the shell, whose
-  // step n+1 values the shell's second step and the C-PML update read.
+  // step n+1 values the shell's second step and the C-PML update read.  The first step's shell items are runs of
+  // `rows` rows, as the core's: they share the launch with the bandwidth-bound core workgroups, and fewer, longer
+  // items spend less of it on their queue warm-up; the second step's are shorter runs (tz_shell rows, else tz):
+  // alone in their launch, more items finish it sooner.
   int pair_za() const { return std::max(kRim, band) + 4; }
   int pair_zb() const { return nz - cpml - 4; }
   int build_pair_items(Batch& b, int rows) {
@@ -575,7 +579,7 @@ struct fwi_plan {
       const int nxs = b.h_shots[s].nx, cx0 = pair_cx0(cpml), cx1 = pair_cx1(nxs, cpml);
       auto band_items = [&](int k, int x, int x1, int z0, int z1, int lo, int hi) {
         for (; x < x1; x += 64)
-          for (auto& r : runs_of(z0, z1, tz)) {
+          for (auto& r : runs_of(z0, z1, k == 0 ? rows : tz_shell > 0 ? tz_shell : tz)) {
             shell[k].push_back(make_int4(s, x, r.first, r.second));
             xr[k].push_back(make_int2(lo, hi));
           }
@@ -1267,6 +1271,34 @@ struct fwi_plan {
           best = ms;
           tz_pair = c;
         }
+      }
+      // rows of the completing launch's shell items, with the chosen core rows
+      if (rows_p.size() > 1 && b.nitems_core > 0 && best < 1e29f) {
+        const int keep = tz_shell;
+        int best_s = keep;
+        for (int sr : {4, 8, 16}) {
+          tz_shell = sr;
+          if (int r = build_pair_items(b, tz_pair)) return r;
+          if (int r = zero_fwd(b, st)) return r;
+          if (int r = fwd_pair(b, m, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, st)) return r;
+          float ms = 1e30f;
+          for (int rep = 0; rep < 2; ++rep) {
+            float t = 0;
+            if (int r = timed([&]() -> int {
+                  for (int n = 2; n < 8; n += 2)
+                    if (int r2 = fwd_pair(b, m, n, 0, nullptr, nullptr, nullptr, nullptr, nullptr, st)) return r2;
+                  return 0;
+                }, t))
+              return r;
+            ms = std::min(ms, t);
+          }
+          if (log) std::fprintf(stderr, "[fwi tune] pair shell rows %2d: %.1f us/step\n", sr, ms * 1e3 / 6);
+          if (ms < best) {
+            best = ms;
+            best_s = sr;
+          }
+        }
+        tz_shell = best_s;
       }
       // tuned pairs are kept only where they beat single steps (six steps each, the chosen tilings)
       if (opt.step_pairs == 1 && best < 1e29f) {
