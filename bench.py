@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--pairs", type=int, default=1,
                     help="step pairs: the core two steps per launch (1 = rows tuned, k > 1 = k rows per core item, "
                          "0 = single steps)")
+    ap.add_argument("--adj-pairs", type=int, default=1,
+                    help="adjoint step pairs: each gradient group's core two adjoint steps per launch (1 = rows tuned "
+                         "and kept where faster than single steps, k > 1 = k rows per core item, 0 = single steps)")
     ap.add_argument("--arith", type=int, default=3,
                     help="0 FP32 two-level, 1 mixed (reference), 2 FP32 difference stencil, 3 FP32 increment form")
     ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned)")
@@ -235,15 +238,18 @@ STEP_KERNELS = [
     (4, "k_fwd_pair<1>", "step pair storing the imaging terms: two steps of the core, one of the band and the shell"),
     (8, "k_fwd_main<0> (pair)", "the band's and the shell's second step of a step pair"),
     (9, "k_fwd_main<1> (pair)", "the band's and the shell's second step of a step pair, storing"),
+    (5, "k_adj_pair", "adjoint step pair: two adjoint steps of every gradient group's core, one of the shell"),
+    (7, "k_adj_shell", "the shell's second adjoint step of an adjoint step pair"),
 ]
 
 
-def kernel_bytes(kind, N, Nb, Nc, Nm, Ncore, grouped, inc):
+def kernel_bytes(kind, N, Nb, Nc, Nm, Ncore, grouped, inc, Nacore=0):
     """Bytes per launch of a step launch: (the survey's algorithmic count, this design's minimum).
 
     Survey count per forward step (FP32): 24 B per cell + 32 per C-PML cell + 12 per model cell (read once per
     batch); adjoint + imaging 56 B per cell + 32 per C-PML cell + 12 per model cell.  A step-pair launch advances the Ncore core cells two steps and the other N - Ncore cells one
-    step, the launch completing the pair those N - Ncore cells one more step: together they count two steps.
+    step, the launch completing the pair those N - Ncore cells one more step: together they count two steps; an
+    adjoint step-pair launch likewise advances the Nacore cells of the gradient groups' cores two adjoint steps.
     Design minimum: the increment form reads P, Q, F, R and writes them (32 B per cell and step or per core cell and
     step pair; + 8 B per cell and step when storing the imaging terms); the FP64 band's Nb cells move two FP64 levels
     of P and F (48 B) + their float rounding (8 B) + the model (12 B, per shot); the adjoint reads mu and mu^{k+2} (or
@@ -266,8 +272,16 @@ def kernel_bytes(kind, N, Nb, Nc, Nm, Ncore, grouped, inc):
         survey = 24.0 * (N - Ncore) + 32.0 * Nc + 12.0 * Nm
         design = shell(N - Ncore)
     else:
-        survey = 56.0 * N + 32.0 * Nc + 12.0 * Nm
-        design = (24.0 + (16.0 if inc else 8.0) + (6.0 if grouped else 24.0)) * N + 16.0 * Nc + 12.0 * Nm
+        adj_shell = (24.0 + (16.0 if inc else 8.0) + (6.0 if grouped else 24.0))
+        if kind == 5:  # core: mu^{k+1}, mu^{k+2} read, mu^k, mu^{k-1} written, both steps' terms, one slot update
+            survey = 56.0 * (2 * Nacore + (N - Nacore)) + 32.0 * Nc + 12.0 * Nm
+            design = (48.0 + (6.0 if grouped else 24.0)) * Nacore + adj_shell * (N - Nacore) + 16.0 * Nc + 12.0 * Nm
+        elif kind == 7:
+            survey = 56.0 * (N - Nacore) + 32.0 * Nc + 12.0 * Nm
+            design = adj_shell * (N - Nacore) + 16.0 * Nc + 12.0 * Nm
+        else:
+            survey = 56.0 * N + 32.0 * Nc + 12.0 * Nm
+            design = adj_shell * N + 16.0 * Nc + 12.0 * Nm
     return survey, design
 
 
@@ -312,13 +326,16 @@ def tiling_of(plan):
          "adjoint_tiling": {"rows_per_item": ar, "item_order": ao, "grouped": bool(g)}}
     if plan.pair_rows:
         t["pair_rows"] = plan.pair_rows
+    if plan.adj_pair_rows:
+        t["adj_pair_rows"] = plan.adj_pair_rows
     return t
 
 
 TRAFFIC_KEYS = {"k_fwd_main<0>": ("k_fwd_main<0, float",), "k_fwd_main<1>": ("k_fwd_main<1, float",),
                 "k_fwd_main<0> (pair)": ("k_fwd_main<0, float",), "k_fwd_main<1> (pair)": ("k_fwd_main<1, float",),
                 "k_fwd_pair<0>": ("k_fwd_pair<0>",), "k_fwd_pair<1>": ("k_fwd_pair<1>",),
-                "k_adj_ex_main": ("k_adj_ex_main<float",)}
+                "k_adj_ex_main": ("k_adj_ex_main<float",), "k_adj_pair": ("k_adj_pair",),
+                "k_adj_shell": ("k_adj_shell",)}
 
 
 def traffic_of(traffic, name):
@@ -367,13 +384,16 @@ def main():
         args.adj_rows, args.adj_order = pinned["adjoint_tiling"]["rows_per_item"], pinned["adjoint_tiling"]["item_order"]
         if args.pairs == 1 and pinned.get("pair_rows"):
             args.pairs = pinned["pair_rows"]
+        if args.adj_pairs == 1:
+            args.adj_pairs = pinned.get("adj_pair_rows", 0)
         log(f"tiling of the counter profile: forward {args.rows} rows order {args.order}, adjoint {args.adj_rows} rows "
             f"order {args.adj_order}")
     plan = Plan(w.nx, w.nz, nt, w.dh, w.dt, w.cpml, w.shots, src, ab, grad_mode=grad_mode, ckpt_interval=args.ckpt,
                 rows_per_item=args.rows, item_order=args.order, adj_rows_per_item=args.adj_rows,
                 adj_item_order=args.adj_order, arith=args.arith,
                 ckpt_format=args.ckpt_format, use_graph=0 if args.no_graph else 1, store_segments=args.keep,
-                grad_group=args.grad_group, step_pairs=args.pairs, surface_rows=args.surface_rows)
+                grad_group=args.grad_group, step_pairs=args.pairs, surface_rows=args.surface_rows,
+                adj_step_pairs=args.adj_pairs)
     log(f"plan: batch {plan.batch_size} shots, K={plan.ckpt_interval}, {plan.kept_steps} steps kept from the forward, "
         f"workspace {plan.workspace_bytes / 1e9:.1f} GB, FP64 band rows 4..{plan.surface_rows - 1}")
     t0 = time.perf_counter()
@@ -422,6 +442,7 @@ def main():
         band = plan.surface_rows
         nband = sum((band - 4) * s.nx for s in w.shots) if band else 0
         ncore = plan.core_cells()
+        nacore = plan.adj_core_cells()
         grouped = bool(plan.adj_tiling[2])
         inc = args.arith == 3
         traffic, tnote = load_traffic(w, len(w.shots), tiling_of(plan))
@@ -430,7 +451,7 @@ def main():
             if not n:
                 continue
             mean = ms / n / 1e3
-            survey, design = kernel_bytes(kind, ncell, nband, nstrip, nmodel, ncore, grouped, inc)
+            survey, design = kernel_bytes(kind, ncell, nband, nstrip, nmodel, ncore, grouped, inc, nacore)
             per_grad = plan.kernel_launches(kind)
             tb = traffic_of(traffic, name)
             k = {"kernel": name, "does": what, "launches_per_gradient": per_grad, "mean_launch_us": round(mean * 1e6, 1),
@@ -558,6 +579,7 @@ def main():
             "ckpt_interval": K,
             "kept_steps": kept,
             "step_pairs": bool(tiling.get("pair_rows")),
+            "adj_step_pairs": bool(tiling.get("adj_pair_rows")),
             "ckpt_format": {0: "fp32", 1: "bf16-scaled", 2: "fp16-scaled"}[args.ckpt_format],
             "workspace_gb": round(wsb / 1e9, 2),
             "batch_shots": bsz,

@@ -38,7 +38,7 @@ class FwiOptions(ctypes.Structure):
                 ("mem_budget", c_double), ("rows_per_item", c_int), ("item_order", c_int),
                 ("grad_group", c_int), ("ckpt_format", c_int), ("store_segments", c_int),
                 ("use_graph", c_int), ("step_pairs", c_int), ("surface_rows", c_int),
-                ("adj_rows_per_item", c_int), ("adj_item_order", c_int)]
+                ("adj_rows_per_item", c_int), ("adj_item_order", c_int), ("adj_step_pairs", c_int)]
 
 
 # Every exported symbol with its signature (argument types, return type).
@@ -104,6 +104,8 @@ BATCHED = {
     "fwi_plan_item_order": ([vp_], c_int),
     "fwi_plan_kept_steps": ([vp_], c_int),
     "fwi_plan_pair_rows": ([vp_], c_int),
+    "fwi_plan_adj_pair_rows": ([vp_], c_int),
+    "fwi_plan_adj_core_cells": ([vp_], c_ll),
     "fwi_plan_surface_rows": ([vp_], c_int),
     "fwi_plan_graph_stats": ([vp_, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)], c_int),
     "fwi_line_search_init": ([ctypes.POINTER(FwiLineSearch)], None),

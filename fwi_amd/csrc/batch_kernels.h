@@ -195,9 +195,11 @@ struct AdjExArgs {
   const int4* items;
   int nitems;
   float dt2;
-  const float* Lc;
-  float* Lo;
-  const float* Mc;
+  const float* Lc;  // mu_P^{k+1}
+  const float* Lp;  // mu_P^{k+2} (read)
+  float* Lo;        // mu_P^k (written; may be Lp)
+  const float* Mc;  // mu_F alike
+  const float* Mp;
   float* Mo;
   const float* sxo;  // C-PML memory variables of step k+1: psi-bar x, zeta-bar x, psi-bar z, zeta-bar z
   const float* zxo;
@@ -217,6 +219,36 @@ struct AdjExArgs {
   float* gh;
   float* gp;
   float* gn;
+  const int2* xr;  // shell items of a step pair: each item's column range (local to its shot), else null
+};
+
+// Two exact adjoint steps k, k-1 per launch on every gradient group's core (no C-PML term, no band row within
+// reach), with step k of the shell (sh, single-step items with column ranges) in the first workgroups.
+struct AdjPairArgs {
+  GridArgs g;
+  ModelArgs m;
+  const int4* items;  // four per workgroup: (member or -(leader+1), local first output column | columns << 16, z0, z1)
+  int nitems;
+  float dt2;
+  const float* L1;  // mu_P^{k+1}, mu_F^{k+1}
+  const float* M1;
+  const float* L2;  // mu^{k+2}
+  const float* M2;
+  float* L0;        // mu^k (written)
+  float* M0;
+  float* Lm;        // mu^{k-1} (written)
+  float* Mm;
+  const float* res;
+  int rit;          // residual time of step k (step k-1: rit - 1)
+  const float* qa0;  // imaging terms of step k, of step k-1
+  const float* qb0;
+  const float* qa1;
+  const float* qb1;
+  float* gh;
+  float* gp;
+  float* gn;
+  AdjExArgs sh;
+  int nshell_wg;
 };
 
 }  // namespace fwi

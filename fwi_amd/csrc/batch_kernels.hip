@@ -964,12 +964,12 @@ __global__ void __launch_bounds__(256) k_adj_ref_main(AdjRefArgs a) {
 // instead of m^{k+2}; D^k = D^{k+1} + (the step's terms) (+ the residual at the receivers), m^k = m^{k+1} + D^k --
 // the same recursion in exact arithmetic, with the time difference stored at its own precision.  Near the
 // receivers, where the residual enters, the two-level form's rounding is what limits the gradient; below, it is not.
-template <class S, bool GROUP, bool IMG, bool INC = false>
-__global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
-  __shared__ float xbuf[4][152];
-  __shared__ float gsum[GROUP ? 2 : 1][4][3][64];
+// SHELL: an item of a step pair's shell (a.xr holds each item's column range): it computes every cell of its band as
+// the single step does but stores, images and updates the gradient only in its range, with the general body.
+// buf: the wave's 152 LDS floats; gsum: the workgroup's imaging buffers.
+template <class S, bool GROUP, bool IMG, bool INC, bool SHELL>
+__device__ __forceinline__ void adj_ex_item(const AdjExArgs& a, int item, float* buf, float (*gsum)[4][3][64]) {
   const int wave = wave_id(), lane = threadIdx.x & 63;
-  const int item = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave;
   if (item >= a.nitems) return;
   int4 itm = a.items[item];
   const bool ghost = GROUP && itm.x < 0;
@@ -979,10 +979,15 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const int ix = itm.y + lane;
   const int z0 = itm.z, z1 = itm.w;
   constexpr bool img = IMG;
+  bool inr = true;  // the lane's cell is one this item updates
+  if constexpr (SHELL) {
+    const int2 r = a.xr[item];
+    inr = ix >= r.x && ix < r.y;
+  }
   // GROUP: field `wave` of the group's slot, read-modify-written by waves 0..2 on the model columns of the
   // union window; other lanes (and wave 3) address past the resource, so their accesses drop out
   const int mc = sd.x0 + ix;  // model column of the lane, the same in every wave of the workgroup
-  const bool gown = GROUP && img && wave < 3 && mc >= sd.gx0 && mc < sd.gx0 + sd.gnx;
+  const bool gown = GROUP && img && wave < 3 && mc >= sd.gx0 && mc < sd.gx0 + sd.gnx && inr;
   const brsrc grs = make_rsrc((wave == 0 ? a.gh : wave == 1 ? a.gn : a.gp) + sd.gslot, 4u * sd.gps * nz);
   const unsigned gcol = gown ? 4u * (unsigned)(mc - sd.gx0) : kOutOfRange;
   const unsigned grow = 4u * (unsigned)sd.gps;
@@ -1033,6 +1038,8 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   const float dh = a.g.dh, dh2 = a.g.dh2, dt2 = a.dt2;
   const float* L = a.Lc + sd.off;
   const float* M = a.Mc + sd.off;
+  const float* Lp = a.Lp + sd.off;
+  const float* Mp = a.Mp + sd.off;
   float* Lo = a.Lo + sd.off;
   float* Mo = a.Mo + sd.off;
   float* DP = a.DP + sd.off;  // increment-form rows: D
@@ -1050,7 +1057,6 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   float* szn = a.szn + sd.off;
   const float* qa = img ? a.qa + sd.off : L;
   const float* qb = img ? a.qb + sd.off : L;
-  float* buf = xbuf[wave];
   const int zb = nz - cpml;      // first row of the bottom C-PML strip
   const int zc = nz - cpml - 4;  // rows at or below this need the C-PML terms in z
   float* gh_ = a.gh + sd.gslot;  // per-shot gradient (ungrouped)
@@ -1075,7 +1081,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
     const float v = dt2 * (vpv * (l + m));
     const bool zs = act && r >= zb && r < nz - kRim;
     const float tz = zs ? zo + v : 0.f;
-    if (zs && r >= z0 && r < z1) stu(zzn, (unsigned)(cx + ps * r), c.y * tz);
+    if (zs && inr && r >= z0 && r < z1) stu(zzn, (unsigned)(cx + ps * r), c.y * tz);
     return (act && r >= kRim && r < nz - kRim) ? v + c.x * tz : 0.f;
   };
   // a_z pt_z of row r from gbar_z of rows r-4..r+4 (g[0..8], Gamma-masked here: rows above the strip only in the
@@ -1093,7 +1099,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       d = d1s<S>(w[0], w[1], w[2], w[3], w[5], w[6], w[7], w[8], dh);
     }
     const float t = sv - d;
-    if (r >= z0 && r < z1) stu(szn, (unsigned)(cx + ps * r), c.y * t);
+    if (inr && r >= z0 && r < z1) stu(szn, (unsigned)(cx + ps * r), c.y * t);
     return c.x * t;
   };
   // The loads of a row are issued two rows ahead, before the current row's stores (the vector memory counter
@@ -1138,8 +1144,8 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       r.pc = ldu(sxo, o);
       r.ph = ldu(sxo, (unsigned)(hx + ps * z));
     }
-    r.lold = ldn(IN ? DP : Lo, o);
-    r.mold = ldn(IN ? DF : Mo, o);
+    r.lold = ldn(IN ? DP : Lp, o);
+    r.mold = ldn(IN ? DF : Mp, o);
     if (img) {
       r.qa = ldn(qa, o);
       r.qb = ldn(qb, o);
@@ -1222,7 +1228,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       // zt_x = zeta-bar_x^{k+1} + abar on chi_x; zeta-bar_x^k of the lane's cell
       const float tx = xso ? cur.xc + go : 0.f;
       const float tw = xsw ? cur.xh + gh : 0.f;
-      if (xso) stu(zxn, o, bxo * tx);
+      if (xso && inr) stu(zxn, o, bxo * tx);
       go = act ? go + axo * tx : 0.f;
       gh = wact ? gh + axw * tw : 0.f;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1246,7 +1252,7 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) w[k] = ((mo >> k) & 1u) ? u[k] : 0.f;
         const float t = cur.pc - d1s<S>(w[0], w[1], w[2], w[3], w[5], w[6], w[7], w[8], dh);
-        stu(sxn, o, bxo * t);
+        if (inr) stu(sxn, o, bxo * t);
         pto = axo * t;
       }
       if (xsh) {
@@ -1286,25 +1292,30 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
       }
     }
     // Every lane stores: a lane off the updated cells (the rim, or a column past the shot clamped onto the
-    // rim) writes back the value it read, which no other wave changes (IN: m^{k+1} there, zero as m^{k+2} is).
+    // rim) stores the rim's value it read, zero at every time level (IN: m^{k+1} there, as m^{k+2}).
     if (IN) {
-      stn(DP, o, act ? ln : cur.lold);
-      stn(DF, o, act ? mn : cur.mold);
+      if (inr) {
+        stn(DP, o, act ? ln : cur.lold);
+        stn(DF, o, act ? mn : cur.mold);
+      }
       ln = cur.lc + ln;
       mn = cur.mc + mn;
-      stn(Lo, o, act ? ln : cur.lc);
-      stn(Mo, o, act ? mn : cur.mc);
-    } else {
+      if (inr) {
+        stn(Lo, o, act ? ln : cur.lc);
+        stn(Mo, o, act ? mn : cur.mc);
+      }
+    } else if (inr) {
       stn(Lo, o, act ? ln : cur.lold);
       stn(Mo, o, act ? mn : cur.mold);
     }
     float ch = 0.f, cn = 0.f, cp = 0.f;  // this shot's imaging contributions (GROUP)
     if (img) {
+      const bool upd = act && inr;
       if (GROUP) {
-        ch = act ? dt2 * ln * cur.qa : 0.f;
-        cn = act ? dt2 * mn * cur.qa : 0.f;
-        cp = act ? dt2 * (ln + mn) * cur.qb : 0.f;
-      } else {
+        ch = upd ? dt2 * ln * cur.qa : 0.f;
+        cn = upd ? dt2 * mn * cur.qa : 0.f;
+        cp = upd ? dt2 * (ln + mn) * cur.qb : 0.f;
+      } else if (inr) {
         stn(gh_, o, act ? cur.g0 + dt2 * ln * cur.qa : cur.g0);
         stn(gn_, o, act ? cur.g1 + dt2 * mn * cur.qa : cur.g1);
         stn(gp_, o, act ? cur.g2 + dt2 * (ln + mn) * cur.qb : cur.g2);
@@ -1366,15 +1377,286 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
   using T = std::true_type;
   using F = std::false_type;
   const bool inc = INC && z1 <= a.band;
+  using P = std::integral_constant<bool, !SHELL>;  // shell items take the general body
   if (xnear) {
     if (z0 >= zc) rows(T{}, T{}, F{}, F{}, z0, z1);
     else if (inc) rows(T{}, F{}, F{}, std::integral_constant<bool, INC>{}, z0, z1);
     else rows(T{}, F{}, F{}, F{}, z0, z1);
   } else {
     if (z0 >= zc) rows(F{}, T{}, F{}, F{}, z0, z1);
-    else if (inc) rows(F{}, F{}, T{}, std::integral_constant<bool, INC>{}, z0, z1);
-    else rows(F{}, F{}, T{}, F{}, z0, z1);
+    else if (inc) rows(F{}, F{}, P{}, std::integral_constant<bool, INC>{}, z0, z1);
+    else rows(F{}, F{}, P{}, F{}, z0, z1);
   }
+}
+
+template <class S, bool GROUP, bool IMG, bool INC = false>
+__global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
+  __shared__ float xbuf[4][152];
+  __shared__ float gsum[GROUP ? 2 : 1][4][3][64];
+  const int wave = wave_id();
+  adj_ex_item<S, GROUP, IMG, INC, false>(a, xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave, xbuf[wave], gsum);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Two exact adjoint steps k, k-1 per launch (temporal blocking) on each gradient group's core: the model columns
+// where every member's window has columns [cpml+8, nx-cpml-8) (rounded inward to 16) and rows
+// [max(4, band)+4, nz-cpml-8), so that neither step, nor the halo of step k that step k-1 reads, involves a C-PML
+// term, a row of the increment-form band or the rim.  A workgroup is one group on one tile of up to 48 output
+// columns, member j on wave j (placeholder waves as in the single step); the lanes of a wave compute step k on
+// columns [x0-4, x0+52) and step k-1 on the tile, marching down: at row r step k at row r, step k-1 at row r-4.
+// gbar_x^{k+1} and gbar_x^k are exchanged within the wave through LDS, gbar_z^{k+1} and gbar_z^k wait in register
+// queues, mu^k, gbar_x^k and v_po in four-row delay lines; step k-1 reads mu^{k+1} of its row again (a cache hit),
+// and the imaging terms and the gradient slot on the owned lanes only (a lane off the tile would fetch a third cache
+// line per row: 10 % of the launch's time).  Per row the members' imaging terms of both
+// steps meet in LDS and waves 0-2 update their field of the group's gradient slot once, as (g + S_k) + S_{k-1}:
+// the two single steps' additions in their order.  Every cell is computed with the single-step expressions in the
+// same order, so the results are identical to two single steps.  Stores go through buffer resources with an
+// out-of-range offset on the lanes that do not own a cell.
+// ---------------------------------------------------------------------------------------------
+constexpr int kAdjPairCols = 48;
+__host__ __device__ inline int adj_pair_cx0(int cpml) { return (cpml + 8 + 15) & ~15; }
+__host__ __device__ inline int adj_pair_cx1(int nx, int cpml) { return (nx - cpml - 8) & ~15; }
+
+// Wave-private exchange of one value per lane: on return u[k] is lane (l-4+k)'s value.  Lanes 0..3 and 60..63 read
+// slots no lane wrote; their results are not used.  buf: 72 floats of the wave's LDS.
+__device__ __forceinline__ void xrow64(float* buf, int lane, float v, float (&u)[9]) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  buf[4 + lane] = v;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int k = 0; k < 9; ++k) u[k] = buf[lane + k];
+}
+
+__device__ __forceinline__ void adj_pair_core(const AdjPairArgs& a, int4 itm, float* xb, float (*gsum)[4][6][64]) {
+  const int wave = wave_id(), lane = threadIdx.x & 63;
+  const bool ghost = itm.x < 0;
+  if (ghost) itm.x = -itm.x - 1;
+  const ShotDev sd = a.g.shots[itm.x];
+  const int nx = sd.nx, ps = sd.ps, nz = a.g.nz, nxm = a.m.nxm;
+  const int x0 = itm.y & 0xffff, ncols = itm.y >> 16;
+  const int z0 = itm.z, z1 = itm.w;
+  const int col = x0 - 8 + lane;  // the lane's column: step k on lanes 4..59, step k-1 on lanes 8..55
+  const bool own = lane >= 8 && lane < 8 + ncols;
+  const int fw = wave < 3 ? wave : 2;  // the field a wave sums (wave 3 sums one it does not store)
+  // field `wave` of the group's gradient slot (waves 0..2) on the owned columns
+  const int mc = sd.x0 + col;
+  const bool gown = wave < 3 && own && mc >= sd.gx0 && mc < sd.gx0 + sd.gnx;
+  const brsrc grs = make_rsrc((wave == 0 ? a.gh : wave == 1 ? a.gn : a.gp) + sd.gslot, 4u * sd.gps * nz);
+  const unsigned gcol = gown ? 4u * (unsigned)(mc - sd.gx0) : kOutOfRange;
+  const unsigned grow = 4u * (unsigned)sd.gps;
+  if (ghost) {
+    // placeholder wave: contributes zeros, takes part in the row barriers and in the slot's update
+    float gv = bld<kBufNT>(grs, gcol + grow * z0);
+    for (int z = z0; z < z1; ++z) {
+      const float g0 = gv;
+      gv = bld<kBufNT>(grs, gcol + grow * min(z + 1, z1 - 1));
+      float(*gs)[6][64] = gsum[z & 1];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) gs[wave][f][lane] = 0.f;
+      __syncthreads();
+      const float s0 = ((gs[0][fw][lane] + gs[1][fw][lane]) + gs[2][fw][lane]) + gs[3][fw][lane];
+      const float s1 = ((gs[0][3 + fw][lane] + gs[1][3 + fw][lane]) + gs[2][3 + fw][lane]) + gs[3][3 + fw][lane];
+      bst<kBufNT>(grs, gcol + grow * z, (g0 + s0) + s1);
+    }
+    return;
+  }
+  const int cc = clampi(col, 0, nx - 1);  // loads: a valid column (lanes outside the shot feed no owned cell)
+  const float dh2 = a.g.dh2, dt2 = a.dt2;
+  const float* L1 = a.L1 + sd.off;
+  const float* M1 = a.M1 + sd.off;
+  const float* L2 = a.L2 + sd.off;
+  const float* M2 = a.M2 + sd.off;
+  const float* vh = a.m.vh + sd.x0;
+  const float* vp = a.m.vp + sd.x0;
+  const float* vn = a.m.vn + sd.x0;
+  const unsigned fbytes = (unsigned)ps * (unsigned)nz * 4u;
+  const brsrc sL0 = make_rsrc(a.L0 + sd.off, fbytes), sM0 = make_rsrc(a.M0 + sd.off, fbytes);
+  const brsrc sLm = make_rsrc(a.Lm + sd.off, fbytes), sMm = make_rsrc(a.Mm + sd.off, fbytes);
+  const brsrc sqa0 = make_rsrc(a.qa0 + sd.off, fbytes), sqb0 = make_rsrc(a.qb0 + sd.off, fbytes);
+  const brsrc sqa1 = make_rsrc(a.qa1 + sd.off, fbytes), sqb1 = make_rsrc(a.qb1 + sd.off, fbytes);
+  const bool rec = sd.nrec > 0 && col >= sd.ir0 && col < sd.ir0 + sd.nrec;
+  const float* res0 = a.res + sd.goff + (long)a.rit * sd.nrec + (rec ? col - sd.ir0 : 0);  // step k
+  const float* res1 = res0 - sd.nrec;                                                       // step k-1
+  const int zend = z1 + 4;  // step k runs on rows [z0-4, zend)
+  struct Row {
+    float lc, mc, vh, vn;        // row r: mu_P^{k+1}, mu_F^{k+1}, v_hor, v_nmo
+    float l4, m4, vp4;           // row r+4: mu^{k+1}, v_po
+    float lp, mp;                // row r: mu^{k+2}
+  };
+  auto load_row = [&](int r, Row& w) __attribute__((always_inline)) {
+    const unsigned o = (unsigned)(cc + ps * r);
+    const unsigned mo = (unsigned)(cc + nxm * r);
+    w.lc = ldu(L1, o);
+    w.mc = ldu(M1, o);
+    w.vh = ldu(vh, mo);
+    w.vn = ldu(vn, mo);
+    w.l4 = ldu(L1, o + 4u * ps);
+    w.m4 = ldu(M1, o + 4u * ps);
+    w.vp4 = ldu(vp, mo + 4u * nxm);
+    w.lp = ldn(L2, o);
+    w.mp = ldn(M2, o);
+  };
+  float gq1[9];  // gbar_z^{k+1}, rows r-4..r+4
+  float gq0[9];  // gbar_z^k, rows r-8..r
+  float dl[4], dm[4], dgo[4], dvp[4];  // rows r-4..r-1: mu^k, gbar_x^k; v_po of rows r..r+3
+  float* xa = xb;       // exchange of gbar_x^{k+1}
+  float* xk = xb + 72;  // of gbar_x^k
+  // MAIN: step k-1 at row r-4 (rows r >= z0+4); otherwise step k only
+  auto iter = [&](int r, const Row& cur, Row& nxt, auto mainc) __attribute__((always_inline)) {
+    constexpr bool MAIN = decltype(mainc)::value;
+    // MAIN: row r-4's mu^{k+1} (a cache hit: read four rows ago), imaging terms and gradient slot (owned lanes),
+    // issued before this row's stores so that their wait does not cover them
+    float lk1 = 0.f, mk1 = 0.f, qa0 = 0.f, qb0 = 0.f, qa1 = 0.f, qb1 = 0.f, gv = 0.f;
+    if (MAIN) {
+      const unsigned ob = (unsigned)(cc + ps * (r - 4));
+      lk1 = ldu(L1, ob);
+      mk1 = ldu(M1, ob);
+      const unsigned oq = own ? 4u * ob : kOutOfRange;
+      qa0 = bld<kBufNT>(sqa0, oq);
+      qb0 = bld<kBufNT>(sqb0, oq);
+      qa1 = bld<kBufNT>(sqa1, oq);
+      qb1 = bld<kBufNT>(sqb1, oq);
+      gv = bld<kBufNT>(grs, gcol + grow * (r - 4));
+    }
+    gq1[8] = dt2 * (cur.vp4 * (cur.l4 + cur.m4));
+    const float go1 = dt2 * (cur.vh * cur.lc + cur.vn * cur.mc);
+    float u[9];
+    xrow64(xa, lane, go1, u);
+    load_row(min(r + 2, zend - 1), nxt);  // two rows ahead; unused repeats on the last rows keep one path
+    // step k at row r (the single step's expressions; no C-PML term in the core and its halo)
+    const float ap = d2s<float>(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8], dh2);
+    const float af = d2s<float>(gq1[0], gq1[1], gq1[2], gq1[3], gq1[4], gq1[5], gq1[6], gq1[7], gq1[8], dh2);
+    float ln = 2.f * cur.lc + ap - cur.lp;
+    float mn = 2.f * cur.mc + af - cur.mp;
+    if (r == sd.irz && rec) {
+      const float rv = res0[0];
+      ln += rv;
+      mn += rv;
+    }
+    if (r >= z0 && r < z1) {
+      const unsigned ob = own ? 4u * (unsigned)(col + ps * r) : kOutOfRange;
+      bst<kBufNT>(sL0, ob, ln);
+      bst<kBufNT>(sM0, ob, mn);
+    }
+    const float go0 = dt2 * (cur.vh * ln + cur.vn * mn);  // gbar_x^k at row r
+    gq0[8] = dt2 * (dvp[0] * (ln + mn));                   // gbar_z^k at row r
+    if (MAIN) {
+      const int rb = r - 4;
+      float w[9];
+      xrow64(xk, lane, dgo[0], w);
+      const float ap1 = d2s<float>(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], dh2);
+      const float af1 = d2s<float>(gq0[0], gq0[1], gq0[2], gq0[3], gq0[4], gq0[5], gq0[6], gq0[7], gq0[8], dh2);
+      float ln1 = 2.f * dl[0] + ap1 - lk1;
+      float mn1 = 2.f * dm[0] + af1 - mk1;
+      if (rb == sd.irz && rec) {
+        const float rv = res1[0];
+        ln1 += rv;
+        mn1 += rv;
+      }
+      const unsigned ob = own ? 4u * (unsigned)(col + ps * rb) : kOutOfRange;
+      bst<kBufNT>(sLm, ob, ln1);
+      bst<kBufNT>(sMm, ob, mn1);
+      // double-buffered by row parity: a wave can only overwrite this buffer two rows later, after every wave has
+      // passed the next row's barrier
+      float(*gs)[6][64] = gsum[rb & 1];
+      gs[wave][0][lane] = own ? dt2 * dl[0] * qa0 : 0.f;
+      gs[wave][1][lane] = own ? dt2 * dm[0] * qa0 : 0.f;
+      gs[wave][2][lane] = own ? dt2 * (dl[0] + dm[0]) * qb0 : 0.f;
+      gs[wave][3][lane] = own ? dt2 * ln1 * qa1 : 0.f;
+      gs[wave][4][lane] = own ? dt2 * mn1 * qa1 : 0.f;
+      gs[wave][5][lane] = own ? dt2 * (ln1 + mn1) * qb1 : 0.f;
+      __syncthreads();
+      const float s0 = ((gs[0][fw][lane] + gs[1][fw][lane]) + gs[2][fw][lane]) + gs[3][fw][lane];
+      const float s1 = ((gs[0][3 + fw][lane] + gs[1][3 + fw][lane]) + gs[2][3 + fw][lane]) + gs[3][3 + fw][lane];
+      bst<kBufNT>(grs, gcol + grow * rb, (gv + s0) + s1);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      gq1[k] = gq1[k + 1];
+      gq0[k] = gq0[k + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      dl[k] = dl[k + 1];
+      dm[k] = dm[k + 1];
+      dgo[k] = dgo[k + 1];
+      dvp[k] = dvp[k + 1];
+    }
+    dl[3] = ln;
+    dm[3] = mn;
+    dgo[3] = go0;
+    dvp[3] = cur.vp4;
+  };
+  using Pro = std::integral_constant<bool, false>;
+  using Main = std::integral_constant<bool, true>;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int r = z0 - 8 + k;
+    const unsigned o = (unsigned)(cc + ps * r);
+    gq1[k] = dt2 * (ldu(vp, (unsigned)(cc + nxm * r)) * (ldu(L1, o) + ldu(M1, o)));
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) dvp[k] = ldu(vp, (unsigned)(cc + nxm * (z0 - 4 + k)));
+  // four row buffers in rotation, two rows in flight: the 8 rows of step k above the first output row, the first
+  // four main rows peeled (so the loop is entered as it repeats), the main rows unrolled by four, the last ones
+  Row b0, b1, b2, b3;
+  load_row(z0 - 4, b0);
+  load_row(z0 - 3, b1);
+  iter(z0 - 4, b0, b2, Pro{});
+  iter(z0 - 3, b1, b3, Pro{});
+  iter(z0 - 2, b2, b0, Pro{});
+  iter(z0 - 1, b3, b1, Pro{});
+  iter(z0, b0, b2, Pro{});
+  iter(z0 + 1, b1, b3, Pro{});
+  iter(z0 + 2, b2, b0, Pro{});
+  iter(z0 + 3, b3, b1, Pro{});
+  int z = z0 + 4;
+  if (z + 3 < zend) {
+    iter(z, b0, b2, Main{});
+    iter(z + 1, b1, b3, Main{});
+    iter(z + 2, b2, b0, Main{});
+    iter(z + 3, b3, b1, Main{});
+    for (z += 4; z + 3 < zend; z += 4) {
+      iter(z, b0, b2, Main{});
+      iter(z + 1, b1, b3, Main{});
+      iter(z + 2, b2, b0, Main{});
+      iter(z + 3, b3, b1, Main{});
+    }
+  }
+  if (z < zend) iter(z, b0, b2, Main{});
+  if (z + 1 < zend) iter(z + 1, b1, b3, Main{});
+  if (z + 2 < zend) iter(z + 2, b2, b0, Main{});
+}
+
+// Grid: [0, nshell_wg) shell workgroups (step k, four single-step items each), then one workgroup per core tile
+// (four items).  Each part is dealt to the XCDs in contiguous logical ranges of its own.  FP32 increment-form
+// arithmetic (the band rows are shell rows), gradient groups, imaging terms for both steps.
+__global__ void __launch_bounds__(256) k_adj_pair(AdjPairArgs a) {
+  __shared__ float xbuf[4][152];
+  __shared__ float gsum[2][4][6][64];
+  const int wave = wave_id();
+  const int bid = (int)blockIdx.x;
+  if (bid < a.nshell_wg) {
+    adj_ex_item<float, true, true, true, true>(a.sh, xcd_logical_block(bid, a.nshell_wg) * 4 + wave, xbuf[wave],
+                                               reinterpret_cast<float(*)[4][3][64]>(&gsum[0][0][0][0]));
+    return;
+  }
+  const int b = xcd_logical_block(bid - a.nshell_wg, (int)gridDim.x - a.nshell_wg);
+  if (4 * b >= a.nitems) return;
+  adj_pair_core(a, a.items[4 * b + wave], xbuf[wave], gsum);
+}
+
+// Step k-1 of a step pair's shell: the single-step body on items with column ranges.
+__global__ void __launch_bounds__(256) k_adj_shell(AdjExArgs a) {
+  __shared__ float xbuf[4][152];
+  __shared__ float gsum[2][4][3][64];
+  const int wave = wave_id();
+  adj_ex_item<float, true, true, true, true>(a, xcd_logical_block(blockIdx.x, gridDim.x) * 4 + wave, xbuf[wave],
+                                             gsum);
 }
 
 // ---------------------------------------------------------------------------------------------

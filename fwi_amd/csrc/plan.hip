@@ -54,6 +54,12 @@ struct Batch {
   int4* d_items_shell1 = nullptr;  // the same for the shell's first step, widened by the core's 4-cell layer
   int2* d_xr_shell1 = nullptr;
   int nitems_shell1 = 0;
+  int4* d_items_acore = nullptr;  // adjoint step pairs: core tiles of the gradient groups (four items each)
+  int nitems_acore = 0;
+  int4* d_items_ash = nullptr;    // their shell: single-step items with column ranges (both steps)
+  int2* d_xr_ash = nullptr;
+  int nitems_ash = 0;
+  long long acore_cells = 0;      // cells of the members' windows in the groups' cores
   struct Group {
     int n = 0;
     int m[4] = {0, 0, 0, 0};  // members (batch indices) in wave order; m[0] leads
@@ -98,6 +104,8 @@ struct fwi_plan {
   int tz_pair = 32;  // rows per fused core item (step pairs)
   int tz_shell = 0;  // rows per shell item of the launch completing a step pair (0: tz)
   bool pairs_off = false;  // step_pairs = 1 and the tuner timed single steps faster (small, latency-bound grids)
+  int tz_apair = 32;       // rows per core item of the adjoint step pairs
+  bool apairs_off = false; // adj_step_pairs = 1 and the tuner timed single adjoint steps faster
   // FP64 near-surface band of the increment form: rows [4, band) of every shot (0: none).  State: seven FP64 arrays
   // of cap_band cells (band rows x row pitch of every shot): P, F and psi_x by step parity, zeta_x.
   int band = 0;
@@ -171,10 +179,12 @@ struct fwi_plan {
   // 6..8, R = F^n - F^{n-1} 9..11; three sets so that step pairs can keep step n while writing n+1 and n+2), in set
   // n mod 2 of P 0..1 and F 3..4 in the two-level form; C-PML memory psi_x^n 12..13 and psi_z^n 14..15 by step
   // parity (work items compute psi^{n+1} from psi^n, their neighbours' included), zx 16, zz 17
-  // adjoint fields: mu_P 0..1 and mu_F 2..3 by step parity; the exact adjoint's C-PML memory variables psi-bar_x
-  // 4..5, zeta-bar_x 6..7, psi-bar_z 8..9, zeta-bar_z 10..11 by step parity (the reference-compatible adjoint's
-  // eight in 4..11) and the increment form's D_P 12, D_F 13
-  static constexpr int kFwdF = 18, kAdjF = 14, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
+  // adjoint fields of the exact adjoint: mu_P^j in 0..3 and mu_F^j in 4..7, set (j+1) mod 4 (four time levels, so
+  // that a step pair reads two and writes two others); the C-PML memory variables psi-bar_x 8..9, zeta-bar_x
+  // 10..11, psi-bar_z 12..13, zeta-bar_z 14..15 by step parity; the increment form's D_P 16, D_F 17.  The
+  // reference-compatible adjoint: mu_P 0..1, mu_F 2..3 by step parity, its eight C-PML fields in 4..11.
+  static constexpr int kFwdF = 18, kAdjF = 18, kGradF = 3, kFixedF = kFwdF + kAdjF + kGradF;
+  static int aset(int j) { return (j + 1) & 3; }
   int fset(int n) const { return inc() ? n % 3 : (n & 1); }
   float* fP(int n) const { return fwd(fset(n)); }
   float* fF(int n) const { return fwd(3 + fset(n)); }
@@ -185,7 +195,7 @@ struct fwi_plan {
   float* fzx() const { return fwd(16); }
   float* fzz() const { return fwd(17); }
   float* fwd(int k) const { return d_work + (size_t)k * cap_cell; }
-  float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..13
+  float* adj(int k) const { return d_work + (size_t)(kFwdF + k) * cap_cell; }                // 0..17
   float* sgrad(int k) const { return d_work + (size_t)(kFwdF + kAdjF + k) * cap_cell; }      // 0..2
   float* qbuf() const { return d_work + (size_t)kFixedF * cap_cell; }                      // qslots * 2 * cap_cell
   // checkpoints j = 1..J-1: 4 cell fields then 4 strip fields, FP32 or 16-bit (ck_esize bytes per value)
@@ -347,6 +357,9 @@ struct fwi_plan {
       (void)hipFree(b.d_xr_shell);
       (void)hipFree(b.d_items_shell1);
       (void)hipFree(b.d_xr_shell1);
+      (void)hipFree(b.d_items_acore);
+      (void)hipFree(b.d_items_ash);
+      (void)hipFree(b.d_xr_ash);
       (void)hipFree(b.d_sblocks);
     }
     (void)hipFree(d_src);
@@ -637,6 +650,93 @@ struct fwi_plan {
     return upload_items(b.d_items_adj, b.nitems_adj, items);
   }
 
+  // Adjoint step pairs (grouped batches).  A group's core is the model columns where every member's window has
+  // [adj_pair_cx0, adj_pair_cx1) and rows [apair_za(), apair_zb()); the core's work items store mu^k and mu^{k-1}
+  // on all of its cells, so the shell -- everything else of the group's union window -- takes single steps k and
+  // k-1 over the same items (64-column bands of the union with the column range each updates, four members per
+  // workgroup as the single-step items, cut at the band and at row nz-cpml-4).  Core items: one tile of up to
+  // kAdjPairCols columns and `rows` rows per workgroup, group -> row run -> tile.  The partition is the same for
+  // every member, so each cell's four imaging terms are still added in one place and order.
+  int apair_za() const { return std::max(kRim, band) + 4; }
+  int apair_zb() const { return nz - cpml - 8; }
+  int build_adj_pair_items(Batch& b, int rows) {
+    std::vector<int4> core, shell;
+    std::vector<int2> xr;
+    const int za = apair_za(), zb = apair_zb(), zc = nz - cpml - 4;
+    b.acore_cells = 0;
+    auto runs_in = [&](int z0, int z1, int step) {  // runs of `step` rows in [z0, z1), cut at zc and the band end
+      std::vector<std::pair<int, int>> r;
+      for (int z = z0; z < z1; z += step) {
+        int a = z;
+        const int e = std::min(z + step, z1);
+        for (int sp : {band > kRim ? band : -1, zc})
+          if (sp > a && sp < e) {
+            r.push_back({a, sp});
+            a = sp;
+          }
+        r.push_back({a, e});
+      }
+      return r;
+    };
+    for (const auto& g : b.groups) {
+      const int gx0 = g.ux0, gx1 = g.ux0 + g.unx;
+      int c0 = gx0, c1 = gx1;
+      for (int j = 0; j < g.n; ++j) {
+        const ShotDev& d = b.h_shots[g.m[j]];
+        c0 = std::max(c0, d.x0 + adj_pair_cx0(cpml));
+        c1 = std::min(c1, d.x0 + adj_pair_cx1(d.nx, cpml));
+      }
+      const bool has_core = c1 > c0 && zb > za;
+      // the group's members (placeholder waves where one is missing or its window misses the band) on model
+      // columns [bm, bm+64), rows [z0, z1), updating model columns [lo, hi)
+      auto band_items = [&](int bm, int z0, int z1, int lo, int hi) {
+        for (auto& r : runs_in(z0, z1, tz_adj))
+          for (int j = 0; j < 4; ++j) {
+            bool member = j < g.n;
+            if (member) {
+              const ShotDev& d = b.h_shots[g.m[j]];
+              member = bm < d.x0 + d.nx && bm + 64 > d.x0;
+            }
+            const int s = member ? g.m[j] : g.m[0];
+            const int xs = b.h_shots[s].x0;
+            shell.push_back(make_int4(member ? s : -(s + 1), bm - xs, r.first, r.second));
+            xr.push_back(make_int2(lo - xs, hi - xs));
+          }
+      };
+      auto region = [&](int x0, int x1, int z0, int z1) {  // the bands meeting model columns [x0, x1)
+        if (x1 <= x0 || z1 <= z0) return;
+        for (int bm = gx0 + ((x0 - gx0) / 64) * 64; bm < x1; bm += 64) band_items(bm, z0, z1, x0, x1);
+      };
+      if (!has_core) {
+        region(gx0, gx1, kRim, nz - kRim);
+        continue;
+      }
+      region(gx0, gx1, kRim, za);       // above the core
+      region(gx0, gx1, zb, nz - kRim);  // below it
+      region(gx0, c0, za, zb);          // left of it
+      region(c1, gx1, za, zb);          // right of it
+      b.acore_cells += (long long)(c1 - c0) * (zb - za) * g.n;
+      for (int z = za; z < zb; z += rows)
+        for (int x = c0; x < c1; x += kAdjPairCols) {
+          const int ncols = std::min(kAdjPairCols, c1 - x);
+          for (int j = 0; j < 4; ++j) {
+            const int s = j < g.n ? g.m[j] : g.m[0];
+            const int xs = b.h_shots[s].x0;
+            core.push_back(make_int4(j < g.n ? s : -(s + 1), (x - xs) | (ncols << 16), z, std::min(z + rows, zb)));
+          }
+        }
+    }
+    if (int r = upload_items(b.d_items_acore, b.nitems_acore, core)) return r;
+    if (int r = upload_items(b.d_items_ash, b.nitems_ash, shell)) return r;
+    (void)hipFree(b.d_xr_ash);
+    b.d_xr_ash = nullptr;
+    if (!xr.empty()) {
+      CK(hipMalloc(&b.d_xr_ash, xr.size() * sizeof(int2)));
+      CK(hipMemcpy(b.d_xr_ash, xr.data(), xr.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    return 0;
+  }
+
   GridArgs grid_args(const Batch& b) const {
     GridArgs g;
     g.shots = b.d_shots;
@@ -836,6 +936,98 @@ struct fwi_plan {
     return 0;
   }
 
+  // Arguments of exact adjoint step k (m^k from m^{k+1}, m^{k+2}) over an item list, without imaging terms
+  AdjExArgs adj_ex_args(const Batch& b, const ModelArgs& m, int k, const int4* items, int nitems) const {
+    AdjExArgs a;
+    a.g = grid_args(b);
+    a.m = m;
+    a.items = items;
+    a.nitems = nitems;
+    a.dt2 = dt * dt;
+    a.Lc = adj(aset(k + 1));
+    a.Lp = adj(aset(k + 2));
+    a.Lo = adj(aset(k));
+    a.Mc = adj(4 + aset(k + 1));
+    a.Mp = adj(4 + aset(k + 2));
+    a.Mo = adj(4 + aset(k));
+    // C-PML memory variables of step k+1 in set (k+1) mod 2, of step k in set k mod 2
+    const int cur = k & 1;
+    a.sxo = adj(8 + (cur ^ 1));
+    a.sxn = adj(8 + cur);
+    a.zxo = adj(10 + (cur ^ 1));
+    a.zxn = adj(10 + cur);
+    a.szo = adj(12 + (cur ^ 1));
+    a.szn = adj(12 + cur);
+    a.zzo = adj(14 + (cur ^ 1));
+    a.zzn = adj(14 + cur);
+    a.DP = adj(16);
+    a.DF = adj(17);
+    a.band = opt.arith == FWI_ARITH_FP32I ? band : 0;
+    a.res = resid();
+    a.rit = k - 1;
+    a.qa = nullptr;
+    a.qb = nullptr;
+    a.gh = sgrad(0);
+    a.gp = sgrad(1);
+    a.gn = sgrad(2);
+    a.xr = nullptr;
+    return a;
+  }
+
+  bool apairs_possible(const Batch& b) const {
+    return opt.grad_mode == FWI_GRAD_EXACT && inc() && opt.adj_step_pairs != 0 && b.grouped;
+  }
+  bool apairs(const Batch& b) const { return apairs_possible(b) && !apairs_off && b.nitems_acore > 0; }
+  // adjoint steps k and k-1 (mu^k, mu^{k-1} from mu^{k+1}, mu^{k+2}): the core's two steps and the shell's step k
+  // in one launch, the shell's step k-1 in a second; the imaging terms of both steps are needed
+  int adj_pair(const Batch& b, const ModelArgs& m, int k, const float* qa0, const float* qb0, const float* qa1,
+               const float* qb1, hipStream_t st) {
+    if (!qa0 || !qa1 || k < 2) return fail("adjoint step pair: needs both steps' imaging terms and k >= 2");
+    AdjPairArgs a;
+    a.g = grid_args(b);
+    a.m = m;
+    a.items = b.d_items_acore;
+    a.nitems = b.nitems_acore;
+    a.dt2 = dt * dt;
+    a.L1 = adj(aset(k + 1));
+    a.M1 = adj(4 + aset(k + 1));
+    a.L2 = adj(aset(k + 2));
+    a.M2 = adj(4 + aset(k + 2));
+    a.L0 = adj(aset(k));
+    a.M0 = adj(4 + aset(k));
+    a.Lm = adj(aset(k - 1));
+    a.Mm = adj(4 + aset(k - 1));
+    a.res = resid();
+    a.rit = k - 1;
+    a.qa0 = qa0;
+    a.qb0 = qb0;
+    a.qa1 = qa1;
+    a.qb1 = qb1;
+    a.gh = sgrad(0);
+    a.gp = sgrad(1);
+    a.gn = sgrad(2);
+    a.sh = adj_ex_args(b, m, k, b.d_items_ash, b.nitems_ash);
+    a.sh.qa = qa0;
+    a.sh.qb = qb0;
+    a.sh.xr = b.d_xr_ash;
+    a.nshell_wg = ((b.nitems_ash + 3) / 4 + 7) / 8 * 8;
+    const int tev = timed_begin(5, st);
+    hipLaunchKernelGGL(k_adj_pair, dim3((unsigned)(a.nshell_wg + b.nitems_acore / 4)), dim3(256), 0, st, a);
+    timed_end(tev, st);
+    CK(hipGetLastError());
+    if (b.nitems_ash > 0) {
+      AdjExArgs c = adj_ex_args(b, m, k - 1, b.d_items_ash, b.nitems_ash);
+      c.qa = qa1;
+      c.qb = qb1;
+      c.xr = b.d_xr_ash;
+      const int tes = timed_begin(7, st);
+      hipLaunchKernelGGL(k_adj_shell, dim3((unsigned)(b.nitems_ash / 4)), dim3(256), 0, st, c);
+      timed_end(tes, st);
+      CK(hipGetLastError());
+    }
+    return 0;
+  }
+
   // one adjoint step k (produces m^k from m^{k+1}, m^{k+2}); q slot pointers may be null
   int adj_step(const Batch& b, const ModelArgs& m, int k, const float* qa, const float* qb, hipStream_t st) {
     const int cur = k & 1;
@@ -887,35 +1079,9 @@ struct fwi_plan {
       ARITH_LAUNCH(k_adj_ref_main, grid, a);
       timed_end(tev, st);
     } else {
-      AdjExArgs a;
-      a.g = grid_args(b);
-      a.m = m;
-      a.items = b.d_items_adj;
-      a.nitems = b.nitems_adj;
-      a.dt2 = dt * dt;
-      a.Lc = Lc;
-      a.Lo = Lo;
-      a.Mc = Mc;
-      a.Mo = Mo;
-      // C-PML memory variables of step k+1 in set (k+1) mod 2, of step k in set k mod 2
-      a.sxo = adj(4 + (cur ^ 1));
-      a.sxn = adj(4 + cur);
-      a.zxo = adj(6 + (cur ^ 1));
-      a.zxn = adj(6 + cur);
-      a.szo = adj(8 + (cur ^ 1));
-      a.szn = adj(8 + cur);
-      a.zzo = adj(10 + (cur ^ 1));
-      a.zzn = adj(10 + cur);
-      a.DP = adj(12);
-      a.DF = adj(13);
-      a.band = opt.arith == FWI_ARITH_FP32I ? band : 0;
-      a.res = resid();
-      a.rit = k - 1;
+      AdjExArgs a = adj_ex_args(b, m, k, b.d_items_adj, b.nitems_adj);
       a.qa = qa;
       a.qb = qb;
-      a.gh = sgrad(0);
-      a.gp = sgrad(1);
-      a.gn = sgrad(2);
       const int tev = timed_begin(2, st);
       if (int r = adj_ex_launch(b.grouped, am, grid, a, st)) return r;
       timed_end(tev, st);
@@ -1084,7 +1250,7 @@ struct fwi_plan {
     // segments below s0 are recomputed from their checkpoints into slots 0..K-1, whose kept terms the
     // adjoint has consumed by then (it walks down from nt)
     int seg = J - S;
-    for (int k = nt; k >= 1; --k) {
+    for (int k = nt; k >= 1;) {
       const int n = k - lag;
       const float* qa = nullptr;
       const float* qb = nullptr;
@@ -1116,8 +1282,17 @@ struct fwi_plan {
         qa = qbuf() + (size_t)(n - seg * K) * slot;
         qb = qa + cap_cell;
       }
+      // a step pair where step k-1's imaging terms lie in the same kept or recomputed range
+      if (qa && k >= 2 && apairs(b) && (n >= s0 ? n - 1 >= s0 : n - 1 >= seg * K)) {
+        const float* qa1 = qa - slot;
+        if (int r = adj_pair(b, m, k, qa, qb, qa1, qa1 + cap_cell, st)) return r;
+        if (int r = step_done()) return r;
+        k -= 2;
+        continue;
+      }
       if (int r = adj_step(b, m, k, qa, qb, st)) return r;
       if (int r = step_done()) return r;
+      --k;
     }
     const long long nmodel = (long long)nx * nz;
     hipLaunchKernelGGL(k_cross_shot_sum, dim3((unsigned)((nmodel + 255) / 256)), dim3(256), 0, st, b.d_shots, b.count,
@@ -1241,6 +1416,56 @@ struct fwi_plan {
         cands.push_back({ms, o, c});
       }
     pick(cands, tz_adj, order_adj);
+    // adjoint step pairs: rows per core item, kept only where they beat single steps (six steps each)
+    if (apairs_possible(b) && nt >= 10) {
+      if (int r = build_adj_items(b, tz_adj, order_adj)) return r;
+      std::vector<int> rows_q = {16, 32, 48, 64};
+      if (opt.adj_step_pairs > 1) rows_q = {opt.adj_step_pairs};
+      auto time_adj = [&](bool pr, float& ms) -> int {
+        ms = 1e30f;
+        CK(hipMemsetAsync(adj(0), 0, kAdjF * (size_t)cap_cell * sizeof(float), st));
+        for (int w = 0; w < 2; ++w)
+          if (int r = adj_step(b, m, nt - w, qa, qa + cap_cell, st)) return r;
+        for (int rep = 0; rep < 2; ++rep) {
+          float t = 0;
+          if (int r = timed([&]() -> int {
+                for (int n = 2; n < 8; n += pr ? 2 : 1)
+                  if (int r2 = pr ? adj_pair(b, m, nt - n, qa, qa + cap_cell, qa, qa + cap_cell, st)
+                                  : adj_step(b, m, nt - n, qa, qa + cap_cell, st))
+                    return r2;
+                return 0;
+              }, t))
+            return r;
+          ms = std::min(ms, t);
+        }
+        return 0;
+      };
+      float best = 1e30f;
+      for (int c : rows_q) {
+        if (c > nz - 8 && c != rows_q[0]) continue;
+        if (int r = build_adj_pair_items(b, c)) return r;
+        if (rows_q.size() == 1 || b.nitems_acore == 0) {
+          tz_apair = c;
+          best = -1.f;
+          break;
+        }
+        float ms = 0;
+        if (int r = time_adj(true, ms)) return r;
+        if (log) std::fprintf(stderr, "[fwi tune] adjoint pair rows %2d: %.1f us/step\n", c, ms * 1e3 / 6);
+        if (ms < best) {
+          best = ms;
+          tz_apair = c;
+        }
+      }
+      if (opt.adj_step_pairs == 1 && best >= 0.f && best < 1e29f) {
+        float single = 0;
+        if (int r = time_adj(false, single)) return r;
+        apairs_off = best >= single;
+        if (log)
+          std::fprintf(stderr, "[fwi tune] adjoint pairs %s (%.1f vs %.1f us/step single)\n", apairs_off ? "off" : "on",
+                       best * 1e3 / 6, single * 1e3 / 6);
+      }
+    }
     // fused core step pairs: rows per item
     if (inc() && opt.step_pairs != 0) {
       std::vector<int> rows_p = {16, 24, 32, 48, 64, 96};
@@ -1328,6 +1553,8 @@ struct fwi_plan {
       if (int r = build_adj_items(bb, tz_adj, order_adj)) return r;
       if (inc() && opt.step_pairs != 0)
         if (int r = build_pair_items(bb, tz_pair)) return r;
+      if (apairs_possible(bb))
+        if (int r = build_adj_pair_items(bb, tz_apair)) return r;
     }
     tuned = true;
     return 0;
@@ -1421,6 +1648,7 @@ void fwi_options_default(fwi_options* o) {
   o->adj_item_order = -1;
   o->use_graph = 1;
   o->step_pairs = 1;
+  o->adj_step_pairs = 1;
 }
 
 int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out) {
@@ -1673,6 +1901,7 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     if (rc == 0) rc = p->build_items(b, p->tz, p->order);
     if (rc == 0) rc = p->build_adj_items(b, p->tz_adj, p->order_adj);
     if (rc == 0 && p->inc() && p->opt.step_pairs != 0) rc = p->build_pair_items(b, p->tz_pair);
+    if (rc == 0 && p->apairs_possible(b)) rc = p->build_adj_pair_items(b, p->tz_apair);
   }
   if (rc) {
     delete p;
@@ -1843,6 +2072,15 @@ int fwi_plan_graph_nodes(const fwi_plan* p, long long* largest, long long* last)
   if (largest) *largest = p->graph_nodes_largest;
   if (last) *last = p->graph_nodes_last;
   return 0;
+}
+long long fwi_plan_adj_core_cells(const fwi_plan* p) {
+  long long c = 0;
+  if (!p->batches.empty() && p->apairs(p->batches[0]))
+    for (auto& b : p->batches) c += b.acore_cells;
+  return c;
+}
+int fwi_plan_adj_pair_rows(const fwi_plan* p) {
+  return (!p->batches.empty() && p->apairs(p->batches[0])) ? p->tz_apair : 0;
 }
 int fwi_plan_pair_rows(const fwi_plan* p) {
   return (!p->batches.empty() && p->pairs(p->batches[0])) ? p->tz_pair : 0;

@@ -103,7 +103,8 @@ def main(argv=None):
         if not lines:
             sys.exit(f"{f}: no bench JSON line")
         c = json.loads(lines[-1])["config"]
-        tilings.append({k: c[k] for k in ("rows_per_item", "item_order", "adjoint_tiling", "pair_rows") if k in c})
+        tilings.append({k: c[k] for k in ("rows_per_item", "item_order", "adjoint_tiling", "pair_rows", "adj_pair_rows")
+                        if k in c})
     if any(t != tilings[0] for t in tilings):
         sys.exit(f"the counter runs used different tilings: {tilings}")
     res = summarize(a.fetch_dir, a.write_dir)

@@ -66,7 +66,7 @@ class Plan:
                  ckpt_interval: int = 0, max_batch: int = 0, mem_budget: float = 0.0, rows_per_item: int = 0,
                  item_order: int = -1, grad_group: int = 0, ckpt_format: int = 0, use_graph: int = 1,
                  store_segments: int = 0, step_pairs: int = 1, surface_rows: int = 0, adj_rows_per_item: int = 0,
-                 adj_item_order: int = -1):
+                 adj_item_order: int = -1, adj_step_pairs: int = 1):
         L = _lib.load()
         self.nx, self.nz, self.nt, self.dh, self.dt, self.cpml = nx, nz, nt, dh, dt, cpml
         self.shots = list(shots)
@@ -97,6 +97,7 @@ class Plan:
         opt.surface_rows = surface_rows
         opt.adj_rows_per_item = adj_rows_per_item
         opt.adj_item_order = adj_item_order
+        opt.adj_step_pairs = adj_step_pairs
         h = ctypes.c_void_p()
         _lib.check(L.fwi_plan_create(ctypes.byref(sv), ctypes.byref(opt), ctypes.byref(h)), "fwi_plan_create")
         self._h = h
@@ -133,6 +134,11 @@ class Plan:
     def pair_rows(self) -> int:
         """Rows per fused core item of the step pairs (0: single steps only)."""
         return int(_lib.load().fwi_plan_pair_rows(self._h))
+
+    @property
+    def adj_pair_rows(self) -> int:
+        """Rows per core item of the adjoint step pairs (0: single adjoint steps only)."""
+        return int(_lib.load().fwi_plan_adj_pair_rows(self._h))
 
     @property
     def kept_steps(self) -> int:
@@ -194,6 +200,10 @@ class Plan:
     def core_cells(self) -> int:
         """Cells the step pairs advance two steps per launch (0 without step pairs)."""
         return int(_lib.load().fwi_plan_core_cells(self._h))
+
+    def adj_core_cells(self) -> int:
+        """Cells the adjoint step pairs advance two steps per launch (0 without adjoint step pairs)."""
+        return int(_lib.load().fwi_plan_adj_core_cells(self._h))
 
     def kernel_time(self, kind: int):
         """(total ms, launches timed) of kind 0 forward step, 1 forward step storing, 2 adjoint step, 3 / 4 fused

@@ -92,6 +92,10 @@ typedef struct fwi_options {
   int adj_rows_per_item; /* z rows per wave in the adjoint kernels: 0 = rows_per_item when that is set, else tuned */
   int adj_item_order;    /* adjoint work-item order 0..3 (1 means 0 with gradient groups), -1 = derived from item_order
                             when that is set (1 -> 0 with groups, 4 -> 3), else tuned */
+  int adj_step_pairs;    /* exact gradient, FP32 increment form, gradient groups: the core of every group (8+ cells from
+                            the C-PML, 4+ below the FP64 band) advances two adjoint steps per launch, the rest by
+                            single steps; 1 (default) = rows per core work item tuned and kept only where faster than
+                            single steps, k > 1 = k rows; 0: single steps.  The result is the same either way. */
 } fwi_options;
 
 typedef struct fwi_plan fwi_plan;
@@ -194,8 +198,9 @@ int fwi_dot3_noalloc(const double* a, const double* b, long long n, double* out,
  * number of timed launches of a kind (0 forward step, 1 forward step storing imaging terms -- each one launch with
  * the FP64 near-surface band --, 2 adjoint step, 3 / 4 step-pair launch without / with stored imaging terms (two
  * steps of the core, one of the band and the shell), 8 / 9 the launch that completes a step pair without / with
- * stored imaging terms (the band's and the shell's second step); 5, 6 and 7 are no longer used: the exact adjoint
- * computes its C-PML terms inside the adjoint step) since the last fwi_plan_set_timing call; fwi_plan_kernel_launches the number of
+ * stored imaging terms (the band's and the shell's second step); 5 the adjoint step-pair launch (two steps of every
+ * gradient group's core, one of the shell), 7 the launch completing it (the shell's second step); 6 is no longer
+ * used: the exact adjoint computes its C-PML terms inside the adjoint step) since the last fwi_plan_set_timing call; fwi_plan_kernel_launches the number of
  * launches of a kind since then (timed or not; counted while stride > 0). */
 void fwi_plan_set_timing(fwi_plan* plan, int stride);
 int fwi_plan_kernel_time(fwi_plan* plan, int kind, double* total_ms, long long* count);
@@ -215,6 +220,10 @@ int fwi_plan_item_order(const fwi_plan* plan);
 int fwi_plan_adj_tiling(const fwi_plan* plan, int* rows_per_item, int* item_order, int* grouped);
 /* Rows per fused core item of the step pairs in use (0 when the plan takes single steps only). */
 int fwi_plan_pair_rows(const fwi_plan* plan);
+/* Rows per core item of the adjoint step pairs in use (0 when the adjoint takes single steps only), and the cells
+ * of the shots' windows that those pairs advance two steps per launch (0 likewise). */
+int fwi_plan_adj_pair_rows(const fwi_plan* plan);
+long long fwi_plan_adj_core_cells(const fwi_plan* plan);
 /* End row of the FP64 near-surface band (rows 4..end-1 of every shot) of the forward in use (0: none). */
 int fwi_plan_surface_rows(const fwi_plan* plan);
 /* Graph captures and replays so far (use_graph). */

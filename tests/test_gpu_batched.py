@@ -505,6 +505,61 @@ def test_step_pairs_identical_to_single_steps(torch, geometry):
             assert np.array_equal(G1, G0), exact
 
 
+def _adj_pair_workloads():
+    """Geometries for the adjoint step pairs, whose core is each gradient group's common core (columns
+    [cpml+8, nx-cpml-8) of every member's window, rows [max(4, band)+4, nz-cpml-8)): four ragged windows in one group
+    with overlapping cores (placeholder waves where a member's window misses a band of the union) and receiver rows
+    inside the core; five full-width shots (a group of one: three placeholder waves in its core tiles); a core four
+    rows tall without the FP64 band."""
+    out = []
+    w = small_workload(nx=320, nz=72, nt=120, cpml=12, nshots=4)
+    w.shots = [M.Shot(x0=0, nx=300, isx=60, isz=6, irz=6, ir0=20, nrec=250),
+               M.Shot(x0=20, nx=280, isx=90, isz=30, irz=30, ir0=12, nrec=250),
+               M.Shot(x0=10, nx=250, isx=40, isz=10, irz=10, ir0=14, nrec=200),
+               M.Shot(x0=40, nx=200, isx=100, isz=6, irz=40, ir0=20, nrec=150)]
+    out.append(w)
+    w = small_workload(nx=200, nz=60, nt=110, cpml=10, nshots=5)
+    for k, s in enumerate(w.shots):
+        w.shots[k] = M.Shot(x0=0, nx=200, isx=s.isx, isz=s.isz, irz=(6, 20, 9, 6, 30)[k], ir0=10, nrec=180)
+    out.append(w)
+    out.append(small_workload(nx=240, nz=28, nt=100, cpml=8, nshots=4))
+    return out
+
+
+@pytest.mark.parametrize("geometry", [0, 1, 2])
+def test_adjoint_step_pairs_identical_to_single_steps(torch, geometry):
+    """Adjoint step pairs (each gradient group's core advanced two adjoint steps per launch, the gradient slot updated
+    once per pair) against single adjoint steps: misfit and exact gradient bit for bit, with and without the FP64
+    near-surface band, an odd checkpoint interval (pairs stop where step k-1's imaging terms lie in another
+    recomputed segment), core tiles of several row-run lengths; and against the oracle."""
+    w = _adj_pair_workloads()[geometry]
+    tab, src, sq_true, sq_init = oracle_setup(w, "f32i")
+    obs_np = oracle_gathers(w, "f32i", sq_true, tab, src)
+    obs = torch.from_numpy(_cat(obs_np)).cuda()
+    model = _dev_model(torch, sq_init)
+    for surface, rows in ((0, 16), (-1, 9), (-1, 32)):
+        say(f"adjoint step pairs vs single steps: {w.nx}x{w.nz} cpml={w.cpml}, {len(w.shots)} shots, K=7, "
+            f"1 kept segment, surface_rows={surface}, pair rows {rows}")
+        res = []
+        for apairs in (rows, 0):
+            plan = _plan(w, tab, src, ckpt_interval=7, store_segments=1, adj_step_pairs=apairs,
+                         surface_rows=surface, rows_per_item=8)
+            band = plan.surface_rows
+            core = w.nz - w.cpml - 8 > max(4, band) + 4
+            assert (plan.adj_pair_rows > 0) == (bool(apairs) and core), (plan.adj_pair_rows, band)
+            phi, G, syn = plan.gradient(model, obs, want_syn=True)
+            torch.cuda.synchronize()
+            res.append((phi, G.cpu().numpy(), syn.cpu().numpy()))
+            plan.close()
+        (p1, G1, s1), (p0, G0, s0) = res
+        assert p1 == p0 and np.array_equal(s1, s0)
+        assert np.array_equal(G1, G0), (surface, rows)
+        if surface == 0:
+            phi_ref, G_ref, _ = oracle_gradient(w, "f32i", True, sq_init, tab, src, obs_np)
+            for k in range(3):
+                assert np.array_equal(G1[k], G_ref[k]), k
+
+
 def test_entry_points_refuse_null_and_pageable_arrays(torch):
     """With a live plan, a null array and a pageable host array (which a kernel would fault on) are refused with
     a message naming the argument, before any kernel runs; the plan works normally afterwards (at the true
