@@ -1718,9 +1718,10 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     return 1;
   }
   if (p->opt.item_order < -1 || p->opt.item_order > 4 || p->opt.adj_item_order < -1 || p->opt.adj_item_order > 3 ||
-      p->opt.rows_per_item < 0 || p->opt.adj_rows_per_item < 0) {
+      p->opt.rows_per_item < 0 || p->opt.adj_rows_per_item < 0 || p->opt.step_pairs < 0 || p->opt.adj_step_pairs < 0) {
     delete p;
-    set_message("item_order must be -1..4, adj_item_order -1..3, rows_per_item and adj_rows_per_item >= 0");
+    set_message("item_order must be -1..4, adj_item_order -1..3, rows_per_item, adj_rows_per_item, step_pairs and "
+                "adj_step_pairs >= 0");
     return 1;
   }
   const int nt = p->nt;
@@ -1737,7 +1738,7 @@ int fwi_plan_create(const fwi_survey* s, const fwi_options* opt, fwi_plan** out)
     strips[i] = strip_count(p->shots[i].nx, p->nz, p->cpml);
     gath[i] = (long long)nt * p->shots[i].nrec;
   }
-  // per-cell floats: 10 fwd + 12 adj + 3 grad + 2K imaging terms + 4 per checkpoint (2 in 16 bits)
+  // per-cell floats: the fixed forward, adjoint and gradient fields + 2K imaging terms + 4 per checkpoint (2 in 16 bits)
   const double ckf = (double)p->ck_esize / 4.0;  // checkpoint values per float
   const double per_cell = (double)fwi_plan::kFixedF + 2.0 * p->K + 4.0 * ckf * (p->J - 1);
   const double per_strip = 4.0 * ckf * (p->J - 1);

@@ -224,7 +224,8 @@ def test_surface_band_option_checks(torch):
         plan = _plan(w, tab, src, **kw)
         assert plan.surface_rows == rows, kw
         plan.close()
-    for kw in ({"item_order": 5}, {"adj_item_order": 4}, {"adj_rows_per_item": -2}):
+    for kw in ({"item_order": 5}, {"adj_item_order": 4}, {"adj_rows_per_item": -2}, {"step_pairs": -1},
+               {"adj_step_pairs": -1}):
         with pytest.raises(RuntimeError, match="item_order"):
             _plan(w, tab, src, **kw)
 
