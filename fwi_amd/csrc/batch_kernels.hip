@@ -1405,9 +1405,11 @@ __global__ void __launch_bounds__(256) k_adj_ex_main(AdjExArgs a) {
 // columns, member j on wave j (placeholder waves as in the single step); the lanes of a wave compute step k on
 // columns [x0-4, x0+52) and step k-1 on the tile, marching down: at row r step k at row r, step k-1 at row r-4.
 // gbar_x^{k+1} and gbar_x^k are exchanged within the wave through LDS, gbar_z^{k+1} and gbar_z^k wait in register
-// queues, mu^k, gbar_x^k and v_po in four-row delay lines; step k-1 reads mu^{k+1} of its row again (a cache hit),
-// and the imaging terms and the gradient slot on the owned lanes only (a lane off the tile would fetch a third cache
-// line per row: 10 % of the launch's time).  Per row the members' imaging terms of both
+// queues, mu^k, gbar_x^k and v_po in four-row delay lines.  mu^{k+1} is loaded once, at row r+4, and waits in a
+// per-lane ring of eight rows in LDS for its uses at rows r (step k) and r-4 (step k-1): loading it again there
+// missed the L2 (16 % more fetched bytes, 5 % of the launch's time) and holding it in registers would cost a wave
+// per SIMD.  The imaging terms and the gradient slot are read on the owned lanes only (a lane off the tile would
+// fetch a third cache line per row: 10 % of the launch's time).  Per row the members' imaging terms of both
 // steps meet in LDS and waves 0-2 update their field of the group's gradient slot once, as (g + S_k) + S_{k-1}:
 // the two single steps' additions in their order.  Every cell is computed with the single-step expressions in the
 // same order, so the results are identical to two single steps.  Stores go through buffer resources with an
@@ -1431,7 +1433,9 @@ __device__ __forceinline__ void xrow64(float* buf, int lane, float v, float (&u)
   for (int k = 0; k < 9; ++k) u[k] = buf[lane + k];
 }
 
-__device__ __forceinline__ void adj_pair_core(const AdjPairArgs& a, int4 itm, float* xb, float (*gsum)[4][6][64]) {
+// ring: the wave's 16 x 64 floats of LDS, mu_P^{k+1} and mu_F^{k+1} of row q in rows 2(q&7) and 2(q&7)+1.
+__device__ __forceinline__ void adj_pair_core(const AdjPairArgs& a, int4 itm, float* xb, float (*gsum)[4][6][64],
+                                              float (*ring)[64]) {
   const int wave = wave_id(), lane = threadIdx.x & 63;
   const bool ghost = itm.x < 0;
   if (ghost) itm.x = -itm.x - 1;
@@ -1483,15 +1487,13 @@ __device__ __forceinline__ void adj_pair_core(const AdjPairArgs& a, int4 itm, fl
   const float* res1 = res0 - sd.nrec;                                                       // step k-1
   const int zend = z1 + 4;  // step k runs on rows [z0-4, zend)
   struct Row {
-    float lc, mc, vh, vn;        // row r: mu_P^{k+1}, mu_F^{k+1}, v_hor, v_nmo
+    float vh, vn;                // row r: v_hor, v_nmo
     float l4, m4, vp4;           // row r+4: mu^{k+1}, v_po
     float lp, mp;                // row r: mu^{k+2}
   };
   auto load_row = [&](int r, Row& w) __attribute__((always_inline)) {
     const unsigned o = (unsigned)(cc + ps * r);
     const unsigned mo = (unsigned)(cc + nxm * r);
-    w.lc = ldu(L1, o);
-    w.mc = ldu(M1, o);
     w.vh = ldu(vh, mo);
     w.vn = ldu(vn, mo);
     w.l4 = ldu(L1, o + 4u * ps);
@@ -1508,13 +1510,17 @@ __device__ __forceinline__ void adj_pair_core(const AdjPairArgs& a, int4 itm, fl
   // MAIN: step k-1 at row r-4 (rows r >= z0+4); otherwise step k only
   auto iter = [&](int r, const Row& cur, Row& nxt, auto mainc) __attribute__((always_inline)) {
     constexpr bool MAIN = decltype(mainc)::value;
-    // MAIN: row r-4's mu^{k+1} (a cache hit: read four rows ago), imaging terms and gradient slot (owned lanes),
-    // issued before this row's stores so that their wait does not cover them
-    float lk1 = 0.f, mk1 = 0.f, qa0 = 0.f, qb0 = 0.f, qa1 = 0.f, qb1 = 0.f, gv = 0.f;
+    // mu^{k+1} of rows r and r-4 from the ring; row r+4 then takes row r-4's slot
+    const int sr = 2 * (r & 7), sb = 2 * ((r - 4) & 7);
+    const float lc = ring[sr][lane], mc = ring[sr + 1][lane];
+    const float lk1 = ring[sb][lane], mk1 = ring[sb + 1][lane];
+    ring[sb][lane] = cur.l4;
+    ring[sb + 1][lane] = cur.m4;
+    // MAIN: row r-4's imaging terms and gradient slot (owned lanes), issued before this row's stores so that their
+    // wait does not cover them
+    float qa0 = 0.f, qb0 = 0.f, qa1 = 0.f, qb1 = 0.f, gv = 0.f;
     if (MAIN) {
       const unsigned ob = (unsigned)(cc + ps * (r - 4));
-      lk1 = ldu(L1, ob);
-      mk1 = ldu(M1, ob);
       const unsigned oq = own ? 4u * ob : kOutOfRange;
       qa0 = bld<kBufNT>(sqa0, oq);
       qb0 = bld<kBufNT>(sqb0, oq);
@@ -1523,15 +1529,15 @@ __device__ __forceinline__ void adj_pair_core(const AdjPairArgs& a, int4 itm, fl
       gv = bld<kBufNT>(grs, gcol + grow * (r - 4));
     }
     gq1[8] = dt2 * (cur.vp4 * (cur.l4 + cur.m4));
-    const float go1 = dt2 * (cur.vh * cur.lc + cur.vn * cur.mc);
+    const float go1 = dt2 * (cur.vh * lc + cur.vn * mc);
     float u[9];
     xrow64(xa, lane, go1, u);
     load_row(min(r + 2, zend - 1), nxt);  // two rows ahead; unused repeats on the last rows keep one path
     // step k at row r (the single step's expressions; no C-PML term in the core and its halo)
     const float ap = d2s<float>(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8], dh2);
     const float af = d2s<float>(gq1[0], gq1[1], gq1[2], gq1[3], gq1[4], gq1[5], gq1[6], gq1[7], gq1[8], dh2);
-    float ln = 2.f * cur.lc + ap - cur.lp;
-    float mn = 2.f * cur.mc + af - cur.mp;
+    float ln = 2.f * lc + ap - cur.lp;
+    float mn = 2.f * mc + af - cur.mp;
     if (r == sd.irz && rec) {
       const float rv = res0[0];
       ln += rv;
@@ -1597,7 +1603,10 @@ __device__ __forceinline__ void adj_pair_core(const AdjPairArgs& a, int4 itm, fl
   for (int k = 0; k < 8; ++k) {
     const int r = z0 - 8 + k;
     const unsigned o = (unsigned)(cc + ps * r);
-    gq1[k] = dt2 * (ldu(vp, (unsigned)(cc + nxm * r)) * (ldu(L1, o) + ldu(M1, o)));
+    const float l = ldu(L1, o), m = ldu(M1, o);
+    ring[2 * (r & 7)][lane] = l;
+    ring[2 * (r & 7) + 1][lane] = m;
+    gq1[k] = dt2 * (ldu(vp, (unsigned)(cc + nxm * r)) * (l + m));
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) dvp[k] = ldu(vp, (unsigned)(cc + nxm * (z0 - 4 + k)));
@@ -1638,6 +1647,7 @@ __device__ __forceinline__ void adj_pair_core(const AdjPairArgs& a, int4 itm, fl
 __global__ void __launch_bounds__(256) k_adj_pair(AdjPairArgs a) {
   __shared__ float xbuf[4][152];
   __shared__ float gsum[2][4][6][64];
+  __shared__ float ring[4][16][64];
   const int wave = wave_id();
   const int bid = (int)blockIdx.x;
   if (bid < a.nshell_wg) {
@@ -1647,7 +1657,7 @@ __global__ void __launch_bounds__(256) k_adj_pair(AdjPairArgs a) {
   }
   const int b = xcd_logical_block(bid - a.nshell_wg, (int)gridDim.x - a.nshell_wg);
   if (4 * b >= a.nitems) return;
-  adj_pair_core(a, a.items[4 * b + wave], xbuf[wave], gsum);
+  adj_pair_core(a, a.items[4 * b + wave], xbuf[wave], gsum, ring[wave]);
 }
 
 // Step k-1 of a step pair's shell: the single-step body on items with column ranges.
