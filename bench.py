@@ -443,6 +443,7 @@ def main():
         nband = sum((band - 4) * s.nx for s in w.shots) if band else 0
         ncore = plan.core_cells()
         nacore = plan.adj_core_cells()
+        nbat = -(-len(w.shots) // plan.batch_size)  # batches of a gradient, each with the same launches
         grouped = bool(plan.adj_tiling[2])
         inc = args.arith == 3
         traffic, tnote = load_traffic(w, len(w.shots), tiling_of(plan))
@@ -451,7 +452,9 @@ def main():
             if not n:
                 continue
             mean = ms / n / 1e3
-            survey, design = kernel_bytes(kind, ncell, nband, nstrip, nmodel, ncore, grouped, inc, nacore)
+            # a launch covers one batch of shots: the mean launch moves the cells of the mean batch
+            survey, design = kernel_bytes(kind, ncell / nbat, nband / nbat, nstrip / nbat, nmodel, ncore / nbat,
+                                          grouped, inc, nacore / nbat)
             per_grad = plan.kernel_launches(kind)
             tb = traffic_of(traffic, name)
             k = {"kernel": name, "does": what, "launches_per_gradient": per_grad, "mean_launch_us": round(mean * 1e6, 1),

@@ -1386,9 +1386,12 @@ struct fwi_plan {
         cands.push_back({ms, o, c});
       }
     pick(cands, tz, order);
-    // adjoint, imaging into zeroed scratch slots
-    float* qa = qbuf();
-    CK(hipMemsetAsync(qa, 0, 2 * (size_t)cap_cell * sizeof(float), st));
+    // adjoint, imaging into zeroed scratch slots: step n reads slot n mod nq, as the gradient reads a slot per step
+    // (one slot for every step would let a step pair read its second step's terms from the cache)
+    const int nq = std::min(qslots(), 8);
+    const size_t qslot = 2 * (size_t)cap_cell;
+    auto qa_of = [&](int n) { return qbuf() + (size_t)(n % nq) * qslot; };
+    CK(hipMemsetAsync(qbuf(), 0, (size_t)nq * qslot * sizeof(float), st));
     cands.clear();
     for (int o : order_a)
       for (int c : rows_a) {
@@ -1400,13 +1403,13 @@ struct fwi_plan {
         }
         CK(hipMemsetAsync(adj(0), 0, kAdjF * (size_t)cap_cell * sizeof(float), st));
         for (int w = 0; w < 2; ++w)
-          if (int r = adj_step(b, m, nt - w, qa, qa + cap_cell, st)) return r;
+          if (int r = adj_step(b, m, nt - w, qa_of(w), qa_of(w) + cap_cell, st)) return r;
         float ms = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
           float t = 0;
           if (int r = timed([&]() -> int {
                 for (int n = 2; n < 8; ++n)
-                  if (int r2 = adj_step(b, m, nt - n, qa, qa + cap_cell, st)) return r2;
+                  if (int r2 = adj_step(b, m, nt - n, qa_of(n), qa_of(n) + cap_cell, st)) return r2;
                 return 0;
               }, t))
             return r;
@@ -1425,13 +1428,14 @@ struct fwi_plan {
         ms = 1e30f;
         CK(hipMemsetAsync(adj(0), 0, kAdjF * (size_t)cap_cell * sizeof(float), st));
         for (int w = 0; w < 2; ++w)
-          if (int r = adj_step(b, m, nt - w, qa, qa + cap_cell, st)) return r;
+          if (int r = adj_step(b, m, nt - w, qa_of(w), qa_of(w) + cap_cell, st)) return r;
         for (int rep = 0; rep < 2; ++rep) {
           float t = 0;
           if (int r = timed([&]() -> int {
                 for (int n = 2; n < 8; n += pr ? 2 : 1)
-                  if (int r2 = pr ? adj_pair(b, m, nt - n, qa, qa + cap_cell, qa, qa + cap_cell, st)
-                                  : adj_step(b, m, nt - n, qa, qa + cap_cell, st))
+                  if (int r2 = pr ? adj_pair(b, m, nt - n, qa_of(n), qa_of(n) + cap_cell, qa_of(n + 1),
+                                             qa_of(n + 1) + cap_cell, st)
+                                  : adj_step(b, m, nt - n, qa_of(n), qa_of(n) + cap_cell, st))
                     return r2;
                 return 0;
               }, t))
