@@ -443,7 +443,7 @@ def main():
         nband = sum((band - 4) * s.nx for s in w.shots) if band else 0
         ncore = plan.core_cells()
         nacore = plan.adj_core_cells()
-        nbat = -(-len(w.shots) // plan.batch_size)  # batches of a gradient, each with the same launches
+        nbat = plan.batch_count  # batches of a gradient, each with the same launches
         grouped = bool(plan.adj_tiling[2])
         inc = args.arith == 3
         traffic, tnote = load_traffic(w, len(w.shots), tiling_of(plan))

@@ -73,6 +73,7 @@ BATCHED = {
     "fwi_plan_set_cpml": ([vp_, ctypes.POINTER(c_float), ctypes.POINTER(c_float)], c_int),
     "fwi_plan_gather_size": ([vp_], c_ll),
     "fwi_plan_batch_size": ([vp_], c_int),
+    "fwi_plan_batch_count": ([vp_], c_int),
     "fwi_plan_workspace_bytes": ([vp_], c_double),
     "fwi_forward": ([vp_, _F, _F, _F, _F, vp_], c_int),
     "fwi_misfit": ([vp_, _F, _F, _F, _F, ctypes.POINTER(c_double), vp_], c_int),

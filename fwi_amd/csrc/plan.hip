@@ -1940,6 +1940,7 @@ int fwi_plan_set_cpml(fwi_plan* p, const float* a, const float* b) {
 
 long long fwi_plan_gather_size(const fwi_plan* p) { return p->gather_total; }
 int fwi_plan_batch_size(const fwi_plan* p) { return p->batches.empty() ? 0 : p->batches[0].count; }
+int fwi_plan_batch_count(const fwi_plan* p) { return (int)p->batches.size(); }
 double fwi_plan_workspace_bytes(const fwi_plan* p) { return (double)p->work_bytes; }
 
 int fwi_forward(fwi_plan* p, const float* vh, const float* vp, const float* vn, float* gathers, void* stream) {

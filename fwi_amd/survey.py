@@ -123,6 +123,10 @@ class Plan:
         return int(_lib.load().fwi_plan_batch_size(self._h))
 
     @property
+    def batch_count(self) -> int:
+        return int(_lib.load().fwi_plan_batch_count(self._h))
+
+    @property
     def workspace_bytes(self) -> float:
         return float(_lib.load().fwi_plan_workspace_bytes(self._h))
 

@@ -114,8 +114,9 @@ int fwi_plan_set_cpml(fwi_plan* plan, const float* cpml_a, const float* cpml_b);
 
 /* Size of the gather array of the whole survey: sum over shots of nt*nrec (floats); layout [shot][it][rec]. */
 long long fwi_plan_gather_size(const fwi_plan* plan);
-/* Shots per batch and number of batches chosen by the plan. */
+/* Shots in the first batch and number of batches chosen by the plan (every batch runs the same launches). */
 int fwi_plan_batch_size(const fwi_plan* plan);
+int fwi_plan_batch_count(const fwi_plan* plan);
 /* Device bytes held by the plan. */
 double fwi_plan_workspace_bytes(const fwi_plan* plan);
 

@@ -254,6 +254,7 @@ def test_checkpoint_interval_and_batching_invariance(torch):
                              grad_group=grp, store_segments=keep)
                 if keep == 1 and K in (7, 13):
                     assert plan.kept_steps < w.nt // 2
+                assert plan.batch_size == (mb or 3) and plan.batch_count == -(-3 // (mb or 3))
                 phi, G, _ = plan.gradient(model, obs)
                 torch.cuda.synchronize()
                 G = G.cpu().numpy()
