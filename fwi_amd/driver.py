@@ -55,6 +55,9 @@ def run(w: M.Workload, stages=(None,), iterations: int = 100, memory: int = 5, i
         plan = Plan(w.nx, w.nz, w.nt, w.dh, w.dt, w.cpml, w.shots, src, cpml_table(p), grad_mode=grad_mode,
                     **plan_kw)
         obs = plan.forward(true_sq)
+        if verbose:
+            torch.cuda.synchronize()
+            print(json.dumps({"stage_f": f, "observed_data_s": round(time.perf_counter() - t0, 3)}), flush=True)
         opt = LBFGS(w.nx, w.nz, memory)
         ls = LineSearch()
         misfits, trials = [], []
@@ -73,7 +76,11 @@ def run(w: M.Workload, stages=(None,), iterations: int = 100, memory: int = 5, i
             def evaluate(alpha):
                 opt.trial(model, alpha)
                 bound()
-                return plan.misfit(model, obs)
+                m = plan.misfit(model, obs)
+                if verbose:
+                    print(json.dumps({"stage_f": f, "iteration": n, "trial_alpha": alpha, "trial_misfit": m}),
+                          flush=True)
+                return m
 
             acc, nt_ = ls.run(phi, evaluate, lambda: opt.restore(model))
             trials.append(nt_)
