@@ -7,8 +7,10 @@ recomputation, imaging, FP64 cross-shot sum) of the batched path.  value = Gcell
 
 Beside it, on the same workload:
   * plain per-shot GPU path: the reference's per-shot flow on the legacy C ABI (launchPropagation with a
-    pinned-host snapshot every step, launchBackPropagation), timed on one shot for a bounded number of
-    steps and scaled to the full survey (it is linear in shots and steps);
+    pinned-host snapshot every step, launchBackPropagation), timed over every shot and step of the gradient
+    when that is at most PLAIN_FULL_CELL_STEPS cell-steps (survey1600: about 75 s), else on a sample of shots and
+    steps scaled to the survey.  Short samples overstate it: this host-bound flow runs its steps faster the
+    longer it runs (survey1600: 0.51 ms per step for one shot, 0.29 ms over all 64 shots);
   * roofline of every step kernel and of the dominant one (the largest share of the gradient's kernel time):
     algorithmic bytes per launch (the survey's count, and this design's minimum) over its average duration,
     measured with HIP events on the stream the kernel runs on during an instrumented gradient, beside the
@@ -34,6 +36,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PLAIN_FULL_CELL_STEPS = 2e11  # largest plain-path gradient timed in full (survey1600: 1.64e11)
 
 
 def log(msg):
@@ -68,7 +71,10 @@ def parse():
     ap.add_argument("--adj-order", type=int, default=-1, help="adjoint work-item order (-1 = from --order or tuned)")
     ap.add_argument("--retune", action="store_true",
                     help="let the tuner choose the tiling even where the counter profile names one")
-    ap.add_argument("--plain-steps", type=int, default=1000)
+    ap.add_argument("--plain-steps", type=int, default=0,
+                    help="steps per shot of the plain path (0: all, or 1000 when the full gradient is too large)")
+    ap.add_argument("--plain-shots", type=int, default=0,
+                    help="shots the plain path runs, the rest extrapolated (0: all, or 1 when too large)")
     ap.add_argument("--cpu-steps", type=int, default=4000)
     ap.add_argument("--cpu1-steps", type=int, default=200, help="steps of the single-thread reference-arithmetic leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the reference-arithmetic (mixed) GPU gradient")
@@ -83,9 +89,10 @@ def parse():
     return ap.parse_args()
 
 
-def plain_path_seconds(w, model_sq_dev, src_dev, ab, nsteps):
-    """Reference flow on the legacy ABI for one shot and `nsteps` steps: forward with a pinned snapshot
-    per step (launchPropagation), residual, adjoint with the snapshot copy back (launchBackPropagation)."""
+def plain_path_seconds(w, model_sq_dev, src_dev, ab, nsteps, nshots=1):
+    """Reference flow on the legacy ABI for `nshots` shots one after the other (spread over the survey; the middle
+    one when 1) and `nsteps` steps each: the shot's fields cleared, forward with a pinned snapshot per step
+    (launchPropagation), residual, adjoint with the snapshot copy back (launchBackPropagation)."""
     import numpy as np
     import torch
 
@@ -95,7 +102,8 @@ def plain_path_seconds(w, model_sq_dev, src_dev, ab, nsteps):
     nx, nz, cpml = w.nx, w.nz, w.cpml
     n = nx * nz
     nt = nsteps
-    shot = w.shots[len(w.shots) // 2]
+    picks = [w.shots[len(w.shots) // 2]] if nshots <= 1 else \
+        [w.shots[(k * len(w.shots)) // nshots] for k in range(min(nshots, len(w.shots)))]
     z = lambda k: torch.zeros(k, dtype=torch.float32, device="cuda")
     names = ("p2", "r2", "psx", "psz", "zx", "zz", "atp", "atf", "PF", "aux1", "aux2", "aux3", "aux4", "p2cx", "p2cz",
              "r2dx", "r2cz", "psixcx", "psixcz", "atenxcx", "atenxcz", "atenzdx", "atenzcz", "gcx", "gcz", "gdx")
@@ -117,33 +125,45 @@ def plain_path_seconds(w, model_sq_dev, src_dev, ab, nsteps):
     s1 = _lib.create_blocking_stream()
     s2 = _lib.create_blocking_stream()
     ptr = lambda k: f[k].data_ptr()
-    sx, sz = shot.isx + 1, shot.isz + 1
+    per_shot = [k for k in names if k not in ("gcx", "gcz", "gdx")]  # the gradient sums over the shots
+    def run_shots(shots):
+        for shot in shots:
+            sx, sz = shot.isx + 1, shot.isz + 1
+            for k in per_shot:
+                f[k].zero_()
+            for t in (Pa, Pb, Pre2):
+                t.zero_()
+            Ppre, Ppast = Pa, Pb
+            for it in range(nt):
+                L.launchPropagation(
+                    s1, s2, PFh.data_ptr() + 8 * it * n, ptr("PF"), Ppast.data_ptr(), Ppre.data_ptr(),
+                    Ppast.data_ptr() + 4 * n, Ppre.data_ptr() + 4 * n, ptr("p2"), ptr("r2"), dh2, nx, nz,
+                    ax.data_ptr(), bx.data_ptr(), az.data_ptr(), bz.data_ptr(), ptr("psx"), ptr("psz"), dh, cpml,
+                    ptr("zx"), ptr("zz"), ptr("atp"), ptr("atf"), vh.data_ptr(), vp.data_ptr(), vn.data_ptr(),
+                    src_dev.data_ptr(), shot2.data_ptr(), C, sx, sz, it)
+                Ppre, Ppast = Ppast, Ppre
+            L.res_h(Res.data_ptr(), shot1.data_ptr(), shot2.data_ptr(), nx, nt)
+            torch.cuda.synchronize()
+            Pa.zero_()
+            Pb.zero_()
+            Ppre, Ppast = Pa, Pb
+            for it in range(nt):
+                L.launchBackPropagation(
+                    s1, s2, nx, nz, nt, PFh.data_ptr() + 8 * (nt - it - 1) * n, Ppre.data_ptr(), Ppast.data_ptr(),
+                    Ppre.data_ptr() + 4 * n, Ppast.data_ptr() + 4 * n, ptr("aux1"), ptr("aux2"), ptr("aux3"),
+                    ptr("aux4"), vh.data_ptr(), vp.data_ptr(), vn.data_ptr(), ptr("p2cx"), ptr("p2cz"), ptr("r2dx"),
+                    ptr("r2cz"), dh2, Pre2.data_ptr(), Pre2.data_ptr() + 4 * n, ax.data_ptr(), bx.data_ptr(),
+                    az.data_ptr(), bz.data_ptr(), ptr("psixcx"), ptr("psixcz"), ptr("psx"), ptr("psz"), dh, cpml,
+                    ptr("zx"), ptr("zz"), ptr("atf"), ptr("atp"), ptr("atenxcx"), ptr("atenxcz"), ptr("atenzdx"),
+                    ptr("atenzcz"), Res.data_ptr(), C, ptr("PF"), it, sx, sz, dt, ptr("gcx"), ptr("gcz"), ptr("gdx"))
+                Ppre, Ppast = Ppast, Ppre
+
+    # one untimed pass over the first shot: a first pass takes about twice as long (first launches, first copies
+    # into the pinned snapshot buffer), and the timed shots stand for shots that run after it
+    run_shots(picks[:1])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    Ppre, Ppast = Pa, Pb
-    for it in range(nt):
-        L.launchPropagation(s1, s2, PFh.data_ptr() + 8 * it * n, ptr("PF"), Ppast.data_ptr(), Ppre.data_ptr(),
-                            Ppast.data_ptr() + 4 * n, Ppre.data_ptr() + 4 * n, ptr("p2"), ptr("r2"), dh2, nx, nz,
-                            ax.data_ptr(), bx.data_ptr(), az.data_ptr(), bz.data_ptr(), ptr("psx"), ptr("psz"), dh,
-                            cpml, ptr("zx"), ptr("zz"), ptr("atp"), ptr("atf"), vh.data_ptr(), vp.data_ptr(),
-                            vn.data_ptr(), src_dev.data_ptr(), shot2.data_ptr(), C, sx, sz, it)
-        Ppre, Ppast = Ppast, Ppre
-    L.res_h(Res.data_ptr(), shot1.data_ptr(), shot2.data_ptr(), nx, nt)
-    torch.cuda.synchronize()
-    Pa.zero_()
-    Pb.zero_()
-    Ppre, Ppast = Pa, Pb
-    for it in range(nt):
-        L.launchBackPropagation(s1, s2, nx, nz, nt, PFh.data_ptr() + 8 * (nt - it - 1) * n, Ppre.data_ptr(),
-                                Ppast.data_ptr(), Ppre.data_ptr() + 4 * n, Ppast.data_ptr() + 4 * n, ptr("aux1"),
-                                ptr("aux2"), ptr("aux3"), ptr("aux4"), vh.data_ptr(), vp.data_ptr(), vn.data_ptr(),
-                                ptr("p2cx"), ptr("p2cz"), ptr("r2dx"), ptr("r2cz"), dh2, Pre2.data_ptr(),
-                                Pre2.data_ptr() + 4 * n, ax.data_ptr(), bx.data_ptr(), az.data_ptr(), bz.data_ptr(),
-                                ptr("psixcx"), ptr("psixcz"), ptr("psx"), ptr("psz"), dh, cpml, ptr("zx"), ptr("zz"),
-                                ptr("atf"), ptr("atp"), ptr("atenxcx"), ptr("atenxcz"), ptr("atenzdx"),
-                                ptr("atenzcz"), Res.data_ptr(), C, ptr("PF"), it, sx, sz, dt, ptr("gcx"), ptr("gcz"),
-                                ptr("gdx"))
-        Ppre, Ppast = Ppast, Ppre
+    run_shots(picks)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     _lib.check_last("plain path")
@@ -533,11 +553,13 @@ def main():
 
     plain = None
     if not args.no_plain:
-        ns = min(args.plain_steps, nt)
-        # the snapshot buffer of the plain path is 2*ns*nx*nz floats of pinned host memory
-        s_plain = plain_path_seconds(w, init_sq, torch.from_numpy(src[:ns].copy()).cuda(), ab, ns)
-        plain = s_plain * (nt / ns) * len(w.shots)
-        log(f"plain per-shot path: {s_plain:.3f} s for 1 shot x {ns} steps -> {plain:.1f} s per gradient")
+        # the snapshot buffer of the plain path is 2*pns*nx*nz floats of pinned host memory (20.5 GB for survey1600)
+        full = len(w.shots) * nt * w.nx * w.nz <= PLAIN_FULL_CELL_STEPS
+        pns = min(args.plain_steps or (nt if full else 1000), nt)
+        pnsh = max(1, min(args.plain_shots or (len(w.shots) if full else 1), len(w.shots)))
+        s_plain = plain_path_seconds(w, init_sq, torch.from_numpy(src[:pns].copy()).cuda(), ab, pns, pnsh)
+        plain = s_plain * (nt / pns) * (len(w.shots) / pnsh)
+        log(f"plain per-shot path: {s_plain:.3f} s for {pnsh} shot(s) x {pns} steps -> {plain:.1f} s per gradient")
 
     cpu = None
     if not args.no_cpu:
@@ -598,7 +620,7 @@ def main():
                        "nodes_gradient": gnodes[1]},
             "s_per_gradient_eager_instrumented": round(sec_instr, 4) if sec_instr else None,
             "s_per_gradient_plain": round(plain, 3) if plain else None,
-            "plain_steps_timed": min(args.plain_steps, nt) if plain else None,
+            "plain_timed": f"{pnsh} shots x {pns} steps" if plain else None,
             "speedup_vs_plain": round(plain / sec, 2) if plain else None,
             "misfit": phi,
         },
